@@ -1,0 +1,217 @@
+"""brpa -- MI355X (gfx950) correlation-energy hot path of beyond-RPA.
+
+Thin Python layer over the C ABI of libbrpa.so (include/brpa.h).  Arrays may
+be numpy (host) or torch CUDA tensors (device, float64, contiguous); the
+library detects device pointers and otherwise stages through its own device
+workspace.  There is no CPU fallback: without the built library every call
+raises.
+
+Layouts (bit-identical to the reference's Fortran arrays):
+  R[ai][P]  (o*v, N)  == Fortran CRG(P, ai) / Rkai(P, a, i),  ai = i*v + a
+  Pi[u][Q][P]         == Fortran PiU(P, Q, u)
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import FP64, SPLIT_BF16  # noqa: F401
+
+PRECISIONS = {"fp64": FP64, "split_bf16": SPLIT_BF16}
+
+
+class BrpaError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("brpa error %d: %s" % (code, msg))
+        self.code = code
+
+
+def _is_torch(x):
+    return type(x).__module__.startswith("torch")
+
+
+def _ptr(x):
+    if x is None:
+        return None
+    if _is_torch(x):
+        if x.dtype.itemsize != 8 or not x.is_contiguous():
+            raise ValueError("torch tensors must be contiguous float64")
+        return C.c_void_p(x.data_ptr())
+    if not (isinstance(x, np.ndarray) and x.dtype == np.float64 and x.flags.c_contiguous):
+        raise ValueError("numpy arrays must be C-contiguous float64")
+    return C.c_void_p(x.ctypes.data)
+
+
+def _f64(x):
+    if _is_torch(x):
+        return x.contiguous()
+    return np.ascontiguousarray(x, dtype=np.float64)
+
+
+def _prec(p):
+    if isinstance(p, str):
+        return PRECISIONS[p]
+    return int(p)
+
+
+def freq_grid(eocc, evir, nfreqs):
+    """Fixed-size Casimir-Polder grid of one fragment (host code in the library).
+
+    Returns (freqs, weights, info) with info = dict(alpha, rmsd, max_rel_error).
+    """
+    lib = _lib.load()
+    eocc = np.ascontiguousarray(eocc, np.float64)
+    evir = np.ascontiguousarray(evir, np.float64)
+    x = np.zeros(nfreqs)
+    w = np.zeros(nfreqs)
+    info = np.zeros(3)
+    rc = lib.brpa_freq_grid(_ptr(eocc), eocc.size, _ptr(evir), evir.size, nfreqs, _ptr(x),
+                            _ptr(w), _ptr(info))
+    if rc != 0:
+        raise BrpaError(rc, "brpa_freq_grid failed")
+    return x, w, dict(alpha=info[0], rmsd=info[1], max_rel_error=info[2])
+
+
+def optimize_freq_grid(eocc, evir, target_error=1e-6, target_rel_error=1e-3, min_nfreqs=8,
+                       max_nfreqs=64):
+    lib = _lib.load()
+    eocc = np.ascontiguousarray(eocc, np.float64)
+    evir = np.ascontiguousarray(evir, np.float64)
+    x = np.zeros(max_nfreqs)
+    w = np.zeros(max_nfreqs)
+    info = np.zeros(3)
+    n = C.c_int64(0)
+    rc = lib.brpa_optimize_freq_grid(_ptr(eocc), eocc.size, _ptr(evir), evir.size, target_error,
+                                     target_rel_error, min_nfreqs, max_nfreqs, _ptr(x), _ptr(w),
+                                     C.byref(n), _ptr(info))
+    if rc != 0:
+        raise BrpaError(rc, "brpa_optimize_freq_grid failed")
+    k = n.value
+    return x[:k].copy(), w[:k].copy(), dict(alpha=info[0], rmsd=info[1], max_rel_error=info[2],
+                                            nfreqs=k)
+
+
+class Context:
+    """One device, one HIP stream; not shared between threads."""
+
+    def __init__(self, device=0):
+        self._lib = _lib.load()
+        h = C.c_void_p()
+        rc = self._lib.brpa_ctx_create(device, C.byref(h))
+        if rc != 0:
+            raise BrpaError(rc, "brpa_ctx_create(device=%d) failed" % device)
+        self._h = h
+        self.device = device
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.brpa_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- plumbing --------------------------------------------------------
+    def _check(self, rc):
+        if rc != 0:
+            msg = self._lib.brpa_last_error(self._h)
+            raise BrpaError(rc, msg.decode() if msg else "")
+
+    def _sync_stream(self, *arrays):
+        """Run on torch's current stream when torch tensors are involved."""
+        if any(_is_torch(a) for a in arrays if a is not None):
+            import torch
+            s = torch.cuda.current_stream().cuda_stream
+            self._lib.brpa_ctx_set_stream(self._h, C.c_void_p(s))
+        else:
+            self._lib.brpa_ctx_set_stream(self._h, None)
+
+    def last_kernel_ms(self):
+        return self._lib.brpa_last_kernel_ms(self._h)
+
+    def last_stage_ms(self):
+        """{stage: (ms, launches)} of the last E_RPA call."""
+        names = {"pi": _lib.STAGE_PI, "split": _lib.STAGE_SPLIT, "chol": _lib.STAGE_CHOL,
+                 "aux": _lib.STAGE_AUX}
+        return {k: (self._lib.brpa_last_stage_ms(self._h, v),
+                    self._lib.brpa_last_stage_launches(self._h, v)) for k, v in names.items()}
+
+    def trim(self):
+        self._check(self._lib.brpa_ctx_trim(self._h))
+
+    # -- reference-equivalent entry points -------------------------------
+    def gpiug_mo(self, gpiug, R, eocc, evir, freq, prefactor=4.0):
+        """gpiug += prefactor * R^T diag(d/(d^2+u^2)) R in place (rpa_GPiUG_MO)."""
+        nai, naux = R.shape
+        eocc, evir = np.ascontiguousarray(eocc, np.float64), np.ascontiguousarray(evir, np.float64)
+        if nai != eocc.size * evir.size or tuple(gpiug.shape) != (naux, naux):
+            raise ValueError("shape mismatch")
+        self._sync_stream(gpiug, R)
+        self._check(self._lib.brpa_gpiug_mo(self._h, _ptr(gpiug), _ptr(R), naux, _ptr(eocc),
+                                            eocc.size, _ptr(evir), evir.size, float(freq),
+                                            float(prefactor)))
+        return gpiug
+
+    def ecrpa_analytic(self, piu, freqs, weights):
+        """E_RPA from stored Pi(u) (rpa_CC_EcRPA_Analytic).  Returns (E, E(u))."""
+        nf, n, n2 = piu.shape
+        if n != n2:
+            raise ValueError("Pi(u) must be square")
+        freqs = np.ascontiguousarray(freqs, np.float64)
+        weights = np.ascontiguousarray(weights, np.float64)
+        e = np.zeros(1)
+        eu = np.zeros(nf)
+        self._sync_stream(piu)
+        self._check(self._lib.brpa_ecrpa_analytic(self._h, _ptr(e), _ptr(piu), _ptr(freqs),
+                                                  _ptr(weights), nf, n, _ptr(eu)))
+        return float(e[0]), eu
+
+    def ecrpa(self, R, eocc, evir, freqs, weights, precision="fp64"):
+        """Fused E_RPA of one fragment.  Returns (E, E(u))."""
+        nai, naux = R.shape
+        eocc, evir = np.ascontiguousarray(eocc, np.float64), np.ascontiguousarray(evir, np.float64)
+        freqs = np.ascontiguousarray(freqs, np.float64)
+        weights = np.ascontiguousarray(weights, np.float64)
+        if nai != eocc.size * evir.size or freqs.size != weights.size:
+            raise ValueError("shape mismatch")
+        e = np.zeros(1)
+        eu = np.zeros(freqs.size)
+        self._sync_stream(R)
+        self._check(self._lib.brpa_ecrpa(self._h, _prec(precision), _ptr(R), naux, _ptr(eocc),
+                                         eocc.size, _ptr(evir), evir.size, _ptr(freqs),
+                                         _ptr(weights), freqs.size, _ptr(e), _ptr(eu)))
+        return float(e[0]), eu
+
+    def ecrpa_batch(self, frags, precision="fp64"):
+        """E_RPA of many fragments in batched launches.
+
+        frags: sequence of (R, eocc, evir, freqs, weights).  Returns an array of E.
+        """
+        keep = []
+        arr = (_lib.brpa_frag_t * len(frags))()
+        for i, (R, eocc, evir, freqs, weights) in enumerate(frags):
+            eocc = np.ascontiguousarray(eocc, np.float64)
+            evir = np.ascontiguousarray(evir, np.float64)
+            freqs = np.ascontiguousarray(freqs, np.float64)
+            weights = np.ascontiguousarray(weights, np.float64)
+            keep.append((R, eocc, evir, freqs, weights))
+            nai, naux = R.shape
+            if nai != eocc.size * evir.size or freqs.size != weights.size:
+                raise ValueError("fragment %d: shape mismatch" % i)
+            arr[i] = _lib.brpa_frag_t(eocc.size, evir.size, naux, freqs.size, _ptr(R).value,
+                                      _ptr(eocc).value, _ptr(evir).value, _ptr(freqs).value,
+                                      _ptr(weights).value)
+        e = np.zeros(len(frags))
+        self._sync_stream(*[k[0] for k in keep])
+        self._check(self._lib.brpa_ecrpa_batch(self._h, _prec(precision), len(frags), arr,
+                                               _ptr(e)))
+        return e

@@ -1,0 +1,68 @@
+"""ctypes binding of libbrpa.so (declarations mirror include/brpa.h)."""
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libbrpa.so")
+
+OK, ERR_ARG, ERR_HIP, ERR_NOT_SPD, ERR_GAP, ERR_NONFINITE, ERR_OOM, ERR_GRID, ERR_SOLVER = range(9)
+FP64, SPLIT_BF16 = 0, 1
+STAGE_PI, STAGE_SPLIT, STAGE_CHOL, STAGE_AUX = range(4)
+
+_i64 = C.c_int64
+_dp = C.POINTER(C.c_double)
+_vp = C.c_void_p
+
+
+class brpa_frag_t(C.Structure):
+    _fields_ = [("nocc", _i64), ("nvir", _i64), ("naux", _i64), ("nfreqs", _i64),
+                ("R", _vp), ("eocc", _vp), ("evir", _vp), ("freqs", _vp), ("weights", _vp)]
+
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "brpa_ctx_create": (C.c_int, [C.c_int, C.POINTER(_vp)]),
+    "brpa_ctx_destroy": (C.c_int, [_vp]),
+    "brpa_last_error": (C.c_char_p, [_vp]),
+    "brpa_ctx_set_stream": (C.c_int, [_vp, _vp]),
+    "brpa_ctx_trim": (C.c_int, [_vp]),
+    "brpa_last_kernel_ms": (C.c_double, [_vp]),
+    "brpa_last_stage_ms": (C.c_double, [_vp, C.c_int]),
+    "brpa_last_stage_launches": (C.c_int, [_vp, C.c_int]),
+    "brpa_freq_grid": (C.c_int, [_vp, _i64, _vp, _i64, _i64, _vp, _vp, _vp]),
+    "brpa_optimize_freq_grid": (C.c_int, [_vp, _i64, _vp, _i64, C.c_double, C.c_double, _i64,
+                                          _i64, _vp, _vp, C.POINTER(_i64), _vp]),
+    "brpa_gpiug_mo": (C.c_int, [_vp, _vp, _vp, _i64, _vp, _i64, _vp, _i64, C.c_double,
+                                C.c_double]),
+    "brpa_ecrpa_analytic": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _i64, _i64, _vp]),
+    "brpa_ecrpa": (C.c_int, [_vp, C.c_int, _vp, _i64, _vp, _i64, _vp, _i64, _vp, _vp, _i64,
+                             _vp, _vp]),
+    "brpa_ecrpa_batch": (C.c_int, [_vp, C.c_int, _i64, C.POINTER(brpa_frag_t), _vp]),
+}
+
+_lib = None
+
+
+def load():
+    """Load libbrpa.so; fail loudly when the HIP library has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("libbrpa.so not found at %s: build it with "
+                           "`python beyond-rpa_amd/build.py` (there is no CPU fallback)" % LIB_PATH)
+    # torch ships its own libamdhip64 (same SONAME, requested under another
+    # name by torch's libraries).  Loading torch first makes libbrpa bind to
+    # that runtime, so a process that uses both has one HIP runtime, not two.
+    if os.environ.get("BRPA_NO_TORCH", "0") != "1":
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
+    lib = C.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib

@@ -1,0 +1,165 @@
+// Context, workspace and error plumbing of the C ABI.
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+
+#include "internal.h"
+
+int brpa_fail(brpa_ctx *ctx, int code, const char *fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if (ctx) ctx->err = buf;
+    return code;
+}
+
+bool is_device_ptr(const void *p) {
+    if (!p) return false;
+    hipPointerAttribute_t attr;
+    std::memset(&attr, 0, sizeof(attr));
+    hipError_t e = hipPointerGetAttributes(&attr, p);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged;
+}
+
+int to_device(brpa_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    if (bytes == 0) return BRPA_OK;
+    hipMemcpyKind kind = is_device_ptr(src) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    BRPA_HIP(ctx, hipMemcpyAsync(dst, src, bytes, kind, ctx->stream));
+    return BRPA_OK;
+}
+
+int from_device(brpa_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    if (bytes == 0) return BRPA_OK;
+    hipMemcpyKind kind = is_device_ptr(dst) ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    BRPA_HIP(ctx, hipMemcpyAsync(dst, src, bytes, kind, ctx->stream));
+    return BRPA_OK;
+}
+
+int ws_reserve(brpa_ctx *ctx, size_t bytes, Arena *arena) {
+    if (bytes > ctx->ws_cap) {
+        BRPA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        if (ctx->ws) {
+            BRPA_HIP(ctx, hipFree(ctx->ws));
+            ctx->ws = nullptr;
+            ctx->ws_cap = 0;
+        }
+        size_t cap = bytes + bytes / 8 + (64u << 20);
+        hipError_t e = hipMalloc(&ctx->ws, cap);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            ctx->ws = nullptr;
+            return brpa_fail(ctx, BRPA_ERR_OOM, "device workspace of %zu bytes: %s", cap,
+                             hipGetErrorString(e));
+        }
+        ctx->ws_cap = cap;
+    }
+    arena->base = ctx->ws;
+    arena->off = 0;
+    return BRPA_OK;
+}
+
+void stage_reset(brpa_ctx *ctx) {
+    for (int s = 0; s < BRPA_NSTAGES; ++s) {
+        ctx->stage_ms[s] = 0.0;
+        ctx->stage_launches[s] = 0;
+    }
+}
+
+int stage_begin(brpa_ctx *ctx, int stage) {
+    BRPA_HIP(ctx, hipEventRecord(ctx->sev[2 * stage], ctx->stream));
+    return BRPA_OK;
+}
+
+int stage_end(brpa_ctx *ctx, int stage) {
+    BRPA_HIP(ctx, hipEventRecord(ctx->sev[2 * stage + 1], ctx->stream));
+    return BRPA_OK;
+}
+
+int stage_collect(brpa_ctx *ctx, int stage) {
+    float ms = 0.f;
+    BRPA_HIP(ctx, hipEventElapsedTime(&ms, ctx->sev[2 * stage], ctx->sev[2 * stage + 1]));
+    ctx->stage_ms[stage] += ms;
+    ctx->stage_launches[stage] += 1;
+    return BRPA_OK;
+}
+
+extern "C" {
+
+int brpa_ctx_create(int device, brpa_ctx **out) {
+    if (!out) return BRPA_ERR_ARG;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+        (void)hipGetLastError();
+        return BRPA_ERR_HIP;
+    }
+    if (device < 0 || device >= n) return BRPA_ERR_ARG;
+    brpa_ctx *ctx = new brpa_ctx();
+    ctx->device = device;
+    if (hipSetDevice(device) != hipSuccess ||
+        hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) {
+        (void)hipGetLastError();
+        delete ctx;
+        return BRPA_ERR_HIP;
+    }
+    for (int i = 0; i < 2 * BRPA_NSTAGES; ++i)
+        if (hipEventCreate(&ctx->sev[i]) != hipSuccess) {
+        (void)hipGetLastError();
+        delete ctx;
+        return BRPA_ERR_HIP;
+    }
+    ctx->stream = ctx->own_stream;
+    *out = ctx;
+    return BRPA_OK;
+}
+
+int brpa_ctx_destroy(brpa_ctx *ctx) {
+    if (!ctx) return BRPA_OK;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->own_stream) (void)hipStreamSynchronize(ctx->own_stream);
+    if (ctx->stream && ctx->stream != ctx->own_stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->ws) (void)hipFree(ctx->ws);
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    for (int i = 0; i < 2 * BRPA_NSTAGES; ++i)
+        if (ctx->sev[i]) (void)hipEventDestroy(ctx->sev[i]);
+    if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
+    delete ctx;
+    return BRPA_OK;
+}
+
+const char *brpa_last_error(const brpa_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int brpa_ctx_set_stream(brpa_ctx *ctx, void *s) {
+    if (!ctx) return BRPA_ERR_ARG;
+    ctx->stream = s ? static_cast<hipStream_t>(s) : ctx->own_stream;
+    return BRPA_OK;
+}
+
+int brpa_ctx_trim(brpa_ctx *ctx) {
+    if (!ctx) return BRPA_ERR_ARG;
+    BRPA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (ctx->ws) BRPA_HIP(ctx, hipFree(ctx->ws));
+    ctx->ws = nullptr;
+    ctx->ws_cap = 0;
+    return BRPA_OK;
+}
+
+double brpa_last_kernel_ms(const brpa_ctx *ctx) { return ctx ? ctx->last_kernel_ms : 0.0; }
+
+double brpa_last_stage_ms(const brpa_ctx *ctx, int stage) {
+    return (ctx && stage >= 0 && stage < BRPA_NSTAGES) ? ctx->stage_ms[stage] : 0.0;
+}
+
+int brpa_last_stage_launches(const brpa_ctx *ctx, int stage) {
+    return (ctx && stage >= 0 && stage < BRPA_NSTAGES) ? ctx->stage_launches[stage] : 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,65 @@
+// Device helpers shared by the gfx950 kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+typedef double d4_t __attribute__((ext_vector_type(4)));
+typedef float f4_t __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8_t __attribute__((ext_vector_type(8)));
+typedef short s8_t __attribute__((ext_vector_type(8)));
+
+// Fragment index f with pre[f] <= b < pre[f+1] (pre has nfrag+1 entries).
+__device__ __forceinline__ int find_frag(const int64_t *__restrict__ pre, int nfrag, int64_t b) {
+    int lo = 0, hi = nfrag - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (pre[mid] <= b)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+// Lower-triangle pair index -> (i, j), i >= j, pair = i*(i+1)/2 + j.
+__device__ __forceinline__ void tri_decode(int pair, int &i, int &j) {
+    int t = (int)((sqrt(8.0 * (double)pair + 1.0) - 1.0) * 0.5);
+    while ((t + 1) * (t + 2) / 2 <= pair) ++t;
+    while (t * (t + 1) / 2 > pair) --t;
+    i = t;
+    j = pair - t * (t + 1) / 2;
+}
+
+// Double-double accumulation (Knuth TwoSum), used where the reference sums
+// in REAL(F128).
+struct dd_t {
+    double hi, lo;
+};
+__device__ __forceinline__ dd_t dd_add(dd_t a, double b) {
+    double s = a.hi + b;
+    double bb = s - a.hi;
+    double err = (a.hi - (s - bb)) + (b - bb);
+    double lo = a.lo + err;
+    double hi = s + lo;
+    lo = lo - (hi - s);
+    return {hi, lo};
+}
+__device__ __forceinline__ dd_t dd_add(dd_t a, dd_t b) {
+    dd_t s = dd_add(a, b.hi);
+    return dd_add(s, b.lo);
+}
+
+// XCD-aware block remap: the hardware hands consecutive work-group ids to the
+// 8 XCDs round-robin; this maps them so that each XCD gets a contiguous range
+// of the logical work list (neighbouring tiles share R panels in that XCD's L2).
+__device__ __forceinline__ int64_t xcd_remap(int64_t b, int64_t nblocks) {
+    const int64_t nx = 8;
+    int64_t per = (nblocks + nx - 1) / nx;
+    int64_t x = b % nx, r = b / nx;
+    int64_t lb = x * per + r;
+    // the last XCD ranges may be short; fall back to identity past the even part
+    int64_t even = (nblocks / nx) * nx;
+    if (b >= even || lb >= nblocks) return b;
+    return (nblocks % nx == 0) ? lb : b;
+}

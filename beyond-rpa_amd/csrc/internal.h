@@ -1,0 +1,122 @@
+// Internal declarations shared by the host orchestration and the kernel files.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "brpa.h"
+
+struct brpa_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;       // stream in use
+    hipStream_t own_stream = nullptr;   // stream created by the context
+    std::string err;
+    char *ws = nullptr;                 // device workspace
+    size_t ws_cap = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_kernel_ms = 0.0;
+    // per-stage device time of the last call (HIP events on ctx->stream)
+    hipEvent_t sev[2 * BRPA_NSTAGES] = {};
+    double stage_ms[BRPA_NSTAGES] = {};
+    int stage_launches[BRPA_NSTAGES] = {};
+};
+
+// Records the start/end of a stage on the context stream; stage_end adds the
+// elapsed time once the stream has been synchronised (stage_collect).
+int stage_begin(brpa_ctx *ctx, int stage);
+int stage_end(brpa_ctx *ctx, int stage);
+int stage_collect(brpa_ctx *ctx, int stage);
+void stage_reset(brpa_ctx *ctx);
+
+int brpa_fail(brpa_ctx *ctx, int code, const char *fmt, ...);
+
+#define BRPA_HIP(ctx, expr)                                                              \
+    do {                                                                                 \
+        hipError_t _e = (expr);                                                          \
+        if (_e != hipSuccess)                                                            \
+            return brpa_fail((ctx), BRPA_ERR_HIP, "%s failed: %s (%s:%d)", #expr,        \
+                             hipGetErrorString(_e), __FILE__, __LINE__);                 \
+    } while (0)
+
+#define BRPA_TRY(expr)            \
+    do {                          \
+        int _rc = (expr);         \
+        if (_rc != BRPA_OK)       \
+            return _rc;           \
+    } while (0)
+
+// Bump allocator over the context workspace.  A first pass with base == nullptr
+// only counts bytes; ws_reserve then makes the workspace large enough.
+struct Arena {
+    char *base = nullptr;
+    size_t off = 0;
+    template <class T>
+    T *take(size_t n) {
+        size_t bytes = ((n * sizeof(T) + 255) / 256) * 256;
+        T *p = base ? reinterpret_cast<T *>(base + off) : nullptr;
+        off += bytes;
+        return p;
+    }
+};
+int ws_reserve(brpa_ctx *ctx, size_t bytes, Arena *arena);
+bool is_device_ptr(const void *p);
+// Copy n elements of a host-or-device array into device memory dst.
+int to_device(brpa_ctx *ctx, void *dst, const void *src, size_t bytes);
+int from_device(brpa_ctx *ctx, void *dst_host_or_dev, const void *src_dev, size_t bytes);
+
+// ---------------------------------------------------------------------------
+// Fragment descriptor used by the batched kernels (device memory).
+//   R     : K x N row-major (R[ai][P]), K = nocc*nvir
+//   dtab  : F x K, dtab[w][k] = 4 d_k/(d_k^2 + u_w^2)   (prefactor included)
+//   Pi    : F x N x N row-major, lower triangle + diagonal is what the
+//           Cholesky reads; the SYRK writes both triangles
+//   diag  : F x N exact FP64 diagonal of Pi
+//   ldpart: F x nsteps x 2 double-double partial log-det sums of the
+//           Cholesky diagonal blocks
+struct FragDev {
+    const double *R;
+    const double *dtab;
+    double *Pi;
+    double *diag;
+    double *ldpart;
+    const double *weights;   // F (device)
+    double *eu;              // F per-frequency E(u) output (device)
+    double *energy;          // 1 (device)
+    int *status;             // F Cholesky status (0 ok)
+    double *linv;            // F x 64 x 64 inverse of the current diagonal block
+    double *dpart;           // split-BF16: F x nblk x N x 2 double-double diagonal partials
+    int K, N, F, nsteps;
+    int nocc, nvir;
+    const double *eocc, *evir, *freqs;   // device copies
+    // split-BF16 slices (only when precision == BRPA_SPLIT_BF16)
+    uint16_t *sl;            // F x 3 x N x Kpad, per column P the Kpad rows contiguous
+    float *scale;            // F x nblk x N, block scale 2^(e-8)
+    int Kpad, nblk;
+};
+
+constexpr int CHOL_NB = 64;   // Cholesky block size
+constexpr int SYRK_T = 64;    // FP64 SYRK tile
+constexpr int SPLIT_KB = 128; // split-BF16 exactness block (rows per scale)
+
+// Launch wrappers (defined in the kernel files).  `pre` arrays are device
+// prefix sums over fragments of the number of work-groups per fragment.
+int launch_dtab(brpa_ctx *, const FragDev *, const int64_t *pre, int nfrag, int64_t total);
+int launch_syrk_f64(brpa_ctx *, const FragDev *, const int64_t *pre, int nfrag, int64_t total);
+int launch_set_diag(brpa_ctx *, const FragDev *, const int64_t *pre, int nfrag, int64_t total,
+                    int from_parts);
+int launch_potrf_diag(brpa_ctx *, const FragDev *, const int64_t *pre, int nfrag, int64_t total, int step);
+int launch_trsm(brpa_ctx *, const FragDev *, const int64_t *pre, int nfrag, int64_t total, int step);
+int launch_chol_update(brpa_ctx *, const FragDev *, const int64_t *pre, int nfrag, int64_t total, int step);
+int launch_energy(brpa_ctx *, const FragDev *, int nfrag);
+int launch_fill_int(brpa_ctx *, int *p, int64_t n, int v);
+int launch_split(brpa_ctx *, const FragDev *, const int64_t *pre, int nfrag, int64_t total);
+int launch_syrk_bf16(brpa_ctx *, const FragDev *, const int64_t *pre, int nfrag, int64_t total);
+
+// Generic single-matrix FP64 SYRK used by brpa_gpiug_mo:
+//   C(N,N) (+)= X^T diag(d) X, X K x N row-major, both triangles written.
+int launch_syrk_single(brpa_ctx *, const double *X, const double *d, int K, int N, double *C,
+                       int accumulate);

@@ -1,0 +1,508 @@
+// FP64 kernels of the E_RPA hot path on gfx950 (CDNA4).
+//
+//   k_dtab         c_w(ai) = 4 d/(d^2+u_w^2)           (rpa_core_MO.f90:379-401)
+//   k_syrk_f64     Pi_w = R^T diag(c_w) R on v_mfma_f64_16x16x4_f64, one R tile in
+//                  LDS shared by 4 waves = 4 frequencies  (dgemm of rpa_core_MO.f90:402)
+//   k_set_diag     diag_w (tr Pi, rpa_CC.f90:444-447); Pi_w[P][P] += 1 (448-450)
+//   k_potrf_diag   Cholesky + inverse of a 64x64 diagonal block (one wave)
+//   k_trsm         L21 = A21 L11^{-T} on MFMA
+//   k_chol_update  trailing update A22 -= L21 L21^T on MFMA
+//   k_energy       E = sum_w w/(2 pi) (2 sum log L_kk - tr Pi_w)  (rpa_CC.f90:451-453)
+//
+// All kernels walk a flattened work list over fragments: work-group b belongs
+// to fragment f with pre[f] <= b < pre[f+1].
+#include "device_common.h"
+#include "internal.h"
+
+#define LAUNCH_CHECK(ctx)                                                                      \
+    do {                                                                                       \
+        hipError_t _e = hipGetLastError();                                                     \
+        if (_e != hipSuccess)                                                                  \
+            return brpa_fail((ctx), BRPA_ERR_HIP, "kernel launch failed: %s (%s:%d)",          \
+                             hipGetErrorString(_e), __FILE__, __LINE__);                       \
+    } while (0)
+
+static inline unsigned grid1(int64_t total) { return (unsigned)total; }
+
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_dtab(const FragDev *__restrict__ fr,
+                                             const int64_t *__restrict__ pre, int nfrag) {
+    int64_t b = blockIdx.x;
+    int f = find_frag(pre, nfrag, b);
+    const FragDev &F = fr[f];
+    int64_t e = (b - pre[f]) * 256 + threadIdx.x;
+    int64_t tot = (int64_t)F.F * F.K;
+    if (e >= tot) return;
+    int w = (int)(e / F.K);
+    int k = (int)(e - (int64_t)w * F.K);
+    int i = k / F.nvir, a = k - i * F.nvir;
+    double d = F.evir[a] - F.eocc[i];
+    double u = F.freqs[w];
+    const_cast<double *>(F.dtab)[e] = 4.0 * d / (d * d + u * u);
+}
+
+int launch_dtab(brpa_ctx *ctx, const FragDev *fr, const int64_t *pre, int nfrag, int64_t total) {
+    if (total <= 0) return BRPA_OK;
+    hipLaunchKernelGGL(k_dtab, dim3(grid1(total)), dim3(256), 0, ctx->stream, fr, pre, nfrag);
+    LAUNCH_CHECK(ctx);
+    return BRPA_OK;
+}
+
+// ---------------------------------------------------------------------------
+// FP64 SYRK on MFMA.  One work-group = one 64x64 tile pair (tp >= tq) and a
+// group of 4 frequencies; wave w computes the whole tile for frequency
+// 4*g + w from the R tiles staged once in LDS.
+//
+// MFMA v_mfma_f64_16x16x4_f64 operand layout (lane l):
+//   A[m = l%16][k = l/16], B[k = l/16][n = l%16],
+//   D: 4 doubles, D[r] = C[m = l/16 + 4*r][n = l%16]  (probed on gfx950; the
+//   f32 shapes use 4*(l/16) + r instead).
+constexpr int KB = 16;        // rows of R per LDS stage
+constexpr int LDS_PAD = 16;   // +16 doubles per row: rows r and r+1 land 32 banks apart
+
+template <bool kSingle>
+__device__ __forceinline__ void syrk_f64_tile(const double *__restrict__ R, int K, int N,
+                                              const double *__restrict__ dtab, int F, int w0,
+                                              double *__restrict__ Pi, int tp, int tq,
+                                              int accumulate) {
+    __shared__ double sA[KB][SYRK_T + LDS_PAD];
+    __shared__ double sB[KB][SYRK_T + LDS_PAD];
+    __shared__ double sD[4][KB];
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6, lane = tid & 63;
+    const int p0 = tp * SYRK_T, q0 = tq * SYRK_T;
+    const int w = w0 + wave;
+
+    d4_t acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = (d4_t){0.0, 0.0, 0.0, 0.0};
+
+    double ra[4], rb[4], rd = 0.0;
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            int e = tid + 256 * i;
+            int r = e >> 6, c = e & 63;
+            int k = k0 + r;
+            bool kin = k < K;
+            ra[i] = (kin && p0 + c < N) ? R[(int64_t)k * N + p0 + c] : 0.0;
+            rb[i] = (kin && q0 + c < N) ? R[(int64_t)k * N + q0 + c] : 0.0;
+        }
+        if (tid < 4 * KB) {
+            int ww = w0 + (tid / KB), k = k0 + (tid % KB);
+            if (kSingle)
+                rd = (tid < KB && k < K) ? dtab[k] : 0.0;
+            else
+                rd = (ww < F && k < K) ? dtab[(int64_t)ww * K + k] : 0.0;
+        }
+    };
+    auto lstore = [&]() {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            int e = tid + 256 * i;
+            int r = e >> 6, c = e & 63;
+            sA[r][c] = ra[i];
+            sB[r][c] = rb[i];
+        }
+        if (tid < 4 * KB) sD[tid / KB][tid % KB] = rd;
+    };
+
+    gload(0);
+    for (int k0 = 0; k0 < K; k0 += KB) {
+        __syncthreads();
+        lstore();
+        __syncthreads();
+        if (k0 + KB < K) gload(k0 + KB);
+#pragma unroll
+        for (int kk = 0; kk < KB; kk += 4) {
+            const int kr = kk + (lane >> 4);
+            const double dv = sD[kSingle ? 0 : wave][kr];
+            double a[4], bv[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) a[i] = sA[kr][16 * i + (lane & 15)] * dv;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) bv[j] = sB[kr][16 * j + (lane & 15)];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], bv[j], acc[i][j], 0, 0, 0);
+        }
+    }
+
+    if (!kSingle && w >= F) return;
+    if (kSingle && wave != 0) return;
+    double *C = kSingle ? Pi : Pi + (int64_t)w * N * N;
+    const bool diag = (tp == tq);
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                int p = p0 + 16 * i + (lane >> 4) + 4 * r;
+                int q = q0 + 16 * j + (lane & 15);
+                if (p < N && q < N) {
+                    double v = acc[i][j][r];
+                    if (accumulate) {
+                        C[(int64_t)p * N + q] += v;
+                        if (!diag) C[(int64_t)q * N + p] += v;
+                    } else {
+                        C[(int64_t)p * N + q] = v;
+                        if (!diag) C[(int64_t)q * N + p] = v;
+                    }
+                }
+            }
+}
+
+// Work-groups per fragment: npairs(ceil(N/64)) * ceil(F/4), pair-major within
+// a frequency group.
+__global__ __launch_bounds__(256) void k_syrk_f64(const FragDev *__restrict__ fr,
+                                                 const int64_t *__restrict__ pre, int nfrag) {
+    int64_t b = blockIdx.x;
+    int f = find_frag(pre, nfrag, b);
+    const FragDev &F = fr[f];
+    int64_t lb = b - pre[f];
+    int nt = (F.N + SYRK_T - 1) / SYRK_T;
+    int npairs = nt * (nt + 1) / 2;
+    int g = (int)(lb / npairs), pair = (int)(lb - (int64_t)g * npairs);
+    int tp, tq;
+    tri_decode(pair, tp, tq);
+    syrk_f64_tile<false>(F.R, F.K, F.N, F.dtab, F.F, 4 * g, F.Pi, tp, tq, 0);
+}
+
+int launch_syrk_f64(brpa_ctx *ctx, const FragDev *fr, const int64_t *pre, int nfrag, int64_t total) {
+    if (total <= 0) return BRPA_OK;
+    hipLaunchKernelGGL(k_syrk_f64, dim3(grid1(total)), dim3(256), 0, ctx->stream, fr, pre, nfrag);
+    LAUNCH_CHECK(ctx);
+    return BRPA_OK;
+}
+
+__global__ __launch_bounds__(256) void k_syrk_single(const double *__restrict__ X,
+                                                    const double *__restrict__ d, int K, int N,
+                                                    double *__restrict__ C, int accumulate) {
+    int nt = (N + SYRK_T - 1) / SYRK_T;
+    (void)nt;
+    int tp, tq;
+    tri_decode((int)blockIdx.x, tp, tq);
+    syrk_f64_tile<true>(X, K, N, d, 1, 0, C, tp, tq, accumulate);
+}
+
+int launch_syrk_single(brpa_ctx *ctx, const double *X, const double *d, int K, int N, double *C,
+                       int accumulate) {
+    int nt = (N + SYRK_T - 1) / SYRK_T;
+    int npairs = nt * (nt + 1) / 2;
+    if (npairs <= 0) return BRPA_OK;
+    hipLaunchKernelGGL(k_syrk_single, dim3(npairs), dim3(256), 0, ctx->stream, X, d, K, N, C,
+                       accumulate);
+    LAUNCH_CHECK(ctx);
+    return BRPA_OK;
+}
+
+// ---------------------------------------------------------------------------
+// diag_w[P] and Pi_w[P][P] = diag_w[P] + 1.  The diagonal is either the one the
+// FP64 SYRK (or the caller) computed, or, for the split-BF16 form, the exact
+// double-double sum of the per-block partials written by k_split.
+// Work-groups per fragment: ceil(F*N/256).
+__global__ __launch_bounds__(256) void k_set_diag(const FragDev *__restrict__ fr,
+                                                 const int64_t *__restrict__ pre, int nfrag,
+                                                 int from_parts) {
+    int64_t b = blockIdx.x;
+    int f = find_frag(pre, nfrag, b);
+    const FragDev &F = fr[f];
+    int64_t e = (b - pre[f]) * 256 + threadIdx.x;
+    if (e >= (int64_t)F.F * F.N) return;
+    int w = (int)(e / F.N), P = (int)(e - (int64_t)w * F.N);
+    double *pp = F.Pi + (int64_t)w * F.N * F.N + (int64_t)P * F.N + P;
+    double d;
+    if (from_parts) {
+        dd_t s = {0.0, 0.0};
+        for (int blk = 0; blk < F.nblk; ++blk) {
+            const double *q = F.dpart + (((int64_t)w * F.nblk + blk) * F.N + P) * 2;
+            s = dd_add(s, (dd_t){q[0], q[1]});
+        }
+        d = s.hi + s.lo;
+    } else {
+        d = *pp;
+    }
+    F.diag[e] = d;
+    *pp = d + 1.0;
+}
+
+int launch_set_diag(brpa_ctx *ctx, const FragDev *fr, const int64_t *pre, int nfrag, int64_t total,
+                    int from_parts) {
+    if (total <= 0) return BRPA_OK;
+    hipLaunchKernelGGL(k_set_diag, dim3(grid1(total)), dim3(256), 0, ctx->stream, fr, pre, nfrag,
+                       from_parts);
+    LAUNCH_CHECK(ctx);
+    return BRPA_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Blocked right-looking Cholesky of A_w = 1 + Pi_w (lower triangle, row-major),
+// three launches per 64-column step j (j0 = 64 j):
+//   k_potrf_diag   one wave per matrix: factors the diagonal block in LDS,
+//                  writes its log-det partial and L11^{-1} (own buffer; A11
+//                  itself is never written back, so no work-group of the same
+//                  step can read a half-factored block)
+//   k_trsm         L21 = A21 L11^{-T} as a 64x64x64 MFMA product per row chunk
+//   k_chol_update  A22 -= L21 L21^T per lower 64x64 tile pair, MFMA
+// Work-groups per fragment: F (diag), F*(nchunks-1) (trsm), F*npairs (update).
+
+// C(64x64) = X Y^T with X, Y 64x64 in LDS; wave (qi,qj) owns a 32x32 quadrant,
+// acc[i][j][r] = C[32 qi + 16 i + l/16 + 4 r][32 qj + 16 j + l%16].
+__device__ __forceinline__ void mma64_xyt(const double (*sX)[CHOL_NB + 1],
+                                          const double (*sY)[CHOL_NB + 1], d4_t acc[2][2]) {
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int qi = wave >> 1, qj = wave & 1;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = (d4_t){0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+    for (int kk = 0; kk < CHOL_NB; kk += 4) {
+        const int kc = kk + (lane >> 4);
+        double a[2], bv[2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) a[i] = sX[32 * qi + 16 * i + (lane & 15)][kc];
+#pragma unroll
+        for (int j = 0; j < 2; ++j) bv[j] = sY[32 * qj + 16 * j + (lane & 15)][kc];
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], bv[j], acc[i][j], 0, 0, 0);
+    }
+}
+
+__global__ __launch_bounds__(64) void k_potrf_diag(const FragDev *__restrict__ fr,
+                                                  const int64_t *__restrict__ pre, int nfrag,
+                                                  int step) {
+    __shared__ double L[CHOL_NB][CHOL_NB + 1];
+    __shared__ double X[CHOL_NB][CHOL_NB + 1];
+    int64_t b = blockIdx.x;
+    int f = find_frag(pre, nfrag, b);
+    const FragDev &F = fr[f];
+    const int N = F.N;
+    const int j0 = step * CHOL_NB;
+    const int w = (int)(b - pre[f]);
+    const int nbj = min(CHOL_NB, N - j0);
+    const double *A = F.Pi + (int64_t)w * N * N;
+    const int lane = threadIdx.x;
+    // lane = row; rows/cols past nbj are padded with the identity
+    for (int c = 0; c < CHOL_NB; ++c) {
+        double v = (lane == c) ? 1.0 : 0.0;
+        if (lane < nbj && c < nbj && c <= lane) v = A[(int64_t)(j0 + lane) * N + j0 + c];
+        L[lane][c] = (c <= lane) ? v : 0.0;
+    }
+    __syncthreads();
+    int bad = 0;
+    for (int c = 0; c < CHOL_NB; ++c) {
+        double piv = L[c][c];
+        if (!(piv > 0.0) || !isfinite(piv)) {
+            bad = 1;
+            piv = 1.0;
+        }
+        const double d = sqrt(piv);
+        __syncthreads();
+        if (lane == c) L[c][c] = d;
+        if (lane > c) L[lane][c] = L[lane][c] / d;
+        __syncthreads();
+        if (lane > c) {
+            const double lic = L[lane][c];
+            for (int k = c + 1; k <= lane; ++k) L[lane][k] -= lic * L[k][c];
+        }
+        __syncthreads();
+    }
+    // column `lane` of L^{-1} by forward substitution (X[i][lane])
+    for (int i = 0; i < CHOL_NB; ++i) {
+        double s = (i == lane) ? 1.0 : 0.0;
+        for (int k = 0; k < i; ++k) s -= L[i][k] * X[k][lane];
+        X[i][lane] = s / L[i][i];
+    }
+    __syncthreads();
+    double *linv = F.linv + (int64_t)w * CHOL_NB * CHOL_NB;
+    for (int i = 0; i < CHOL_NB; ++i) linv[i * CHOL_NB + lane] = X[i][lane];
+    if (lane == 0) {
+        dd_t s = {0.0, 0.0};
+        for (int c = 0; c < nbj; ++c) s = dd_add(s, log(L[c][c]));
+        double *part = F.ldpart + ((int64_t)w * F.nsteps + step) * 2;
+        part[0] = s.hi;
+        part[1] = s.lo;
+        if (bad) F.status[w] = 1;
+    }
+}
+
+int launch_potrf_diag(brpa_ctx *ctx, const FragDev *fr, const int64_t *pre, int nfrag,
+                      int64_t total, int step) {
+    if (total <= 0) return BRPA_OK;
+    hipLaunchKernelGGL(k_potrf_diag, dim3(grid1(total)), dim3(64), 0, ctx->stream, fr, pre,
+                       nfrag, step);
+    LAUNCH_CHECK(ctx);
+    return BRPA_OK;
+}
+
+__global__ __launch_bounds__(256) void k_trsm(const FragDev *__restrict__ fr,
+                                             const int64_t *__restrict__ pre, int nfrag, int step) {
+    __shared__ double sB[CHOL_NB][CHOL_NB + 1];
+    __shared__ double sL[CHOL_NB][CHOL_NB + 1];
+    int64_t b = blockIdx.x;
+    int f = find_frag(pre, nfrag, b);
+    const FragDev &F = fr[f];
+    const int N = F.N;
+    const int j0 = step * CHOL_NB;
+    const int nsub = (N - j0 + CHOL_NB - 1) / CHOL_NB - 1;   // chunks below the diagonal
+    int64_t lb = b - pre[f];
+    const int w = (int)(lb / nsub), r = 1 + (int)(lb - (int64_t)w * nsub);
+    double *A = F.Pi + (int64_t)w * N * N;
+    const double *linv = F.linv + (int64_t)w * CHOL_NB * CHOL_NB;
+    const int R0 = j0 + CHOL_NB * r;
+    const int tid = threadIdx.x;
+    for (int e = tid; e < CHOL_NB * CHOL_NB; e += 256) {
+        int rr = e / CHOL_NB, cc = e % CHOL_NB;
+        sB[rr][cc] = (R0 + rr < N) ? A[(int64_t)(R0 + rr) * N + j0 + cc] : 0.0;
+        sL[rr][cc] = linv[e];
+    }
+    __syncthreads();
+    d4_t acc[2][2];
+    mma64_xyt(sB, sL, acc);   // L21[i][c] = sum_k A21[i][k] Linv[c][k]
+    const int wave = tid >> 6, lane = tid & 63, qi = wave >> 1, qj = wave & 1;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                int row = R0 + 32 * qi + 16 * i + (lane >> 4) + 4 * q;
+                int col = 32 * qj + 16 * j + (lane & 15);
+                if (row < N) A[(int64_t)row * N + j0 + col] = acc[i][j][q];
+            }
+}
+
+int launch_trsm(brpa_ctx *ctx, const FragDev *fr, const int64_t *pre, int nfrag, int64_t total,
+                int step) {
+    if (total <= 0) return BRPA_OK;
+    hipLaunchKernelGGL(k_trsm, dim3(grid1(total)), dim3(256), 0, ctx->stream, fr, pre, nfrag, step);
+    LAUNCH_CHECK(ctx);
+    return BRPA_OK;
+}
+
+__global__ __launch_bounds__(256) void k_chol_update(const FragDev *__restrict__ fr,
+                                                    const int64_t *__restrict__ pre, int nfrag,
+                                                    int step) {
+    __shared__ double sI[CHOL_NB][CHOL_NB + 1];
+    __shared__ double sJ[CHOL_NB][CHOL_NB + 1];
+    int64_t b = blockIdx.x;
+    int f = find_frag(pre, nfrag, b);
+    const FragDev &F = fr[f];
+    const int N = F.N;
+    const int j0 = step * CHOL_NB;
+    const int t0 = j0 + CHOL_NB;
+    const int nt = (N - t0 + CHOL_NB - 1) / CHOL_NB;
+    const int npairs = nt * (nt + 1) / 2;
+    int64_t lb = b - pre[f];
+    const int w = (int)(lb / npairs), pair = (int)(lb - (int64_t)w * npairs);
+    int ti, tj;
+    tri_decode(pair, ti, tj);
+    double *A = F.Pi + (int64_t)w * N * N;
+    const int Ri = t0 + CHOL_NB * ti, Rj = t0 + CHOL_NB * tj;
+    const int tid = threadIdx.x;
+    for (int e = tid; e < CHOL_NB * CHOL_NB; e += 256) {
+        int rr = e / CHOL_NB, cc = e % CHOL_NB;
+        sI[rr][cc] = (Ri + rr < N) ? A[(int64_t)(Ri + rr) * N + j0 + cc] : 0.0;
+        sJ[rr][cc] = (Rj + rr < N) ? A[(int64_t)(Rj + rr) * N + j0 + cc] : 0.0;
+    }
+    __syncthreads();
+    d4_t acc[2][2];
+    mma64_xyt(sI, sJ, acc);
+    const int wave = tid >> 6, lane = tid & 63, qi = wave >> 1, qj = wave & 1;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                int row = Ri + 32 * qi + 16 * i + (lane >> 4) + 4 * q;
+                int col = Rj + 32 * qj + 16 * j + (lane & 15);
+                if (row < N && col < N && col <= row) A[(int64_t)row * N + col] -= acc[i][j][q];
+            }
+}
+
+int launch_chol_update(brpa_ctx *ctx, const FragDev *fr, const int64_t *pre, int nfrag,
+                       int64_t total, int step) {
+    if (total <= 0) return BRPA_OK;
+    hipLaunchKernelGGL(k_chol_update, dim3(grid1(total)), dim3(256), 0, ctx->stream, fr, pre,
+                       nfrag, step);
+    LAUNCH_CHECK(ctx);
+    return BRPA_OK;
+}
+
+// ---------------------------------------------------------------------------
+// One work-group per fragment: E = sum_w w_w/(2 pi) [2 sum_j ldpart - tr].
+__global__ __launch_bounds__(256) void k_energy(const FragDev *__restrict__ fr) {
+    const FragDev &F = fr[blockIdx.x];
+    __shared__ double sh[256], sl[256];
+    const int tid = threadIdx.x;
+    dd_t etot = {0.0, 0.0};
+    for (int w = 0; w < F.F; ++w) {
+        dd_t t = {0.0, 0.0};
+        for (int P = tid; P < F.N; P += 256) t = dd_add(t, F.diag[(int64_t)w * F.N + P]);
+        sh[tid] = t.hi;
+        sl[tid] = t.lo;
+        __syncthreads();
+        for (int s = 128; s > 0; s >>= 1) {
+            if (tid < s) {
+                dd_t x = {sh[tid], sl[tid]}, y = {sh[tid + s], sl[tid + s]};
+                x = dd_add(x, y);
+                sh[tid] = x.hi;
+                sl[tid] = x.lo;
+            }
+            __syncthreads();
+        }
+        if (tid == 0) {
+            dd_t tr = {sh[0], sl[0]};
+            dd_t ld = {0.0, 0.0};
+            // only this fragment's own Cholesky steps wrote a partial
+            const int ns = (F.N + CHOL_NB - 1) / CHOL_NB;
+            for (int j = 0; j < ns; ++j) {
+                const double *p = F.ldpart + ((int64_t)w * F.nsteps + j) * 2;
+                ld = dd_add(ld, (dd_t){p[0], p[1]});
+            }
+            ld.hi *= 2.0;
+            ld.lo *= 2.0;
+            dd_t eu = dd_add(ld, (dd_t){-tr.hi, -tr.lo});
+            F.eu[w] = eu.hi + eu.lo;
+            const double c = F.weights[w] / (2.0 * 3.14159265358979323846);
+            double ph = c * eu.hi;
+            double pl = fma(c, eu.hi, -ph) + c * eu.lo;
+            etot = dd_add(etot, ph);
+            etot.lo += pl;
+        }
+        __syncthreads();
+    }
+    if (tid == 0) F.energy[0] = etot.hi + etot.lo;
+}
+
+int launch_energy(brpa_ctx *ctx, const FragDev *fr, int nfrag) {
+    if (nfrag <= 0) return BRPA_OK;
+    hipLaunchKernelGGL(k_energy, dim3(nfrag), dim3(256), 0, ctx->stream, fr);
+    LAUNCH_CHECK(ctx);
+    return BRPA_OK;
+}
+
+// Fill n ints with a value (used instead of hipMemsetAsync so that the
+// sequence stays valid under stream capture).
+__global__ void k_fill_int(int *p, int64_t n, int v) {
+    int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+int launch_fill_int(brpa_ctx *ctx, int *p, int64_t n, int v) {
+    if (n <= 0) return BRPA_OK;
+    hipLaunchKernelGGL(k_fill_int, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream,
+                       p, n, v);
+    LAUNCH_CHECK(ctx);
+    return BRPA_OK;
+}

@@ -1,0 +1,145 @@
+/*
+ * brpa.h -- C ABI of the MI355X (gfx950) correlation-energy hot path.
+ *
+ * Drop-in for the MO-basis direct-RPA energy and SOSEX correction of
+ * modrzejewski/beyond-rpa.  Every entry point keeps the meaning of the
+ * reference routine it replaces (cited per function, paths relative to the
+ * reference's src/).  The reference's Fortran arrays are column-major with
+ * default 64-bit INTEGER; the C layouts below are the same bytes:
+ *
+ *   Fortran CRG(P, ai) / Rkai(P, a, i)  ==  C row-major R[ai][P], ai = i*nvir + a
+ *   Fortran GPiUG(P, Q), PiU(P, Q, u)   ==  C row-major Pi[u][Q][P] (symmetric)
+ *   Fortran V(mu, ai) / Vkai(mu, a, i)  ==  C row-major V[ai][mu]
+ *
+ * Pointers may be host or device (HBM) memory; the library detects device
+ * pointers and otherwise stages through its own device workspace.  Calls are
+ * synchronous: results are visible to the host on return.  A context owns one
+ * HIP stream on one device and must not be shared between threads.
+ *
+ * Every function returns BRPA_OK (0) or a BRPA_ERR_* code; the message of the
+ * last failure is brpa_last_error(ctx).  The reference has no return codes:
+ * it prints a message and executes `error stop`, which is what the Fortran
+ * shim does with a non-zero code.
+ */
+#ifndef BRPA_H
+#define BRPA_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct brpa_ctx brpa_ctx;
+
+enum {
+    BRPA_OK = 0,
+    BRPA_ERR_ARG = 1,        /* bad size / null pointer / unsupported option */
+    BRPA_ERR_HIP = 2,        /* HIP runtime error (message has the detail) */
+    BRPA_ERR_NOT_SPD = 3,    /* Cholesky breakdown of 1 + Pi (reference: DGEQRF info /= 0) */
+    BRPA_ERR_GAP = 4,        /* non-positive HOMO-LUMO gap (OrbDiffHist.f90:77-80) */
+    BRPA_ERR_NONFINITE = 5,  /* NaN/Inf in an input */
+    BRPA_ERR_OOM = 6,        /* device allocation failed */
+    BRPA_ERR_GRID = 7,       /* frequency quadrature did not converge (OptFreqQuad.f90:68-71) */
+    BRPA_ERR_SOLVER = 8      /* eigensolver failure (real_linalg.f90:1527-1531) */
+};
+
+/* Arithmetic of the Pi and amplitude contractions.  BRPA_FP64 is the drop-in. */
+enum {
+    BRPA_FP64 = 0,
+    BRPA_SPLIT_BF16 = 1      /* 3 fixed-point bf16 slices, 6 slice products on MFMA,
+                                exact FP32 accumulation per 128-row block, FP64 between
+                                blocks; exact FP64 diagonal and trace; FP64 log-det */
+};
+
+/* ---- context ------------------------------------------------------------ */
+int brpa_ctx_create(int device, brpa_ctx **ctx);
+int brpa_ctx_destroy(brpa_ctx *ctx);
+const char *brpa_last_error(const brpa_ctx *ctx);
+/* Use an external HIP stream (e.g. torch's current stream); NULL restores the
+ * context's own stream.  The caller keeps ownership of the stream. */
+int brpa_ctx_set_stream(brpa_ctx *ctx, void *hip_stream);
+/* Release the device workspace (it grows on demand and is kept otherwise). */
+int brpa_ctx_trim(brpa_ctx *ctx);
+/* Device time in milliseconds of the last hot-path kernel set of the last call
+ * (HIP events recorded on the context's stream around the kernels only). */
+double brpa_last_kernel_ms(const brpa_ctx *ctx);
+
+/* Device time (ms) of one stage of the last E_RPA call, summed over its
+ * launches, and the number of launches.  Stages: */
+enum {
+    BRPA_STAGE_PI = 0,     /* Pi(u) SYRK kernel (FP64 or split-BF16 MFMA) */
+    BRPA_STAGE_SPLIT = 1,  /* split-BF16 slicing pass */
+    BRPA_STAGE_CHOL = 2,   /* blocked Cholesky log-det (panel + trailing updates) */
+    BRPA_STAGE_AUX = 3,    /* Delta table, exact diagonal, quadrature */
+    BRPA_NSTAGES = 4
+};
+double brpa_last_stage_ms(const brpa_ctx *ctx, int stage);
+int brpa_last_stage_launches(const brpa_ctx *ctx, int stage);
+
+/* ---- frequency grid (host code) ----------------------------------------- */
+/* Fixed-size grid of one fragment: 100-bin gap histogram, 15000-point scan of
+ * the Casimir-Polder scale alpha in [0.1, 15], transformed Gauss-Legendre rule.
+ * Replaces rpa_DaiHistogram (rpa/OrbDiffHist.f90:42-123) + rpa_OptQuadScaling
+ * (rpa/OptFreqQuad.f90:75-122) + quad_CasimirPolder (rpa/quadratures.f90:570-643)
+ * at a given NFreqs.  eocc: active occupied energies ascending; evir ascending.
+ * info (may be NULL): {alpha, rmsd, max_rel_error}. */
+int brpa_freq_grid(const double *eocc, int64_t nocc, const double *evir, int64_t nvir,
+                   int64_t nfreqs, double *freqs, double *weights, double *info);
+
+/* Adaptive grid: the smallest n in [min_nfreqs, max_nfreqs] meeting both
+ * targets.  Replaces rpa_OptimizeFreqQuad (rpa/OptFreqQuad.f90:12-72).
+ * freqs/weights must hold max_nfreqs entries. */
+int brpa_optimize_freq_grid(const double *eocc, int64_t nocc, const double *evir, int64_t nvir,
+                            double target_error, double target_rel_error,
+                            int64_t min_nfreqs, int64_t max_nfreqs,
+                            double *freqs, double *weights, int64_t *nfreqs, double *info);
+
+/* ---- Pi(u) build --------------------------------------------------------- */
+/* GPiUG(N,N) += prefactor * CRG * diag(W) * CRG^T, W_ai = d_ai/(d_ai^2+u^2),
+ * d_ai = evir[a] - eocc[i], ai = i*nvir + a.  Full symmetric matrix updated.
+ * Replaces rpa_GPiUG_MO (rpa/rpa_core_MO.f90:347-406) for one spin block;
+ * prefactor is 4 for closed shells and 2 per spin otherwise (379-383).
+ * crg is the Fortran CRG(N, nocc*nvir) (C: R[ai][P]). */
+int brpa_gpiug_mo(brpa_ctx *ctx, double *gpiug, const double *crg, int64_t naux,
+                  const double *eocc, int64_t nocc, const double *evir, int64_t nvir,
+                  double freq, double prefactor);
+
+/* ---- E_RPA from stored Pi(u) ---------------------------------------------- */
+/* EcRPA = sum_u w_u/(2 pi) [log det(1 + Pi_u) - tr Pi_u], Pi_u the N x N slices
+ * of piu (N*N*nfreqs doubles).  Replaces rpa_CC_EcRPA_Analytic
+ * (rpa/rpa_CC.f90:412-471); the log-det is a blocked FP64 Cholesky instead of
+ * DGEQRF (1+Pi is SPD), sums in double-double instead of REAL(F128).
+ * eu (may be NULL) receives the nfreqs per-frequency terms E(u). */
+int brpa_ecrpa_analytic(brpa_ctx *ctx, double *ecrpa, const double *piu,
+                        const double *freqs, const double *weights, int64_t nfreqs,
+                        int64_t naux, double *eu);
+
+/* ---- fused E_RPA of one fragment ---------------------------------------- */
+/* Pi(u_k) build + exact diagonal/trace + Cholesky log-det + quadrature, Pi
+ * kept in device memory.  Equivalent to rpa_CC_PiU's frequency loop
+ * (rpa/rpa_CC.f90:523-526) followed by rpa_CC_EcRPA_Analytic (785). */
+int brpa_ecrpa(brpa_ctx *ctx, int precision, const double *R, int64_t naux,
+               const double *eocc, int64_t nocc, const double *evir, int64_t nvir,
+               const double *freqs, const double *weights, int64_t nfreqs,
+               double *ecrpa, double *eu);
+
+/* ---- batched fragments ---------------------------------------------------- */
+typedef struct {
+    int64_t nocc, nvir, naux, nfreqs;
+    const double *R;        /* (nocc*nvir) x naux, host or device */
+    const double *eocc;     /* nocc */
+    const double *evir;     /* nvir */
+    const double *freqs;    /* nfreqs */
+    const double *weights;  /* nfreqs */
+} brpa_frag_t;
+
+/* E_RPA of many independent fragments in one set of batched launches (the
+ * reference runs subsystems one after another, rpa/rpa_driver.f90:176-223). */
+int brpa_ecrpa_batch(brpa_ctx *ctx, int precision, int64_t nfrag, const brpa_frag_t *frags,
+                     double *ecrpa);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BRPA_H */

@@ -1,0 +1,168 @@
+"""Factorised direct-ring doubles amplitudes (oracle restatement of rpa_CC_T2_v5).
+
+Reference: src/rpa/rpa_CC_Doubles.f90:491-642 (rpa_CC_T2_v5) with
+  rpa_CC_NormalizeColumns 80-91, rpa_CC_Chi 276-292, rpa_CC_RChiM 295-316,
+  rpa_CC_ChiRTQWQTRChiM 319-345, rpa_CC_D 348-366, rpa_CC_IM_v5 369-422,
+  rpa_CC_MIM_v5 425-488, rpa_T2_EigRPA 987-1009,
+and rpa_CC_Diagonalize_PiU (src/rpa/rpa_CC.f90:1108-1129).
+
+Algorithm (lambda = T2CouplingStrength, default 1):
+  1. Guess vectors Omega (ov x NG), entries U(-1/2, 1/2), columns normalised.
+  2. I*Omega = -sum_k w_k/(4 pi) chi_k R^T Q_k diag(l/(1+l W_k)) Q_k^T R chi_k Omega
+     with chi_k = -4 d/(d^2+u_k^2); columns normalised again.
+  3. S = (I Omega)^T (I Omega); keep the eigenvalues above the cutoff
+     (counted from the largest down, stopping at the first one below).
+  4. T = (I Omega) U_s s^{-1/2};  TIT = T^T I T (MIM_v5);  eig -> a.
+  5. A = a / (2 (1 + a));  V = eigvecs^T T^T.
+
+The guess vectors: the reference draws them from gfortran's
+random_init(repeatable=.true., image_distinct=.true.) stream
+(rpa_CC_Doubles.f90:571-588), which no other program can reproduce.  Here
+they come from ``guess_vectors``, a SplitMix64 counter hash of
+(seed, column, row) that the HIP library evaluates bit-identically; the
+uniform-(-1/2,1/2) distribution and column normalisation are the reference's.
+
+Pinned by (tests/test_oracle_t2.py): with NG >= ov and a zero cutoff the
+factorisation equals the explicit T = I (1+I)^{-1} / 2 of the full ov x ov
+matrix I; that explicit T matches the Riccati closed form at exact frequency
+integration up to the grid error; E_T2 = 2 sum_mu A_mu |R^T V_mu|^2 reproduces
+E_RPA the way the reference's EigRPA diagnostic does.  The eigenvalue cutoff
+and the random subspace make the factorised result depend on the guess
+vectors, so bit-level parity with the reference's own (A, V) is impossible by
+construction; A, V are compared with this oracle on the same guess vectors.
+"""
+import math
+
+import numpy as np
+
+SMALL_EIGENVAL_CUTOFF_T2 = 1.0e-10   # src/rpa/rpa_definitions.f90:305
+GUESS_NVECS_T2 = 3.0                 # src/rpa/rpa_definitions.f90:428
+MAX_BATCH_DIM_T2 = 100               # src/rpa/rpa_definitions.f90:356
+
+_M64 = (1 << 64) - 1
+_GOLDEN = 0x9E3779B97F4A7C15
+
+
+def guess_count(naux, nai, factor=GUESS_NVECS_T2):
+    """ceiling(GuessNVecsT2 * NVecsPiU) (rpa.f90:1696), capped at the ov dimension."""
+    return int(min(nai, math.ceil(factor * naux)))
+
+
+def guess_vectors(seed, nrows, ncols):
+    """Omega[row, col] = u - 1/2 with u = SplitMix64(seed + (col*nrows+row+1)*golden) / 2^64.
+
+    Same bits as the device generator; columns are NOT yet normalised.
+    """
+    idx = (np.arange(ncols, dtype=np.uint64)[None, :] * np.uint64(nrows)
+           + np.arange(nrows, dtype=np.uint64)[:, None] + np.uint64(1))
+    with np.errstate(over="ignore"):
+        z = np.uint64(seed & _M64) + idx * np.uint64(_GOLDEN)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    u = (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+    return u - 0.5
+
+
+def normalize_columns(M):
+    """rpa_CC_NormalizeColumns (rpa_CC_Doubles.f90:80-91)."""
+    return M / np.linalg.norm(M, axis=0)[None, :]
+
+
+def diagonalize_pi(Pis):
+    """Ascending eigenvalues W_k and eigenvectors Q_k of each Pi(u_k) (rpa_CC.f90:1108-1129)."""
+    W, Q = [], []
+    for P in Pis:
+        w, q = np.linalg.eigh(P)
+        W.append(w)
+        Q.append(q)
+    return W, Q
+
+
+def chi(gaps, freq):
+    """rpa_CC_Chi: -4 d / (d^2 + u^2)."""
+    return -(4.0 * gaps) / (gaps ** 2 + freq ** 2)
+
+
+def im_v5(M, Q, W, R, gaps, freqs, weights, lam=1.0):
+    """I*M following rpa_CC_IM_v5 (rpa_CC_Doubles.f90:369-422); R is R[ai][P]."""
+    IM = np.zeros_like(M)
+    for k in range(len(freqs)):
+        c = chi(gaps, freqs[k])
+        RChiM = R.T @ (c[:, None] * M)                  # (N, nq)
+        MTChiRTQ = RChiM.T @ Q[k]                       # (nq, N)
+        MTChiRTQ = MTChiRTQ * (lam / (1.0 + lam * W[k]))[None, :]
+        MTChiRTQWQT = MTChiRTQ @ Q[k].T                 # (nq, N)
+        X = (R @ MTChiRTQWQT.T) * c[:, None]            # (ov, nq)
+        IM += (-weights[k] / (4.0 * math.pi)) * X
+    return IM
+
+
+def mim_v5(T, Q, W, R, gaps, freqs, weights, lam=1.0):
+    """T^T I T following rpa_CC_MIM_v5 (rpa_CC_Doubles.f90:425-488)."""
+    nt = T.shape[1]
+    MIM = np.zeros((nt, nt))
+    for k in range(len(freqs)):
+        c = chi(gaps, freqs[k])
+        QTRChi = (Q[k].T @ R.T) * c[None, :]            # (N, ov)
+        MTChiRTQ = T.T @ QTRChi.T                       # (nt, N)
+        MTChiRTQW = MTChiRTQ * (lam / (1.0 + lam * W[k]))[None, :]
+        MIM += (-weights[k] / (4.0 * math.pi)) * (MTChiRTQ @ MTChiRTQW.T)
+    return MIM
+
+
+def t2_v5(R, gaps, freqs, weights, Pis, lam=1.0, cutoff=SMALL_EIGENVAL_CUTOFF_T2,
+          nguess=None, seed=0, maxbatch=MAX_BATCH_DIM_T2):
+    """(A, V) with A ascending (NT2,) and V as C row-major V[ai][mu] (ov x NT2).
+
+    V[ai][mu] is bit-for-bit the layout of the reference's Fortran V(mu, ai).
+    ``Pis`` are the Pi(u_k) matrices of the same grid (reference sign).
+    """
+    nai, naux = R.shape
+    ng = guess_count(naux, nai) if nguess is None else nguess
+    W, Q = diagonalize_pi(Pis)
+    omega = normalize_columns(guess_vectors(seed, nai, ng))
+    IOmega = np.empty((nai, ng))
+    for q0 in range(0, ng, maxbatch):
+        q1 = min(ng, q0 + maxbatch)
+        IOmega[:, q0:q1] = normalize_columns(
+            im_v5(omega[:, q0:q1], Q, W, R, gaps, freqs, weights, lam))
+    S = IOmega.T @ IOmega
+    s, U = np.linalg.eigh(S)
+    nt = 0
+    for i in range(ng - 1, -1, -1):
+        if s[i] > cutoff:
+            nt += 1
+        else:
+            break
+    Wm = U[:, ng - nt:] / np.sqrt(s[ng - nt:])[None, :]
+    T = IOmega @ Wm
+    TIT = mim_v5(T, Q, W, R, gaps, freqs, weights, lam)
+    a, E = np.linalg.eigh(TIT)
+    A = 0.5 * (a / (a + 1.0))
+    V = T @ E
+    return A, V
+
+
+def i_matrix_explicit(R, gaps, freqs, weights, Pis, lam=1.0):
+    """Full ov x ov I = -sum_k w_k/(4 pi) chi_k R l(1+l Pi_k)^{-1} R^T chi_k (tiny inputs)."""
+    nai, n = R.shape
+    I = np.zeros((nai, nai))
+    for k in range(len(freqs)):
+        c = chi(gaps, freqs[k])
+        Minv = lam * np.linalg.inv(np.eye(n) + lam * Pis[k])
+        X = c[:, None] * R
+        I += (-weights[k] / (4.0 * math.pi)) * (X @ Minv @ X.T)
+    return I
+
+
+def t2_explicit(I):
+    """T2 = I (1+I)^{-1} / 2, i.e. A = a/(2(1+a)) on the eigenvalues of I."""
+    a, E = np.linalg.eigh(I)
+    return (E * (0.5 * a / (1.0 + a))[None, :]) @ E.T
+
+
+def eig_rpa(A, V, R):
+    """rpa_T2_EigRPA: EigRPA_mu = 2 A_mu |R^T V_mu|^2 (rpa_CC_Doubles.f90:987-1009)."""
+    RU = R.T @ V
+    return 2.0 * A * np.sum(RU * RU, axis=0)
