@@ -26,7 +26,7 @@ sys.path.insert(0, ROOT)
 
 PEAK_FP64_TFLOPS = 78.6      # MI355X dense FP64 matrix
 PEAK_BF16_TFLOPS = 2500.0    # MI355X dense BF16 matrix
-SPLIT_PRODUCTS = 6           # bf16 slice products per FP64-equivalent product
+SPLIT_PRODUCTS = 10          # bf16 slice products per FP64-equivalent product (4 slices, level <= 3)
 
 
 def syrk_flops(o, v, n, nf):
@@ -157,7 +157,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "bf16x6->fp32 (split), fp64 log-det" if a.precision == "split_bf16" else "f64",
+        "dtype": "bf16 (4-slice split, 10 products, fp32 acc) + f64 log-det" if a.precision == "split_bf16" else "f64",
         "data": "synthetic (seeded PCG64, brpa.synth)",
         "config": {"workload": "batch of %d uneven fragments, o~U{8..40}, v~U{70..350}, "
                                "N=round(4.5(o+v)), %d frequencies" % (a.nfrag, a.nfreqs),

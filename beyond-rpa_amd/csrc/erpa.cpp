@@ -110,7 +110,7 @@ size_t frag_bytes(const HostFrag &h, int precision) {
     size_t b = 8 * (F * K + F * N * N + 4 * F * N + 64 * F + F * CHOL_NB * CHOL_NB) + 8 * K * N;
     if (precision == BRPA_SPLIT_BF16) {
         size_t kpad = cdiv(K, SPLIT_KB) * SPLIT_KB;
-        b += 2 * F * 3 * N * kpad + (4 + 16) * F * (kpad / SPLIT_KB) * N;
+        b += 2 * F * NSLICE * N * kpad + (4 + 16) * F * (kpad / SPLIT_KB) * N;
     }
     return b;
 }
@@ -182,7 +182,7 @@ int run_chunk(brpa_ctx *ctx, int precision, const std::vector<HostFrag> &fr, dou
             d.Kpad = (int)(cdiv(K, SPLIT_KB) * SPLIT_KB);
             d.nblk = d.Kpad / SPLIT_KB;
             if (precision == BRPA_SPLIT_BF16) {
-                d.sl = ar.take<uint16_t>((size_t)F * 3 * N * d.Kpad);
+                d.sl = ar.take<uint16_t>((size_t)F * NSLICE * N * d.Kpad);
                 d.scale = ar.take<float>((size_t)F * d.nblk * N);
                 d.dpart = ar.take<double>((size_t)F * d.nblk * N * 2);
             } else {

@@ -93,7 +93,7 @@ struct FragDev {
     int nocc, nvir;
     const double *eocc, *evir, *freqs;   // device copies
     // split-BF16 slices (only when precision == BRPA_SPLIT_BF16)
-    uint16_t *sl;            // F x 3 x N x Kpad, per column P the Kpad rows contiguous
+    uint16_t *sl;            // F x NSLICE x N x Kpad, per column P the Kpad rows contiguous
     float *scale;            // F x nblk x N, block scale 2^(e-8)
     int Kpad, nblk;
 };
@@ -101,6 +101,7 @@ struct FragDev {
 constexpr int CHOL_NB = 64;   // Cholesky block size
 constexpr int SYRK_T = 64;    // FP64 SYRK tile
 constexpr int SPLIT_KB = 128; // split-BF16 exactness block (rows per scale)
+constexpr int NSLICE = 4;     // bf16 slices per FP64 value in the split-BF16 form
 
 // Launch wrappers (defined in the kernel files).  `pre` arrays are device
 // prefix sums over fragments of the number of work-groups per fragment.

@@ -2,16 +2,17 @@
 //
 // X_w = diag(sqrt(c_w)) R is cut into 128-row blocks.  In block b, column P
 // gets a power-of-two scale s = 2^(e-8) with max_k |X[k][P]| < 2^e, and every
-// element is written as s (n0 + n1/256 + n2/65536) with integers |n0| <= 256,
-// |n1|, |n2| <= 128: exact in bf16, representation error <= s/2^17.
-// Pi_w[p][q] = sum_b s_p s_q sum_k (x_p x_q in slices), and the six slice
-// products with level a+b <= 2 are accumulated per level in FP32:
-//   L0 = n0 n0,  L1 = n0 n1 + n1 n0,  L2 = n0 n2 + n2 n0 + n1 n1.
-// Over 128 rows every partial sum is an integer below 2^24, so the FP32 MFMA
-// accumulation is exact; each block is flushed to FP64 as
-// s_p s_q (L0 + L1/2^8 + L2/2^16).  The diagonal (and so tr Pi) is replaced
-// by the exact FP64 value afterwards (k_diag / k_set_diag), which removes the
-// systematic diagonal bias of the dropped level-3/4 products.
+// element is written as s (n0 + n1 2^-8 + n2 2^-16 + n3 2^-24) with integers
+// |n0| <= 256, |n1|,|n2|,|n3| <= 128.  Slice k is stored as the bf16 value
+// n_k 2^(-8k), which is exact.  Pi_w[p][q] = sum_b s_p s_q sum_k x_p x_q, and
+// the ten slice products of level a+b <= 3 go to three FP32 accumulators:
+//   L0  = n0 n0                          (integer sums < 2^24: exact)
+//   L1  = (n0 n1 + n1 n0) 2^-8           (integer sums < 2^24 times 2^-8: exact)
+//   L23 = levels 2 and 3                 (rounding ~2^-40 relative to Pi)
+// Each 128-row block is flushed to FP64 as s_p s_q (L0 + L1 + L23).  The
+// dropped level >= 4 products are ~2^-32 relative.  The diagonal (and so
+// tr Pi) is then replaced by its exact FP64 value (k_split partials, summed in
+// k_set_diag), which removes the systematic diagonal bias of the truncation.
 #include "device_common.h"
 #include "internal.h"
 
@@ -23,8 +24,10 @@
                              hipGetErrorString(_e), __FILE__, __LINE__);                       \
     } while (0)
 
+// n 2^-8k with an integer |n| <= 256: exact in float and in bf16 (truncating
+// the float's low 16 bits, which are zero, is the exact conversion).
 __device__ __forceinline__ uint16_t int_to_bf16_bits(double n) {
-    float f = (float)n;   // |n| <= 256 integer: exact in float and in bf16
+    float f = (float)n;
     return (uint16_t)(__float_as_uint(f) >> 16);
 }
 
@@ -98,33 +101,28 @@ __global__ __launch_bounds__(256) void k_split(const FragDev *__restrict__ fr,
     const int P = P0 + c;
     if (P >= N) return;
     const int e = sExp[c];
-    uint16_t *s0 = F.sl + (((int64_t)w * 3 + 0) * N + P) * F.Kpad + k0 + r0;
-    uint16_t *s1 = F.sl + (((int64_t)w * 3 + 1) * N + P) * F.Kpad + k0 + r0;
-    uint16_t *s2 = F.sl + (((int64_t)w * 3 + 2) * N + P) * F.Kpad + k0 + r0;
     for (int r = 0; r < 32; r += 8) {
-        uint16_t v0[8], v1[8], v2[8];
+        uint16_t v[NSLICE][8];
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
-            double x = ldexp(sX[r0 + r + i][c], 8 - e);   // |x| < 256, exact
-            double n0 = rint(x);
-            double t1 = (x - n0) * 256.0;                  // exact
-            double n1 = rint(t1);
-            double t2 = (t1 - n1) * 256.0;                 // exact
-            double n2 = rint(t2);
-            v0[i] = int_to_bf16_bits(n0);
-            v1[i] = int_to_bf16_bits(n1);
-            v2[i] = int_to_bf16_bits(n2);
+            double t = ldexp(sX[r0 + r + i][c], 8 - e);   // |t| < 256, exact
+#pragma unroll
+            for (int sl = 0; sl < NSLICE; ++sl) {
+                double n = rint(t);
+                t = (t - n) * 256.0;                      // exact
+                v[sl][i] = int_to_bf16_bits(ldexp(n, -8 * sl));
+            }
         }
-        uint4 p0, p1, p2;
-        p0.x = v0[0] | ((uint32_t)v0[1] << 16); p0.y = v0[2] | ((uint32_t)v0[3] << 16);
-        p0.z = v0[4] | ((uint32_t)v0[5] << 16); p0.w = v0[6] | ((uint32_t)v0[7] << 16);
-        p1.x = v1[0] | ((uint32_t)v1[1] << 16); p1.y = v1[2] | ((uint32_t)v1[3] << 16);
-        p1.z = v1[4] | ((uint32_t)v1[5] << 16); p1.w = v1[6] | ((uint32_t)v1[7] << 16);
-        p2.x = v2[0] | ((uint32_t)v2[1] << 16); p2.y = v2[2] | ((uint32_t)v2[3] << 16);
-        p2.z = v2[4] | ((uint32_t)v2[5] << 16); p2.w = v2[6] | ((uint32_t)v2[7] << 16);
-        *reinterpret_cast<uint4 *>(s0 + r) = p0;
-        *reinterpret_cast<uint4 *>(s1 + r) = p1;
-        *reinterpret_cast<uint4 *>(s2 + r) = p2;
+#pragma unroll
+        for (int sl = 0; sl < NSLICE; ++sl) {
+            uint4 pk;
+            pk.x = v[sl][0] | ((uint32_t)v[sl][1] << 16);
+            pk.y = v[sl][2] | ((uint32_t)v[sl][3] << 16);
+            pk.z = v[sl][4] | ((uint32_t)v[sl][5] << 16);
+            pk.w = v[sl][6] | ((uint32_t)v[sl][7] << 16);
+            uint16_t *dst = F.sl + (((int64_t)w * NSLICE + sl) * N + P) * F.Kpad + k0 + r0 + r;
+            *reinterpret_cast<uint4 *>(dst) = pk;
+        }
     }
 }
 
@@ -137,22 +135,33 @@ int launch_split(brpa_ctx *ctx, const FragDev *fr, const int64_t *pre, int nfrag
 
 // ---------------------------------------------------------------------------
 // 128x128 tile pair per work-group, 8 waves as 2 (rows) x 4 (cols) sub-tiles
-// of 64x32: 2 waves per SIMD, FP32 level accumulators in AGPRs, the FP64 block
-// accumulator, stage prefetch and operand fragments in VGPRs.
+// of 64x32 (2 waves per SIMD).
 // MFMA v_mfma_f32_16x16x32_bf16 operand layout (lane l):
 //   A[m = l%16][k = 8*(l/16) .. +7], B[k = 8*(l/16) .. +7][n = l%16],
 //   D: 4 floats, D[r] = C[m = 4*(l/16) + r][n = l%16].
 // Both operands are columns of X (k contiguous per column), so A and B are
 // read from LDS the same way.
+//
+// Stages of 32 rows go straight from HBM to LDS (global_load_lds_dwordx4,
+// 16 bytes per lane, no register staging), double-buffered.  One wave load
+// fills 1 KB = 16 columns x 64 bytes; lane l lands in column l/4, slot l%4,
+// and fetches k-piece (l%4) ^ ((l/16)%4) of that column, so that the MFMA
+// operand reads (16 columns at one k-piece per 16 lanes) hit 16 distinct
+// 4-bank groups.
 constexpr int T16 = 128;
-constexpr int KS = 32;          // rows per LDS stage (one MFMA k-depth)
-constexpr int LROW = KS + 8;    // bf16 per LDS row (+8 pad: 80 B row stride)
+constexpr int KS = 32;          // rows per stage (one MFMA k-depth)
+
+typedef __attribute__((address_space(3))) void lds_void_t;
+
+__device__ __forceinline__ int swz(int col_local, int piece) {
+    return piece ^ ((col_local >> 2) & 3);
+}
 
 __global__ __launch_bounds__(512) void k_syrk_bf16(const FragDev *__restrict__ fr,
                                                   const int64_t *__restrict__ pre, int nfrag) {
-    __shared__ __attribute__((aligned(16))) uint16_t sA[3][T16][LROW];
-    __shared__ __attribute__((aligned(16))) uint16_t sB[3][T16][LROW];
-    __shared__ float sSP[T16], sSQ[T16];
+    // [buffer][side][slice][column][slot] of 16-byte pieces
+    __shared__ __attribute__((aligned(16))) uint4 sbuf[2][2][NSLICE][T16][4];
+    __shared__ float sSc[2][2 * T16];
     int64_t b = blockIdx.x;
     int f = find_frag(pre, nfrag, b);
     const FragDev &F = fr[f];
@@ -167,132 +176,132 @@ __global__ __launch_bounds__(512) void k_syrk_bf16(const FragDev *__restrict__ f
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int wi = wave >> 2, wj = wave & 3;     // 64-row half, 32-col quarter
 
-    const uint16_t *slw = F.sl + (int64_t)w * 3 * N * F.Kpad;
+    const uint16_t *slw = F.sl + (int64_t)w * NSLICE * N * F.Kpad;
     const float *scw = F.scale + (int64_t)w * F.nblk * N;
+    const int64_t Kpad = F.Kpad;
 
-    f4_t L0[4][2], L1[4][2], L2[4][2];
+    f4_t L0[4][2], L1[4][2], L23[4][2];
     double acc[4][2][4];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
-            L0[i][j] = L1[i][j] = L2[i][j] = (f4_t){0.f, 0.f, 0.f, 0.f};
+            L0[i][j] = L1[i][j] = L23[i][j] = (f4_t){0.f, 0.f, 0.f, 0.f};
 #pragma unroll
             for (int r = 0; r < 4; ++r) acc[i][j][r] = 0.0;
         }
 
-    // global -> register prefetch of one stage: 2 sides x 3 slices x 128 columns
-    // x 4 pieces of 8 bf16 (16 bytes) = 3072 pieces, 6 per thread.  Thread t
-    // always takes column t/4, piece t%4 (consecutive threads read one
-    // column's 64 contiguous bytes) of every slice on both sides.
-    uint4 pf[6];
-    const int lcol = tid >> 2, lkof = 8 * (tid & 3);
-    const bool okP = p0 + lcol < N, okQ = q0 + lcol < N;
-    const int64_t sstride = (int64_t)N * F.Kpad;
-    const uint16_t *gA = slw + (int64_t)(okP ? p0 + lcol : 0) * F.Kpad + lkof;
-    const uint16_t *gB = slw + (int64_t)(okQ ? q0 + lcol : 0) * F.Kpad + lkof;
-    auto gload = [&](int k0) {
+    // This wave's 8 of the 64 wave loads of a stage: load id = 8*wave + t
+    // -> side = id/32, slice = (id/8)%4, column group = id%8.  Per-lane global
+    // offsets (columns past N are clamped to N-1: they only feed outputs that
+    // are never written).
+    const int lcol = lane >> 2;
+    const int lpiece = (lane & 3) ^ ((lane >> 4) & 3);
+    int64_t goff[8];
 #pragma unroll
-        for (int s = 0; s < 3; ++s) {
-            pf[s] = okP ? *reinterpret_cast<const uint4 *>(gA + s * sstride + k0) : make_uint4(0, 0, 0, 0);
-            pf[3 + s] = okQ ? *reinterpret_cast<const uint4 *>(gB + s * sstride + k0) : make_uint4(0, 0, 0, 0);
+    for (int t = 0; t < 8; ++t) {
+        const int id = 8 * wave + t;
+        const int side = id >> 5, sl = (id >> 3) & 3, g = id & 7;
+        int P = (side ? q0 : p0) + 16 * g + lcol;
+        P = P < N ? P : N - 1;
+        goff[t] = ((int64_t)sl * N + P) * Kpad + 8 * lpiece;
+    }
+    auto issue = [&](int buf, int k0) {
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            const int id = 8 * wave + t;
+            const int side = id >> 5, sl = (id >> 3) & 3, g = id & 7;
+            __builtin_amdgcn_global_load_lds((const void *)(slw + goff[t] + k0),
+                                             (lds_void_t *)&sbuf[buf][side][sl][16 * g][0], 16, 0, 0);
         }
     };
-    auto lstore = [&]() {
-#pragma unroll
-        for (int s = 0; s < 3; ++s) {
-            *reinterpret_cast<uint4 *>(&sA[s][lcol][lkof]) = pf[s];
-            *reinterpret_cast<uint4 *>(&sB[s][lcol][lkof]) = pf[3 + s];
-        }
+    auto frag = [&](int buf, int side, int s, int row) {
+        const int cl = row & 15;
+        const uint4 v = sbuf[buf][side][s][row][swz(cl, lane >> 4)];
+        return __builtin_bit_cast(bf8_t, v);
     };
-    auto afrag = [&](int s, int i) {
-        return *reinterpret_cast<const bf8_t *>(&sA[s][64 * wi + 16 * i + (lane & 15)][8 * (lane >> 4)]);
-    };
-    auto bfrag = [&](int s, int j) {
-        return *reinterpret_cast<const bf8_t *>(&sB[s][32 * wj + 16 * j + (lane & 15)][8 * (lane >> 4)]);
-    };
+#define MFMA(a, b, c) c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0)
 
     const int nblk = F.nblk;
     const int ksteps = SPLIT_KB / KS;
-    gload(0);
-    for (int blk = 0; blk < nblk; ++blk) {
-        for (int ks = 0; ks < ksteps; ++ks) {
-            const int k0 = blk * SPLIT_KB + ks * KS;
-            __syncthreads();
-            lstore();
-            if (ks == 0 && tid < 2 * T16) {
-                int side = tid / T16, c = tid - side * T16;
-                int P = (side ? q0 : p0) + c;
-                float sc = (P < N) ? scw[(int64_t)blk * N + P] : 0.0f;
-                if (side)
-                    sSQ[c] = sc;
-                else
-                    sSP[c] = sc;
-            }
-            __syncthreads();
-            if (k0 + KS < F.Kpad) gload(k0 + KS);
-            // products grouped by A slice and by pairs of 16-row blocks so that
-            // few fragments are live at once
-            {
-                bf8_t b0[2], b1[2], b2[2];
+    const int nst = nblk * ksteps;
+    issue(0, 0);
+    for (int st = 0; st < nst; ++st) {
+        const int buf = st & 1, blk = st / ksteps, ks = st - blk * ksteps;
+        if (ks == 0 && tid < 2 * T16) {
+            int side = tid / T16, c = tid - side * T16;
+            int P = (side ? q0 : p0) + c;
+            sSc[blk & 1][tid] = (P < N) ? scw[(int64_t)blk * N + P] : 0.0f;
+        }
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();   // stage st in sbuf[buf]; everyone is done with sbuf[buf^1]
+        if (st + 1 < nst) issue(buf ^ 1, (st + 1) * KS);
+        // the ten products of level <= 3, grouped by the A slice and by pairs
+        // of 16-row blocks so that few fragments are live at once
+        bf8_t bq[NSLICE][2];
+#pragma unroll
+        for (int s2 = 0; s2 < NSLICE; ++s2)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) bq[s2][j] = frag(buf, 1, s2, 32 * wj + 16 * j + (lane & 15));
+#pragma unroll
+        for (int ih = 0; ih < 4; ih += 2) {
+            bf8_t a[2];
+#pragma unroll
+            for (int ii = 0; ii < 2; ++ii) a[ii] = frag(buf, 0, 0, 64 * wi + 16 * (ih + ii) + (lane & 15));
+#pragma unroll
+            for (int ii = 0; ii < 2; ++ii)
 #pragma unroll
                 for (int j = 0; j < 2; ++j) {
-                    b0[j] = bfrag(0, j);
-                    b1[j] = bfrag(1, j);
-                    b2[j] = bfrag(2, j);
+                    MFMA(a[ii], bq[0][j], L0[ih + ii][j]);
+                    MFMA(a[ii], bq[1][j], L1[ih + ii][j]);
+                    MFMA(a[ii], bq[2][j], L23[ih + ii][j]);
+                    MFMA(a[ii], bq[3][j], L23[ih + ii][j]);
                 }
 #pragma unroll
-                for (int ih = 0; ih < 4; ih += 2) {
-                    bf8_t a[2];
+            for (int ii = 0; ii < 2; ++ii) a[ii] = frag(buf, 0, 1, 64 * wi + 16 * (ih + ii) + (lane & 15));
 #pragma unroll
-                    for (int ii = 0; ii < 2; ++ii) a[ii] = afrag(0, ih + ii);
+            for (int ii = 0; ii < 2; ++ii)
 #pragma unroll
-                    for (int ii = 0; ii < 2; ++ii)
-#pragma unroll
-                        for (int j = 0; j < 2; ++j) {
-                            const int i = ih + ii;
-                            L0[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[ii], b0[j], L0[i][j], 0, 0, 0);
-                            L1[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[ii], b1[j], L1[i][j], 0, 0, 0);
-                            L2[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[ii], b2[j], L2[i][j], 0, 0, 0);
-                        }
-#pragma unroll
-                    for (int ii = 0; ii < 2; ++ii) a[ii] = afrag(1, ih + ii);
-#pragma unroll
-                    for (int ii = 0; ii < 2; ++ii)
-#pragma unroll
-                        for (int j = 0; j < 2; ++j) {
-                            const int i = ih + ii;
-                            L1[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[ii], b0[j], L1[i][j], 0, 0, 0);
-                            L2[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[ii], b1[j], L2[i][j], 0, 0, 0);
-                        }
-#pragma unroll
-                    for (int ii = 0; ii < 2; ++ii) a[ii] = afrag(2, ih + ii);
-#pragma unroll
-                    for (int ii = 0; ii < 2; ++ii)
-#pragma unroll
-                        for (int j = 0; j < 2; ++j) {
-                            const int i = ih + ii;
-                            L2[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[ii], b0[j], L2[i][j], 0, 0, 0);
-                        }
+                for (int j = 0; j < 2; ++j) {
+                    MFMA(a[ii], bq[0][j], L1[ih + ii][j]);
+                    MFMA(a[ii], bq[1][j], L23[ih + ii][j]);
+                    MFMA(a[ii], bq[2][j], L23[ih + ii][j]);
                 }
-            }
+#pragma unroll
+            for (int ii = 0; ii < 2; ++ii) a[ii] = frag(buf, 0, 2, 64 * wi + 16 * (ih + ii) + (lane & 15));
+#pragma unroll
+            for (int ii = 0; ii < 2; ++ii)
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    MFMA(a[ii], bq[0][j], L23[ih + ii][j]);
+                    MFMA(a[ii], bq[1][j], L23[ih + ii][j]);
+                }
+#pragma unroll
+            for (int ii = 0; ii < 2; ++ii) a[ii] = frag(buf, 0, 3, 64 * wi + 16 * (ih + ii) + (lane & 15));
+#pragma unroll
+            for (int ii = 0; ii < 2; ++ii)
+#pragma unroll
+                for (int j = 0; j < 2; ++j) MFMA(a[ii], bq[0][j], L23[ih + ii][j]);
         }
-        // flush the exact level sums of this block to FP64
+        if (ks == ksteps - 1) {
+            // flush this block's level sums to FP64
+            const float *sc = sSc[blk & 1];
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
+            for (int i = 0; i < 4; ++i)
 #pragma unroll
-            for (int j = 0; j < 2; ++j) {
-                const double sq = (double)sSQ[32 * wj + 16 * j + (lane & 15)];
+                for (int j = 0; j < 2; ++j) {
+                    const double sq = (double)sc[T16 + 32 * wj + 16 * j + (lane & 15)];
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const double sp = (double)sSP[64 * wi + 16 * i + 4 * (lane >> 4) + r];
-                    double v = (double)L0[i][j][r] + (double)L1[i][j][r] * (1.0 / 256.0) +
-                               (double)L2[i][j][r] * (1.0 / 65536.0);
-                    acc[i][j][r] = fma(v, sp * sq, acc[i][j][r]);
+                    for (int r = 0; r < 4; ++r) {
+                        const double sp = (double)sc[64 * wi + 16 * i + 4 * (lane >> 4) + r];
+                        double v = (double)L0[i][j][r] + (double)L1[i][j][r] + (double)L23[i][j][r];
+                        acc[i][j][r] = fma(v, sp * sq, acc[i][j][r]);
+                    }
+                    L0[i][j] = L1[i][j] = L23[i][j] = (f4_t){0.f, 0.f, 0.f, 0.f};
                 }
-                L0[i][j] = L1[i][j] = L2[i][j] = (f4_t){0.f, 0.f, 0.f, 0.f};
-            }
+        }
     }
+#undef MFMA
 
     double *C = F.Pi + (int64_t)w * N * N;
     const bool diag = (tp == tq);

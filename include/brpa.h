@@ -47,9 +47,11 @@ enum {
 /* Arithmetic of the Pi and amplitude contractions.  BRPA_FP64 is the drop-in. */
 enum {
     BRPA_FP64 = 0,
-    BRPA_SPLIT_BF16 = 1      /* 3 fixed-point bf16 slices, 6 slice products on MFMA,
-                                exact FP32 accumulation per 128-row block, FP64 between
-                                blocks; exact FP64 diagonal and trace; FP64 log-det */
+    BRPA_SPLIT_BF16 = 1      /* 4 fixed-point bf16 slices per value, the 10 slice
+                                products of level <= 3 on bf16 MFMA with FP32
+                                accumulation (levels 0 and 1 exact per 128-row block),
+                                FP64 between blocks; exact FP64 diagonal and trace;
+                                FP64 Cholesky log-det and quadrature */
 };
 
 /* ---- context ------------------------------------------------------------ */
