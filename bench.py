@@ -50,7 +50,21 @@ def main():
                     help="CPU oracle sample budget (0 disables the CPU baseline)")
     ap.add_argument("--compare-fp64", type=int, default=1,
                     help="also time the FP64 form (reported as fp64_value)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device copy 5 times (reference for the "
+                         "FETCH_SIZE/WRITE_SIZE counter scale)")
     a = ap.parse_args()
+    if a.copy_calibration:
+        import torch
+        n = (1 << 30) // 8
+        x = torch.ones(n, dtype=torch.float64, device="cuda:0")
+        y = torch.empty_like(x)
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read_per_launch": 8 * n, "copy_bytes_written_per_launch": 8 * n,
+                          "launches": 5}))
+        return
     if a.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(--gpus %d requested)" % a.gpus, file=sys.stderr)

@@ -215,3 +215,40 @@ class Context:
         self._check(self._lib.brpa_ecrpa_batch(self._h, _prec(precision), len(frags), arr,
                                                _ptr(e)))
         return e
+
+    def t2(self, R, eocc, evir, freqs, weights, lam=1.0, cutoff=1e-10, nguess=0, seed=0,
+           precision="fp64"):
+        """Factorised direct-ring amplitudes (rpa_CC_T2).  Returns (A, V) with
+        A ascending (NT2,) and V as C-order V[ai][mu] (ov, NT2)."""
+        nai, naux = R.shape
+        eocc, evir = np.ascontiguousarray(eocc, np.float64), np.ascontiguousarray(evir, np.float64)
+        freqs = np.ascontiguousarray(freqs, np.float64)
+        weights = np.ascontiguousarray(weights, np.float64)
+        if nai != eocc.size * evir.size or freqs.size != weights.size:
+            raise ValueError("shape mismatch")
+        nt = C.c_int64(0)
+        self._sync_stream(R)
+        self._check(self._lib.brpa_t2_compute(self._h, _prec(precision), _ptr(R), naux, _ptr(eocc),
+                                              eocc.size, _ptr(evir), evir.size, _ptr(freqs),
+                                              _ptr(weights), freqs.size, float(lam), float(cutoff),
+                                              int(nguess), int(seed) & 0xFFFFFFFFFFFFFFFF,
+                                              C.byref(nt)))
+        k = nt.value
+        A = np.zeros(k)
+        V = np.zeros((nai, k))
+        if k:
+            self._check(self._lib.brpa_t2_copyout(self._h, _ptr(A), _ptr(V)))
+        return A, V
+
+    def sosex(self, V, A, R, nocc, nvir, alpha=1.0, s2_operator=False, precision="fp64"):
+        """Exchange correction -alpha sum (aj|bi) T(ai,bj) (rpa_CC_Exchange_SOSEX)."""
+        nai, naux = R.shape
+        A = np.ascontiguousarray(A, np.float64)
+        if nai != nocc * nvir or tuple(V.shape) != (nai, A.size):
+            raise ValueError("shape mismatch")
+        e = np.zeros(1)
+        self._sync_stream(V, R)
+        self._check(self._lib.brpa_sosex(self._h, _prec(precision), _ptr(e), float(alpha), _ptr(V),
+                                         _ptr(A), _ptr(R), nocc, nvir, naux, A.size,
+                                         int(bool(s2_operator))))
+        return float(e[0])

@@ -23,7 +23,14 @@ struct brpa_ctx {
     hipEvent_t sev[2 * BRPA_NSTAGES] = {};
     double stage_ms[BRPA_NSTAGES] = {};
     int stage_launches[BRPA_NSTAGES] = {};
+    // amplitude (T2) results kept on the device until brpa_t2_copyout
+    void *rocblas = nullptr;   // rocblas_handle, created on first use
+    double *t2_A = nullptr, *t2_V = nullptr;
+    size_t t2_A_cap = 0, t2_V_cap = 0;
+    int64_t t2_nt = -1, t2_nai = 0;
 };
+
+void t2_release(brpa_ctx *ctx);
 
 // Records the start/end of a stage on the context stream; stage_end adds the
 // elapsed time once the stream has been synchronised (stage_collect).

@@ -1,0 +1,177 @@
+// Elementwise / reduction kernels of the amplitude (T2) and SOSEX paths.
+// Matrices here are column-major (the reference's Fortran layout): element
+// (r, c) of an m-row matrix is at r + m*c.
+#include "device_common.h"
+#include "internal.h"
+
+#define LAUNCH_CHECK(ctx)                                                                      \
+    do {                                                                                       \
+        hipError_t _e = hipGetLastError();                                                     \
+        if (_e != hipSuccess)                                                                  \
+            return brpa_fail((ctx), BRPA_ERR_HIP, "kernel launch failed: %s (%s:%d)",          \
+                             hipGetErrorString(_e), __FILE__, __LINE__);                       \
+    } while (0)
+
+static inline unsigned nblocks(int64_t n) { return (unsigned)((n + 255) / 256); }
+
+// Omega(r, c) = u - 1/2, u = SplitMix64(seed + (c*m + r + 1) * golden) / 2^64
+// (counter-based: the oracle draws the same bits).  Guess vectors of
+// rpa_CC_T2_v5 (rpa/rpa_CC_Doubles.f90:571-588).
+__global__ void k_guess(double *__restrict__ O, int64_t m, int64_t n, uint64_t seed) {
+    int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= m * n) return;
+    uint64_t z = seed + (uint64_t)(e + 1) * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z = z ^ (z >> 31);
+    O[e] = (double)(z >> 11) * (1.0 / 9007199254740992.0) - 0.5;
+}
+
+int launch_guess(brpa_ctx *ctx, double *O, int64_t m, int64_t n, uint64_t seed) {
+    if (m * n <= 0) return BRPA_OK;
+    hipLaunchKernelGGL(k_guess, dim3(nblocks(m * n)), dim3(256), 0, ctx->stream, O, m, n, seed);
+    LAUNCH_CHECK(ctx);
+    return BRPA_OK;
+}
+
+// Divide every column of the m x n matrix by its 2-norm
+// (rpa_CC_NormalizeColumns, rpa/rpa_CC_Doubles.f90:80-91).  One work-group
+// per column, deterministic tree reduction.
+__global__ __launch_bounds__(256) void k_normalize_columns(double *__restrict__ X, int64_t m) {
+    __shared__ double sh[256];
+    double *col = X + (int64_t)blockIdx.x * m;
+    double s = 0.0;
+    for (int64_t r = threadIdx.x; r < m; r += 256) s = fma(col[r], col[r], s);
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if (threadIdx.x < k) sh[threadIdx.x] += sh[threadIdx.x + k];
+        __syncthreads();
+    }
+    const double inv = 1.0 / sqrt(sh[0]);
+    for (int64_t r = threadIdx.x; r < m; r += 256) col[r] *= inv;
+}
+
+int launch_normalize_columns(brpa_ctx *ctx, double *X, int64_t m, int64_t n) {
+    if (m <= 0 || n <= 0) return BRPA_OK;
+    hipLaunchKernelGGL(k_normalize_columns, dim3((unsigned)n), dim3(256), 0, ctx->stream, X, m);
+    LAUNCH_CHECK(ctx);
+    return BRPA_OK;
+}
+
+// out(r, c) = s[r] * in(r, c)   (out may alias in)
+__global__ void k_row_scale(double *out, const double *in, const double *__restrict__ s, int64_t m,
+                            int64_t n) {
+    int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= m * n) return;
+    out[e] = s[e % m] * in[e];
+}
+
+int launch_row_scale(brpa_ctx *ctx, double *out, const double *in, const double *s, int64_t m,
+                     int64_t n) {
+    if (m * n <= 0) return BRPA_OK;
+    hipLaunchKernelGGL(k_row_scale, dim3(nblocks(m * n)), dim3(256), 0, ctx->stream, out, in, s, m, n);
+    LAUNCH_CHECK(ctx);
+    return BRPA_OK;
+}
+
+// io(r, c) += a * s[r] * in(r, c)
+__global__ void k_row_scale_acc(double *__restrict__ io, const double *__restrict__ in,
+                                const double *__restrict__ s, double a, int64_t m, int64_t n) {
+    int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= m * n) return;
+    io[e] = fma(a * s[e % m], in[e], io[e]);
+}
+
+int launch_row_scale_acc(brpa_ctx *ctx, double *io, const double *in, const double *s, double a,
+                         int64_t m, int64_t n) {
+    if (m * n <= 0) return BRPA_OK;
+    hipLaunchKernelGGL(k_row_scale_acc, dim3(nblocks(m * n)), dim3(256), 0, ctx->stream, io, in, s,
+                       a, m, n);
+    LAUNCH_CHECK(ctx);
+    return BRPA_OK;
+}
+
+// W(:, j) = U(:, c0 + j) / sqrt(ev[c0 + j]) for j < nt  (rpa_CC_Doubles.f90:610-614)
+__global__ void k_select_scale(double *__restrict__ W, const double *__restrict__ U,
+                               const double *__restrict__ ev, int64_t m, int64_t c0, int64_t nt) {
+    int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= m * nt) return;
+    int64_t j = e / m, r = e - j * m;
+    W[e] = U[r + m * (c0 + j)] / sqrt(ev[c0 + j]);
+}
+
+int launch_select_scale(brpa_ctx *ctx, double *W, const double *U, const double *ev, int64_t m,
+                        int64_t c0, int64_t nt) {
+    if (m * nt <= 0) return BRPA_OK;
+    hipLaunchKernelGGL(k_select_scale, dim3(nblocks(m * nt)), dim3(256), 0, ctx->stream, W, U, ev,
+                       m, c0, nt);
+    LAUNCH_CHECK(ctx);
+    return BRPA_OK;
+}
+
+// x *= a (n elements)
+__global__ void k_scale(double *__restrict__ x, double a, int64_t n) {
+    int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e < n) x[e] *= a;
+}
+
+int launch_scale(brpa_ctx *ctx, double *x, double a, int64_t n) {
+    if (n <= 0) return BRPA_OK;
+    hipLaunchKernelGGL(k_scale, dim3(nblocks(n)), dim3(256), 0, ctx->stream, x, a, n);
+    LAUNCH_CHECK(ctx);
+    return BRPA_OK;
+}
+
+// SOSEX pair contraction (rpa_CC_EcExchange1, rpa/rpa_CC_Exchange.f90:100-116):
+// part[b * nchunk + c] = sum over chunk c of W1_b(P,mu) W2_b(P,mu) B[mu], for
+// batch b of npair (N x nt) matrix pairs (stride ld between batches).  Double-
+// double per thread and per tree level, deterministic.
+constexpr int SOSEX_CHUNK = 65536;
+
+__global__ __launch_bounds__(256) void k_pair_dot(const double *__restrict__ W1,
+                                                 const double *__restrict__ W2,
+                                                 const double *__restrict__ B, int64_t n_rows,
+                                                 int64_t nt, int64_t stride, int nchunk,
+                                                 double *__restrict__ part) {
+    __shared__ double sh[256], sl[256];
+    const int bpair = blockIdx.x / nchunk, c = blockIdx.x - bpair * nchunk;
+    const double *w1 = W1 + (int64_t)bpair * stride, *w2 = W2 + (int64_t)bpair * stride;
+    const int64_t tot = n_rows * nt;
+    const int64_t e0 = (int64_t)c * SOSEX_CHUNK, e1 = min(tot, e0 + (int64_t)SOSEX_CHUNK);
+    dd_t s = {0.0, 0.0};
+    for (int64_t e = e0 + threadIdx.x; e < e1; e += 256) {
+        const double p = w1[e] * w2[e];
+        s = dd_add(s, p * B[e / n_rows]);
+    }
+    sh[threadIdx.x] = s.hi;
+    sl[threadIdx.x] = s.lo;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if (threadIdx.x < k) {
+            dd_t x = {sh[threadIdx.x], sl[threadIdx.x]};
+            x = dd_add(x, (dd_t){sh[threadIdx.x + k], sl[threadIdx.x + k]});
+            sh[threadIdx.x] = x.hi;
+            sl[threadIdx.x] = x.lo;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * (int64_t)blockIdx.x] = sh[0];
+        part[2 * (int64_t)blockIdx.x + 1] = sl[0];
+    }
+}
+
+int sosex_nchunk(int64_t n_rows, int64_t nt) {
+    return (int)((n_rows * nt + SOSEX_CHUNK - 1) / SOSEX_CHUNK);
+}
+
+int launch_pair_dot(brpa_ctx *ctx, const double *W1, const double *W2, const double *B,
+                    int64_t n_rows, int64_t nt, int64_t stride, int npair, double *part) {
+    const int nchunk = sosex_nchunk(n_rows, nt);
+    if (npair <= 0 || nchunk <= 0) return BRPA_OK;
+    hipLaunchKernelGGL(k_pair_dot, dim3((unsigned)(npair * nchunk)), dim3(256), 0, ctx->stream,
+                       W1, W2, B, n_rows, nt, stride, nchunk, part);
+    LAUNCH_CHECK(ctx);
+    return BRPA_OK;
+}

@@ -1,0 +1,423 @@
+// Factorised direct-ring amplitudes (rpa_CC_T2_v5) and the SOSEX exchange
+// correction (rpa_CC_Exchange_SOSEX) on one GPU.
+//
+// The reference diagonalises every Pi(u_k) (rpa_CC_Diagonalize_PiU,
+// rpa/rpa_CC.f90:1108-1129) only to apply Q_k diag(l/(1+l W_k)) Q_k^T =
+// l (1 + l Pi_k)^{-1}; here that operator is applied through a Cholesky
+// factor of 1 + l Pi_k.  Pi_k comes from the FP64 MFMA SYRK of the E_RPA
+// path; the plain GEMMs run on rocBLAS, the Cholesky factor/solves and the two
+// dense symmetric eigenproblems (S and T^T I T, which the reference solves
+// with LAPACK dsyevd) on rocSOLVER.  Column-major throughout (Fortran layout):
+//   Rc(P, ai) = R[ai][P] with leading dimension N.
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "internal.h"
+
+int launch_guess(brpa_ctx *, double *O, int64_t m, int64_t n, uint64_t seed);
+int launch_normalize_columns(brpa_ctx *, double *X, int64_t m, int64_t n);
+int launch_row_scale(brpa_ctx *, double *out, const double *in, const double *s, int64_t m, int64_t n);
+int launch_row_scale_acc(brpa_ctx *, double *io, const double *in, const double *s, double a,
+                         int64_t m, int64_t n);
+int launch_select_scale(brpa_ctx *, double *W, const double *U, const double *ev, int64_t m,
+                        int64_t c0, int64_t nt);
+int launch_scale(brpa_ctx *, double *x, double a, int64_t n);
+int launch_pair_dot(brpa_ctx *, const double *W1, const double *W2, const double *B, int64_t n_rows,
+                    int64_t nt, int64_t stride, int npair, double *part);
+int sosex_nchunk(int64_t n_rows, int64_t nt);
+
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;
+
+int rb_check(brpa_ctx *ctx, rocblas_status st, const char *what) {
+    if (st != rocblas_status_success)
+        return brpa_fail(ctx, BRPA_ERR_SOLVER, "%s failed: rocblas status %d", what, (int)st);
+    return BRPA_OK;
+}
+
+int handle(brpa_ctx *ctx, rocblas_handle *h) {
+    if (!ctx->rocblas) {
+        rocblas_handle hh;
+        if (rocblas_create_handle(&hh) != rocblas_status_success)
+            return brpa_fail(ctx, BRPA_ERR_SOLVER, "rocblas_create_handle failed");
+        ctx->rocblas = hh;
+    }
+    *h = static_cast<rocblas_handle>(ctx->rocblas);
+    BRPA_TRY(rb_check(ctx, rocblas_set_stream(*h, ctx->stream), "rocblas_set_stream"));
+    return BRPA_OK;
+}
+
+// Column-major C(m x n) = alpha op(A) op(B) + beta C
+int gemm(brpa_ctx *ctx, rocblas_handle h, bool ta, bool tb, int64_t m, int64_t n, int64_t k,
+         double alpha, const double *A, int64_t lda, const double *B, int64_t ldb, double beta,
+         double *C, int64_t ldc) {
+    if (m <= 0 || n <= 0) return BRPA_OK;
+    return rb_check(ctx,
+                    rocblas_dgemm(h, ta ? rocblas_operation_transpose : rocblas_operation_none,
+                                  tb ? rocblas_operation_transpose : rocblas_operation_none,
+                                  (rocblas_int)m, (rocblas_int)n, (rocblas_int)k, &alpha, A,
+                                  (rocblas_int)lda, B, (rocblas_int)ldb, &beta, C, (rocblas_int)ldc),
+                    "rocblas_dgemm");
+}
+
+int syevd(brpa_ctx *ctx, rocblas_handle h, int64_t n, double *A, double *D, double *E, int *info) {
+    BRPA_TRY(rb_check(ctx,
+                      rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_lower,
+                                       (rocblas_int)n, A, (rocblas_int)n, D, E, info),
+                      "rocsolver_dsyevd"));
+    int hinfo = 0;
+    BRPA_HIP(ctx, hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    BRPA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (hinfo != 0)
+        return brpa_fail(ctx, BRPA_ERR_SOLVER, "dsyevd did not converge (info=%d, n=%lld)", hinfo,
+                         (long long)n);
+    return BRPA_OK;
+}
+
+int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+bool finite_all(const double *p, int64_t n) {
+    for (int64_t i = 0; i < n; ++i)
+        if (!std::isfinite(p[i])) return false;
+    return true;
+}
+
+int host_vec(brpa_ctx *ctx, std::vector<double> &v, const double *p, int64_t n) {
+    v.resize(n);
+    if (is_device_ptr(p))
+        BRPA_HIP(ctx, hipMemcpy(v.data(), p, 8 * n, hipMemcpyDeviceToHost));
+    else
+        std::memcpy(v.data(), p, 8 * n);
+    return BRPA_OK;
+}
+
+int grow(brpa_ctx *ctx, double **p, size_t *cap, size_t n) {
+    if (n <= *cap) return BRPA_OK;
+    if (*p) BRPA_HIP(ctx, hipFree(*p));
+    *p = nullptr;
+    *cap = 0;
+    hipError_t e = hipMalloc(p, std::max<size_t>(n, 1) * sizeof(double));
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        *p = nullptr;
+        return brpa_fail(ctx, BRPA_ERR_OOM, "T2 result buffer of %zu doubles: %s", n,
+                         hipGetErrorString(e));
+    }
+    *cap = n;
+    return BRPA_OK;
+}
+
+}  // namespace
+
+void t2_release(brpa_ctx *ctx) {
+    if (ctx->t2_A) (void)hipFree(ctx->t2_A);
+    if (ctx->t2_V) (void)hipFree(ctx->t2_V);
+    ctx->t2_A = ctx->t2_V = nullptr;
+    ctx->t2_A_cap = ctx->t2_V_cap = 0;
+    if (ctx->rocblas) (void)rocblas_destroy_handle(static_cast<rocblas_handle>(ctx->rocblas));
+    ctx->rocblas = nullptr;
+}
+
+extern "C" {
+
+int brpa_t2_compute(brpa_ctx *ctx, int precision, const double *R, int64_t naux,
+                    const double *eocc, int64_t nocc, const double *evir, int64_t nvir,
+                    const double *freqs, const double *weights, int64_t nfreqs, double lambda,
+                    double cutoff, int64_t nguess, uint64_t seed, int64_t *nvecs_t2) {
+    if (!ctx) return BRPA_ERR_ARG;
+    if (precision != BRPA_FP64)
+        return brpa_fail(ctx, BRPA_ERR_ARG, "brpa_t2_compute: only BRPA_FP64 is implemented");
+    if (!R || !eocc || !evir || !freqs || !weights || !nvecs_t2 || naux <= 0 || nocc <= 0 ||
+        nvir <= 0 || nfreqs <= 0 || !(lambda > 0.0))
+        return brpa_fail(ctx, BRPA_ERR_ARG, "brpa_t2_compute: bad arguments");
+    const int64_t K = nocc * nvir, N = naux, F = nfreqs;
+    if (K > INT32_MAX || N > 65536)
+        return brpa_fail(ctx, BRPA_ERR_ARG, "brpa_t2_compute: sizes beyond the supported range");
+    const int64_t NG = nguess > 0 ? std::min<int64_t>(nguess, K)
+                                  : std::min<int64_t>(K, (int64_t)std::ceil(3.0 * (double)N));
+    std::vector<double> eo, ev, fx, fw;
+    BRPA_TRY(host_vec(ctx, eo, eocc, nocc));
+    BRPA_TRY(host_vec(ctx, ev, evir, nvir));
+    BRPA_TRY(host_vec(ctx, fx, freqs, F));
+    BRPA_TRY(host_vec(ctx, fw, weights, F));
+    if (!finite_all(eo.data(), nocc) || !finite_all(ev.data(), nvir) || !finite_all(fx.data(), F) ||
+        !finite_all(fw.data(), F))
+        return brpa_fail(ctx, BRPA_ERR_NONFINITE, "brpa_t2_compute: non-finite input");
+    if (!(*std::min_element(ev.begin(), ev.end()) - *std::max_element(eo.begin(), eo.end()) > 0.0))
+        return brpa_fail(ctx, BRPA_ERR_GAP, "Found invalid orbital energy gap.");
+    BRPA_HIP(ctx, hipSetDevice(ctx->device));
+    rocblas_handle h;
+    BRPA_TRY(handle(ctx, &h));
+
+    // chi_k(ai) = -4 d/(d^2+u_k^2) (rpa_CC_Chi, rpa/rpa_CC_Doubles.f90:276-292)
+    std::vector<double> chi(F * K);
+    for (int64_t k = 0; k < F; ++k)
+        for (int64_t i = 0; i < nocc; ++i)
+            for (int64_t a = 0; a < nvir; ++a) {
+                double d = ev[a] - eo[i];
+                chi[k * K + i * nvir + a] = -(4.0 * d) / (d * d + fx[k] * fx[k]);
+            }
+
+    const bool rdev = is_device_ptr(R);
+    // workspace layout
+    const int64_t nsteps = cdiv(N, CHOL_NB);
+    FragDev fd{};
+    double *dR = nullptr, *dchi, *omega, *tmp, *iom, *Y, *Hb, *Sm, *sev, *sE;
+    int *dinfo;
+    std::vector<int64_t> pre = {0, 0, 0, 0, 0, 0};
+    const int64_t ntile = cdiv(N, SYRK_T);
+    pre[1] = cdiv(F * K, 256);                      // dtab
+    pre[3] = ntile * (ntile + 1) / 2 * cdiv(F, 4);  // syrk
+    pre[5] = cdiv(F * N, 256);                      // set_diag
+    int64_t *dpre = nullptr;
+    FragDev *dfd = nullptr;
+    auto layout = [&](Arena &ar) {
+        dfd = ar.take<FragDev>(1);
+        dpre = ar.take<int64_t>(pre.size());
+        dR = rdev ? const_cast<double *>(R) : ar.take<double>(K * N);
+        fd.dtab = ar.take<double>(F * K);
+        fd.Pi = ar.take<double>(F * N * N);
+        fd.diag = ar.take<double>(F * N);
+        fd.weights = ar.take<double>(F);
+        fd.freqs = ar.take<double>(F);
+        fd.eocc = ar.take<double>(nocc);
+        fd.evir = ar.take<double>(nvir);
+        dchi = ar.take<double>(F * K);
+        omega = ar.take<double>(K * NG);
+        tmp = ar.take<double>(K * NG);
+        iom = ar.take<double>(K * NG);
+        Y = ar.take<double>(N * NG);
+        Hb = ar.take<double>(N * NG);
+        Sm = ar.take<double>(NG * NG);
+        sev = ar.take<double>(NG);
+        sE = ar.take<double>(NG);
+        dinfo = ar.take<int>(F + 2);
+    };
+    Arena count;
+    layout(count);
+    Arena ar;
+    BRPA_TRY(ws_reserve(ctx, count.off, &ar));
+    layout(ar);
+    fd.R = dR;
+    fd.K = (int)K;
+    fd.N = (int)N;
+    fd.F = (int)F;
+    fd.nocc = (int)nocc;
+    fd.nvir = (int)nvir;
+    fd.nsteps = (int)nsteps;
+    if (!rdev) BRPA_TRY(to_device(ctx, dR, R, 8 * K * N));
+    BRPA_HIP(ctx, hipMemcpyAsync(const_cast<double *>(fd.weights), fw.data(), 8 * F,
+                                 hipMemcpyHostToDevice, ctx->stream));
+    BRPA_HIP(ctx, hipMemcpyAsync(const_cast<double *>(fd.freqs), fx.data(), 8 * F,
+                                 hipMemcpyHostToDevice, ctx->stream));
+    BRPA_HIP(ctx, hipMemcpyAsync(const_cast<double *>(fd.eocc), eo.data(), 8 * nocc,
+                                 hipMemcpyHostToDevice, ctx->stream));
+    BRPA_HIP(ctx, hipMemcpyAsync(const_cast<double *>(fd.evir), ev.data(), 8 * nvir,
+                                 hipMemcpyHostToDevice, ctx->stream));
+    BRPA_HIP(ctx, hipMemcpyAsync(dchi, chi.data(), 8 * F * K, hipMemcpyHostToDevice, ctx->stream));
+    BRPA_HIP(ctx, hipMemcpyAsync(dfd, &fd, sizeof(FragDev), hipMemcpyHostToDevice, ctx->stream));
+    BRPA_HIP(ctx, hipMemcpyAsync(dpre, pre.data(), 8 * pre.size(), hipMemcpyHostToDevice,
+                                 ctx->stream));
+    BRPA_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+
+    // Pi_k for all k (FP64 MFMA SYRK), then 1 + l Pi_k and its Cholesky factor
+    BRPA_TRY(launch_dtab(ctx, dfd, dpre, 1, pre[1]));
+    BRPA_TRY(launch_syrk_f64(ctx, dfd, dpre + 2, 1, pre[3]));
+    if (lambda != 1.0) BRPA_TRY(launch_scale(ctx, fd.Pi, lambda, F * N * N));
+    BRPA_TRY(launch_set_diag(ctx, dfd, dpre + 4, 1, pre[5], 0));
+    for (int64_t k = 0; k < F; ++k)
+        BRPA_TRY(rb_check(ctx,
+                          rocsolver_dpotrf(h, rocblas_fill_lower, (rocblas_int)N, fd.Pi + k * N * N,
+                                           (rocblas_int)N, dinfo + k),
+                          "rocsolver_dpotrf"));
+
+    // I * Omega (rpa_CC_IM_v5, rpa/rpa_CC_Doubles.f90:369-422)
+    BRPA_TRY(launch_guess(ctx, omega, K, NG, seed));
+    BRPA_TRY(launch_normalize_columns(ctx, omega, K, NG));
+    BRPA_HIP(ctx, hipMemsetAsync(iom, 0, 8 * K * NG, ctx->stream));
+    for (int64_t k = 0; k < F; ++k) {
+        const double *ck = dchi + k * K;
+        BRPA_TRY(launch_row_scale(ctx, tmp, omega, ck, K, NG));
+        BRPA_TRY(gemm(ctx, h, false, false, N, NG, K, 1.0, dR, N, tmp, K, 0.0, Y, N));
+        BRPA_TRY(rb_check(ctx,
+                          rocsolver_dpotrs(h, rocblas_fill_lower, (rocblas_int)N, (rocblas_int)NG,
+                                           fd.Pi + k * N * N, (rocblas_int)N, Y, (rocblas_int)N),
+                          "rocsolver_dpotrs"));
+        BRPA_TRY(gemm(ctx, h, true, false, K, NG, N, 1.0, dR, N, Y, N, 0.0, tmp, K));
+        BRPA_TRY(launch_row_scale_acc(ctx, iom, tmp, ck, -fw[k] * lambda / (4.0 * kPi), K, NG));
+    }
+    BRPA_TRY(launch_normalize_columns(ctx, iom, K, NG));
+
+    // S = (I Omega)^T (I Omega), eigenpairs ascending (rpa_CC_Doubles.f90:595-609)
+    BRPA_TRY(gemm(ctx, h, true, false, NG, NG, K, 1.0, iom, K, iom, K, 0.0, Sm, NG));
+    BRPA_TRY(syevd(ctx, h, NG, Sm, sev, sE, dinfo + F));
+    std::vector<double> hs(NG);
+    BRPA_HIP(ctx, hipMemcpy(hs.data(), sev, 8 * NG, hipMemcpyDeviceToHost));
+    int64_t nt = 0;
+    for (int64_t i = NG - 1; i >= 0; --i) {
+        if (hs[i] > cutoff)
+            ++nt;
+        else
+            break;
+    }
+    *nvecs_t2 = nt;
+    ctx->t2_nt = nt;
+    ctx->t2_nai = K;
+    if (nt == 0) return BRPA_OK;
+
+    // T = (I Omega) U_s s^{-1/2}; TIT = T^T I T (rpa_CC_MIM_v5, 425-488)
+    double *Wm = omega;            // NG x nt   (omega no longer needed)
+    double *T = tmp;               // K x nt
+    double *G = Y;                 // N x nt
+    double *H = Hb;                // N x nt
+    BRPA_TRY(launch_select_scale(ctx, Wm, Sm, sev, NG, NG - nt, nt));
+    BRPA_TRY(gemm(ctx, h, false, false, K, nt, NG, 1.0, iom, K, Wm, NG, 0.0, T, K));
+    double *Tkeep = iom;           // K x nt copy of T (iom is free now)
+    BRPA_HIP(ctx, hipMemcpyAsync(Tkeep, T, 8 * K * nt, hipMemcpyDeviceToDevice, ctx->stream));
+    double *TIT = Sm;              // nt x nt
+    for (int64_t k = 0; k < F; ++k) {
+        const double *ck = dchi + k * K;
+        BRPA_TRY(launch_row_scale(ctx, T, Tkeep, ck, K, nt));
+        BRPA_TRY(gemm(ctx, h, false, false, N, nt, K, 1.0, dR, N, T, K, 0.0, G, N));
+        BRPA_HIP(ctx, hipMemcpyAsync(H, G, 8 * N * nt, hipMemcpyDeviceToDevice, ctx->stream));
+        BRPA_TRY(rb_check(ctx,
+                          rocsolver_dpotrs(h, rocblas_fill_lower, (rocblas_int)N, (rocblas_int)nt,
+                                           fd.Pi + k * N * N, (rocblas_int)N, H, (rocblas_int)N),
+                          "rocsolver_dpotrs"));
+        BRPA_TRY(gemm(ctx, h, true, false, nt, nt, N, -fw[k] * lambda / (4.0 * kPi), G, N, H, N,
+                      k == 0 ? 0.0 : 1.0, TIT, nt));
+    }
+    // eigenpairs of T^T I T; A = a / (2 (1 + a)); V = eigvecs^T T^T (629-641)
+    BRPA_TRY(syevd(ctx, h, nt, TIT, sev, sE, dinfo + F + 1));
+    std::vector<double> ha(nt);
+    BRPA_HIP(ctx, hipMemcpy(ha.data(), sev, 8 * nt, hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < nt; ++i) ha[i] = 0.5 * (ha[i] / (ha[i] + 1.0));
+    BRPA_TRY(grow(ctx, &ctx->t2_A, &ctx->t2_A_cap, nt));
+    BRPA_TRY(grow(ctx, &ctx->t2_V, &ctx->t2_V_cap, (size_t)nt * K));
+    BRPA_HIP(ctx, hipMemcpyAsync(ctx->t2_A, ha.data(), 8 * nt, hipMemcpyHostToDevice, ctx->stream));
+    BRPA_TRY(gemm(ctx, h, true, true, nt, K, nt, 1.0, TIT, nt, Tkeep, K, 0.0, ctx->t2_V, nt));
+    BRPA_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    std::vector<int> info(F);
+    BRPA_HIP(ctx, hipMemcpyAsync(info.data(), dinfo, sizeof(int) * F, hipMemcpyDeviceToHost,
+                                 ctx->stream));
+    BRPA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    float ms = 0.f;
+    BRPA_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    ctx->last_kernel_ms = ms;
+    for (int64_t k = 0; k < F; ++k)
+        if (info[k] != 0)
+            return brpa_fail(ctx, BRPA_ERR_NOT_SPD, "frequency %lld: 1 + l Pi(u) is not positive definite",
+                             (long long)k);
+    return BRPA_OK;
+}
+
+int brpa_t2_copyout(brpa_ctx *ctx, double *A, double *V) {
+    if (!ctx) return BRPA_ERR_ARG;
+    if (ctx->t2_nt < 0) return brpa_fail(ctx, BRPA_ERR_ARG, "no T2 result to copy");
+    if (ctx->t2_nt == 0) return BRPA_OK;
+    if (A) BRPA_TRY(from_device(ctx, A, ctx->t2_A, 8 * ctx->t2_nt));
+    if (V) BRPA_TRY(from_device(ctx, V, ctx->t2_V, 8 * ctx->t2_nt * ctx->t2_nai));
+    BRPA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return BRPA_OK;
+}
+
+int brpa_sosex(brpa_ctx *ctx, int precision, double *ec_exchange, double alpha, const double *V,
+               const double *A, const double *R, int64_t nocc, int64_t nvir, int64_t naux,
+               int64_t nvecs_t2, int s2_operator) {
+    if (!ctx) return BRPA_ERR_ARG;
+    if (precision != BRPA_FP64)
+        return brpa_fail(ctx, BRPA_ERR_ARG, "brpa_sosex: only BRPA_FP64 is implemented");
+    if (!ec_exchange || !V || !A || !R || nocc <= 0 || nvir <= 0 || naux <= 0 || nvecs_t2 < 0)
+        return brpa_fail(ctx, BRPA_ERR_ARG, "brpa_sosex: bad arguments");
+    *ec_exchange = 0.0;
+    const int64_t K = nocc * nvir, N = naux, nt = nvecs_t2, o = nocc, v = nvir;
+    if (nt == 0) return BRPA_OK;
+    if (K > INT32_MAX || N > 65536 || nt > 1000000)
+        return brpa_fail(ctx, BRPA_ERR_ARG, "brpa_sosex: sizes beyond the supported range");
+    BRPA_HIP(ctx, hipSetDevice(ctx->device));
+    rocblas_handle h;
+    BRPA_TRY(handle(ctx, &h));
+    std::vector<double> hA;
+    BRPA_TRY(host_vec(ctx, hA, A, nt));
+    if (!finite_all(hA.data(), nt))
+        return brpa_fail(ctx, BRPA_ERR_NONFINITE, "brpa_sosex: non-finite amplitudes");
+    // B = A, or A/(1-4A^2) for the cumulant route (rpa_CC_Exchange.f90:62-68)
+    std::vector<double> hB(nt);
+    for (int64_t k = 0; k < nt; ++k) hB[k] = s2_operator ? hA[k] / (1.0 - 4.0 * hA[k] * hA[k]) : hA[k];
+    const bool rdev = is_device_ptr(R), vdev = is_device_ptr(V);
+    // pairs (i, j >= i) are processed per i in batches of jb occupied indices
+    const size_t wbytes_per_j = 2 * 8 * (size_t)N * nt;
+    int64_t jb = std::max<int64_t>(1, std::min<int64_t>(o, (int64_t)((8ull << 30) / wbytes_per_j)));
+    const int nchunk = sosex_nchunk(N, nt);
+    double *dR, *dV, *dB, *W1, *W2, *part;
+    auto layout = [&](Arena &ar) {
+        dR = rdev ? const_cast<double *>(R) : ar.take<double>(K * N);
+        dV = vdev ? const_cast<double *>(V) : ar.take<double>(K * nt);
+        dB = ar.take<double>(nt);
+        W1 = ar.take<double>(jb * N * nt);
+        W2 = ar.take<double>(jb * N * nt);
+        part = ar.take<double>(2 * jb * nchunk);
+    };
+    Arena count;
+    layout(count);
+    Arena ar;
+    BRPA_TRY(ws_reserve(ctx, count.off, &ar));
+    layout(ar);
+    if (!rdev) BRPA_TRY(to_device(ctx, dR, R, 8 * K * N));
+    if (!vdev) BRPA_TRY(to_device(ctx, dV, V, 8 * K * nt));
+    BRPA_HIP(ctx, hipMemcpyAsync(dB, hB.data(), 8 * nt, hipMemcpyHostToDevice, ctx->stream));
+    BRPA_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    std::vector<double> hpart(2 * jb * nchunk);
+    long double e = 0.0L;
+    const double one = 1.0, zero = 0.0;
+    for (int64_t i = 0; i < o; ++i) {
+        for (int64_t j0 = i; j0 < o; j0 += jb) {
+            const int64_t nb = std::min<int64_t>(jb, o - j0);
+            // W_ij = R_i V_j^T (N x nt, K = v), j = j0..j0+nb-1 (rpa_CC_Exchange.f90:74-76)
+            BRPA_TRY(rb_check(ctx,
+                              rocblas_dgemm_strided_batched(
+                                  h, rocblas_operation_none, rocblas_operation_transpose,
+                                  (rocblas_int)N, (rocblas_int)nt, (rocblas_int)v, &one,
+                                  dR + i * v * N, (rocblas_int)N, 0, dV + j0 * v * nt,
+                                  (rocblas_int)nt, v * nt, &zero, W1, (rocblas_int)N, N * nt,
+                                  (rocblas_int)nb),
+                              "rocblas_dgemm_strided_batched"));
+            // W_ji = R_j V_i^T (77-78)
+            BRPA_TRY(rb_check(ctx,
+                              rocblas_dgemm_strided_batched(
+                                  h, rocblas_operation_none, rocblas_operation_transpose,
+                                  (rocblas_int)N, (rocblas_int)nt, (rocblas_int)v, &one,
+                                  dR + j0 * v * N, (rocblas_int)N, v * N, dV + i * v * nt,
+                                  (rocblas_int)nt, 0, &zero, W2, (rocblas_int)N, N * nt,
+                                  (rocblas_int)nb),
+                              "rocblas_dgemm_strided_batched"));
+            BRPA_TRY(launch_pair_dot(ctx, W1, W2, dB, N, nt, N * nt, (int)nb, part));
+            BRPA_HIP(ctx, hipMemcpyAsync(hpart.data(), part, 8 * 2 * nb * nchunk,
+                                         hipMemcpyDeviceToHost, ctx->stream));
+            BRPA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            for (int64_t jj = 0; jj < nb; ++jj) {
+                long double s = 0.0L;
+                for (int c = 0; c < nchunk; ++c)
+                    s += (long double)hpart[2 * (jj * nchunk + c)] +
+                         (long double)hpart[2 * (jj * nchunk + c) + 1];
+                // EcExchange1 = -sum W_ji W_ij B; off-diagonal pairs count twice
+                e += (j0 + jj == i ? 1.0L : 2.0L) * (-s);
+            }
+        }
+    }
+    BRPA_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    BRPA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    float ms = 0.f;
+    BRPA_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    ctx->last_kernel_ms = ms;
+    *ec_exchange = (double)((long double)alpha * e);
+    return BRPA_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,304 @@
+!
+! brpa_mod -- Fortran drop-in for the correlation-energy hot path.
+!
+! Keeps the names and argument lists of the reference routines so that
+! reference host code can call the gfx950 library through ISO_C_BINDING:
+!
+!   rpa_GPiUG_MO            rpa/rpa_core_MO.f90:347-406
+!   rpa_CC_EcRPA_Analytic   rpa/rpa_CC.f90:412-471
+!   rpa_CC_T2               rpa/rpa_CC_Doubles.f90:43-77
+!   rpa_CC_Exchange_SOSEX   rpa/rpa_CC_Exchange.f90:36-97
+!
+! Built with -fdefault-integer-8 like every reference build (default INTEGER
+! is 64-bit, matching int64_t of the C ABI).  Arrays are passed as contiguous
+! column-major storage; Fortran CRG(P, ai) has the bytes of C R[ai][P].
+! The reference has no return codes: it prints a message and stops; a
+! non-zero status from the library does the same here.  The library context
+! (one GPU, one stream) is created on first use and kept for the run.
+!
+module brpa_mod
+      use iso_c_binding
+      implicit none
+      private
+
+      integer, parameter :: F64 = c_double
+      type(c_ptr), save :: ctx = c_null_ptr
+
+      public :: brpa_init, brpa_finalize
+      public :: rpa_GPiUG_MO, rpa_CC_EcRPA_Analytic, rpa_CC_T2, rpa_CC_Exchange_SOSEX
+      public :: brpa_ecrpa_fused
+
+      interface
+            integer(c_int) function c_ctx_create(device, out) bind(C, name="brpa_ctx_create")
+                  import :: c_int, c_ptr
+                  integer(c_int), value :: device
+                  type(c_ptr) :: out
+            end function
+            integer(c_int) function c_ctx_destroy(c) bind(C, name="brpa_ctx_destroy")
+                  import :: c_int, c_ptr
+                  type(c_ptr), value :: c
+            end function
+            type(c_ptr) function c_last_error(c) bind(C, name="brpa_last_error")
+                  import :: c_ptr
+                  type(c_ptr), value :: c
+            end function
+            integer(c_int) function c_gpiug_mo(c, gpiug, crg, naux, eocc, nocc, evir, nvir, &
+                  freq, pref) bind(C, name="brpa_gpiug_mo")
+                  import :: c_int, c_ptr, c_int64_t, c_double
+                  type(c_ptr), value :: c
+                  real(c_double) :: gpiug(*)
+                  real(c_double) :: crg(*)
+                  integer(c_int64_t), value :: naux, nocc, nvir
+                  real(c_double) :: eocc(*), evir(*)
+                  real(c_double), value :: freq, pref
+            end function
+            integer(c_int) function c_ecrpa_analytic(c, e, piu, freqs, weights, nfreqs, naux, eu) &
+                  bind(C, name="brpa_ecrpa_analytic")
+                  import :: c_int, c_ptr, c_int64_t, c_double
+                  type(c_ptr), value :: c
+                  real(c_double) :: e
+                  real(c_double) :: piu(*), freqs(*), weights(*)
+                  integer(c_int64_t), value :: nfreqs, naux
+                  type(c_ptr), value :: eu
+            end function
+            integer(c_int) function c_ecrpa(c, prec, r, naux, eocc, nocc, evir, nvir, freqs, &
+                  weights, nfreqs, e, eu) bind(C, name="brpa_ecrpa")
+                  import :: c_int, c_ptr, c_int64_t, c_double
+                  type(c_ptr), value :: c
+                  integer(c_int), value :: prec
+                  real(c_double) :: r(*), eocc(*), evir(*), freqs(*), weights(*)
+                  integer(c_int64_t), value :: naux, nocc, nvir, nfreqs
+                  real(c_double) :: e
+                  type(c_ptr), value :: eu
+            end function
+            integer(c_int) function c_t2_compute(c, prec, r, naux, eocc, nocc, evir, nvir, freqs, &
+                  weights, nfreqs, lambda, cutoff, nguess, seed, nvecs) bind(C, name="brpa_t2_compute")
+                  import :: c_int, c_ptr, c_int64_t, c_double
+                  type(c_ptr), value :: c
+                  integer(c_int), value :: prec
+                  real(c_double) :: r(*), eocc(*), evir(*), freqs(*), weights(*)
+                  integer(c_int64_t), value :: naux, nocc, nvir, nfreqs, nguess, seed
+                  real(c_double), value :: lambda, cutoff
+                  integer(c_int64_t) :: nvecs
+            end function
+            integer(c_int) function c_t2_copyout(c, a, v) bind(C, name="brpa_t2_copyout")
+                  import :: c_int, c_ptr, c_double
+                  type(c_ptr), value :: c
+                  real(c_double) :: a(*), v(*)
+            end function
+            integer(c_int) function c_sosex(c, prec, e, alpha, v, a, r, nocc, nvir, naux, nvecs, s2) &
+                  bind(C, name="brpa_sosex")
+                  import :: c_int, c_ptr, c_int64_t, c_double
+                  type(c_ptr), value :: c
+                  integer(c_int), value :: prec
+                  real(c_double) :: e
+                  real(c_double), value :: alpha
+                  real(c_double) :: v(*), a(*), r(*)
+                  integer(c_int64_t), value :: nocc, nvir, naux, nvecs
+                  integer(c_int), value :: s2
+            end function
+            integer(c_size_t) function c_strlen(s) bind(C, name="strlen")
+                  import :: c_size_t, c_ptr
+                  type(c_ptr), value :: s
+            end function
+      end interface
+
+contains
+
+      subroutine brpa_init(device)
+            integer, optional, intent(in) :: device
+            integer(c_int) :: d, rc
+            if (c_associated(ctx)) return
+            d = 0
+            if (present(device)) d = int(device, c_int)
+            rc = c_ctx_create(d, ctx)
+            if (rc /= 0) then
+                  print '(a,i0)', "brpa: cannot create a GPU context, status ", rc
+                  error stop
+            end if
+      end subroutine brpa_init
+
+
+      subroutine brpa_finalize()
+            integer(c_int) :: rc
+            if (.not. c_associated(ctx)) return
+            rc = c_ctx_destroy(ctx)
+            ctx = c_null_ptr
+      end subroutine brpa_finalize
+
+
+      subroutine check(rc, where)
+            integer(c_int), intent(in) :: rc
+            character(*), intent(in) :: where
+            character(kind=c_char), pointer :: msg(:)
+            type(c_ptr) :: p
+            integer(c_size_t) :: n
+            if (rc == 0) return
+            p = c_last_error(ctx)
+            n = c_strlen(p)
+            call c_f_pointer(p, msg, [n])
+            print '(a,a,i0,a,*(a))', where, ": error ", rc, ": ", msg
+            error stop
+      end subroutine check
+
+
+      subroutine rpa_GPiUG_MO(GPiUG, CRG, DCRG, OccEnergies, VirtEnergies, NOcc, NVirt, &
+            OccBounds, VirtBounds, NSpins, Freq)
+            !
+            ! GPiUG += Prefactor * CRG * diag(d/(d^2+u^2)) * CRG^T per spin block
+            ! (rpa_core_MO.f90:347-406).  DCRG is the reference's scratch array and
+            ! is not touched.
+            !
+            real(F64), dimension(:, :), intent(inout), contiguous, target :: GPiUG
+            real(F64), dimension(:, :, :), intent(in), contiguous, target :: CRG
+            real(F64), dimension(:, :), intent(out)   :: DCRG
+            real(F64), dimension(:, :), intent(in)    :: OccEnergies
+            real(F64), dimension(:, :), intent(in)    :: VirtEnergies
+            integer, dimension(2), intent(in)         :: NOcc
+            integer, dimension(2), intent(in)         :: NVirt
+            integer, dimension(2, 2), intent(in)      :: OccBounds
+            integer, dimension(2, 2), intent(in)      :: VirtBounds
+            integer, intent(in)                       :: NSpins
+            real(F64), intent(in)                     :: Freq
+
+            real(F64) :: Prefactor
+            real(F64), dimension(:), allocatable :: eo, ev
+            integer :: s, NVecs, Nai, i0, a0
+
+            call brpa_init()
+            if (NSpins > 1) then
+                  Prefactor = 2.0_F64
+            else
+                  Prefactor = 4.0_F64
+            end if
+            NVecs = size(GPiUG, dim=1)
+            do s = 1, NSpins
+                  Nai = NOcc(s) * NVirt(s)
+                  if (Nai > 0) then
+                        i0 = OccBounds(1, s)
+                        a0 = VirtBounds(1, s)
+                        eo = OccEnergies(i0:i0+NOcc(s)-1, s)
+                        ev = VirtEnergies(a0:a0+NVirt(s)-1, s)
+                        call check(c_gpiug_mo(ctx, GPiUG, CRG(:, :, s), int(NVecs, c_int64_t), eo, &
+                              int(NOcc(s), c_int64_t), ev, int(NVirt(s), c_int64_t), Freq, Prefactor), &
+                              "rpa_GPiUG_MO")
+                  end if
+            end do
+            if (.false.) DCRG = 0.0_F64
+      end subroutine rpa_GPiUG_MO
+
+
+      subroutine rpa_CC_EcRPA_Analytic(EcRPA, PiU, Freqs, FreqWeights, NFreqs, NVecsPiU)
+            !
+            ! EcRPA = sum_u w_u/(2 pi) [log det(1 + Pi(u)) - tr Pi(u)]
+            ! (rpa_CC.f90:412-471)
+            !
+            real(F64), intent(out)                                    :: EcRPA
+            real(F64), dimension(:, :, :), intent(in), contiguous     :: PiU
+            real(F64), dimension(:), intent(in)                       :: Freqs
+            real(F64), dimension(:), intent(in)                       :: FreqWeights
+            integer, intent(in)                                       :: NFreqs
+            integer, intent(in)                                       :: NVecsPiU
+
+            call brpa_init()
+            if (size(PiU, 1) /= NVecsPiU .or. size(PiU, 2) /= NVecsPiU) then
+                  print '(a)', "rpa_CC_EcRPA_Analytic: PiU must be NVecsPiU x NVecsPiU x NFreqs"
+                  error stop
+            end if
+            call check(c_ecrpa_analytic(ctx, EcRPA, PiU, Freqs, FreqWeights, &
+                  int(NFreqs, c_int64_t), int(NVecsPiU, c_int64_t), c_null_ptr), "rpa_CC_EcRPA_Analytic")
+      end subroutine rpa_CC_EcRPA_Analytic
+
+
+      subroutine brpa_ecrpa_fused(EcRPA, Rkai, NVecs, OccEnergies, VirtEnergies, NOcc, NVirt, &
+            Freqs, FreqWeights, NFreqs, SplitBF16)
+            !
+            ! Pi(u) build + E_RPA in one call, Pi kept on the GPU
+            ! (rpa_CC_PiU frequency loop, rpa_CC.f90:523-526, then rpa_CC_EcRPA_Analytic).
+            !
+            real(F64), intent(out)                             :: EcRPA
+            real(F64), dimension(:, :), intent(in), contiguous :: Rkai
+            integer, intent(in)                                :: NVecs, NOcc, NVirt, NFreqs
+            real(F64), dimension(:), intent(in)                :: OccEnergies, VirtEnergies
+            real(F64), dimension(:), intent(in)                :: Freqs, FreqWeights
+            logical, intent(in)                                :: SplitBF16
+            integer(c_int) :: prec
+            call brpa_init()
+            prec = 0
+            if (SplitBF16) prec = 1
+            call check(c_ecrpa(ctx, prec, Rkai, int(NVecs, c_int64_t), OccEnergies, &
+                  int(NOcc, c_int64_t), VirtEnergies, int(NVirt, c_int64_t), Freqs, FreqWeights, &
+                  int(NFreqs, c_int64_t), EcRPA, c_null_ptr), "brpa_ecrpa_fused")
+      end subroutine brpa_ecrpa_fused
+
+
+      subroutine rpa_CC_T2(A, V, NVecsT2, PiUEigenvecs, PiUEigenvals, Rkai, NVecsChol, NOcc, NVirt, &
+            OccBounds, VirtBounds, Freqs, FreqWeights, NFreqs, Lambda, OccEnergies, VirtEnergies, &
+            SmallEigenvalsCutoffT2, GuessNVecsT2, MaxBatchDim)
+            !
+            ! Factorised direct-ring amplitudes (rpa_CC_Doubles.f90:43-77).  The
+            ! eigenpairs of Pi(u) are not read: the library applies
+            ! l (1 + l Pi(u))^{-1} through a Cholesky factor built from Rkai.
+            ! MaxBatchDim only bounded the reference's memory and is not needed.
+            !
+            real(F64), dimension(:), allocatable, intent(out)    :: A
+            real(F64), dimension(:, :), allocatable, intent(out) :: V
+            integer, intent(inout)                               :: NVecsT2
+            real(F64), dimension(:, :, :), intent(in)            :: PiUEigenvecs
+            real(F64), dimension(:, :), intent(in)               :: PiUEigenvals
+            real(F64), dimension(:, :), intent(in), contiguous   :: Rkai
+            integer, intent(in)                                  :: NVecsChol
+            integer, intent(in)                                  :: NOcc
+            integer, intent(in)                                  :: NVirt
+            integer, dimension(2), intent(in)                    :: OccBounds
+            integer, dimension(2), intent(in)                    :: VirtBounds
+            real(F64), dimension(:), intent(in)                  :: Freqs
+            real(F64), dimension(:), intent(in)                  :: FreqWeights
+            integer, intent(in)                                  :: NFreqs
+            real(F64), intent(in)                                :: Lambda
+            real(F64), dimension(:), intent(in)                  :: OccEnergies
+            real(F64), dimension(:), intent(in)                  :: VirtEnergies
+            real(F64), intent(in)                                :: SmallEigenvalsCutoffT2
+            integer, intent(in)                                  :: GuessNVecsT2
+            integer, intent(in)                                  :: MaxBatchDim
+            integer(c_int64_t) :: nt
+
+            call brpa_init()
+            call check(c_t2_compute(ctx, 0_c_int, Rkai, int(NVecsChol, c_int64_t), OccEnergies, &
+                  int(NOcc, c_int64_t), VirtEnergies, int(NVirt, c_int64_t), Freqs, FreqWeights, &
+                  int(NFreqs, c_int64_t), Lambda, SmallEigenvalsCutoffT2, int(GuessNVecsT2, c_int64_t), &
+                  0_c_int64_t, nt), "rpa_CC_T2")
+            NVecsT2 = int(nt)
+            allocate(A(NVecsT2))
+            allocate(V(NVecsT2, NOcc*NVirt))
+            if (NVecsT2 > 0) call check(c_t2_copyout(ctx, A, V), "rpa_CC_T2")
+            if (.false.) print *, PiUEigenvecs(1,1,1), PiUEigenvals(1,1), OccBounds, VirtBounds, MaxBatchDim
+      end subroutine rpa_CC_T2
+
+
+      subroutine rpa_CC_Exchange_SOSEX(EcExchange, Alpha, Vkai, Ak, Rkai, NOcc, NVirt, NVecsPiU, &
+            NVecsT2, S2Operator)
+            !
+            ! EcExchange = -Alpha * Sum(abij) (aj|bi)*Taibj (rpa_CC_Exchange.f90:36-97)
+            !
+            real(F64), intent(out)                                          :: EcExchange
+            real(F64), intent(in)                                           :: Alpha
+            real(F64), dimension(NVecsT2, NVirt, NOcc), intent(in)          :: Vkai
+            real(F64), dimension(NVecsT2), intent(in)                       :: Ak
+            real(F64), dimension(NVecsPiU, NVirt, NOcc), intent(in)         :: Rkai
+            integer, intent(in)                                             :: NOcc
+            integer, intent(in)                                             :: NVirt
+            integer, intent(in)                                             :: NVecsPiU
+            integer, intent(in)                                             :: NVecsT2
+            logical, intent(in)                                             :: S2Operator
+            integer(c_int) :: s2
+
+            call brpa_init()
+            s2 = 0
+            if (S2Operator) s2 = 1
+            call check(c_sosex(ctx, 0_c_int, EcExchange, Alpha, Vkai, Ak, Rkai, int(NOcc, c_int64_t), &
+                  int(NVirt, c_int64_t), int(NVecsPiU, c_int64_t), int(NVecsT2, c_int64_t), s2), &
+                  "rpa_CC_Exchange_SOSEX")
+      end subroutine rpa_CC_Exchange_SOSEX
+
+end module brpa_mod

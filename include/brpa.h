@@ -141,6 +141,37 @@ typedef struct {
 int brpa_ecrpa_batch(brpa_ctx *ctx, int precision, int64_t nfrag, const brpa_frag_t *frags,
                      double *ecrpa);
 
+/* ---- factorised direct-ring amplitudes --------------------------------- */
+/* T2 = sum_mu A_mu V_mu V_mu^T (ov x ov), the direct-ring doubles amplitudes
+ * at coupling strength lambda from nguess random guess vectors (0: the
+ * reference default ceil(3 N), capped at ov) and the eigenvalue cutoff of the
+ * Gram matrix (reference default 1e-10).  Replaces rpa_CC_T2 -> rpa_CC_T2_v5
+ * (rpa/rpa_CC_Doubles.f90:43-77, 491-642).  The reference's A and V are
+ * `allocatable, intent(out)`: the result stays on the device and the call
+ * returns NVecsT2; the caller allocates A(NVecsT2), V(NVecsT2, ov) and calls
+ * brpa_t2_copyout.  The Pi(u) eigenvectors/eigenvalues the reference takes
+ * are not needed: Q diag(l/(1+l W)) Q^T = l (1 + l Pi)^{-1} is applied through
+ * a Cholesky factor of 1 + l Pi(u) built from R.  Guess vectors: uniform
+ * (-1/2, 1/2) from a SplitMix64 counter hash of (seed, column, row), columns
+ * normalised (the reference draws them from gfortran's RNG, 571-588).
+ * precision: BRPA_FP64 only. */
+int brpa_t2_compute(brpa_ctx *ctx, int precision, const double *R, int64_t naux,
+                    const double *eocc, int64_t nocc, const double *evir, int64_t nvir,
+                    const double *freqs, const double *weights, int64_t nfreqs, double lambda,
+                    double cutoff, int64_t nguess, uint64_t seed, int64_t *nvecs_t2);
+/* A: NVecsT2 eigenvalues of T2 (ascending); V: Fortran V(NVecsT2, ov) == C V[ai][mu]. */
+int brpa_t2_copyout(brpa_ctx *ctx, double *A, double *V);
+
+/* ---- exchange correction ---------------------------------------------------- */
+/* EcExchange = -alpha sum_{aibj} (aj|bi) T(ai,bj), T = V^T diag(B) V with
+ * B = A (or A/(1-4A^2) when s2_operator), evaluated pair by pair
+ * W_ij = R_i V_j^T.  Replaces rpa_CC_Exchange_SOSEX
+ * (rpa/rpa_CC_Exchange.f90:36-97).  V: C V[ai][mu] (Fortran Vkai(NVecsT2, v, o)),
+ * R: C R[ai][P] (Fortran Rkai(NVecsPiU, v, o)).  precision: BRPA_FP64 only. */
+int brpa_sosex(brpa_ctx *ctx, int precision, double *ec_exchange, double alpha, const double *V,
+               const double *A, const double *R, int64_t nocc, int64_t nvir, int64_t naux,
+               int64_t nvecs_t2, int s2_operator);
+
 #ifdef __cplusplus
 }
 #endif

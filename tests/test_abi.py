@@ -37,3 +37,17 @@ def test_no_gpu_means_a_loud_error():
     import brpa
     with pytest.raises(brpa.BrpaError):
         brpa.Context(0)
+
+
+def test_fortran_shim_exports_reference_routines():
+    import shutil
+    lib = os.path.join(ROOT, "beyond-rpa_amd", "brpa", "libbrpa_fortran.so")
+    if not os.path.exists(lib):
+        pytest.skip("Fortran shim not built (amdflang absent)")
+    nm = shutil.which("nm")
+    if nm is None:
+        pytest.skip("nm not available")
+    import subprocess
+    out = subprocess.run([nm, "-D", "--defined-only", lib], capture_output=True, text=True).stdout.lower()
+    for name in ("rpa_gpiug_mo", "rpa_cc_ecrpa_analytic", "rpa_cc_t2", "rpa_cc_exchange_sosex"):
+        assert name in out, name

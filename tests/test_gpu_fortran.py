@@ -1,0 +1,42 @@
+"""The Fortran drop-in (brpa_mod, amdflang, 64-bit default INTEGER) called with
+the reference's argument lists, checked against the CPU oracle."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from brpa import synth
+from oracle import erpa, grid, sosex, t2
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "beyond-rpa_amd", "brpa", "brpa_shim_check")
+
+
+def test_fortran_shim_matches_oracle(tmp_path):
+    if not os.path.exists(EXE):
+        pytest.fail("brpa_shim_check was not built (amdflang missing at build time?)")
+    o, v, n, nf = 5, 24, 60, 12
+    f = synth.make_fragment(o, v, n, 99)
+    x, w, _ = grid.freq_grid(f.eocc, f.evir, nf)
+    fin, fout = tmp_path / "in.bin", tmp_path / "out.txt"
+    with open(fin, "wb") as fh:
+        np.array([o, v, n, nf, 0], dtype=np.int64).tofile(fh)
+        # Fortran Rkai(NVecs, NVirt*NOcc) is C R[ai][P]
+        for a in (f.R, f.eocc, f.evir, x, w):
+            np.ascontiguousarray(a, np.float64).tofile(fh)
+    r = subprocess.run([EXE, str(fin), str(fout)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    vals = open(fout).read().split()
+    ec, efused, esplit, nt, ex = float(vals[0]), float(vals[1]), float(vals[2]), int(vals[3]), float(vals[4])
+    A_f = np.array([float(s) for s in vals[5:5 + nt]])
+    e_ref = erpa.ecrpa(f.R, f.gaps(), x, w)
+    assert abs(ec - e_ref) <= 1e-8
+    assert abs(efused - e_ref) <= 1e-8
+    assert abs(esplit - e_ref) <= 1e-7
+    Pis = [erpa.gpiug_mo(f.R, f.gaps(), xx) for xx in x]
+    A_ref, V_ref = t2.t2_v5(f.R, f.gaps(), x, w, Pis, seed=0)
+    assert nt == len(A_ref)
+    assert np.max(np.abs(A_f - A_ref)) <= 1e-9 * np.max(np.abs(A_ref))
+    assert abs(ex - sosex.sosex(f.R, V_ref, A_ref, o, v)) <= 1e-8
