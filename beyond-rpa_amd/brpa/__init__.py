@@ -252,3 +252,26 @@ class Context:
                                          _ptr(A), _ptr(R), nocc, nvir, naux, A.size,
                                          int(bool(s2_operator))))
         return float(e[0])
+
+    def ecrpa_grad(self, R, eocc, evir, freqs, weights):
+        """(dE/dR, dE/d eps_occ, dE/d eps_vir) of the FP64 E_RPA, grid held fixed.
+
+        dE/dR has R's type and device (numpy array or torch tensor)."""
+        nai, naux = R.shape
+        eocc, evir = np.ascontiguousarray(eocc, np.float64), np.ascontiguousarray(evir, np.float64)
+        freqs = np.ascontiguousarray(freqs, np.float64)
+        weights = np.ascontiguousarray(weights, np.float64)
+        if nai != eocc.size * evir.size or freqs.size != weights.size:
+            raise ValueError("shape mismatch")
+        if _is_torch(R):
+            import torch
+            dR = torch.empty_like(R)
+        else:
+            dR = np.empty_like(R)
+        deo = np.zeros(eocc.size)
+        dev = np.zeros(evir.size)
+        self._sync_stream(R)
+        self._check(self._lib.brpa_ecrpa_grad(self._h, _ptr(R), naux, _ptr(eocc), eocc.size,
+                                              _ptr(evir), evir.size, _ptr(freqs), _ptr(weights),
+                                              freqs.size, _ptr(dR), _ptr(deo), _ptr(dev)))
+        return dR, deo, dev

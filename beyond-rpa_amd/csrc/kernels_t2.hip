@@ -175,3 +175,51 @@ int launch_pair_dot(brpa_ctx *ctx, const double *W1, const double *W2, const dou
     LAUNCH_CHECK(ctx);
     return BRPA_OK;
 }
+
+// Adjoint accumulation for one frequency (column-major, Rc(P, ai) and
+// X = M Rc with M = (1 + Pi_k)^{-1}):
+//   dR(P, ai) += 2 c_k(ai) coef (X - Rc)(P, ai)
+//   dd(ai)    += coef dc_k(ai) sum_P (X - Rc)(P, ai) Rc(P, ai)
+// coef = w_k/(2 pi).  One wave per column ai, fixed-order wave reduction.
+__global__ __launch_bounds__(256) void k_grad_acc(const double *__restrict__ X,
+                                                 const double *__restrict__ Rc, int64_t N,
+                                                 int64_t K, const double *__restrict__ c,
+                                                 const double *__restrict__ dc, double coef,
+                                                 double *__restrict__ dR, double *__restrict__ dd) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t ai = (int64_t)blockIdx.x * 4 + wave;
+    if (ai >= K) return;
+    const double *x = X + ai * N, *r = Rc + ai * N;
+    double *g = dR + ai * N;
+    const double s2 = 2.0 * c[ai] * coef;
+    double acc = 0.0;
+    for (int64_t P = lane; P < N; P += 64) {
+        const double d = x[P] - r[P];
+        g[P] = fma(s2, d, g[P]);
+        acc = fma(d, r[P], acc);
+    }
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    if (lane == 0) dd[ai] += coef * dc[ai] * acc;
+}
+
+int launch_grad_acc(brpa_ctx *ctx, const double *X, const double *Rc, int64_t N, int64_t K,
+                    const double *c, const double *dc, double coef, double *dR, double *dd) {
+    if (K <= 0) return BRPA_OK;
+    hipLaunchKernelGGL(k_grad_acc, dim3((unsigned)((K + 3) / 4)), dim3(256), 0, ctx->stream, X, Rc,
+                       N, K, c, dc, coef, dR, dd);
+    LAUNCH_CHECK(ctx);
+    return BRPA_OK;
+}
+
+// fill n doubles with a value (graph-capture safe)
+__global__ void k_fill(double *p, int64_t n, double v) {
+    int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+int launch_fill(brpa_ctx *ctx, double *p, int64_t n, double v) {
+    if (n <= 0) return BRPA_OK;
+    hipLaunchKernelGGL(k_fill, dim3(nblocks(n)), dim3(256), 0, ctx->stream, p, n, v);
+    LAUNCH_CHECK(ctx);
+    return BRPA_OK;
+}

@@ -115,6 +115,13 @@ int grow(brpa_ctx *ctx, double **p, size_t *cap, size_t n) {
 
 }  // namespace
 
+int rocblas_handle_of(brpa_ctx *ctx, void **h) {
+    rocblas_handle hh;
+    BRPA_TRY(handle(ctx, &hh));
+    *h = hh;
+    return BRPA_OK;
+}
+
 void t2_release(brpa_ctx *ctx) {
     if (ctx->t2_A) (void)hipFree(ctx->t2_A);
     if (ctx->t2_V) (void)hipFree(ctx->t2_V);

@@ -172,6 +172,17 @@ int brpa_sosex(brpa_ctx *ctx, int precision, double *ec_exchange, double alpha, 
                const double *A, const double *R, int64_t nocc, int64_t nvir, int64_t naux,
                int64_t nvecs_t2, int s2_operator);
 
+/* ---- adjoint ----------------------------------------------------------------- */
+/* dE_RPA/dR (C R[ai][P], same shape as R) and dE_RPA/d(eps_occ), dE_RPA/d(eps_vir)
+ * of the FP64 energy of brpa_ecrpa, with the frequency grid held fixed.  Pi(u)
+ * is recomputed per frequency, never stored from the forward pass.  No
+ * reference counterpart: the reference computes no RPA gradients.  deocc/devir
+ * may be NULL. */
+int brpa_ecrpa_grad(brpa_ctx *ctx, const double *R, int64_t naux, const double *eocc,
+                    int64_t nocc, const double *evir, int64_t nvir, const double *freqs,
+                    const double *weights, int64_t nfreqs, double *dR, double *deocc,
+                    double *devir);
+
 #ifdef __cplusplus
 }
 #endif

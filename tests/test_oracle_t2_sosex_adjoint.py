@@ -118,3 +118,26 @@ def test_adjoint_central_differences():
         em[k] -= h
         fd = (E(f.R, f.eocc, ep) - E(f.R, f.eocc, em)) / (2 * h)
         assert abs(fd - dev[k]) <= 1e-6 * max(abs(dev[k]), 1e-3)
+
+
+def test_cancellation_free_fd_equals_plain_fd_on_small_energies():
+    """On a tiny fragment (E ~ 0.1 Ha) and h = 1e-2, where the rounding of the
+    plain FP64 difference (~1e-16 |E| / h) is below 1e-12, the cancellation-free
+    central difference equals the plain one."""
+    f = synth.make_fragment(2, 3, 4, seed=7)
+    x, w, _ = grid.freq_grid(f.eocc, f.evir, 12)
+    h = 1e-2
+    fd = adjoint.central_fd_R(f.R, f.eocc, f.evir, x, w, [(3, 1)], h)[(3, 1)]
+    Rp, Rm = f.R.copy(), f.R.copy()
+    Rp[3, 1] += h
+    Rm[3, 1] -= h
+    plain = (erpa.ecrpa(Rp, f.gaps(), x, w, logdet=erpa.log_det_cholesky)
+             - erpa.ecrpa(Rm, f.gaps(), x, w, logdet=erpa.log_det_cholesky)) / (2 * h)
+    assert abs(fd - plain) < 1e-9 * abs(plain)
+    v = adjoint.central_fd_eps(f.R, f.eocc, f.evir, x, w, "v", 1, h)
+    ep, em = f.evir.copy(), f.evir.copy()
+    ep[1] += h
+    em[1] -= h
+    plain = (erpa.ecrpa(f.R, (ep[None] - f.eocc[:, None]).ravel(), x, w)
+             - erpa.ecrpa(f.R, (em[None] - f.eocc[:, None]).ravel(), x, w)) / (2 * h)
+    assert abs(v - plain) < 1e-8 * abs(plain)
