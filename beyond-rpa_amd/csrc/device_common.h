@@ -22,6 +22,18 @@ __device__ __forceinline__ int find_frag(const int64_t *__restrict__ pre, int nf
     return lo;
 }
 
+// The same search on a copy of pre staged in LDS (one parallel load instead of
+// a chain of dependent global loads; matters for small work-groups).  Every
+// thread of the work-group must call it (it synchronises).
+constexpr int FIND_LDS_MAX = 512;
+__device__ __forceinline__ int find_frag_lds(const int64_t *__restrict__ pre, int nfrag, int64_t b,
+                                             int64_t *s_pre) {
+    if (nfrag > FIND_LDS_MAX) return find_frag(pre, nfrag, b);
+    for (int i = threadIdx.x; i <= nfrag; i += blockDim.x) s_pre[i] = pre[i];
+    __syncthreads();
+    return find_frag(s_pre, nfrag, b);
+}
+
 // Lower-triangle pair index -> (i, j), i >= j, pair = i*(i+1)/2 + j.
 __device__ __forceinline__ void tri_decode(int pair, int &i, int &j) {
     int t = (int)((sqrt(8.0 * (double)pair + 1.0) - 1.0) * 0.5);

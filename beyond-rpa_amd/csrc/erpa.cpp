@@ -97,7 +97,7 @@ Counts counts_of(const HostFrag &f, int nsteps_max, int precision) {
         if (N > j0) c.pdiag[j] = F;
         if (N > j0 + CHOL_NB) c.trsm[j] = F * (cdiv(N - j0, CHOL_NB) - 1);
         if (N > j0 + CHOL_NB) {
-            int64_t t = cdiv(N - j0 - CHOL_NB, CHOL_NB);
+            int64_t t = cdiv(N - j0 - CHOL_NB, 128);   // 128x128 update super-tiles
             c.update[j] = F * (t * (t + 1) / 2);
         }
     }
@@ -406,7 +406,7 @@ int brpa_ecrpa_analytic(brpa_ctx *ctx, double *ecrpa, const double *piu, const d
         int64_t j0 = (int64_t)j * CHOL_NB;
         pre[2 * (1 + j) + 1] = (N > j0) ? F : 0;
         pre[2 * (1 + nsteps + j) + 1] = (N > j0 + CHOL_NB) ? F * (cdiv(N - j0, CHOL_NB) - 1) : 0;
-        int64_t t = (N > j0 + CHOL_NB) ? cdiv(N - j0 - CHOL_NB, CHOL_NB) : 0;
+        int64_t t = (N > j0 + CHOL_NB) ? cdiv(N - j0 - CHOL_NB, 128) : 0;   // update super-tiles
         pre[2 * (1 + 2 * nsteps + j) + 1] = F * (t * (t + 1) / 2);
     }
     FragDev hd{};

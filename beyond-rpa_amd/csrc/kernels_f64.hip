@@ -280,14 +280,17 @@ __device__ __forceinline__ void mma64_xyt(const double (*sX)[CHOL_NB + 1],
 __global__ __launch_bounds__(64) void k_potrf_diag(const FragDev *__restrict__ fr,
                                                   const int64_t *__restrict__ pre, int nfrag,
                                                   int step) {
+    // L in the lower triangle; column `lane` of L^{-1} (lower triangular) is kept
+    // transposed in the unused upper triangle, its diagonal in Xd.
     __shared__ double L[CHOL_NB][CHOL_NB + 1];
-    __shared__ double X[CHOL_NB][CHOL_NB + 1];
+    __shared__ double Xd[CHOL_NB];
+    __shared__ int64_t s_pre[FIND_LDS_MAX + 1];
     int64_t b = blockIdx.x;
-    int f = find_frag(pre, nfrag, b);
+    int f = find_frag_lds(pre, nfrag, b, s_pre);
     const FragDev &F = fr[f];
     const int N = F.N;
     const int j0 = step * CHOL_NB;
-    const int w = (int)(b - pre[f]);
+    const int w = (int)(b - (nfrag <= FIND_LDS_MAX ? s_pre[f] : pre[f]));
     const int nbj = min(CHOL_NB, N - j0);
     const double *A = F.Pi + (int64_t)w * N * N;
     const int lane = threadIdx.x;
@@ -316,15 +319,21 @@ __global__ __launch_bounds__(64) void k_potrf_diag(const FragDev *__restrict__ f
         }
         __syncthreads();
     }
-    // column `lane` of L^{-1} by forward substitution (X[i][lane])
-    for (int i = 0; i < CHOL_NB; ++i) {
-        double s = (i == lane) ? 1.0 : 0.0;
-        for (int k = 0; k < i; ++k) s -= L[i][k] * X[k][lane];
-        X[i][lane] = s / L[i][i];
+    // column `lane` of L^{-1} by forward substitution: x_i = X[i][lane], i >= lane,
+    // stored at L[lane][i] (i > lane, upper triangle) and Xd[lane] (i == lane)
+    Xd[lane] = 1.0 / L[lane][lane];
+    for (int i = lane + 1; i < CHOL_NB; ++i) {
+        double s = -L[i][lane] * Xd[lane];
+        for (int k = lane + 1; k < i; ++k) s -= L[i][k] * L[lane][k];
+        L[lane][i] = s / L[i][i];
     }
     __syncthreads();
     double *linv = F.linv + (int64_t)w * CHOL_NB * CHOL_NB;
-    for (int i = 0; i < CHOL_NB; ++i) linv[i * CHOL_NB + lane] = X[i][lane];
+    for (int i = 0; i < CHOL_NB; ++i) {
+        // X[i][lane]: zero above the diagonal, Xd on it, upper-stored below it
+        const double x = (i < lane) ? 0.0 : (i == lane ? Xd[lane] : L[lane][i]);
+        linv[i * CHOL_NB + lane] = x;
+    }
     if (lane == 0) {
         dd_t s = {0.0, 0.0};
         for (int c = 0; c < nbj; ++c) s = dd_add(s, log(L[c][c]));
@@ -348,13 +357,14 @@ __global__ __launch_bounds__(256) void k_trsm(const FragDev *__restrict__ fr,
                                              const int64_t *__restrict__ pre, int nfrag, int step) {
     __shared__ double sB[CHOL_NB][CHOL_NB + 1];
     __shared__ double sL[CHOL_NB][CHOL_NB + 1];
+    __shared__ int64_t s_pre[FIND_LDS_MAX + 1];
     int64_t b = blockIdx.x;
-    int f = find_frag(pre, nfrag, b);
+    int f = find_frag_lds(pre, nfrag, b, s_pre);
     const FragDev &F = fr[f];
     const int N = F.N;
     const int j0 = step * CHOL_NB;
     const int nsub = (N - j0 + CHOL_NB - 1) / CHOL_NB - 1;   // chunks below the diagonal
-    int64_t lb = b - pre[f];
+    int64_t lb = b - (nfrag <= FIND_LDS_MAX ? s_pre[f] : pre[f]);
     const int w = (int)(lb / nsub), r = 1 + (int)(lb - (int64_t)w * nsub);
     double *A = F.Pi + (int64_t)w * N * N;
     const double *linv = F.linv + (int64_t)w * CHOL_NB * CHOL_NB;
@@ -389,43 +399,71 @@ int launch_trsm(brpa_ctx *ctx, const FragDev *fr, const int64_t *pre, int nfrag,
     return BRPA_OK;
 }
 
-__global__ __launch_bounds__(256) void k_chol_update(const FragDev *__restrict__ fr,
+// Trailing update on 128x128 super-tiles (TI >= TJ in 128-row units of the
+// trailing matrix that starts at t0 = j0 + 64), K = 64, 8 waves as 2 (rows) x
+// 4 (cols) of 64x32.  Waves entirely above the diagonal of a diagonal
+// super-tile skip their products; element writes keep col <= row.
+constexpr int UPD_T = 128;
+
+__global__ __launch_bounds__(512) void k_chol_update(const FragDev *__restrict__ fr,
                                                     const int64_t *__restrict__ pre, int nfrag,
                                                     int step) {
-    __shared__ double sI[CHOL_NB][CHOL_NB + 1];
-    __shared__ double sJ[CHOL_NB][CHOL_NB + 1];
+    __shared__ double sI[UPD_T][CHOL_NB + 1];
+    __shared__ double sJ[UPD_T][CHOL_NB + 1];
+    __shared__ int64_t s_pre[FIND_LDS_MAX + 1];
     int64_t b = blockIdx.x;
-    int f = find_frag(pre, nfrag, b);
+    int f = find_frag_lds(pre, nfrag, b, s_pre);
     const FragDev &F = fr[f];
     const int N = F.N;
     const int j0 = step * CHOL_NB;
     const int t0 = j0 + CHOL_NB;
-    const int nt = (N - t0 + CHOL_NB - 1) / CHOL_NB;
+    const int nt = (N - t0 + UPD_T - 1) / UPD_T;
     const int npairs = nt * (nt + 1) / 2;
-    int64_t lb = b - pre[f];
+    int64_t lb = b - (nfrag <= FIND_LDS_MAX ? s_pre[f] : pre[f]);
     const int w = (int)(lb / npairs), pair = (int)(lb - (int64_t)w * npairs);
     int ti, tj;
     tri_decode(pair, ti, tj);
     double *A = F.Pi + (int64_t)w * N * N;
-    const int Ri = t0 + CHOL_NB * ti, Rj = t0 + CHOL_NB * tj;
+    const int Ri = t0 + UPD_T * ti, Rj = t0 + UPD_T * tj;
     const int tid = threadIdx.x;
-    for (int e = tid; e < CHOL_NB * CHOL_NB; e += 256) {
+    for (int e = tid; e < UPD_T * CHOL_NB; e += 512) {
         int rr = e / CHOL_NB, cc = e % CHOL_NB;
         sI[rr][cc] = (Ri + rr < N) ? A[(int64_t)(Ri + rr) * N + j0 + cc] : 0.0;
         sJ[rr][cc] = (Rj + rr < N) ? A[(int64_t)(Rj + rr) * N + j0 + cc] : 0.0;
     }
     __syncthreads();
-    d4_t acc[2][2];
-    mma64_xyt(sI, sJ, acc);
-    const int wave = tid >> 6, lane = tid & 63, qi = wave >> 1, qj = wave & 1;
+    const int wave = tid >> 6, lane = tid & 63;
+    const int wa = wave >> 2, wq = wave & 3;   // rows 64*wa.., cols 32*wq..
+    const int r0 = 64 * wa, c0 = 32 * wq;
+    if (ti == tj && c0 > r0 + 63) return;      // wave entirely above the diagonal
+    if (Ri + r0 >= N || Rj + c0 >= N) return;
+    d4_t acc[4][2];
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = (d4_t){0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+    for (int kk = 0; kk < CHOL_NB; kk += 4) {
+        const int kc = kk + (lane >> 4);
+        double a[4], bv[2];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) a[i] = sI[r0 + 16 * i + (lane & 15)][kc];
+#pragma unroll
+        for (int j = 0; j < 2; ++j) bv[j] = sJ[c0 + 16 * j + (lane & 15)][kc];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], bv[j], acc[i][j], 0, 0, 0);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j)
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                int row = Ri + 32 * qi + 16 * i + (lane >> 4) + 4 * q;
-                int col = Rj + 32 * qj + 16 * j + (lane & 15);
+                int row = Ri + r0 + 16 * i + (lane >> 4) + 4 * q;
+                int col = Rj + c0 + 16 * j + (lane & 15);
                 if (row < N && col < N && col <= row) A[(int64_t)row * N + col] -= acc[i][j][q];
             }
 }
@@ -433,7 +471,7 @@ __global__ __launch_bounds__(256) void k_chol_update(const FragDev *__restrict__
 int launch_chol_update(brpa_ctx *ctx, const FragDev *fr, const int64_t *pre, int nfrag,
                        int64_t total, int step) {
     if (total <= 0) return BRPA_OK;
-    hipLaunchKernelGGL(k_chol_update, dim3(grid1(total)), dim3(256), 0, ctx->stream, fr, pre,
+    hipLaunchKernelGGL(k_chol_update, dim3(grid1(total)), dim3(512), 0, ctx->stream, fr, pre,
                        nfrag, step);
     LAUNCH_CHECK(ctx);
     return BRPA_OK;
