@@ -1,6 +1,6 @@
 """Throughput of the batched correlation-energy hot path on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision split_bf16|fp64]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision split_i8|split_bf16|fp64]
 
 Workload: 256 uneven fragments, o ~ U{8..40}, v ~ U{70..350},
 N = round(4.5 (o+v)), 16 frequencies each, seeded synthetic inputs
@@ -26,7 +26,16 @@ sys.path.insert(0, ROOT)
 
 PEAK_FP64_TFLOPS = 78.6      # MI355X dense FP64 matrix
 PEAK_BF16_TFLOPS = 2500.0    # MI355X dense BF16 matrix
-SPLIT_PRODUCTS = 10          # bf16 slice products per FP64-equivalent product (4 slices, level <= 3)
+PEAK_I8_TOPS = 5000.0        # MI355X dense INT8 matrix
+SPLIT_PRODUCTS = 10          # slice products per FP64-equivalent product (4 slices, level <= 3)
+PRECISIONS = ("split_i8", "split_bf16", "fp64")
+DTYPES = {"split_i8": "i8 (4-slice split, 10 products, exact i32 acc) + f64 log-det",
+          "split_bf16": "bf16 (4-slice split, 10 products, fp32 acc) + f64 log-det",
+          "fp64": "f64"}
+# dominant Pi kernel and the peak its algorithmic FP64-equivalent flops are priced at
+PI_KERNEL = {"split_i8": ("k_syrk_i8", PEAK_I8_TOPS / SPLIT_PRODUCTS),
+             "split_bf16": ("k_syrk_bf16", PEAK_BF16_TFLOPS / SPLIT_PRODUCTS),
+             "fp64": ("k_syrk_f64", PEAK_FP64_TFLOPS)}
 
 
 def syrk_flops(o, v, n, nf):
@@ -43,13 +52,13 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--precision", default="split_bf16", choices=["split_bf16", "fp64"])
+    ap.add_argument("--precision", default="split_i8", choices=list(PRECISIONS))
     ap.add_argument("--nfrag", type=int, default=256)
     ap.add_argument("--nfreqs", type=int, default=16)
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="CPU oracle sample budget (0 disables the CPU baseline)")
     ap.add_argument("--compare-fp64", type=int, default=1,
-                    help="also time the FP64 form (reported as fp64_value)")
+                    help="also time the other forms (reported under `forms`)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy 5 times (reference for the "
                          "FETCH_SIZE/WRITE_SIZE counter scale)")
@@ -108,28 +117,26 @@ def main():
     e, dt, stage_ms, stage_n = run(a.precision, a.warmup, a.steps)
     print("# %s: %.1f ms/step stages %s" % (a.precision, dt * 1e3, stage_ms), file=sys.stderr,
           flush=True)
-    fp64_value = None
-    e64 = None
-    if a.compare_fp64 and a.precision != "fp64":
-        e64, dt64, st64, _ = run("fp64", 1, max(1, a.steps))
-        fp64_value = a.nfrag / dt64
-        print("# fp64: %.1f ms/step stages %s" % (dt64 * 1e3, st64), file=sys.stderr, flush=True)
+    forms = {a.precision: {"energies": e, "ms_per_step": dt * 1e3, "stage_ms": stage_ms}}
+    if a.compare_fp64:
+        for prec in PRECISIONS:
+            if prec in forms:
+                continue
+            ep, dtp, stp, _ = run(prec, 1, max(1, a.steps))
+            forms[prec] = {"energies": ep, "ms_per_step": dtp * 1e3, "stage_ms": stp}
+            print("# %s: %.1f ms/step stages %s" % (prec, dtp * 1e3, stp), file=sys.stderr,
+                  flush=True)
 
     tot_syrk = sum(syrk_flops(o, v, n, a.nfreqs) for (o, v, n) in shapes)
     tot_chol = sum(chol_flops(n, a.nfreqs) for (o, v, n) in shapes)
     dom = max(("pi", "chol"), key=lambda k: stage_ms[k])
     if dom == "pi":
-        if a.precision == "fp64":
-            peak, bound = PEAK_FP64_TFLOPS, "mfma"
-        else:
-            peak, bound = PEAK_BF16_TFLOPS / SPLIT_PRODUCTS, "mfma"
+        kernel, peak = PI_KERNEL[a.precision]
         achieved = tot_syrk / (stage_ms["pi"] * 1e-3) / 1e12
-        kernel = "k_syrk_f64" if a.precision == "fp64" else "k_syrk_bf16"
     else:
-        peak, bound = PEAK_FP64_TFLOPS, "mfma"
+        kernel, peak = "k_potrf_diag+k_trsm+k_chol_update", PEAK_FP64_TFLOPS
         achieved = tot_chol / (stage_ms["chol"] * 1e-3) / 1e12
-        kernel = "k_potrf_panel+k_chol_update"
-    roofline = {"bound": bound, "achieved": round(achieved, 3), "peak": round(peak, 3),
+    roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": round(peak, 3),
                 "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": None,
                 "kernel": kernel, "kernel_ms_per_step": round(stage_ms[dom], 3),
                 "launches_per_step": stage_n[dom]}
@@ -143,22 +150,20 @@ def main():
         threads = max([p.get("num_threads", 1) for p in threadpool_info()] or [1])
         t = time.perf_counter()
         k = 0
-        dmax = 0.0
-        d64max = 0.0
+        dmax = {p: 0.0 for p in forms}
         while k < a.nfrag and time.perf_counter() - t < a.cpu_seconds:
             f = frags[k]
             eref = oerpa.ecrpa(f.R, f.gaps(), grids[k][0], grids[k][1])
-            dmax = max(dmax, abs(e[k] - eref))
-            if e64 is not None:
-                d64max = max(d64max, abs(e64[k] - eref))
+            for p in forms:
+                dmax[p] = max(dmax[p], abs(forms[p]["energies"][k] - eref))
             k += 1
         tc = time.perf_counter() - t
         cpu = {"value": round(k / tc, 4), "unit": "fragments/sec", "cores": threads,
                "kind": "port",
                "sample": "first %d of the %d fragments, E_RPA by the numpy FP64 oracle "
                          "(dgemm Pi + dgeqrf log-det), %.1f s" % (k, a.nfrag, tc)}
-        parity = {"n_checked": k, "max_abs_dE_Ha": dmax,
-                  "max_abs_dE_fp64_Ha": d64max if e64 is not None else None}
+        parity = {"n_checked": k, "max_abs_dE_Ha": dmax[a.precision],
+                  "max_abs_dE_Ha_by_form": dmax}
 
     out = {
         "metric": "fragments/sec (E_RPA+exchange corr) split-BF16 vs FP64 at |ΔE|≤1e-7 Ha",
@@ -171,15 +176,17 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "bf16 (4-slice split, 10 products, fp32 acc) + f64 log-det" if a.precision == "split_bf16" else "f64",
+        "dtype": DTYPES[a.precision],
         "data": "synthetic (seeded PCG64, brpa.synth)",
         "config": {"workload": "batch of %d uneven fragments, o~U{8..40}, v~U{70..350}, "
                                "N=round(4.5(o+v)), %d frequencies" % (a.nfrag, a.nfreqs),
                    "precision": a.precision, "components": "E_RPA (exchange correction not "
                    "in the batched step yet)", "algorithmic_tflop_per_step":
                    round((tot_syrk + tot_chol) / 1e12, 3)},
-        "fp64_value": round(fp64_value, 4) if fp64_value else None,
-        "stage_ms": {k: round(v, 3) for k, v in stage_ms.items()},
+        "forms": {p: {"fragments_per_sec": round(a.nfrag / (d["ms_per_step"] * 1e-3), 4),
+                      "ms_per_step": round(d["ms_per_step"], 3),
+                      "stage_ms": {k: round(v, 3) for k, v in d["stage_ms"].items()}}
+                  for p, d in forms.items()},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "parity": parity,

@@ -15,9 +15,9 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import FP64, SPLIT_BF16  # noqa: F401
+from ._lib import FP64, SPLIT_BF16, SPLIT_I8  # noqa: F401
 
-PRECISIONS = {"fp64": FP64, "split_bf16": SPLIT_BF16}
+PRECISIONS = {"fp64": FP64, "split_bf16": SPLIT_BF16, "split_i8": SPLIT_I8}
 
 
 class BrpaError(RuntimeError):
@@ -147,6 +147,19 @@ class Context:
 
     def trim(self):
         self._check(self._lib.brpa_ctx_trim(self._h))
+
+    def set_graphs(self, enable=True):
+        """Capture/replay the batched E_RPA launch sequence as a hipGraph."""
+        self._check(self._lib.brpa_ctx_set_graphs(self._h, int(bool(enable))))
+
+    def last_kernel_ms(self):
+        return self._lib.brpa_last_kernel_ms(self._h)
+
+    def graph_stats(self):
+        """(captures, replays) of the last ecrpa_batch call."""
+        c, r = C.c_int(0), C.c_int(0)
+        self._check(self._lib.brpa_last_graph_stats(self._h, C.byref(c), C.byref(r)))
+        return c.value, r.value
 
     # -- reference-equivalent entry points -------------------------------
     def gpiug_mo(self, gpiug, R, eocc, evir, freq, prefactor=4.0):

@@ -6,7 +6,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libbrpa.so")
 
 OK, ERR_ARG, ERR_HIP, ERR_NOT_SPD, ERR_GAP, ERR_NONFINITE, ERR_OOM, ERR_GRID, ERR_SOLVER = range(9)
-FP64, SPLIT_BF16 = 0, 1
+FP64, SPLIT_BF16, SPLIT_I8 = 0, 1, 2
 STAGE_PI, STAGE_SPLIT, STAGE_CHOL, STAGE_AUX = range(4)
 
 _i64 = C.c_int64
@@ -26,6 +26,8 @@ SIGNATURES = {
     "brpa_last_error": (C.c_char_p, [_vp]),
     "brpa_ctx_set_stream": (C.c_int, [_vp, _vp]),
     "brpa_ctx_trim": (C.c_int, [_vp]),
+    "brpa_ctx_set_graphs": (C.c_int, [_vp, C.c_int]),
+    "brpa_last_graph_stats": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "brpa_last_kernel_ms": (C.c_double, [_vp]),
     "brpa_last_stage_ms": (C.c_double, [_vp, C.c_int]),
     "brpa_last_stage_launches": (C.c_int, [_vp, C.c_int]),

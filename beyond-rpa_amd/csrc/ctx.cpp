@@ -44,6 +44,7 @@ int from_device(brpa_ctx *ctx, void *dst, const void *src, size_t bytes) {
 int ws_reserve(brpa_ctx *ctx, size_t bytes, Arena *arena) {
     if (bytes > ctx->ws_cap) {
         BRPA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        graphs_release(ctx);   // captured graphs hold workspace addresses
         if (ctx->ws) {
             BRPA_HIP(ctx, hipFree(ctx->ws));
             ctx->ws = nullptr;
@@ -62,6 +63,32 @@ int ws_reserve(brpa_ctx *ctx, size_t bytes, Arena *arena) {
     arena->base = ctx->ws;
     arena->off = 0;
     return BRPA_OK;
+}
+
+int pin_reserve(brpa_ctx *ctx, size_t bytes) {
+    if (bytes <= ctx->pin_cap) return BRPA_OK;
+    BRPA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    // captured graphs hold the old staging address
+    graphs_release(ctx);
+    if (ctx->pin) BRPA_HIP(ctx, hipHostFree(ctx->pin));
+    ctx->pin = nullptr;
+    ctx->pin_cap = 0;
+    size_t cap = bytes + bytes / 4 + (1u << 20);
+    hipError_t e = hipHostMalloc((void **)&ctx->pin, cap, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        ctx->pin = nullptr;
+        return brpa_fail(ctx, BRPA_ERR_OOM, "pinned host staging of %zu bytes: %s", cap,
+                         hipGetErrorString(e));
+    }
+    ctx->pin_cap = cap;
+    return BRPA_OK;
+}
+
+void graphs_release(brpa_ctx *ctx) {
+    for (auto &g : ctx->graphs)
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    ctx->graphs.clear();
 }
 
 void stage_reset(brpa_ctx *ctx) {
@@ -125,7 +152,9 @@ int brpa_ctx_destroy(brpa_ctx *ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->own_stream) (void)hipStreamSynchronize(ctx->own_stream);
     if (ctx->stream && ctx->stream != ctx->own_stream) (void)hipStreamSynchronize(ctx->stream);
+    graphs_release(ctx);
     if (ctx->ws) (void)hipFree(ctx->ws);
+    if (ctx->pin) (void)hipHostFree(ctx->pin);
     t2_release(ctx);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
@@ -144,9 +173,27 @@ int brpa_ctx_set_stream(brpa_ctx *ctx, void *s) {
     return BRPA_OK;
 }
 
+int brpa_ctx_set_graphs(brpa_ctx *ctx, int enable) {
+    if (!ctx) return BRPA_ERR_ARG;
+    ctx->use_graphs = enable ? 1 : 0;
+    if (!enable) {
+        BRPA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        graphs_release(ctx);
+    }
+    return BRPA_OK;
+}
+
+int brpa_last_graph_stats(const brpa_ctx *ctx, int *captures, int *replays) {
+    if (!ctx) return BRPA_ERR_ARG;
+    if (captures) *captures = ctx->last_graph_captures;
+    if (replays) *replays = ctx->last_graph_replays;
+    return BRPA_OK;
+}
+
 int brpa_ctx_trim(brpa_ctx *ctx) {
     if (!ctx) return BRPA_ERR_ARG;
     BRPA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    graphs_release(ctx);
     if (ctx->ws) BRPA_HIP(ctx, hipFree(ctx->ws));
     ctx->ws = nullptr;
     ctx->ws_cap = 0;

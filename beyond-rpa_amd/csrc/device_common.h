@@ -62,16 +62,13 @@ __device__ __forceinline__ dd_t dd_add(dd_t a, dd_t b) {
     return dd_add(s, b.lo);
 }
 
-// XCD-aware block remap: the hardware hands consecutive work-group ids to the
-// 8 XCDs round-robin; this maps them so that each XCD gets a contiguous range
-// of the logical work list (neighbouring tiles share R panels in that XCD's L2).
+// XCD-aware block order.  Work-group ids are handed to the 8 XCDs round-robin
+// (id % 8), and each XCD has its own L2.  This bijection on [0, nblocks) gives
+// XCD x a contiguous range of logical work items, so tiles that share panels
+// (neighbouring items of one matrix) run on the same XCD and hit its L2.
 __device__ __forceinline__ int64_t xcd_remap(int64_t b, int64_t nblocks) {
-    const int64_t nx = 8;
-    int64_t per = (nblocks + nx - 1) / nx;
-    int64_t x = b % nx, r = b / nx;
-    int64_t lb = x * per + r;
-    // the last XCD ranges may be short; fall back to identity past the even part
-    int64_t even = (nblocks / nx) * nx;
-    if (b >= even || lb >= nblocks) return b;
-    return (nblocks % nx == 0) ? lb : b;
+    constexpr int64_t NX = 8;
+    const int64_t q = nblocks / NX, rem = nblocks % NX;
+    const int64_t x = b % NX, r = b / NX;
+    return x * q + (x < rem ? x : rem) + r;
 }

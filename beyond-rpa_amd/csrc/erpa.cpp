@@ -74,7 +74,7 @@ int host_copy(brpa_ctx *ctx, double *dst, const double *src, int64_t n) {
 
 // Work-group counts per fragment for every stage.
 struct Counts {
-    int64_t dtab, syrk, setdiag, split, syrk16;
+    int64_t dtab, syrk, setdiag, split, syrk16, colscale8, split8, syrk8;
     std::vector<int64_t> pdiag, trsm, update;   // per Cholesky step
 };
 
@@ -89,6 +89,10 @@ Counts counts_of(const HostFrag &f, int nsteps_max, int precision) {
     c.split = (precision == BRPA_SPLIT_BF16) ? F * (kpad / SPLIT_KB) * cdiv(N, 64) : 0;
     const int64_t nt16 = cdiv(N, 128);
     c.syrk16 = (precision == BRPA_SPLIT_BF16) ? F * (nt16 * (nt16 + 1) / 2) : 0;
+    const bool i8 = precision == BRPA_SPLIT_I8;
+    c.colscale8 = i8 ? cdiv(N, 64) : 0;
+    c.split8 = i8 ? cdiv(K, I8_KB) * cdiv(N, 64) : 0;
+    c.syrk8 = i8 ? F * (nt16 * (nt16 + 1) / 2) : 0;
     c.pdiag.assign(nsteps_max, 0);
     c.trsm.assign(nsteps_max, 0);
     c.update.assign(nsteps_max, 0);
@@ -112,6 +116,10 @@ size_t frag_bytes(const HostFrag &h, int precision) {
         size_t kpad = cdiv(K, SPLIT_KB) * SPLIT_KB;
         b += 2 * F * NSLICE * N * kpad + (4 + 16) * F * (kpad / SPLIT_KB) * N;
     }
+    if (precision == BRPA_SPLIT_I8) {
+        size_t kpad = cdiv(K, I8_KB) * I8_KB;
+        b += F * NSLICE * N * kpad + 8 * F * N + 8 * F * K;
+    }
     return b;
 }
 
@@ -128,36 +136,59 @@ int run_chunk(brpa_ctx *ctx, int precision, const std::vector<HostFrag> &fr, dou
     std::vector<Counts> cnt(nfrag);
     for (int f = 0; f < nfrag; ++f) cnt[f] = counts_of(fr[f], nsteps_max, precision);
 
-    // prefix arrays: [dtab, syrk, setdiag, split, syrk16, pdiag_0..S-1, trsm_0..S-1, update_0..]
-    const int nstage = 5 + 3 * nsteps_max;
+    // prefix arrays: [dtab, syrk, setdiag, split, syrk16, colscale8, split8, syrk8,
+    //                 pdiag_0..S-1, trsm_0..S-1, update_0..S-1]
+    constexpr int NFIX = 8;
+    const int nstage = NFIX + 3 * nsteps_max;
     std::vector<int64_t> pre((size_t)nstage * (nfrag + 1), 0);
     auto P = [&](int s) { return pre.data() + (size_t)s * (nfrag + 1); };
     for (int f = 0; f < nfrag; ++f) {
         const Counts &c = cnt[f];
-        int64_t v[5] = {c.dtab, c.syrk, c.setdiag, c.split, c.syrk16};
-        for (int s = 0; s < 5; ++s) P(s)[f + 1] = P(s)[f] + v[s];
+        int64_t v[NFIX] = {c.dtab,  c.syrk,      c.setdiag, c.split,
+                           c.syrk16, c.colscale8, c.split8,  c.syrk8};
+        for (int s = 0; s < NFIX; ++s) P(s)[f + 1] = P(s)[f] + v[s];
         for (int j = 0; j < nsteps_max; ++j) {
-            P(5 + j)[f + 1] = P(5 + j)[f] + c.pdiag[j];
-            P(5 + nsteps_max + j)[f + 1] = P(5 + nsteps_max + j)[f] + c.trsm[j];
-            P(5 + 2 * nsteps_max + j)[f + 1] = P(5 + 2 * nsteps_max + j)[f] + c.update[j];
+            P(NFIX + j)[f + 1] = P(NFIX + j)[f] + c.pdiag[j];
+            P(NFIX + nsteps_max + j)[f + 1] = P(NFIX + nsteps_max + j)[f] + c.trsm[j];
+            P(NFIX + 2 * nsteps_max + j)[f + 1] = P(NFIX + 2 * nsteps_max + j)[f] + c.update[j];
         }
     }
 
-    // small per-fragment arrays (weights, freqs, eocc, evir) are packed into one
-    // host buffer and uploaded with a single copy
-    size_t nsmall = 0;
-    for (int f = 0; f < nfrag; ++f) nsmall += 2 * fr[f].nfreqs + fr[f].nocc + fr[f].nvir;
-    std::vector<double> hsmall(nsmall);
+    // Inputs reach the device through one upload of a pinned staging region
+    //   [small arrays (weights, freqs, eocc, evir per fragment) | FragDev[] | pre]
+    // and results come back in one download of
+    //   [energy (nfrag) | E(u) (sum F) | status (sum F)].
+    size_t nsmall = 0, nfreq_all = 0;
+    for (int f = 0; f < nfrag; ++f) {
+        nsmall += 2 * fr[f].nfreqs + fr[f].nocc + fr[f].nvir;
+        nfreq_all += fr[f].nfreqs;
+    }
+    auto pad = [](size_t b) { return ((b + 255) / 256) * 256; };
+    const size_t up_small = pad(8 * nsmall), up_fr = pad(sizeof(FragDev) * nfrag),
+                 up_pre = pad(sizeof(int64_t) * pre.size());
+    const size_t up_bytes = up_small + up_fr + up_pre;
+    const size_t dn_e = pad(8 * (size_t)nfrag), dn_eu = pad(8 * nfreq_all),
+                 dn_st = pad(4 * nfreq_all);
+    const size_t dn_bytes = dn_e + dn_eu + dn_st;
+    BRPA_TRY(pin_reserve(ctx, up_bytes + dn_bytes));
+    char *h_up = ctx->pin, *h_dn = ctx->pin + up_bytes;
+    double *hsmall = reinterpret_cast<double *>(h_up);
+    FragDev *hd = reinterpret_cast<FragDev *>(h_up + up_small);
+    int64_t *hpre = reinterpret_cast<int64_t *>(h_up + up_small + up_fr);
 
     // workspace layout (two passes: count, then assign)
-    std::vector<FragDev> hd(nfrag);
-    double *dsmall = nullptr;
+    char *d_up = nullptr, *d_dn = nullptr;
     auto layout = [&](Arena &ar) {
-        dsmall = ar.take<double>(nsmall);
-        size_t so = 0;
+        d_up = ar.take<char>(up_bytes);
+        d_dn = ar.take<char>(dn_bytes);
+        double *dsmall = reinterpret_cast<double *>(d_up);
+        double *d_energy = reinterpret_cast<double *>(d_dn);
+        double *d_eu = reinterpret_cast<double *>(d_dn + dn_e);
+        int *d_status = reinterpret_cast<int *>(d_dn + dn_e + dn_eu);
+        size_t so = 0, fo = 0;
         for (int f = 0; f < nfrag; ++f) {
             const HostFrag &h = fr[f];
-            FragDev &d = hd[f];
+            FragDev d{};
             const int64_t K = h.nocc * h.nvir, N = h.naux, F = h.nfreqs;
             d.K = (int)K;
             d.N = (int)N;
@@ -170,14 +201,15 @@ int run_chunk(brpa_ctx *ctx, int precision, const std::vector<HostFrag> &fr, dou
             d.Pi = ar.take<double>(F * N * N);
             d.diag = ar.take<double>(F * N);
             d.ldpart = ar.take<double>(F * nsteps_max * 2);
-            d.weights = dsmall ? dsmall + so : nullptr;
-            d.freqs = dsmall ? dsmall + so + F : nullptr;
-            d.eocc = dsmall ? dsmall + so + 2 * F : nullptr;
-            d.evir = dsmall ? dsmall + so + 2 * F + h.nocc : nullptr;
+            d.weights = d_up ? dsmall + so : nullptr;
+            d.freqs = d_up ? dsmall + so + F : nullptr;
+            d.eocc = d_up ? dsmall + so + 2 * F : nullptr;
+            d.evir = d_up ? dsmall + so + 2 * F + h.nocc : nullptr;
             so += 2 * F + h.nocc + h.nvir;
-            d.eu = ar.take<double>(F);
-            d.energy = ar.take<double>(1);
-            d.status = ar.take<int>(F);
+            d.eu = d_dn ? d_eu + fo : nullptr;
+            d.energy = d_dn ? d_energy + f : nullptr;
+            d.status = d_dn ? d_status + fo : nullptr;
+            fo += F;
             d.linv = ar.take<double>(F * CHOL_NB * CHOL_NB);
             d.Kpad = (int)(cdiv(K, SPLIT_KB) * SPLIT_KB);
             d.nblk = d.Kpad / SPLIT_KB;
@@ -185,99 +217,164 @@ int run_chunk(brpa_ctx *ctx, int precision, const std::vector<HostFrag> &fr, dou
                 d.sl = ar.take<uint16_t>((size_t)F * NSLICE * N * d.Kpad);
                 d.scale = ar.take<float>((size_t)F * d.nblk * N);
                 d.dpart = ar.take<double>((size_t)F * d.nblk * N * 2);
-            } else {
-                d.sl = nullptr;
-                d.scale = nullptr;
-                d.dpart = nullptr;
             }
+            d.Kpad8 = (int)(cdiv(K, I8_KB) * I8_KB);
+            if (precision == BRPA_SPLIT_I8) {
+                d.sdtab = ar.take<double>((size_t)F * K);
+                d.sl8 = ar.take<int8_t>((size_t)F * NSLICE * N * d.Kpad8);
+                d.scale8 = ar.take<double>((size_t)F * N);
+            }
+            if (ar.base) hd[f] = d;
         }
     };
     Arena count;
     layout(count);
-    const size_t desc_bytes = ((sizeof(FragDev) * nfrag + 255) / 256) * 256;
-    const size_t pre_bytes = ((sizeof(int64_t) * pre.size() + 255) / 256) * 256;
     Arena ar;
-    BRPA_TRY(ws_reserve(ctx, count.off + desc_bytes + pre_bytes, &ar));
-    FragDev *d_fr = ar.take<FragDev>(nfrag);
-    int64_t *d_pre = ar.take<int64_t>(pre.size());
+    BRPA_TRY(ws_reserve(ctx, count.off, &ar));
     layout(ar);
+    const FragDev *d_fr = reinterpret_cast<const FragDev *>(d_up + up_small);
+    const int64_t *d_pre = reinterpret_cast<const int64_t *>(d_up + up_small + up_fr);
 
-    size_t so = 0;
-    for (int f = 0; f < nfrag; ++f) {
-        const HostFrag &h = fr[f];
-        const FragDev &d = hd[f];
-        const int64_t K = h.nocc * h.nvir, N = h.naux, F = h.nfreqs;
-        if (d.R != h.R) BRPA_TRY(to_device(ctx, const_cast<double *>(d.R), h.R, 8 * K * N));
-        BRPA_TRY(host_copy(ctx, hsmall.data() + so, h.weights, F));
-        BRPA_TRY(host_copy(ctx, hsmall.data() + so + F, h.freqs, F));
-        BRPA_TRY(host_copy(ctx, hsmall.data() + so + 2 * F, h.eocc, h.nocc));
-        BRPA_TRY(host_copy(ctx, hsmall.data() + so + 2 * F + h.nocc, h.evir, h.nvir));
-        so += 2 * F + h.nocc + h.nvir;
-        BRPA_TRY(launch_fill_int(ctx, d.status, F, 0));
+    // host staging: small arrays, descriptors (filled by layout), prefix sums
+    bool all_dev = true;
+    {
+        size_t so = 0;
+        for (int f = 0; f < nfrag; ++f) {
+            const HostFrag &h = fr[f];
+            const int64_t F = h.nfreqs;
+            BRPA_TRY(host_copy(ctx, hsmall + so, h.weights, F));
+            BRPA_TRY(host_copy(ctx, hsmall + so + F, h.freqs, F));
+            BRPA_TRY(host_copy(ctx, hsmall + so + 2 * F, h.eocc, h.nocc));
+            BRPA_TRY(host_copy(ctx, hsmall + so + 2 * F + h.nocc, h.evir, h.nvir));
+            so += 2 * F + h.nocc + h.nvir;
+            if (hd[f].R != h.R) all_dev = false;
+        }
+        std::memcpy(hpre, pre.data(), sizeof(int64_t) * pre.size());
     }
-    BRPA_HIP(ctx, hipMemcpyAsync(dsmall, hsmall.data(), 8 * nsmall, hipMemcpyHostToDevice,
-                                 ctx->stream));
-    BRPA_HIP(ctx, hipMemcpyAsync(d_fr, hd.data(), sizeof(FragDev) * nfrag, hipMemcpyHostToDevice,
-                                 ctx->stream));
-    BRPA_HIP(ctx, hipMemcpyAsync(d_pre, pre.data(), sizeof(int64_t) * pre.size(),
-                                 hipMemcpyHostToDevice, ctx->stream));
     auto DP = [&](int s) { return d_pre + (size_t)s * (nfrag + 1); };
     auto TOT = [&](int s) { return P(s)[nfrag]; };
 
-    BRPA_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-    BRPA_TRY(stage_begin(ctx, BRPA_STAGE_AUX));
-    BRPA_TRY(launch_dtab(ctx, d_fr, DP(0), nfrag, TOT(0)));
-    BRPA_TRY(stage_end(ctx, BRPA_STAGE_AUX));
-    if (precision == BRPA_FP64) {
-        BRPA_TRY(stage_begin(ctx, BRPA_STAGE_PI));
-        BRPA_TRY(launch_syrk_f64(ctx, d_fr, DP(1), nfrag, TOT(1)));
-        BRPA_TRY(stage_end(ctx, BRPA_STAGE_PI));
-    } else {
-        BRPA_TRY(stage_begin(ctx, BRPA_STAGE_SPLIT));
-        BRPA_TRY(launch_split(ctx, d_fr, DP(3), nfrag, TOT(3)));
-        BRPA_TRY(stage_end(ctx, BRPA_STAGE_SPLIT));
-        BRPA_TRY(stage_begin(ctx, BRPA_STAGE_PI));
-        BRPA_TRY(launch_syrk_bf16(ctx, d_fr, DP(4), nfrag, TOT(4)));
-        BRPA_TRY(stage_end(ctx, BRPA_STAGE_PI));
-    }
-    BRPA_TRY(stage_begin(ctx, BRPA_STAGE_CHOL));
-    BRPA_TRY(launch_set_diag(ctx, d_fr, DP(2), nfrag, TOT(2), precision == BRPA_SPLIT_BF16));
-    for (int j = 0; j < nsteps_max; ++j) {
-        BRPA_TRY(launch_potrf_diag(ctx, d_fr, DP(5 + j), nfrag, TOT(5 + j), j));
-        BRPA_TRY(launch_trsm(ctx, d_fr, DP(5 + nsteps_max + j), nfrag, TOT(5 + nsteps_max + j), j));
-        BRPA_TRY(launch_chol_update(ctx, d_fr, DP(5 + 2 * nsteps_max + j), nfrag,
-                                    TOT(5 + 2 * nsteps_max + j), j));
-    }
-    BRPA_TRY(stage_end(ctx, BRPA_STAGE_CHOL));
-    BRPA_TRY(launch_energy(ctx, d_fr, nfrag));
-    BRPA_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    // the launch sequence; `timed` records the per-stage events (not in graphs)
+    auto enqueue = [&](bool timed) -> int {
+        auto sb = [&](int st) { return timed ? stage_begin(ctx, st) : BRPA_OK; };
+        auto se = [&](int st) { return timed ? stage_end(ctx, st) : BRPA_OK; };
+        BRPA_HIP(ctx, hipMemcpyAsync(d_up, h_up, up_bytes, hipMemcpyHostToDevice, ctx->stream));
+        BRPA_TRY(launch_fill_int(ctx, reinterpret_cast<int *>(d_dn + dn_e + dn_eu),
+                                 (int64_t)nfreq_all, 0));
+        BRPA_TRY(sb(BRPA_STAGE_AUX));
+        BRPA_TRY(launch_dtab(ctx, d_fr, DP(0), nfrag, TOT(0)));
+        BRPA_TRY(se(BRPA_STAGE_AUX));
+        if (precision == BRPA_FP64) {
+            BRPA_TRY(sb(BRPA_STAGE_PI));
+            BRPA_TRY(launch_syrk_f64(ctx, d_fr, DP(1), nfrag, TOT(1)));
+            BRPA_TRY(se(BRPA_STAGE_PI));
+        } else if (precision == BRPA_SPLIT_I8) {
+            BRPA_TRY(sb(BRPA_STAGE_SPLIT));
+            BRPA_TRY(launch_colscale_i8(ctx, d_fr, DP(5), nfrag, TOT(5)));
+            BRPA_TRY(launch_split_i8(ctx, d_fr, DP(6), nfrag, TOT(6)));
+            BRPA_TRY(se(BRPA_STAGE_SPLIT));
+            BRPA_TRY(sb(BRPA_STAGE_PI));
+            BRPA_TRY(launch_syrk_i8(ctx, d_fr, DP(7), nfrag, TOT(7)));
+            BRPA_TRY(se(BRPA_STAGE_PI));
+        } else {
+            BRPA_TRY(sb(BRPA_STAGE_SPLIT));
+            BRPA_TRY(launch_split(ctx, d_fr, DP(3), nfrag, TOT(3)));
+            BRPA_TRY(se(BRPA_STAGE_SPLIT));
+            BRPA_TRY(sb(BRPA_STAGE_PI));
+            BRPA_TRY(launch_syrk_bf16(ctx, d_fr, DP(4), nfrag, TOT(4)));
+            BRPA_TRY(se(BRPA_STAGE_PI));
+        }
+        BRPA_TRY(sb(BRPA_STAGE_CHOL));
+        const int diag_mode =
+            precision == BRPA_SPLIT_BF16 ? 1 : (precision == BRPA_SPLIT_I8 ? 2 : 0);
+        BRPA_TRY(launch_set_diag(ctx, d_fr, DP(2), nfrag, TOT(2), diag_mode));
+        for (int j = 0; j < nsteps_max; ++j) {
+            BRPA_TRY(launch_potrf_diag(ctx, d_fr, DP(NFIX + j), nfrag, TOT(NFIX + j), j));
+            BRPA_TRY(launch_trsm(ctx, d_fr, DP(NFIX + nsteps_max + j), nfrag,
+                                 TOT(NFIX + nsteps_max + j), j));
+            BRPA_TRY(launch_chol_update(ctx, d_fr, DP(NFIX + 2 * nsteps_max + j), nfrag,
+                                        TOT(NFIX + 2 * nsteps_max + j), j));
+        }
+        BRPA_TRY(se(BRPA_STAGE_CHOL));
+        BRPA_TRY(launch_energy(ctx, d_fr, nfrag));
+        BRPA_HIP(ctx, hipMemcpyAsync(h_dn, d_dn, dn_bytes, hipMemcpyDeviceToHost, ctx->stream));
+        return BRPA_OK;
+    };
 
-    std::vector<int> status;
-    for (int f = 0; f < nfrag; ++f) {
-        const FragDev &d = hd[f];
-        BRPA_TRY(from_device(ctx, energies + f, d.energy, sizeof(double)));
-        if (eu_out && eu_out[f]) BRPA_TRY(from_device(ctx, eu_out[f], d.eu, 8 * (size_t)d.F));
-    }
-    std::vector<std::vector<int>> st(nfrag);
-    for (int f = 0; f < nfrag; ++f) {
-        st[f].resize(hd[f].F);
-        BRPA_HIP(ctx, hipMemcpyAsync(st[f].data(), hd[f].status, sizeof(int) * hd[f].F,
-                                     hipMemcpyDeviceToHost, ctx->stream));
+    const bool graph = ctx->use_graphs && all_dev;
+    if (graph) {
+        // key: everything the captured addresses and launch shapes depend on
+        std::string key = std::to_string(precision) + "/" + std::to_string(nfrag) + "/" +
+                          std::to_string((uintptr_t)ctx->ws) + "/" +
+                          std::to_string((uintptr_t)ctx->pin) + "/" +
+                          std::to_string((uintptr_t)ctx->stream);
+        for (int f = 0; f < nfrag; ++f)
+            key += ":" + std::to_string(fr[f].nocc) + "," + std::to_string(fr[f].nvir) + "," +
+                   std::to_string(fr[f].naux) + "," + std::to_string(fr[f].nfreqs);
+        hipGraphExec_t exec = nullptr;
+        for (auto &g : ctx->graphs)
+            if (g.key == key) exec = g.exec;
+        if (!exec) {
+            BRPA_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+            int rc = enqueue(false);
+            hipGraph_t gr = nullptr;
+            hipError_t ce = hipStreamEndCapture(ctx->stream, &gr);
+            if (rc != BRPA_OK) {
+                if (gr) (void)hipGraphDestroy(gr);
+                return rc;
+            }
+            BRPA_HIP(ctx, ce);
+            hipError_t ie = hipGraphInstantiate(&exec, gr, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(gr);
+            BRPA_HIP(ctx, ie);
+            ctx->graphs.push_back({key, exec});
+            ctx->last_graph_captures += 1;
+        } else {
+            ctx->last_graph_replays += 1;
+        }
+        BRPA_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+        BRPA_HIP(ctx, hipGraphLaunch(exec, ctx->stream));
+        BRPA_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    } else {
+        // fragments whose R is a host array are copied into the workspace first
+        for (int f = 0; f < nfrag; ++f)
+            if (hd[f].R != fr[f].R)
+                BRPA_TRY(to_device(ctx, const_cast<double *>(hd[f].R), fr[f].R,
+                                   8 * (size_t)fr[f].nocc * fr[f].nvir * fr[f].naux));
+        BRPA_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+        BRPA_TRY(enqueue(true));
+        BRPA_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
     }
     BRPA_HIP(ctx, hipStreamSynchronize(ctx->stream));
     float ms = 0.f;
     BRPA_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     *kernel_ms += ms;
-    BRPA_TRY(stage_collect(ctx, BRPA_STAGE_AUX));
-    BRPA_TRY(stage_collect(ctx, BRPA_STAGE_PI));
-    if (precision != BRPA_FP64) BRPA_TRY(stage_collect(ctx, BRPA_STAGE_SPLIT));
-    BRPA_TRY(stage_collect(ctx, BRPA_STAGE_CHOL));
-    for (int f = 0; f < nfrag; ++f)
-        for (int w = 0; w < hd[f].F; ++w)
-            if (st[f][w])
+    if (!graph) {
+        BRPA_TRY(stage_collect(ctx, BRPA_STAGE_AUX));
+        BRPA_TRY(stage_collect(ctx, BRPA_STAGE_PI));
+        if (precision != BRPA_FP64) BRPA_TRY(stage_collect(ctx, BRPA_STAGE_SPLIT));
+        BRPA_TRY(stage_collect(ctx, BRPA_STAGE_CHOL));
+    }
+    const double *h_energy = reinterpret_cast<const double *>(h_dn);
+    const double *h_eu = reinterpret_cast<const double *>(h_dn + dn_e);
+    const int *h_status = reinterpret_cast<const int *>(h_dn + dn_e + dn_eu);
+    size_t fo = 0;
+    for (int f = 0; f < nfrag; ++f) {
+        energies[f] = h_energy[f];
+        if (eu_out && eu_out[f]) {
+            if (is_device_ptr(eu_out[f]))
+                BRPA_HIP(ctx, hipMemcpy(eu_out[f], h_eu + fo, 8 * (size_t)fr[f].nfreqs,
+                                        hipMemcpyHostToDevice));
+            else
+                std::memcpy(eu_out[f], h_eu + fo, 8 * (size_t)fr[f].nfreqs);
+        }
+        for (int w = 0; w < fr[f].nfreqs; ++w)
+            if (h_status[fo + w])
                 return brpa_fail(ctx, BRPA_ERR_NOT_SPD,
                                  "fragment %d, frequency %d: 1 + Pi(u) is not positive definite",
                                  f, w);
+        fo += fr[f].nfreqs;
+    }
     return BRPA_OK;
 }
 
@@ -285,15 +382,30 @@ int run_chunk(brpa_ctx *ctx, int precision, const std::vector<HostFrag> &fr, dou
 // workspace stays under the budget (BRPA_WS_BUDGET_GB, default 64 GiB).
 int run_ecrpa(brpa_ctx *ctx, int precision, const std::vector<HostFrag> &fr, double *energies,
               double *const *eu_out) {
-    if (precision != BRPA_FP64 && precision != BRPA_SPLIT_BF16)
+    if (precision != BRPA_FP64 && precision != BRPA_SPLIT_BF16 && precision != BRPA_SPLIT_I8)
         return brpa_fail(ctx, BRPA_ERR_ARG, "unknown precision %d", precision);
-    for (size_t f = 0; f < fr.size(); ++f) BRPA_TRY(check_frag(ctx, fr[f], (int64_t)f));
+    for (size_t f = 0; f < fr.size(); ++f) {
+        BRPA_TRY(check_frag(ctx, fr[f], (int64_t)f));
+        if (precision == BRPA_SPLIT_I8 && fr[f].nocc * fr[f].nvir > I8_KMAX)
+            return brpa_fail(ctx, BRPA_ERR_ARG,
+                             "fragment %lld: nocc*nvir = %lld exceeds %d, the largest K with "
+                             "exact int32 sums in the split-int8 form",
+                             (long long)f, (long long)(fr[f].nocc * fr[f].nvir), I8_KMAX);
+    }
     BRPA_HIP(ctx, hipSetDevice(ctx->device));
-    double budget_gb = 64.0;
-    if (const char *e = std::getenv("BRPA_WS_BUDGET_GB")) budget_gb = std::atof(e);
-    const size_t budget = (size_t)(budget_gb * (1ull << 30));
+    // workspace budget: BRPA_WS_BUDGET_GB, else 3/4 of the device memory that is
+    // free now plus what the context already holds
+    size_t budget;
+    if (const char *e = std::getenv("BRPA_WS_BUDGET_GB")) {
+        budget = (size_t)(std::atof(e) * (1ull << 30));
+    } else {
+        size_t free_b = 0, total_b = 0;
+        BRPA_HIP(ctx, hipMemGetInfo(&free_b, &total_b));
+        budget = (free_b + ctx->ws_cap) / 4 * 3;
+    }
     double ms = 0.0;
     stage_reset(ctx);
+    ctx->last_graph_captures = ctx->last_graph_replays = 0;
     size_t i = 0;
     while (i < fr.size()) {
         size_t j = i, bytes = 0;

@@ -28,7 +28,22 @@ struct brpa_ctx {
     double *t2_A = nullptr, *t2_V = nullptr;
     size_t t2_A_cap = 0, t2_V_cap = 0;
     int64_t t2_nt = -1, t2_nai = 0;
+    // pinned host staging of the batched E_RPA path (uploads and results)
+    char *pin = nullptr;
+    size_t pin_cap = 0;
+    // hipGraph replay of the batched E_RPA launch sequence (brpa_ctx_set_graphs)
+    int use_graphs = 0;
+    struct GraphEntry {
+        std::string key;
+        hipGraphExec_t exec = nullptr;
+    };
+    std::vector<GraphEntry> graphs;
+    int last_graph_replays = 0, last_graph_captures = 0;
 };
+
+// Pinned host buffer of at least `bytes` owned by the context.
+int pin_reserve(brpa_ctx *ctx, size_t bytes);
+void graphs_release(brpa_ctx *ctx);
 
 void t2_release(brpa_ctx *ctx);
 
@@ -103,19 +118,27 @@ struct FragDev {
     uint16_t *sl;            // F x NSLICE x N x Kpad, per column P the Kpad rows contiguous
     float *scale;            // F x nblk x N, block scale 2^(e-8)
     int Kpad, nblk;
+    // split-int8 slices (only when precision == BRPA_SPLIT_I8)
+    const double *sdtab;     // F x K, sqrt(c_w(k))
+    int8_t *sl8;             // F x NSLICE x N x Kpad8, per column P the Kpad8 rows contiguous
+    double *scale8;          // F x N, column scale max_k |X_w[k][P]| / 127
+    int Kpad8;
 };
 
 constexpr int CHOL_NB = 64;   // Cholesky block size
 constexpr int SYRK_T = 64;    // FP64 SYRK tile
 constexpr int SPLIT_KB = 128; // split-BF16 exactness block (rows per scale)
-constexpr int NSLICE = 4;     // bf16 slices per FP64 value in the split-BF16 form
+constexpr int NSLICE = 4;     // bf16 (or int8) slices per FP64 value in the split forms
+constexpr int I8_KB = 128;    // split-int8: rows per split work-group (Kpad8 is a multiple)
+constexpr int I8_KMAX = 65536;// split-int8: largest K with exact int32 level sums
 
 // Launch wrappers (defined in the kernel files).  `pre` arrays are device
 // prefix sums over fragments of the number of work-groups per fragment.
 int launch_dtab(brpa_ctx *, const FragDev *, const int64_t *pre, int nfrag, int64_t total);
 int launch_syrk_f64(brpa_ctx *, const FragDev *, const int64_t *pre, int nfrag, int64_t total);
+// mode: 0 = Pi's own diagonal, 1 = split-BF16 partials, 2 = diag precomputed
 int launch_set_diag(brpa_ctx *, const FragDev *, const int64_t *pre, int nfrag, int64_t total,
-                    int from_parts);
+                    int mode);
 int launch_potrf_diag(brpa_ctx *, const FragDev *, const int64_t *pre, int nfrag, int64_t total, int step);
 int launch_trsm(brpa_ctx *, const FragDev *, const int64_t *pre, int nfrag, int64_t total, int step);
 int launch_chol_update(brpa_ctx *, const FragDev *, const int64_t *pre, int nfrag, int64_t total, int step);
@@ -123,6 +146,9 @@ int launch_energy(brpa_ctx *, const FragDev *, int nfrag);
 int launch_fill_int(brpa_ctx *, int *p, int64_t n, int v);
 int launch_split(brpa_ctx *, const FragDev *, const int64_t *pre, int nfrag, int64_t total);
 int launch_syrk_bf16(brpa_ctx *, const FragDev *, const int64_t *pre, int nfrag, int64_t total);
+int launch_colscale_i8(brpa_ctx *, const FragDev *, const int64_t *pre, int nfrag, int64_t total);
+int launch_split_i8(brpa_ctx *, const FragDev *, const int64_t *pre, int nfrag, int64_t total);
+int launch_syrk_i8(brpa_ctx *, const FragDev *, const int64_t *pre, int nfrag, int64_t total);
 
 // Generic single-matrix FP64 SYRK used by brpa_gpiug_mo:
 //   C(N,N) (+)= X^T diag(d) X, X K x N row-major, both triangles written.

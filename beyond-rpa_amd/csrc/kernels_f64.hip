@@ -38,7 +38,9 @@ __global__ __launch_bounds__(256) void k_dtab(const FragDev *__restrict__ fr,
     int i = k / F.nvir, a = k - i * F.nvir;
     double d = F.evir[a] - F.eocc[i];
     double u = F.freqs[w];
-    const_cast<double *>(F.dtab)[e] = 4.0 * d / (d * d + u * u);
+    const double c = 4.0 * d / (d * d + u * u);
+    const_cast<double *>(F.dtab)[e] = c;
+    if (F.sdtab) const_cast<double *>(F.sdtab)[e] = sqrt(c);
 }
 
 int launch_dtab(brpa_ctx *ctx, const FragDev *fr, const int64_t *pre, int nfrag, int64_t total) {
@@ -208,7 +210,7 @@ int launch_syrk_single(brpa_ctx *ctx, const double *X, const double *d, int K, i
 // Work-groups per fragment: ceil(F*N/256).
 __global__ __launch_bounds__(256) void k_set_diag(const FragDev *__restrict__ fr,
                                                  const int64_t *__restrict__ pre, int nfrag,
-                                                 int from_parts) {
+                                                 int mode) {
     int64_t b = blockIdx.x;
     int f = find_frag(pre, nfrag, b);
     const FragDev &F = fr[f];
@@ -216,14 +218,18 @@ __global__ __launch_bounds__(256) void k_set_diag(const FragDev *__restrict__ fr
     if (e >= (int64_t)F.F * F.N) return;
     int w = (int)(e / F.N), P = (int)(e - (int64_t)w * F.N);
     double *pp = F.Pi + (int64_t)w * F.N * F.N + (int64_t)P * F.N + P;
+    // mode 0: Pi's own diagonal; 1: sum of the split-BF16 double-double block
+    // partials; 2: diag already holds the exact value (split-int8 column pass)
     double d;
-    if (from_parts) {
+    if (mode == 1) {
         dd_t s = {0.0, 0.0};
         for (int blk = 0; blk < F.nblk; ++blk) {
             const double *q = F.dpart + (((int64_t)w * F.nblk + blk) * F.N + P) * 2;
             s = dd_add(s, (dd_t){q[0], q[1]});
         }
         d = s.hi + s.lo;
+    } else if (mode == 2) {
+        d = F.diag[e];
     } else {
         d = *pp;
     }
@@ -232,10 +238,10 @@ __global__ __launch_bounds__(256) void k_set_diag(const FragDev *__restrict__ fr
 }
 
 int launch_set_diag(brpa_ctx *ctx, const FragDev *fr, const int64_t *pre, int nfrag, int64_t total,
-                    int from_parts) {
+                    int mode) {
     if (total <= 0) return BRPA_OK;
     hipLaunchKernelGGL(k_set_diag, dim3(grid1(total)), dim3(256), 0, ctx->stream, fr, pre, nfrag,
-                       from_parts);
+                       mode);
     LAUNCH_CHECK(ctx);
     return BRPA_OK;
 }
@@ -277,6 +283,50 @@ __device__ __forceinline__ void mma64_xyt(const double (*sX)[CHOL_NB + 1],
     }
 }
 
+// Loads rows [row0, row0 + ROWS) x columns [col0, col0 + 64) of a row-major
+// matrix (leading dimension ld) into an LDS panel; rows >= rowmax and columns
+// >= colmax read as 0.
+// Every load is issued before the first LDS store (the loop is unrolled into
+// registers), so the work-group waits for one memory round trip, not ROWS*64/THREADS.
+// Out-of-range rows load from A[0] (always valid) and are then zeroed.
+typedef __attribute__((address_space(1))) const double gdouble_t;
+typedef __attribute__((address_space(1))) double gdouble_mut_t;
+
+template <int ROWS, int THREADS, int NP>
+__device__ __forceinline__ void load_panels64(double (*const *s)[CHOL_NB + 1],
+                                              const double *__restrict__ A, int64_t ld,
+                                              const int *row0, int rowmax, int col0, int colmax,
+                                              int tid) {
+    constexpr int PER = ROWS * CHOL_NB / THREADS;
+    constexpr int RSTEP = THREADS / CHOL_NB;
+    const int c = tid & (CHOL_NB - 1), r0 = tid / CHOL_NB;
+    const gdouble_t *G = (const gdouble_t *)A;
+    double v[NP][PER];
+#pragma unroll
+    for (int p = 0; p < NP; ++p)
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int r = r0 + RSTEP * u;
+            const bool ok = row0[p] + r < rowmax && col0 + c < colmax;
+            const int64_t idx = ok ? (int64_t)(row0[p] + r) * ld + col0 + c : 0;
+            v[p][u] = G[idx];
+            v[p][u] = ok ? v[p][u] : 0.0;
+        }
+#pragma unroll
+    for (int p = 0; p < NP; ++p)
+#pragma unroll
+        for (int u = 0; u < PER; ++u) s[p][r0 + RSTEP * u][c] = v[p][u];
+}
+
+template <int ROWS, int THREADS>
+__device__ __forceinline__ void load_panel64(double (*s)[CHOL_NB + 1], const double *__restrict__ A,
+                                             int64_t ld, int row0, int rowmax, int col0, int colmax,
+                                             int tid) {
+    double(*sp[1])[CHOL_NB + 1] = {s};
+    const int r[1] = {row0};
+    load_panels64<ROWS, THREADS, 1>(sp, A, ld, r, rowmax, col0, colmax, tid);
+}
+
 __global__ __launch_bounds__(64) void k_potrf_diag(const FragDev *__restrict__ fr,
                                                   const int64_t *__restrict__ pre, int nfrag,
                                                   int step) {
@@ -294,11 +344,12 @@ __global__ __launch_bounds__(64) void k_potrf_diag(const FragDev *__restrict__ f
     const int nbj = min(CHOL_NB, N - j0);
     const double *A = F.Pi + (int64_t)w * N * N;
     const int lane = threadIdx.x;
-    // lane = row; rows/cols past nbj are padded with the identity
+    // rows/cols past nbj are padded with the identity; upper triangle zero
+    load_panel64<CHOL_NB, 64>(L, A, N, j0, j0 + nbj, j0, j0 + nbj, lane);
+    __syncthreads();
     for (int c = 0; c < CHOL_NB; ++c) {
-        double v = (lane == c) ? 1.0 : 0.0;
-        if (lane < nbj && c < nbj && c <= lane) v = A[(int64_t)(j0 + lane) * N + j0 + c];
-        L[lane][c] = (c <= lane) ? v : 0.0;
+        const double v = L[lane][c];
+        L[lane][c] = (lane < nbj && c < nbj) ? (c <= lane ? v : 0.0) : (lane == c ? 1.0 : 0.0);
     }
     __syncthreads();
     int bad = 0;
@@ -370,11 +421,8 @@ __global__ __launch_bounds__(256) void k_trsm(const FragDev *__restrict__ fr,
     const double *linv = F.linv + (int64_t)w * CHOL_NB * CHOL_NB;
     const int R0 = j0 + CHOL_NB * r;
     const int tid = threadIdx.x;
-    for (int e = tid; e < CHOL_NB * CHOL_NB; e += 256) {
-        int rr = e / CHOL_NB, cc = e % CHOL_NB;
-        sB[rr][cc] = (R0 + rr < N) ? A[(int64_t)(R0 + rr) * N + j0 + cc] : 0.0;
-        sL[rr][cc] = linv[e];
-    }
+    load_panel64<CHOL_NB, 256>(sB, A, N, R0, N, j0, N, tid);
+    load_panel64<CHOL_NB, 256>(sL, linv, CHOL_NB, 0, CHOL_NB, 0, CHOL_NB, tid);
     __syncthreads();
     d4_t acc[2][2];
     mma64_xyt(sB, sL, acc);   // L21[i][c] = sum_k A21[i][k] Linv[c][k]
@@ -426,10 +474,10 @@ __global__ __launch_bounds__(512) void k_chol_update(const FragDev *__restrict__
     double *A = F.Pi + (int64_t)w * N * N;
     const int Ri = t0 + UPD_T * ti, Rj = t0 + UPD_T * tj;
     const int tid = threadIdx.x;
-    for (int e = tid; e < UPD_T * CHOL_NB; e += 512) {
-        int rr = e / CHOL_NB, cc = e % CHOL_NB;
-        sI[rr][cc] = (Ri + rr < N) ? A[(int64_t)(Ri + rr) * N + j0 + cc] : 0.0;
-        sJ[rr][cc] = (Rj + rr < N) ? A[(int64_t)(Rj + rr) * N + j0 + cc] : 0.0;
+    {
+        double(*sp[2])[CHOL_NB + 1] = {sI, sJ};
+        const int rows[2] = {Ri, Rj};
+        load_panels64<UPD_T, 512, 2>(sp, A, N, rows, N, j0, N, tid);
     }
     __syncthreads();
     const int wave = tid >> 6, lane = tid & 63;
@@ -456,16 +504,29 @@ __global__ __launch_bounds__(512) void k_chol_update(const FragDev *__restrict__
             for (int j = 0; j < 2; ++j)
                 acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], bv[j], acc[i][j], 0, 0, 0);
     }
+    // read-modify-write: all 32 reads first (out-of-range ones from A[0]), then the stores
+    double old[4][2][4];
+    int64_t idx[4][2][4];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j)
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                int row = Ri + r0 + 16 * i + (lane >> 4) + 4 * q;
-                int col = Rj + c0 + 16 * j + (lane & 15);
-                if (row < N && col < N && col <= row) A[(int64_t)row * N + col] -= acc[i][j][q];
+                const int row = Ri + r0 + 16 * i + (lane >> 4) + 4 * q;
+                const int col = Rj + c0 + 16 * j + (lane & 15);
+                const bool ok = row < N && col < N && col <= row;
+                idx[i][j][q] = ok ? (int64_t)row * N + col : -1;
+                old[i][j][q] = ((const gdouble_t *)A)[ok ? idx[i][j][q] : 0];
             }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (idx[i][j][q] >= 0)
+                    ((gdouble_mut_t *)A)[idx[i][j][q]] = old[i][j][q] - acc[i][j][q];
 }
 
 int launch_chol_update(brpa_ctx *ctx, const FragDev *fr, const int64_t *pre, int nfrag,

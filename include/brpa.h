@@ -47,11 +47,16 @@ enum {
 /* Arithmetic of the Pi and amplitude contractions.  BRPA_FP64 is the drop-in. */
 enum {
     BRPA_FP64 = 0,
-    BRPA_SPLIT_BF16 = 1      /* 4 fixed-point bf16 slices per value, the 10 slice
+    BRPA_SPLIT_BF16 = 1,     /* 4 fixed-point bf16 slices per value, the 10 slice
                                 products of level <= 3 on bf16 MFMA with FP32
                                 accumulation (levels 0 and 1 exact per 128-row block),
                                 FP64 between blocks; exact FP64 diagonal and trace;
                                 FP64 Cholesky log-det and quadrature */
+    BRPA_SPLIT_I8 = 2        /* 4 fixed-point int8 slices per value (scale per column
+                                over the whole K), the 10 slice products of level <= 3
+                                on int8 MFMA with exact int32 accumulation (K <= 65536),
+                                one FP64 combination per element; exact FP64 diagonal
+                                and trace; FP64 Cholesky log-det and quadrature */
 };
 
 /* ---- context ------------------------------------------------------------ */
@@ -63,6 +68,16 @@ const char *brpa_last_error(const brpa_ctx *ctx);
 int brpa_ctx_set_stream(brpa_ctx *ctx, void *hip_stream);
 /* Release the device workspace (it grows on demand and is kept otherwise). */
 int brpa_ctx_trim(brpa_ctx *ctx);
+/* hipGraph replay of the batched E_RPA launch sequence (no counterpart in the
+ * reference, which has no GPU path).  When enabled and every fragment's R is
+ * device-resident, brpa_ecrpa_batch captures the whole launch sequence of a
+ * workspace chunk (uploads from pinned staging, all kernels, result download)
+ * once per (precision, fragment shapes, workspace address) and replays it on
+ * later calls; inputs are refreshed through the staging buffer every call.
+ * Per-stage timings are not recorded in replayed calls. */
+int brpa_ctx_set_graphs(brpa_ctx *ctx, int enable);
+/* Graphs captured and replayed by the last brpa_ecrpa_batch call. */
+int brpa_last_graph_stats(const brpa_ctx *ctx, int *captures, int *replays);
 /* Device time in milliseconds of the last hot-path kernel set of the last call
  * (HIP events recorded on the context's stream around the kernels only). */
 double brpa_last_kernel_ms(const brpa_ctx *ctx);

@@ -1,7 +1,7 @@
 """GPU parity of the E_RPA hot path against the CPU oracle (through the C ABI).
 
-Tolerances: 1e-8 Ha for the FP64 form, 1e-7 Ha for the split-BF16 form
-(absolute, per fragment), relative 1e-12 for FP64 Pi(u) elements.
+Tolerances: 1e-8 Ha for the FP64 form, 1e-7 Ha for the split forms (bf16 and
+int8 slices; absolute, per fragment), relative 1e-12 for FP64 Pi(u) elements.
 """
 import numpy as np
 import pytest
@@ -13,6 +13,7 @@ pytestmark = pytest.mark.gpu
 
 TOL_FP64 = 1e-8
 TOL_SPLIT = 1e-7
+SPLITS = ("split_bf16", "split_i8")
 
 
 def _frag_and_grid(o, v, n, nf, seed):
@@ -55,14 +56,36 @@ def test_ecrpa_fp64_matches_oracle(ctx, o, v, n, nf):
     assert np.max(np.abs(eu - np.array(eu_ref))) <= 1e-8
 
 
+@pytest.mark.parametrize("prec", SPLITS)
 @pytest.mark.parametrize("o,v,n,nf", [(8, 72, 400, 16), (3, 11, 70, 8), (6, 45, 257, 10),
-                                      (12, 100, 500, 16)])
-def test_ecrpa_split_bf16_matches_oracle(ctx, o, v, n, nf):
+                                      (12, 100, 500, 16), (1, 3, 200, 8), (2, 9, 3, 8)])
+def test_ecrpa_split_matches_oracle(ctx, prec, o, v, n, nf):
     f, x, w = _frag_and_grid(o, v, n, nf, 20261202)
     e_ref = erpa.ecrpa(f.R, f.gaps(), x, w)
-    e, _ = ctx.ecrpa(f.R, f.eocc, f.evir, x, w, "split_bf16")
-    print("split o=%d v=%d n=%d: E=%.12f ref=%.12f dE=%.2e" % (o, v, n, e, e_ref, e - e_ref))
+    e, _ = ctx.ecrpa(f.R, f.eocc, f.evir, x, w, prec)
+    print("%s o=%d v=%d n=%d: E=%.12f ref=%.12f dE=%.2e" % (prec, o, v, n, e, e_ref, e - e_ref))
     assert abs(e - e_ref) <= TOL_SPLIT
+
+
+def test_ecrpa_single_large_fragment_both_splits(ctx):
+    """o=36, v=300, N=1800, 24 frequencies: both split forms within 1e-7 Ha
+    of the oracle on the single large fragment."""
+    f, x, w = _frag_and_grid(36, 300, 1800, 24, 20261202)
+    e_ref = erpa.ecrpa(f.R, f.gaps(), x, w)
+    for prec in SPLITS:
+        e, _ = ctx.ecrpa(f.R, f.eocc, f.evir, x, w, prec)
+        print("%s large: E=%.10f dE=%.2e" % (prec, e, e - e_ref))
+        assert abs(e - e_ref) <= TOL_SPLIT
+
+
+def test_split_i8_rejects_k_beyond_exact_range(ctx):
+    import brpa
+    o, v, n = 2, 40000, 4
+    f = synth.make_fragment(o, v, n, 3)
+    x, w, _ = grid.freq_grid(f.eocc, f.evir, 8)
+    with pytest.raises(brpa.BrpaError) as ei:
+        ctx.ecrpa(f.R, f.eocc, f.evir, x, w, "split_i8")
+    assert ei.value.code == brpa._lib.ERR_ARG
 
 
 def test_batch_uneven_fragments(ctx):
@@ -73,10 +96,12 @@ def test_batch_uneven_fragments(ctx):
         frags.append((f.R, f.eocc, f.evir, x, w))
         refs.append(erpa.ecrpa(f.R, f.gaps(), x, w))
     e64 = ctx.ecrpa_batch(frags, "fp64")
-    e16 = ctx.ecrpa_batch(frags, "split_bf16")
-    print("batch dE fp64", e64 - refs, "split", e16 - refs)
+    print("batch dE fp64", e64 - refs)
     assert np.max(np.abs(e64 - refs)) <= TOL_FP64
-    assert np.max(np.abs(e16 - refs)) <= TOL_SPLIT
+    for prec in SPLITS:
+        es = ctx.ecrpa_batch(frags, prec)
+        print("batch dE", prec, es - refs)
+        assert np.max(np.abs(es - refs)) <= TOL_SPLIT
     # the batched result equals the single-fragment call bit for bit
     e_single = [ctx.ecrpa(*fr, precision="fp64")[0] for fr in frags]
     assert np.array_equal(e64, np.array(e_single))
@@ -93,9 +118,9 @@ def test_device_resident_inputs(ctx):
 
 def test_deterministic(ctx):
     f, x, w = _frag_and_grid(5, 50, 300, 12, 9)
-    a = [ctx.ecrpa(f.R, f.eocc, f.evir, x, w, p)[0] for p in ("fp64", "fp64", "split_bf16",
-                                                               "split_bf16")]
-    assert a[0] == a[1] and a[2] == a[3]
+    a = [ctx.ecrpa(f.R, f.eocc, f.evir, x, w, p)[0]
+         for p in ("fp64", "fp64", "split_bf16", "split_bf16", "split_i8", "split_i8")]
+    assert a[0] == a[1] and a[2] == a[3] and a[4] == a[5]
 
 
 def test_error_codes(ctx):
@@ -130,7 +155,7 @@ def test_batch_many_fragments_cholesky_steps(ctx):
         f, x, w = _frag_and_grid(o, v, n, 8, 500 + i)
         frags.append((f.R, f.eocc, f.evir, x, w))
         refs.append(erpa.ecrpa(f.R, f.gaps(), x, w))
-    for prec, tol in (("fp64", TOL_FP64), ("split_bf16", TOL_SPLIT)):
+    for prec, tol in (("fp64", TOL_FP64), ("split_bf16", TOL_SPLIT), ("split_i8", TOL_SPLIT)):
         e = ctx.ecrpa_batch(frags, prec)
         print(prec, "max |dE|", np.max(np.abs(e - refs)))
         assert np.max(np.abs(e - refs)) <= tol

@@ -1,0 +1,71 @@
+"""hipGraph replay of the batched E_RPA launch sequence (brpa_ctx_set_graphs).
+
+Every replay is compared bit for bit with the eager (non-graph) result of the
+same inputs, with eager library work, torch work and synchronisations between
+the replays, and with new input values of the same shapes (the replay must
+read the refreshed staging, not the captured values).
+"""
+import numpy as np
+import pytest
+
+from brpa import synth
+from oracle import erpa, grid
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(3, 20, 60), (8, 72, 400), (5, 33, 129), (2, 5, 7), (7, 61, 300)]
+
+
+def _batch(torch, scale=1.0, seed0=100):
+    out, refs = [], []
+    for i, (o, v, n) in enumerate(SHAPES):
+        f = synth.make_fragment(o, v, n, seed0 + i)
+        x, w, _ = grid.freq_grid(f.eocc, f.evir, 8 + 2 * i)
+        R = f.R * scale
+        out.append((torch.from_numpy(R).cuda(), f.eocc, f.evir, x, w))
+        refs.append((R, f.gaps(), x, w))
+    return out, refs
+
+
+@pytest.mark.parametrize("prec", ["fp64", "split_bf16", "split_i8"])
+def test_graph_replays_match_eager(ctx, prec):
+    torch = pytest.importorskip("torch")
+    frags, refs = _batch(torch)
+    frags2, refs2 = _batch(torch, scale=1.05, seed0=300)
+    ctx.set_graphs(False)
+    eager = ctx.ecrpa_batch(frags, prec)
+    eager2 = ctx.ecrpa_batch(frags2, prec)
+    ctx.set_graphs(True)
+    try:
+        for rep in range(4):
+            e = ctx.ecrpa_batch(frags, prec)
+            cap, rpl = ctx.graph_stats()
+            assert (cap, rpl) == ((1, 0) if rep == 0 else (0, 1))
+            assert np.array_equal(e, eager), (rep, e - eager)
+            # eager work between replays: a host-R call (not graphed) and torch work
+            f = frags[1]
+            ctx.ecrpa(f[0].cpu().numpy(), f[1], f[2], f[3], f[4], prec)
+            t = torch.randn(512, 512, device="cuda", dtype=torch.float64)
+            (t @ t).sum().item()
+            torch.cuda.synchronize()
+            # same shapes, new values: replayed, reads the new staging
+            e2 = ctx.ecrpa_batch(frags2, prec)
+            assert ctx.graph_stats() == (0, 1)
+            assert np.array_equal(e2, eager2), (rep, e2 - eager2)
+    finally:
+        ctx.set_graphs(False)
+    tol = 1e-8 if prec == "fp64" else 1e-7
+    for (R, g, x, w), ev in zip(refs2, eager2):
+        assert abs(ev - erpa.ecrpa(R, g, x, w)) <= tol
+
+
+def test_graph_not_used_for_host_inputs(ctx):
+    f = synth.make_fragment(4, 30, 100, 5)
+    x, w, _ = grid.freq_grid(f.eocc, f.evir, 8)
+    ctx.set_graphs(True)
+    try:
+        e = ctx.ecrpa_batch([(f.R, f.eocc, f.evir, x, w)], "fp64")
+        assert ctx.graph_stats() == (0, 0)
+    finally:
+        ctx.set_graphs(False)
+    assert abs(e[0] - erpa.ecrpa(f.R, f.gaps(), x, w)) <= 1e-8
