@@ -6,8 +6,14 @@ Workload: 256 uneven fragments, o ~ U{8..40}, v ~ U{70..350},
 N = round(4.5 (o+v)), 16 frequencies each, seeded synthetic inputs
 (brpa.synth).  One step = E_RPA of all 256 fragments in batched launches
 (Pi(u) build, exact diagonal/trace, blocked FP64 Cholesky log-det,
-quadrature), with R already resident in HBM.  The frequency grids (host
-code, independent of R) are built once before timing.
+quadrature), with R already resident in HBM, replayed as a hipGraph per
+workspace chunk (one eager pass supplies the per-stage kernel times).  The
+frequency grids (host code, independent of R) are built once before timing.
+
+Exchange correction: the factorised direct-ring amplitudes (T2) and the SOSEX
+contraction are timed per fragment on the first fragments of the same batch
+(bounded by --exchange-seconds), and the E_RPA + exchange rate is
+1 / (E_RPA time per fragment + mean exchange time per fragment).
 
 Prints one JSON line.  `roofline` is measured for the dominant kernel with
 HIP events inside the library over the timed steps; `cpu_baseline` times the
@@ -59,6 +65,10 @@ def main():
                     help="CPU oracle sample budget (0 disables the CPU baseline)")
     ap.add_argument("--compare-fp64", type=int, default=1,
                     help="also time the other forms (reported under `forms`)")
+    ap.add_argument("--graphs", type=int, default=1,
+                    help="replay the batched launch sequence as a hipGraph (1) or launch eagerly")
+    ap.add_argument("--exchange-seconds", type=float, default=20.0,
+                    help="time budget of the T2 + SOSEX sample (0 skips the exchange part)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy 5 times (reference for the "
                          "FETCH_SIZE/WRITE_SIZE counter scale)")
@@ -99,20 +109,24 @@ def main():
     ctx = brpa.Context(0)
 
     def run(prec, warmup, steps):
+        # one eager pass: per-stage kernel times (HIP events inside the library)
+        ctx.set_graphs(False)
+        e_eager = ctx.ecrpa_batch(batch, prec)
+        stage = {k: v[0] for k, v in ctx.last_stage_ms().items()}
+        launches = {k: v[1] for k, v in ctx.last_stage_ms().items()}
+        ctx.set_graphs(bool(a.graphs))
         for _ in range(warmup):
             e = ctx.ecrpa_batch(batch, prec)
         torch.cuda.synchronize()
-        stage = {k: 0.0 for k in ("pi", "split", "chol", "aux")}
-        launches = {k: 0 for k in stage}
         t = time.perf_counter()
         for _ in range(steps):
             e = ctx.ecrpa_batch(batch, prec)
-            for k, (ms, n) in ctx.last_stage_ms().items():
-                stage[k] += ms
-                launches[k] += n
         torch.cuda.synchronize()
         dt = (time.perf_counter() - t) / steps
-        return e, dt, {k: stage[k] / steps for k in stage}, {k: launches[k] // steps for k in stage}
+        if not np.array_equal(e, e_eager):
+            raise RuntimeError("%s: replayed energies differ from the eager pass" % prec)
+        ctx.set_graphs(False)
+        return e, dt, stage, launches
 
     e, dt, stage_ms, stage_n = run(a.precision, a.warmup, a.steps)
     print("# %s: %.1f ms/step stages %s" % (a.precision, dt * 1e3, stage_ms), file=sys.stderr,
@@ -141,6 +155,34 @@ def main():
                 "kernel": kernel, "kernel_ms_per_step": round(stage_ms[dom], 3),
                 "launches_per_step": stage_n[dom]}
 
+    exch = None
+    if a.exchange_seconds > 0:
+        import math
+        ts, exs = [], []
+        t = time.perf_counter()
+        k = 0
+        while k < a.nfrag and (k == 0 or time.perf_counter() - t < a.exchange_seconds):
+            o, v, n = shapes[k]
+            f = frags[k]
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            A, V = ctx.t2(Rd[k], f.eocc, f.evir, grids[k][0], grids[k][1],
+                          nguess=math.ceil(3.0 * n), seed=synth.BASE_SEED + k, device=True)
+            ex = ctx.sosex(V, A, Rd[k], o, v)
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t1)
+            exs.append(ex)
+            del A, V
+            k += 1
+        t_ex = sum(ts) / len(ts)
+        exch = {"n_fragments_timed": k, "mean_ms_per_fragment": round(t_ex * 1e3, 3),
+                "precision": "fp64 (T2 amplitudes and SOSEX contraction)",
+                "nguess": "ceil(3.0 N)"}
+        for p, d in forms.items():
+            d["with_exchange_fps"] = 1.0 / (d["ms_per_step"] * 1e-3 / a.nfrag + t_ex)
+        print("# exchange: %d fragments, %.1f ms each" % (k, t_ex * 1e3), file=sys.stderr,
+              flush=True)
+
     cpu = None
     parity = None
     if a.cpu_seconds > 0:
@@ -167,7 +209,7 @@ def main():
 
     out = {
         "metric": "fragments/sec (E_RPA+exchange corr) split-BF16 vs FP64 at |ΔE|≤1e-7 Ha",
-        "value": round(a.nfrag / dt, 4),
+        "value": round(forms[a.precision]["with_exchange_fps"], 4) if exch else round(a.nfrag / dt, 4),
         "unit": "fragments/sec",
         "n_gpus": 1,
         "steps": a.steps,
@@ -180,10 +222,16 @@ def main():
         "data": "synthetic (seeded PCG64, brpa.synth)",
         "config": {"workload": "batch of %d uneven fragments, o~U{8..40}, v~U{70..350}, "
                                "N=round(4.5(o+v)), %d frequencies" % (a.nfrag, a.nfreqs),
-                   "precision": a.precision, "components": "E_RPA (exchange correction not "
-                   "in the batched step yet)", "algorithmic_tflop_per_step":
-                   round((tot_syrk + tot_chol) / 1e12, 3)},
-        "forms": {p: {"fragments_per_sec": round(a.nfrag / (d["ms_per_step"] * 1e-3), 4),
+                   "precision": a.precision,
+                   "components": ("E_RPA (batched, %s) + SOSEX exchange (T2 + contraction, fp64)"
+                                  % a.precision) if exch else "E_RPA only",
+                   "graphs": bool(a.graphs),
+                   "algorithmic_tflop_per_step": round((tot_syrk + tot_chol) / 1e12, 3)},
+        "erpa_only_fragments_per_sec": round(a.nfrag / dt, 4),
+        "exchange": exch,
+        "forms": {p: {"erpa_fragments_per_sec": round(a.nfrag / (d["ms_per_step"] * 1e-3), 4),
+                      "with_exchange_fragments_per_sec": (round(d["with_exchange_fps"], 4)
+                                                          if exch else None),
                       "ms_per_step": round(d["ms_per_step"], 3),
                       "stage_ms": {k: round(v, 3) for k, v in d["stage_ms"].items()}}
                   for p, d in forms.items()},

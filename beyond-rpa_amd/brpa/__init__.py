@@ -204,6 +204,38 @@ class Context:
                                          _ptr(weights), freqs.size, _ptr(e), _ptr(eu)))
         return float(e[0]), eu
 
+    def _frag_array(self, frags):
+        keep = []
+        arr = (_lib.brpa_frag_t * len(frags))()
+        for i, (R, eocc, evir, freqs, weights) in enumerate(frags):
+            eocc = np.ascontiguousarray(eocc, np.float64)
+            evir = np.ascontiguousarray(evir, np.float64)
+            freqs = np.ascontiguousarray(freqs, np.float64)
+            weights = np.ascontiguousarray(weights, np.float64)
+            keep.append((R, eocc, evir, freqs, weights))
+            nai, naux = R.shape
+            if nai != eocc.size * evir.size or freqs.size != weights.size:
+                raise ValueError("fragment %d: shape mismatch" % i)
+            arr[i] = _lib.brpa_frag_t(eocc.size, evir.size, naux, freqs.size, _ptr(R).value,
+                                      _ptr(eocc).value, _ptr(evir).value, _ptr(freqs).value,
+                                      _ptr(weights).value)
+        return arr, keep
+
+    def autotune(self, frags, precision="split_i8"):
+        """Time the product-kernel variants on these fragments; returns
+        {bucket: (chosen variant, [ms per variant])} for the buckets present."""
+        arr, keep = self._frag_array(frags)
+        self._sync_stream(*[k[0] for k in keep])
+        self._check(self._lib.brpa_autotune(self._h, _prec(precision), len(frags), arr))
+        out = {}
+        for b in range(3):
+            v = C.c_int(-1)
+            ms = np.zeros(3)
+            self._check(self._lib.brpa_autotune_result(self._h, b, C.byref(v), _ptr(ms)))
+            if v.value >= 0:
+                out[b] = (v.value, ms.tolist())
+        return out
+
     def ecrpa_batch(self, frags, precision="fp64"):
         """E_RPA of many fragments in batched launches.
 
@@ -230,9 +262,10 @@ class Context:
         return e
 
     def t2(self, R, eocc, evir, freqs, weights, lam=1.0, cutoff=1e-10, nguess=0, seed=0,
-           precision="fp64"):
+           precision="fp64", device=False):
         """Factorised direct-ring amplitudes (rpa_CC_T2).  Returns (A, V) with
-        A ascending (NT2,) and V as C-order V[ai][mu] (ov, NT2)."""
+        A ascending (NT2,) and V as C-order V[ai][mu] (ov, NT2); numpy arrays,
+        or float64 torch tensors on R's device when device=True."""
         nai, naux = R.shape
         eocc, evir = np.ascontiguousarray(eocc, np.float64), np.ascontiguousarray(evir, np.float64)
         freqs = np.ascontiguousarray(freqs, np.float64)
@@ -247,8 +280,14 @@ class Context:
                                               int(nguess), int(seed) & 0xFFFFFFFFFFFFFFFF,
                                               C.byref(nt)))
         k = nt.value
-        A = np.zeros(k)
-        V = np.zeros((nai, k))
+        if device:
+            import torch
+            dev = R.device if _is_torch(R) else torch.device("cuda", self.device)
+            A = torch.zeros(k, dtype=torch.float64, device=dev)
+            V = torch.zeros((nai, k), dtype=torch.float64, device=dev)
+        else:
+            A = np.zeros(k)
+            V = np.zeros((nai, k))
         if k:
             self._check(self._lib.brpa_t2_copyout(self._h, _ptr(A), _ptr(V)))
         return A, V

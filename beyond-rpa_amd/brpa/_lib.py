@@ -5,7 +5,8 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libbrpa.so")
 
-OK, ERR_ARG, ERR_HIP, ERR_NOT_SPD, ERR_GAP, ERR_NONFINITE, ERR_OOM, ERR_GRID, ERR_SOLVER = range(9)
+(OK, ERR_ARG, ERR_HIP, ERR_NOT_SPD, ERR_GAP, ERR_NONFINITE, ERR_OOM, ERR_GRID, ERR_SOLVER,
+ ERR_INTERNAL) = range(10)
 FP64, SPLIT_BF16, SPLIT_I8 = 0, 1, 2
 STAGE_PI, STAGE_SPLIT, STAGE_CHOL, STAGE_AUX = range(4)
 
@@ -40,6 +41,8 @@ SIGNATURES = {
     "brpa_ecrpa": (C.c_int, [_vp, C.c_int, _vp, _i64, _vp, _i64, _vp, _i64, _vp, _vp, _i64,
                              _vp, _vp]),
     "brpa_ecrpa_batch": (C.c_int, [_vp, C.c_int, _i64, C.POINTER(brpa_frag_t), _vp]),
+    "brpa_autotune": (C.c_int, [_vp, C.c_int, _i64, C.POINTER(brpa_frag_t)]),
+    "brpa_autotune_result": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int), _vp]),
     "brpa_t2_compute": (C.c_int, [_vp, C.c_int, _vp, _i64, _vp, _i64, _vp, _i64, _vp, _vp, _i64,
                                   C.c_double, C.c_double, _i64, C.c_uint64, C.POINTER(_i64)]),
     "brpa_t2_copyout": (C.c_int, [_vp, _vp, _vp]),

@@ -74,11 +74,12 @@ int host_copy(brpa_ctx *ctx, double *dst, const double *src, int64_t n) {
 
 // Work-group counts per fragment for every stage.
 struct Counts {
-    int64_t dtab, syrk, setdiag, split, syrk16, colscale8, split8, syrk8;
+    int64_t dtab, syrk, setdiag, split, syrk16, colscale8, split8;
+    int64_t syrk8[I8V_COUNT];   // split-int8 product work-groups, per kernel variant
     std::vector<int64_t> pdiag, trsm, update;   // per Cholesky step
 };
 
-Counts counts_of(const HostFrag &f, int nsteps_max, int precision) {
+Counts counts_of(const HostFrag &f, int nsteps_max, int precision, int i8_variant) {
     Counts c;
     const int64_t K = f.nocc * f.nvir, N = f.naux, F = f.nfreqs;
     c.dtab = cdiv(F * K, 256);
@@ -92,7 +93,10 @@ Counts counts_of(const HostFrag &f, int nsteps_max, int precision) {
     const bool i8 = precision == BRPA_SPLIT_I8;
     c.colscale8 = i8 ? cdiv(N, 64) : 0;
     c.split8 = i8 ? cdiv(K, I8_KB) * cdiv(N, 64) : 0;
-    c.syrk8 = i8 ? F * (nt16 * (nt16 + 1) / 2) : 0;
+    for (int v = 0; v < I8V_COUNT; ++v) {
+        const int64_t t = cdiv(N, syrk_i8_tile(v));
+        c.syrk8[v] = (i8 && v == i8_variant) ? F * (t * (t + 1) / 2) : 0;
+    }
     c.pdiag.assign(nsteps_max, 0);
     c.trsm.assign(nsteps_max, 0);
     c.update.assign(nsteps_max, 0);
@@ -133,19 +137,25 @@ int run_chunk(brpa_ctx *ctx, int precision, const std::vector<HostFrag> &fr, dou
     for (int f = 0; f < nfrag; ++f)
         nsteps_max = std::max<int>(nsteps_max, (int)cdiv(fr[f].naux, CHOL_NB));
 
+    // split-int8 product variant of every fragment (autotuner choice per size bucket)
+    std::vector<int> var(nfrag, I8V_T128);
+    for (int f = 0; f < nfrag; ++f) {
+        const int c = ctx->i8_choice[tune_bucket(fr[f].naux)];
+        var[f] = ctx->tune_force >= 0 ? ctx->tune_force : (c >= 0 ? c : I8V_T128);
+    }
     std::vector<Counts> cnt(nfrag);
-    for (int f = 0; f < nfrag; ++f) cnt[f] = counts_of(fr[f], nsteps_max, precision);
+    for (int f = 0; f < nfrag; ++f) cnt[f] = counts_of(fr[f], nsteps_max, precision, var[f]);
 
-    // prefix arrays: [dtab, syrk, setdiag, split, syrk16, colscale8, split8, syrk8,
-    //                 pdiag_0..S-1, trsm_0..S-1, update_0..S-1]
-    constexpr int NFIX = 8;
+    // prefix arrays: [dtab, syrk, setdiag, split, syrk16, colscale8, split8,
+    //                 syrk8 per variant, pdiag_0..S-1, trsm_0..S-1, update_0..S-1]
+    constexpr int NFIX = 7 + I8V_COUNT;
     const int nstage = NFIX + 3 * nsteps_max;
     std::vector<int64_t> pre((size_t)nstage * (nfrag + 1), 0);
     auto P = [&](int s) { return pre.data() + (size_t)s * (nfrag + 1); };
     for (int f = 0; f < nfrag; ++f) {
         const Counts &c = cnt[f];
-        int64_t v[NFIX] = {c.dtab,  c.syrk,      c.setdiag, c.split,
-                           c.syrk16, c.colscale8, c.split8,  c.syrk8};
+        int64_t v[NFIX] = {c.dtab,     c.syrk,     c.setdiag,  c.split,   c.syrk16,
+                           c.colscale8, c.split8,  c.syrk8[0], c.syrk8[1], c.syrk8[2]};
         for (int s = 0; s < NFIX; ++s) P(s)[f + 1] = P(s)[f] + v[s];
         for (int j = 0; j < nsteps_max; ++j) {
             P(NFIX + j)[f + 1] = P(NFIX + j)[f] + c.pdiag[j];
@@ -274,7 +284,8 @@ int run_chunk(brpa_ctx *ctx, int precision, const std::vector<HostFrag> &fr, dou
             BRPA_TRY(launch_split_i8(ctx, d_fr, DP(6), nfrag, TOT(6)));
             BRPA_TRY(se(BRPA_STAGE_SPLIT));
             BRPA_TRY(sb(BRPA_STAGE_PI));
-            BRPA_TRY(launch_syrk_i8(ctx, d_fr, DP(7), nfrag, TOT(7)));
+            for (int v = 0; v < I8V_COUNT; ++v)
+                BRPA_TRY(launch_syrk_i8(ctx, d_fr, DP(7 + v), nfrag, TOT(7 + v), v));
             BRPA_TRY(se(BRPA_STAGE_PI));
         } else {
             BRPA_TRY(sb(BRPA_STAGE_SPLIT));
@@ -310,7 +321,8 @@ int run_chunk(brpa_ctx *ctx, int precision, const std::vector<HostFrag> &fr, dou
                           std::to_string((uintptr_t)ctx->stream);
         for (int f = 0; f < nfrag; ++f)
             key += ":" + std::to_string(fr[f].nocc) + "," + std::to_string(fr[f].nvir) + "," +
-                   std::to_string(fr[f].naux) + "," + std::to_string(fr[f].nfreqs);
+                   std::to_string(fr[f].naux) + "," + std::to_string(fr[f].nfreqs) + "," +
+                   std::to_string(var[f]);
         hipGraphExec_t exec = nullptr;
         for (auto &g : ctx->graphs)
             if (g.key == key) exec = g.exec;
@@ -432,6 +444,61 @@ int brpa_ecrpa(brpa_ctx *ctx, int precision, const double *R, int64_t naux, cons
     fr[0] = {nocc, nvir, naux, nfreqs, R, eocc, evir, freqs, weights};
     double *eus[1] = {eu};
     return run_ecrpa(ctx, precision, fr, ecrpa, eu ? eus : nullptr);
+}
+
+int brpa_autotune(brpa_ctx *ctx, int precision, int64_t nfrag, const brpa_frag_t *frags) {
+    if (!ctx) return BRPA_ERR_ARG;
+    if (nfrag < 0 || (nfrag > 0 && !frags)) return brpa_fail(ctx, BRPA_ERR_ARG, "bad batch arguments");
+    if (precision != BRPA_SPLIT_I8) return BRPA_OK;   // the other forms have one kernel shape
+    std::vector<HostFrag> bucket[3];
+    for (int64_t f = 0; f < nfrag; ++f)
+        bucket[tune_bucket(frags[f].naux)].push_back(
+            {frags[f].nocc, frags[f].nvir, frags[f].naux, frags[f].nfreqs, frags[f].R,
+             frags[f].eocc, frags[f].evir, frags[f].freqs, frags[f].weights});
+    BRPA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    graphs_release(ctx);
+    const int saved_graphs = ctx->use_graphs;
+    ctx->use_graphs = 0;   // eager runs: the per-stage events time the product kernel
+    int rc = BRPA_OK;
+    for (int b = 0; b < 3 && rc == BRPA_OK; ++b) {
+        if (bucket[b].empty()) continue;
+        std::vector<double> e(bucket[b].size()), e0;
+        double best = 0.0;
+        int best_v = I8V_T128;
+        for (int v = 0; v < I8V_COUNT && rc == BRPA_OK; ++v) {
+            ctx->tune_force = v;
+            // the first run allocates and warms up; the second is timed
+            rc = run_ecrpa(ctx, precision, bucket[b], e.data(), nullptr);
+            if (rc == BRPA_OK) rc = run_ecrpa(ctx, precision, bucket[b], e.data(), nullptr);
+            if (rc != BRPA_OK) break;
+            // the int32 level sums are exact, so every tile shape gives the same Pi
+            if (v == 0) {
+                e0 = e;
+            } else if (std::memcmp(e.data(), e0.data(), 8 * e.size()) != 0) {
+                rc = brpa_fail(ctx, BRPA_ERR_INTERNAL,
+                               "autotuner: split-int8 variant %d differs from variant 0", v);
+                break;
+            }
+            const double ms = ctx->stage_ms[BRPA_STAGE_PI];
+            ctx->i8_tune_ms[b][v] = ms;
+            if (v == 0 || ms < best) {
+                best = ms;
+                best_v = v;
+            }
+        }
+        if (rc == BRPA_OK) ctx->i8_choice[b] = best_v;
+    }
+    ctx->tune_force = -1;
+    ctx->use_graphs = saved_graphs;
+    return rc;
+}
+
+int brpa_autotune_result(const brpa_ctx *ctx, int bucket, int *variant, double *ms_per_variant) {
+    if (!ctx || bucket < 0 || bucket >= 3) return BRPA_ERR_ARG;
+    if (variant) *variant = ctx->i8_choice[bucket];
+    if (ms_per_variant)
+        for (int v = 0; v < I8V_COUNT; ++v) ms_per_variant[v] = ctx->i8_tune_ms[bucket][v];
+    return BRPA_OK;
 }
 
 int brpa_ecrpa_batch(brpa_ctx *ctx, int precision, int64_t nfrag, const brpa_frag_t *frags,

@@ -39,7 +39,14 @@ struct brpa_ctx {
     };
     std::vector<GraphEntry> graphs;
     int last_graph_replays = 0, last_graph_captures = 0;
+    // autotuner: split-int8 product variant per fragment-size bucket (-1: default)
+    int tune_force = -1;                // variant forced for every fragment (tuning runs)
+    int i8_choice[3] = {-1, -1, -1};
+    double i8_tune_ms[3][3] = {};      // [bucket][variant], 0 = not measured
 };
+
+// Fragment-size bucket of the autotuner (by NAux).
+inline int tune_bucket(int64_t naux) { return naux < 640 ? 0 : (naux < 1280 ? 1 : 2); }
 
 // Pinned host buffer of at least `bytes` owned by the context.
 int pin_reserve(brpa_ctx *ctx, size_t bytes);
@@ -148,7 +155,13 @@ int launch_split(brpa_ctx *, const FragDev *, const int64_t *pre, int nfrag, int
 int launch_syrk_bf16(brpa_ctx *, const FragDev *, const int64_t *pre, int nfrag, int64_t total);
 int launch_colscale_i8(brpa_ctx *, const FragDev *, const int64_t *pre, int nfrag, int64_t total);
 int launch_split_i8(brpa_ctx *, const FragDev *, const int64_t *pre, int nfrag, int64_t total);
-int launch_syrk_i8(brpa_ctx *, const FragDev *, const int64_t *pre, int nfrag, int64_t total);
+// split-int8 product kernel variants (chosen per fragment-size bucket by the
+// autotuner): 128x128 tiles in launch order, the same with one contiguous tile
+// range per XCD, 64x64 tiles
+enum { I8V_T128 = 0, I8V_T128_XCD = 1, I8V_T64 = 2, I8V_COUNT = 3 };
+int syrk_i8_tile(int variant);
+int launch_syrk_i8(brpa_ctx *, const FragDev *, const int64_t *pre, int nfrag, int64_t total,
+                   int variant);
 
 // Generic single-matrix FP64 SYRK used by brpa_gpiug_mo:
 //   C(N,N) (+)= X^T diag(d) X, X K x N row-major, both triangles written.

@@ -212,9 +212,9 @@ int launch_split_i8(brpa_ctx *ctx, const FragDev *fr, const int64_t *pre, int nf
 }
 
 // ---------------------------------------------------------------------------
-// Product.  One 128x128 output tile pair (p-tile >= q-tile) of one frequency
-// per work-group; work-groups per fragment: F * npairs(ceil(N/128)).
-// 8 waves as 2 (rows) x 4 (cols) sub-tiles of 64x32, 2 waves per SIMD.
+// Product.  One TxT output tile pair (p-tile >= q-tile) of one frequency per
+// work-group; work-groups per fragment: F * npairs(ceil(N/T)).  The waves
+// own 64x32 sub-tiles: T = 128 -> 8 waves (2 x 4), T = 64 -> 2 waves (1 x 2).
 //
 // v_mfma_i32_16x16x64_i8 lane layout (lane l, 16 bytes per operand):
 //   A[m = l%16][k = 16*(l/16) .. +15], B[k = 16*(l/16) .. +15][n = l%16],
@@ -227,7 +227,6 @@ int launch_split_i8(brpa_ctx *ctx, const FragDev *fr, const int64_t *pre, int nf
 // lane l lands in column l/4, slot l%4, and fetches k-piece
 // (l%4) ^ ((l/16)%4) of that column, so that the MFMA operand reads (16
 // columns at one k-piece per 16 lanes) hit 16 distinct 4-bank groups.
-constexpr int T8 = 128;
 constexpr int KS8 = 64;   // k-values (bytes) per stage and slice
 
 typedef __attribute__((address_space(3))) void lds_void_t;
@@ -236,26 +235,31 @@ __device__ __forceinline__ int swz8(int col_local, int piece) {
     return piece ^ ((col_local >> 2) & 3);
 }
 
-__global__ __launch_bounds__(512) void k_syrk_i8(const FragDev *__restrict__ fr,
-                                                const int64_t *__restrict__ pre, int nfrag,
-                                                int xcd) {
+template <int T>
+__global__ __launch_bounds__((T / 64) * (T / 32) * 64) void k_syrk_i8(
+    const FragDev *__restrict__ fr, const int64_t *__restrict__ pre, int nfrag, int xcd) {
+    constexpr int WJN = T / 32;                 // waves along columns
+    constexpr int NW = (T / 64) * WJN;          // waves
+    constexpr int GPS = T / 16;                 // 16-column groups per side and slice
+    constexpr int LPS = NSLICE * GPS;           // wave loads per side and stage
+    constexpr int LPW = 2 * LPS / NW;           // wave loads per wave and stage
     // [buffer][side][slice][column][slot] of 16-byte pieces
-    __shared__ __attribute__((aligned(16))) uint4 sbuf[2][2][NSLICE][T8][4];
+    __shared__ __attribute__((aligned(16))) uint4 sbuf[2][2][NSLICE][T][4];
     __shared__ int64_t s_pre[FIND_LDS_MAX + 1];
     // xcd = 1: one contiguous range of tiles per XCD (see xcd_remap)
     const int64_t b = xcd ? xcd_remap(blockIdx.x, gridDim.x) : (int64_t)blockIdx.x;
     const int f = find_frag_lds(pre, nfrag, b, s_pre);
     const FragDev &F = fr[f];
     const int N = F.N;
-    const int nt = (N + T8 - 1) / T8;
+    const int nt = (N + T - 1) / T;
     const int npairs = nt * (nt + 1) / 2;
     const int64_t lb = b - (nfrag <= FIND_LDS_MAX ? s_pre[f] : pre[f]);
     const int w = (int)(lb / npairs), pair = (int)(lb - (int64_t)w * npairs);
     int tp, tq;
     tri_decode(pair, tp, tq);
-    const int p0 = tp * T8, q0 = tq * T8;
+    const int p0 = tp * T, q0 = tq * T;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    const int wi = wave >> 2, wj = wave & 3;   // 64-row half, 32-col quarter
+    const int wi = wave / WJN, wj = wave % WJN;   // 64-row block, 32-col block
 
     const int64_t Kp = F.Kpad8;
     const int8_t *slw = F.sl8 + (int64_t)w * NSLICE * N * Kp;
@@ -266,25 +270,25 @@ __global__ __launch_bounds__(512) void k_syrk_i8(const FragDev *__restrict__ fr,
 #pragma unroll
         for (int j = 0; j < 2; ++j) L0[i][j] = L1[i][j] = L2[i][j] = L3[i][j] = (v4i_t){0, 0, 0, 0};
 
-    // This wave's 8 of the 64 wave loads of a stage: load id = 8*wave + t
-    // -> side = id/32, slice = (id/8)%4, column group = id%8.  Columns past N
-    // are clamped to N-1: they only feed outputs that are never written.
+    // This wave's LPW wave loads of a stage: load id = LPW*wave + t -> side =
+    // id/LPS, slice = (id%LPS)/GPS, column group = id%GPS.  Columns past N are
+    // clamped to N-1: they only feed outputs that are never written.
     const int lcol = lane >> 2;
     const int lpiece = (lane & 3) ^ ((lane >> 4) & 3);
-    int64_t goff[8];
+    int64_t goff[LPW];
 #pragma unroll
-    for (int t = 0; t < 8; ++t) {
-        const int id = 8 * wave + t;
-        const int side = id >> 5, sl = (id >> 3) & 3, g = id & 7;
+    for (int t = 0; t < LPW; ++t) {
+        const int id = LPW * wave + t;
+        const int side = id / LPS, sl = (id % LPS) / GPS, g = id % GPS;
         int P = (side ? q0 : p0) + 16 * g + lcol;
         P = P < N ? P : N - 1;
         goff[t] = ((int64_t)sl * N + P) * Kp + 16 * lpiece;
     }
     auto issue = [&](int buf, int k0) {
 #pragma unroll
-        for (int t = 0; t < 8; ++t) {
-            const int id = 8 * wave + t;
-            const int side = id >> 5, sl = (id >> 3) & 3, g = id & 7;
+        for (int t = 0; t < LPW; ++t) {
+            const int id = LPW * wave + t;
+            const int side = id / LPS, sl = (id % LPS) / GPS, g = id % GPS;
             __builtin_amdgcn_global_load_lds((const void *)(slw + goff[t] + k0),
                                              (lds_void_t *)&sbuf[buf][side][sl][16 * g][0], 16, 0, 0);
         }
@@ -375,10 +379,27 @@ __global__ __launch_bounds__(512) void k_syrk_i8(const FragDev *__restrict__ fr,
         }
 }
 
-int launch_syrk_i8(brpa_ctx *ctx, const FragDev *fr, const int64_t *pre, int nfrag, int64_t total) {
+int syrk_i8_tile(int variant) { return variant == I8V_T64 ? 64 : 128; }
+
+int launch_syrk_i8(brpa_ctx *ctx, const FragDev *fr, const int64_t *pre, int nfrag, int64_t total,
+                   int variant) {
     if (total <= 0) return BRPA_OK;
-    hipLaunchKernelGGL(k_syrk_i8, dim3((unsigned)total), dim3(512), 0, ctx->stream, fr, pre,
-                       nfrag, 0);
+    switch (variant) {
+    case I8V_T128:
+        hipLaunchKernelGGL(k_syrk_i8<128>, dim3((unsigned)total), dim3(512), 0, ctx->stream, fr,
+                           pre, nfrag, 0);
+        break;
+    case I8V_T128_XCD:
+        hipLaunchKernelGGL(k_syrk_i8<128>, dim3((unsigned)total), dim3(512), 0, ctx->stream, fr,
+                           pre, nfrag, 1);
+        break;
+    case I8V_T64:
+        hipLaunchKernelGGL(k_syrk_i8<64>, dim3((unsigned)total), dim3(128), 0, ctx->stream, fr,
+                           pre, nfrag, 0);
+        break;
+    default:
+        return brpa_fail(ctx, BRPA_ERR_ARG, "unknown split-int8 kernel variant %d", variant);
+    }
     LAUNCH_CHECK(ctx);
     return BRPA_OK;
 }

@@ -41,7 +41,9 @@ enum {
     BRPA_ERR_NONFINITE = 5,  /* NaN/Inf in an input */
     BRPA_ERR_OOM = 6,        /* device allocation failed */
     BRPA_ERR_GRID = 7,       /* frequency quadrature did not converge (OptFreqQuad.f90:68-71) */
-    BRPA_ERR_SOLVER = 8      /* eigensolver failure (real_linalg.f90:1527-1531) */
+    BRPA_ERR_SOLVER = 8,     /* eigensolver failure (real_linalg.f90:1527-1531) */
+    BRPA_ERR_INTERNAL = 9    /* a self-check of the library failed (e.g. kernel variants
+                                that must agree bit for bit did not) */
 };
 
 /* Arithmetic of the Pi and amplitude contractions.  BRPA_FP64 is the drop-in. */
@@ -85,8 +87,9 @@ double brpa_last_kernel_ms(const brpa_ctx *ctx);
 /* Device time (ms) of one stage of the last E_RPA call, summed over its
  * launches, and the number of launches.  Stages: */
 enum {
-    BRPA_STAGE_PI = 0,     /* Pi(u) SYRK kernel (FP64 or split-BF16 MFMA) */
-    BRPA_STAGE_SPLIT = 1,  /* split-BF16 slicing pass */
+    BRPA_STAGE_PI = 0,     /* Pi(u) SYRK kernel (FP64, split-BF16 or split-int8 MFMA) */
+    BRPA_STAGE_SPLIT = 1,  /* slicing passes of the split forms (int8: column scales,
+                              exact diagonal and slices) */
     BRPA_STAGE_CHOL = 2,   /* blocked Cholesky log-det (panel + trailing updates) */
     BRPA_STAGE_AUX = 3,    /* Delta table, exact diagonal, quadrature */
     BRPA_NSTAGES = 4
@@ -155,6 +158,17 @@ typedef struct {
  * reference runs subsystems one after another, rpa/rpa_driver.f90:176-223). */
 int brpa_ecrpa_batch(brpa_ctx *ctx, int precision, int64_t nfrag, const brpa_frag_t *frags,
                      double *ecrpa);
+
+/* Autotuner (no counterpart in the reference).  Times every tile variant of the
+ * split-int8 product kernel (128x128 tiles, the same with one contiguous tile
+ * range per XCD, 64x64 tiles) on the given fragments, separately for three
+ * NAux buckets (< 640, < 1280, >= 1280), and keeps the fastest per bucket in
+ * the context for later calls.  Other precisions have one kernel shape (no-op).
+ * Runs the pipeline twice per variant and bucket (warm-up + timed). */
+int brpa_autotune(brpa_ctx *ctx, int precision, int64_t nfrag, const brpa_frag_t *frags);
+/* Chosen variant of a bucket (-1: not tuned) and the product-kernel time (ms) of
+ * each of the 3 variants measured by the last brpa_autotune (0: not measured). */
+int brpa_autotune_result(const brpa_ctx *ctx, int bucket, int *variant, double *ms_per_variant);
 
 /* ---- factorised direct-ring amplitudes --------------------------------- */
 /* T2 = sum_mu A_mu V_mu V_mu^T (ov x ov), the direct-ring doubles amplitudes

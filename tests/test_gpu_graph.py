@@ -69,3 +69,23 @@ def test_graph_not_used_for_host_inputs(ctx):
     finally:
         ctx.set_graphs(False)
     assert abs(e[0] - erpa.ecrpa(f.R, f.gaps(), x, w)) <= 1e-8
+
+
+def test_autotune_chooses_and_agrees(ctx):
+    """Every split-int8 product variant runs on fragments of all three NAux
+    buckets (the tuner checks that they agree bit for bit); the tuned batch
+    stays within 1e-7 Ha of the oracle."""
+    shapes = [(4, 60, 300), (9, 80, 700), (6, 250, 1300), (3, 40, 200)]
+    frags, refs = [], []
+    for i, (o, v, n) in enumerate(shapes):
+        f = synth.make_fragment(o, v, n, 700 + i)
+        x, w, _ = grid.freq_grid(f.eocc, f.evir, 8)
+        frags.append((f.R, f.eocc, f.evir, x, w))
+        refs.append(erpa.ecrpa(f.R, f.gaps(), x, w))
+    res = ctx.autotune(frags, "split_i8")
+    print("autotune", res)
+    assert sorted(res) == [0, 1, 2]
+    for b, (v, ms) in res.items():
+        assert 0 <= v < 3 and all(m > 0 for m in ms)
+    e = ctx.ecrpa_batch(frags, "split_i8")
+    assert np.max(np.abs(e - np.array(refs))) <= 1e-7
