@@ -295,13 +295,15 @@ class Context:
     def sosex(self, V, A, R, nocc, nvir, alpha=1.0, s2_operator=False, precision="fp64"):
         """Exchange correction -alpha sum (aj|bi) T(ai,bj) (rpa_CC_Exchange_SOSEX)."""
         nai, naux = R.shape
-        A = np.ascontiguousarray(A, np.float64)
-        if nai != nocc * nvir or tuple(V.shape) != (nai, A.size):
+        if not _is_torch(A):
+            A = np.ascontiguousarray(A, np.float64)
+        nt = A.numel() if _is_torch(A) else A.size
+        if nai != nocc * nvir or tuple(V.shape) != (nai, nt):
             raise ValueError("shape mismatch")
         e = np.zeros(1)
         self._sync_stream(V, R)
         self._check(self._lib.brpa_sosex(self._h, _prec(precision), _ptr(e), float(alpha), _ptr(V),
-                                         _ptr(A), _ptr(R), nocc, nvir, naux, A.size,
+                                         _ptr(A), _ptr(R), nocc, nvir, naux, nt,
                                          int(bool(s2_operator))))
         return float(e[0])
 

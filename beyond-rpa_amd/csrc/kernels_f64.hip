@@ -451,13 +451,19 @@ int launch_trsm(brpa_ctx *ctx, const FragDev *fr, const int64_t *pre, int nfrag,
 // trailing matrix that starts at t0 = j0 + 64), K = 64, 8 waves as 2 (rows) x
 // 4 (cols) of 64x32.  Waves entirely above the diagonal of a diagonal
 // super-tile skip their products; element writes keep col <= row.
+//
+// The C tile is loaded straight into the accumulators (the MFMA subtracts via
+// a negated A operand), and the two panels pass through LDS in four K-phases
+// of 16 columns (35 KB), so two work-groups fit on a CU (128 registers per
+// lane) and one's loads overlap the other's products.
 constexpr int UPD_T = 128;
+constexpr int UPD_KH = 16;   // K columns per LDS phase
 
-__global__ __launch_bounds__(512) void k_chol_update(const FragDev *__restrict__ fr,
-                                                    const int64_t *__restrict__ pre, int nfrag,
-                                                    int step) {
-    __shared__ double sI[UPD_T][CHOL_NB + 1];
-    __shared__ double sJ[UPD_T][CHOL_NB + 1];
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_chol_update(const FragDev *__restrict__ fr,
+                                                       const int64_t *__restrict__ pre,
+                                                       int nfrag, int step) {
+    __shared__ double sI[UPD_T][UPD_KH + 1];
+    __shared__ double sJ[UPD_T][UPD_KH + 1];
     __shared__ int64_t s_pre[FIND_LDS_MAX + 1];
     int64_t b = blockIdx.x;
     int f = find_frag_lds(pre, nfrag, b, s_pre);
@@ -472,41 +478,44 @@ __global__ __launch_bounds__(512) void k_chol_update(const FragDev *__restrict__
     int ti, tj;
     tri_decode(pair, ti, tj);
     double *A = F.Pi + (int64_t)w * N * N;
+    const gdouble_t *G = (const gdouble_t *)A;
     const int Ri = t0 + UPD_T * ti, Rj = t0 + UPD_T * tj;
     const int tid = threadIdx.x;
-    {
-        double(*sp[2])[CHOL_NB + 1] = {sI, sJ};
-        const int rows[2] = {Ri, Rj};
-        load_panels64<UPD_T, 512, 2>(sp, A, N, rows, N, j0, N, tid);
-    }
-    __syncthreads();
     const int wave = tid >> 6, lane = tid & 63;
     const int wa = wave >> 2, wq = wave & 3;   // rows 64*wa.., cols 32*wq..
     const int r0 = 64 * wa, c0 = 32 * wq;
-    if (ti == tj && c0 > r0 + 63) return;      // wave entirely above the diagonal
-    if (Ri + r0 >= N || Rj + c0 >= N) return;
+    const bool active = !(ti == tj && c0 > r0 + 63) && Ri + r0 < N && Rj + c0 < N;
+
+    // panel phase h: rows Ri.. / Rj.. (128 each) x columns j0 + 16 h .. + 16;
+    // thread -> column tid%16, rows tid/16 + 32 u (u < 4) of both panels
+    constexpr int RS = 512 / UPD_KH;        // rows per load sweep
+    constexpr int PH = UPD_T / RS;
+    const int lc = tid % UPD_KH, lr = tid / UPD_KH;
+    auto load_half = [&](int h, double (&v)[2][PH]) {
+#pragma unroll
+        for (int u = 0; u < PH; ++u) {
+            const int r = lr + RS * u;
+            const bool oki = Ri + r < N, okj = Rj + r < N;
+            v[0][u] = G[oki ? (int64_t)(Ri + r) * N + j0 + UPD_KH * h + lc : 0];
+            v[1][u] = G[okj ? (int64_t)(Rj + r) * N + j0 + UPD_KH * h + lc : 0];
+            v[0][u] = oki ? v[0][u] : 0.0;
+            v[1][u] = okj ? v[1][u] : 0.0;
+        }
+    };
+    auto store_half = [&](const double (&v)[2][PH]) {
+#pragma unroll
+        for (int u = 0; u < PH; ++u) {
+            sI[lr + RS * u][lc] = v[0][u];
+            sJ[lr + RS * u][lc] = v[1][u];
+        }
+    };
+
+    double v0[2][PH];
+    load_half(0, v0);
+    store_half(v0);
+    __syncthreads();
+    // the C tile into the accumulators (out-of-range / upper elements read A[0], then 0)
     d4_t acc[4][2];
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j) acc[i][j] = (d4_t){0.0, 0.0, 0.0, 0.0};
-#pragma unroll 4
-    for (int kk = 0; kk < CHOL_NB; kk += 4) {
-        const int kc = kk + (lane >> 4);
-        double a[4], bv[2];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) a[i] = sI[r0 + 16 * i + (lane & 15)][kc];
-#pragma unroll
-        for (int j = 0; j < 2; ++j) bv[j] = sJ[c0 + 16 * j + (lane & 15)][kc];
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-            for (int j = 0; j < 2; ++j)
-                acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], bv[j], acc[i][j], 0, 0, 0);
-    }
-    // read-modify-write: all 32 reads first (out-of-range ones from A[0]), then the stores
-    double old[4][2][4];
-    int64_t idx[4][2][4];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -515,18 +524,47 @@ __global__ __launch_bounds__(512) void k_chol_update(const FragDev *__restrict__
             for (int q = 0; q < 4; ++q) {
                 const int row = Ri + r0 + 16 * i + (lane >> 4) + 4 * q;
                 const int col = Rj + c0 + 16 * j + (lane & 15);
-                const bool ok = row < N && col < N && col <= row;
-                idx[i][j][q] = ok ? (int64_t)row * N + col : -1;
-                old[i][j][q] = ((const gdouble_t *)A)[ok ? idx[i][j][q] : 0];
+                const bool ok = active && row < N && col < N && col <= row;
+                const double v = G[ok ? (int64_t)row * N + col : 0];
+                acc[i][j][q] = ok ? v : 0.0;
             }
+    auto mult = [&]() {
+        if (!active) return;
+#pragma unroll 4
+        for (int kk = 0; kk < UPD_KH; kk += 4) {
+            const int kc = kk + (lane >> 4);
+            double a[4], bv[2];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) a[i] = -sI[r0 + 16 * i + (lane & 15)][kc];
+#pragma unroll
+            for (int j = 0; j < 2; ++j) bv[j] = sJ[c0 + 16 * j + (lane & 15)][kc];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], bv[j], acc[i][j], 0, 0, 0);
+        }
+    };
+    mult();
+    for (int h = 1; h < CHOL_NB / UPD_KH; ++h) {
+        load_half(h, v0);
+        __syncthreads();   // everyone is done with the previous phase
+        store_half(v0);
+        __syncthreads();
+        mult();
+    }
+    if (!active) return;
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j)
 #pragma unroll
-            for (int q = 0; q < 4; ++q)
-                if (idx[i][j][q] >= 0)
-                    ((gdouble_mut_t *)A)[idx[i][j][q]] = old[i][j][q] - acc[i][j][q];
+            for (int q = 0; q < 4; ++q) {
+                const int row = Ri + r0 + 16 * i + (lane >> 4) + 4 * q;
+                const int col = Rj + c0 + 16 * j + (lane & 15);
+                if (row < N && col < N && col <= row)
+                    ((gdouble_mut_t *)A)[(int64_t)row * N + col] = acc[i][j][q];
+            }
 }
 
 int launch_chol_update(brpa_ctx *ctx, const FragDev *fr, const int64_t *pre, int nfrag,
