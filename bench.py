@@ -11,9 +11,10 @@ workspace chunk (one eager pass supplies the per-stage kernel times).  The
 frequency grids (host code, independent of R) are built once before timing.
 
 Exchange correction: the factorised direct-ring amplitudes (T2) and the SOSEX
-contraction are timed per fragment on the first fragments of the same batch
-(bounded by --exchange-seconds), and the E_RPA + exchange rate is
-1 / (E_RPA time per fragment + mean exchange time per fragment).
+contraction are timed on the first --exchange-frags fragments of the same
+batch (brpa_exchange_batch, one fragment after another), and the
+E_RPA + exchange rate is 1 / (E_RPA time per fragment + exchange time per
+fragment).
 
 Prints one JSON line.  `roofline` is measured for the dominant kernel with
 HIP events inside the library over the timed steps; `cpu_baseline` times the
@@ -67,8 +68,10 @@ def main():
                     help="also time the other forms (reported under `forms`)")
     ap.add_argument("--graphs", type=int, default=1,
                     help="replay the batched launch sequence as a hipGraph (1) or launch eagerly")
-    ap.add_argument("--exchange-seconds", type=float, default=20.0,
-                    help="time budget of the T2 + SOSEX sample (0 skips the exchange part)")
+    ap.add_argument("--exchange-frags", type=int, default=64,
+                    help="fragments in the T2 + SOSEX sample (0 skips the exchange part)")
+    ap.add_argument("--exchange-workers", type=int, default=1,
+                    help="worker count passed to brpa_exchange_batch (only 1 is supported)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy 5 times (reference for the "
                          "FETCH_SIZE/WRITE_SIZE counter scale)")
@@ -156,32 +159,25 @@ def main():
                 "launches_per_step": stage_n[dom]}
 
     exch = None
-    if a.exchange_seconds > 0:
-        import math
-        ts, exs = [], []
-        t = time.perf_counter()
-        k = 0
-        while k < a.nfrag and (k == 0 or time.perf_counter() - t < a.exchange_seconds):
-            o, v, n = shapes[k]
-            f = frags[k]
-            torch.cuda.synchronize()
-            t1 = time.perf_counter()
-            A, V = ctx.t2(Rd[k], f.eocc, f.evir, grids[k][0], grids[k][1],
-                          nguess=math.ceil(3.0 * n), seed=synth.BASE_SEED + k, device=True)
-            ex = ctx.sosex(V, A, Rd[k], o, v)
-            torch.cuda.synchronize()
-            ts.append(time.perf_counter() - t1)
-            exs.append(ex)
-            del A, V
-            k += 1
-        t_ex = sum(ts) / len(ts)
-        exch = {"n_fragments_timed": k, "mean_ms_per_fragment": round(t_ex * 1e3, 3),
+    if a.exchange_frags > 0:
+        # T2 + SOSEX of the first fragments of the batch, several in flight
+        kx = min(a.exchange_frags, a.nfrag)
+        sub = batch[:kx]
+        ctx.exchange_batch(sub[:min(kx, a.exchange_workers)], seed=synth.BASE_SEED,
+                           nworkers=a.exchange_workers)          # warm-up (workspaces)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        exs, nts = ctx.exchange_batch(sub, seed=synth.BASE_SEED, nworkers=a.exchange_workers)
+        torch.cuda.synchronize()
+        t_ex = (time.perf_counter() - t1) / kx
+        exch = {"n_fragments_timed": kx, "workers": a.exchange_workers,
+                "ms_per_fragment": round(t_ex * 1e3, 3),
                 "precision": "fp64 (T2 amplitudes and SOSEX contraction)",
-                "nguess": "ceil(3.0 N)"}
+                "nguess": "ceil(3.0 N)", "mean_nt2": float(np.mean(nts))}
         for p, d in forms.items():
             d["with_exchange_fps"] = 1.0 / (d["ms_per_step"] * 1e-3 / a.nfrag + t_ex)
-        print("# exchange: %d fragments, %.1f ms each" % (k, t_ex * 1e3), file=sys.stderr,
-              flush=True)
+        print("# exchange: %d fragments, %d workers, %.1f ms per fragment" %
+              (kx, a.exchange_workers, t_ex * 1e3), file=sys.stderr, flush=True)
 
     cpu = None
     parity = None

@@ -236,6 +236,21 @@ class Context:
                 out[b] = (v.value, ms.tolist())
         return out
 
+    def exchange_batch(self, frags, lam=1.0, cutoff=1e-10, guess_factor=3.0, seed=0,
+                       alpha=1.0, s2_operator=False, nworkers=1, precision="fp64"):
+        """SOSEX exchange energy of many fragments (T2 then SOSEX per fragment,
+        on device-resident amplitudes).  Fragment f uses guess seed seed + f.
+        Returns (E_x array, NT2 array)."""
+        arr, keep = self._frag_array(frags)
+        ex = np.zeros(len(frags))
+        nt = np.zeros(len(frags), dtype=np.int64)
+        self._sync_stream(*[k[0] for k in keep])
+        self._check(self._lib.brpa_exchange_batch(
+            self._h, _prec(precision), len(frags), arr, float(lam), float(cutoff),
+            float(guess_factor), int(seed) & 0xFFFFFFFFFFFFFFFF, float(alpha),
+            int(bool(s2_operator)), int(nworkers), _ptr(ex), C.c_void_p(nt.ctypes.data)))
+        return ex, nt
+
     def ecrpa_batch(self, frags, precision="fp64"):
         """E_RPA of many fragments in batched launches.
 

@@ -46,6 +46,9 @@ SIGNATURES = {
     "brpa_t2_compute": (C.c_int, [_vp, C.c_int, _vp, _i64, _vp, _i64, _vp, _i64, _vp, _vp, _i64,
                                   C.c_double, C.c_double, _i64, C.c_uint64, C.POINTER(_i64)]),
     "brpa_t2_copyout": (C.c_int, [_vp, _vp, _vp]),
+    "brpa_exchange_batch": (C.c_int, [_vp, C.c_int, _i64, C.POINTER(brpa_frag_t), C.c_double,
+                                      C.c_double, C.c_double, C.c_uint64, C.c_double, C.c_int,
+                                      C.c_int, _vp, _vp]),
     "brpa_ecrpa_grad": (C.c_int, [_vp, _vp, _i64, _vp, _i64, _vp, _i64, _vp, _vp, _i64, _vp, _vp,
                                   _vp]),
     "brpa_sosex": (C.c_int, [_vp, C.c_int, _vp, C.c_double, _vp, _vp, _vp, _i64, _i64, _i64,

@@ -41,6 +41,13 @@ int from_device(brpa_ctx *ctx, void *dst, const void *src, size_t bytes) {
     return BRPA_OK;
 }
 
+int copy_sync(brpa_ctx *ctx, void *dst, const void *src, size_t bytes, hipMemcpyKind kind) {
+    if (bytes == 0) return BRPA_OK;
+    BRPA_HIP(ctx, hipMemcpyAsync(dst, src, bytes, kind, ctx->stream));
+    BRPA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return BRPA_OK;
+}
+
 int ws_reserve(brpa_ctx *ctx, size_t bytes, Arena *arena) {
     if (bytes > ctx->ws_cap) {
         BRPA_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -153,6 +160,8 @@ int brpa_ctx_destroy(brpa_ctx *ctx) {
     if (ctx->own_stream) (void)hipStreamSynchronize(ctx->own_stream);
     if (ctx->stream && ctx->stream != ctx->own_stream) (void)hipStreamSynchronize(ctx->stream);
     graphs_release(ctx);
+    for (brpa_ctx *w : ctx->workers) brpa_ctx_destroy(w);
+    ctx->workers.clear();
     if (ctx->ws) (void)hipFree(ctx->ws);
     if (ctx->pin) (void)hipHostFree(ctx->pin);
     t2_release(ctx);
@@ -194,6 +203,8 @@ int brpa_ctx_trim(brpa_ctx *ctx) {
     if (!ctx) return BRPA_ERR_ARG;
     BRPA_HIP(ctx, hipStreamSynchronize(ctx->stream));
     graphs_release(ctx);
+    for (brpa_ctx *w : ctx->workers) brpa_ctx_destroy(w);
+    ctx->workers.clear();
     if (ctx->ws) BRPA_HIP(ctx, hipFree(ctx->ws));
     ctx->ws = nullptr;
     ctx->ws_cap = 0;

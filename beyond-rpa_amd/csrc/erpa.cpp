@@ -36,15 +36,15 @@ int check_frag(brpa_ctx *ctx, const HostFrag &f, int64_t idx) {
     std::vector<double> eo(f.nocc), ev(f.nvir), fx(f.nfreqs), fw(f.nfreqs);
     const bool dev = is_device_ptr(f.eocc);
     if (dev) {
-        BRPA_HIP(ctx, hipMemcpy(eo.data(), f.eocc, 8 * f.nocc, hipMemcpyDeviceToHost));
-        BRPA_HIP(ctx, hipMemcpy(ev.data(), f.evir, 8 * f.nvir, hipMemcpyDeviceToHost));
+        BRPA_TRY(copy_sync(ctx, eo.data(), f.eocc, 8 * f.nocc, hipMemcpyDeviceToHost));
+        BRPA_TRY(copy_sync(ctx, ev.data(), f.evir, 8 * f.nvir, hipMemcpyDeviceToHost));
     } else {
         std::memcpy(eo.data(), f.eocc, 8 * f.nocc);
         std::memcpy(ev.data(), f.evir, 8 * f.nvir);
     }
     if (is_device_ptr(f.freqs)) {
-        BRPA_HIP(ctx, hipMemcpy(fx.data(), f.freqs, 8 * f.nfreqs, hipMemcpyDeviceToHost));
-        BRPA_HIP(ctx, hipMemcpy(fw.data(), f.weights, 8 * f.nfreqs, hipMemcpyDeviceToHost));
+        BRPA_TRY(copy_sync(ctx, fx.data(), f.freqs, 8 * f.nfreqs, hipMemcpyDeviceToHost));
+        BRPA_TRY(copy_sync(ctx, fw.data(), f.weights, 8 * f.nfreqs, hipMemcpyDeviceToHost));
     } else {
         std::memcpy(fx.data(), f.freqs, 8 * f.nfreqs);
         std::memcpy(fw.data(), f.weights, 8 * f.nfreqs);
@@ -66,7 +66,7 @@ int check_frag(brpa_ctx *ctx, const HostFrag &f, int64_t idx) {
 // Copies n doubles of a host-or-device array into host memory.
 int host_copy(brpa_ctx *ctx, double *dst, const double *src, int64_t n) {
     if (is_device_ptr(src))
-        BRPA_HIP(ctx, hipMemcpy(dst, src, 8 * n, hipMemcpyDeviceToHost));
+        BRPA_TRY(copy_sync(ctx, dst, src, 8 * n, hipMemcpyDeviceToHost));
     else
         std::memcpy(dst, src, 8 * n);
     return BRPA_OK;
@@ -375,7 +375,7 @@ int run_chunk(brpa_ctx *ctx, int precision, const std::vector<HostFrag> &fr, dou
         energies[f] = h_energy[f];
         if (eu_out && eu_out[f]) {
             if (is_device_ptr(eu_out[f]))
-                BRPA_HIP(ctx, hipMemcpy(eu_out[f], h_eu + fo, 8 * (size_t)fr[f].nfreqs,
+                BRPA_TRY(copy_sync(ctx, eu_out[f], h_eu + fo, 8 * (size_t)fr[f].nfreqs,
                                         hipMemcpyHostToDevice));
             else
                 std::memcpy(eu_out[f], h_eu + fo, 8 * (size_t)fr[f].nfreqs);
@@ -530,8 +530,8 @@ int brpa_gpiug_mo(brpa_ctx *ctx, double *gpiug, const double *crg, int64_t naux,
     BRPA_HIP(ctx, hipSetDevice(ctx->device));
     std::vector<double> eo(nocc), ev(nvir);
     if (is_device_ptr(eocc)) {
-        BRPA_HIP(ctx, hipMemcpy(eo.data(), eocc, 8 * nocc, hipMemcpyDeviceToHost));
-        BRPA_HIP(ctx, hipMemcpy(ev.data(), evir, 8 * nvir, hipMemcpyDeviceToHost));
+        BRPA_TRY(copy_sync(ctx, eo.data(), eocc, 8 * nocc, hipMemcpyDeviceToHost));
+        BRPA_TRY(copy_sync(ctx, ev.data(), evir, 8 * nvir, hipMemcpyDeviceToHost));
     } else {
         std::memcpy(eo.data(), eocc, 8 * nocc);
         std::memcpy(ev.data(), evir, 8 * nvir);

@@ -48,7 +48,7 @@ extern "C" int brpa_ecrpa_grad(brpa_ctx *ctx, const double *R, int64_t naux, con
     std::vector<double> eo(nocc), ev(nvir), fx(F), fw(F);
     auto hv = [&](std::vector<double> &v, const double *p, int64_t n) -> int {
         if (is_device_ptr(p))
-            BRPA_HIP(ctx, hipMemcpy(v.data(), p, 8 * n, hipMemcpyDeviceToHost));
+            BRPA_TRY(copy_sync(ctx, v.data(), p, 8 * n, hipMemcpyDeviceToHost));
         else
             std::memcpy(v.data(), p, 8 * n);
         for (double x : v)
@@ -170,13 +170,13 @@ extern "C" int brpa_ecrpa_grad(brpa_ctx *ctx, const double *R, int64_t naux, con
     std::vector<double> go_d(go.begin(), go.end()), gv_d(gv.begin(), gv.end());
     if (deocc) {
         if (is_device_ptr(deocc))
-            BRPA_HIP(ctx, hipMemcpy(deocc, go_d.data(), 8 * nocc, hipMemcpyHostToDevice));
+            BRPA_TRY(copy_sync(ctx, deocc, go_d.data(), 8 * nocc, hipMemcpyHostToDevice));
         else
             std::memcpy(deocc, go_d.data(), 8 * nocc);
     }
     if (devir) {
         if (is_device_ptr(devir))
-            BRPA_HIP(ctx, hipMemcpy(devir, gv_d.data(), 8 * nvir, hipMemcpyHostToDevice));
+            BRPA_TRY(copy_sync(ctx, devir, gv_d.data(), 8 * nvir, hipMemcpyHostToDevice));
         else
             std::memcpy(devir, gv_d.data(), 8 * nvir);
     }

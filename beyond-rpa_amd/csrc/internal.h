@@ -43,6 +43,8 @@ struct brpa_ctx {
     int tune_force = -1;                // variant forced for every fragment (tuning runs)
     int i8_choice[3] = {-1, -1, -1};
     double i8_tune_ms[3][3] = {};      // [bucket][variant], 0 = not measured
+    // worker contexts (own stream, rocBLAS handle, workspace) of brpa_exchange_batch
+    std::vector<brpa_ctx *> workers;
 };
 
 // Fragment-size bucket of the autotuner (by NAux).
@@ -96,6 +98,9 @@ bool is_device_ptr(const void *p);
 // Copy n elements of a host-or-device array into device memory dst.
 int to_device(brpa_ctx *ctx, void *dst, const void *src, size_t bytes);
 int from_device(brpa_ctx *ctx, void *dst_host_or_dev, const void *src_dev, size_t bytes);
+// Blocking copy ordered on ctx->stream.  (A plain hipMemcpy runs on the null
+// stream, which is not ordered with the context's non-blocking stream.)
+int copy_sync(brpa_ctx *ctx, void *dst, const void *src, size_t bytes, hipMemcpyKind kind);
 
 // ---------------------------------------------------------------------------
 // Fragment descriptor used by the batched kernels (device memory).

@@ -30,6 +30,14 @@ int launch_scale(brpa_ctx *, double *x, double a, int64_t n);
 int launch_pair_dot(brpa_ctx *, const double *W1, const double *W2, const double *B, int64_t n_rows,
                     int64_t nt, int64_t stride, int npair, double *part);
 int sosex_nchunk(int64_t n_rows, int64_t nt);
+int sosex_i8_vpad(int64_t v);
+size_t sosex_i8_slice_bytes(int64_t nblk, int64_t ncols, int64_t v);
+int launch_sosex_i8_split(brpa_ctx *, const double *X, int64_t nblk, int64_t v, int64_t ncols,
+                          double *scale, int8_t *out);
+int64_t sosex_i8_nparts(int64_t o, int64_t N, int64_t NT);
+int launch_sosex_i8_pairs(brpa_ctx *, const int8_t *SR, const int8_t *SV, const double *scR,
+                          const double *scV, const double *B, int64_t o, int64_t v, int64_t N,
+                          int64_t NT, double *part, double *out2);
 
 namespace {
 
@@ -91,7 +99,7 @@ bool finite_all(const double *p, int64_t n) {
 int host_vec(brpa_ctx *ctx, std::vector<double> &v, const double *p, int64_t n) {
     v.resize(n);
     if (is_device_ptr(p))
-        BRPA_HIP(ctx, hipMemcpy(v.data(), p, 8 * n, hipMemcpyDeviceToHost));
+        BRPA_TRY(copy_sync(ctx, v.data(), p, 8 * n, hipMemcpyDeviceToHost));
     else
         std::memcpy(v.data(), p, 8 * n);
     return BRPA_OK;
@@ -265,7 +273,7 @@ int brpa_t2_compute(brpa_ctx *ctx, int precision, const double *R, int64_t naux,
     BRPA_TRY(gemm(ctx, h, true, false, NG, NG, K, 1.0, iom, K, iom, K, 0.0, Sm, NG));
     BRPA_TRY(syevd(ctx, h, NG, Sm, sev, sE, dinfo + F));
     std::vector<double> hs(NG);
-    BRPA_HIP(ctx, hipMemcpy(hs.data(), sev, 8 * NG, hipMemcpyDeviceToHost));
+    BRPA_TRY(copy_sync(ctx, hs.data(), sev, 8 * NG, hipMemcpyDeviceToHost));
     int64_t nt = 0;
     for (int64_t i = NG - 1; i >= 0; --i) {
         if (hs[i] > cutoff)
@@ -303,7 +311,7 @@ int brpa_t2_compute(brpa_ctx *ctx, int precision, const double *R, int64_t naux,
     // eigenpairs of T^T I T; A = a / (2 (1 + a)); V = eigvecs^T T^T (629-641)
     BRPA_TRY(syevd(ctx, h, nt, TIT, sev, sE, dinfo + F + 1));
     std::vector<double> ha(nt);
-    BRPA_HIP(ctx, hipMemcpy(ha.data(), sev, 8 * nt, hipMemcpyDeviceToHost));
+    BRPA_TRY(copy_sync(ctx, ha.data(), sev, 8 * nt, hipMemcpyDeviceToHost));
     for (int64_t i = 0; i < nt; ++i) ha[i] = 0.5 * (ha[i] / (ha[i] + 1.0));
     BRPA_TRY(grow(ctx, &ctx->t2_A, &ctx->t2_A_cap, nt));
     BRPA_TRY(grow(ctx, &ctx->t2_V, &ctx->t2_V_cap, (size_t)nt * K));
@@ -338,10 +346,15 @@ int brpa_sosex(brpa_ctx *ctx, int precision, double *ec_exchange, double alpha, 
                const double *A, const double *R, int64_t nocc, int64_t nvir, int64_t naux,
                int64_t nvecs_t2, int s2_operator) {
     if (!ctx) return BRPA_ERR_ARG;
-    if (precision != BRPA_FP64)
-        return brpa_fail(ctx, BRPA_ERR_ARG, "brpa_sosex: only BRPA_FP64 is implemented");
+    if (precision != BRPA_FP64 && precision != BRPA_SPLIT_I8)
+        return brpa_fail(ctx, BRPA_ERR_ARG,
+                         "brpa_sosex: precision must be BRPA_FP64 or BRPA_SPLIT_I8");
     if (!ec_exchange || !V || !A || !R || nocc <= 0 || nvir <= 0 || naux <= 0 || nvecs_t2 < 0)
         return brpa_fail(ctx, BRPA_ERR_ARG, "brpa_sosex: bad arguments");
+    if (precision == BRPA_SPLIT_I8 && nvir > 65536)
+        return brpa_fail(ctx, BRPA_ERR_ARG,
+                         "brpa_sosex: nvir = %lld exceeds 65536, the largest K with exact int32 "
+                         "sums in the split-int8 form", (long long)nvir);
     *ec_exchange = 0.0;
     const int64_t K = nocc * nvir, N = naux, nt = nvecs_t2, o = nocc, v = nvir;
     if (nt == 0) return BRPA_OK;
@@ -358,6 +371,45 @@ int brpa_sosex(brpa_ctx *ctx, int precision, double *ec_exchange, double alpha, 
     std::vector<double> hB(nt);
     for (int64_t k = 0; k < nt; ++k) hB[k] = s2_operator ? hA[k] / (1.0 - 4.0 * hA[k] * hA[k]) : hA[k];
     const bool rdev = is_device_ptr(R), vdev = is_device_ptr(V);
+    if (precision == BRPA_SPLIT_I8) {
+        // fused split-int8 pair kernel: 5 int8 slices per value, 15 products,
+        // exact int32 level sums, W_ij and W_ji never stored
+        const int64_t nparts = sosex_i8_nparts(o, N, nt);
+        double *dR, *dV, *dB, *scR, *scV, *part, *out2;
+        int8_t *SR, *SV;
+        auto layout = [&](Arena &ar) {
+            dR = rdev ? const_cast<double *>(R) : ar.take<double>(K * N);
+            dV = vdev ? const_cast<double *>(V) : ar.take<double>(K * nt);
+            dB = ar.take<double>(nt);
+            scR = ar.take<double>(o * N);
+            scV = ar.take<double>(o * nt);
+            SR = ar.take<int8_t>(sosex_i8_slice_bytes(o, N, v));
+            SV = ar.take<int8_t>(sosex_i8_slice_bytes(o, nt, v));
+            part = ar.take<double>(2 * nparts);
+            out2 = ar.take<double>(2);
+        };
+        Arena count;
+        layout(count);
+        Arena ar;
+        BRPA_TRY(ws_reserve(ctx, count.off, &ar));
+        layout(ar);
+        if (!rdev) BRPA_TRY(to_device(ctx, dR, R, 8 * K * N));
+        if (!vdev) BRPA_TRY(to_device(ctx, dV, V, 8 * K * nt));
+        BRPA_HIP(ctx, hipMemcpyAsync(dB, hB.data(), 8 * nt, hipMemcpyHostToDevice, ctx->stream));
+        BRPA_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+        BRPA_TRY(launch_sosex_i8_split(ctx, dR, o, v, N, scR, SR));
+        BRPA_TRY(launch_sosex_i8_split(ctx, dV, o, v, nt, scV, SV));
+        BRPA_TRY(launch_sosex_i8_pairs(ctx, SR, SV, scR, scV, dB, o, v, N, nt, part, out2));
+        BRPA_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+        double h2[2];
+        BRPA_TRY(copy_sync(ctx, h2, out2, sizeof(h2), hipMemcpyDeviceToHost));
+        float ms = 0.f;
+        BRPA_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        ctx->last_kernel_ms = ms;
+        // EcExchange1 = -sum W_ji W_ij B per pair; the partials carry (2 - delta_ij)
+        *ec_exchange = (double)(-(long double)alpha * ((long double)h2[0] + (long double)h2[1]));
+        return BRPA_OK;
+    }
     // pairs (i, j >= i) are processed per i in batches of jb occupied indices
     const size_t wbytes_per_j = 2 * 8 * (size_t)N * nt;
     int64_t jb = std::max<int64_t>(1, std::min<int64_t>(o, (int64_t)((8ull << 30) / wbytes_per_j)));

@@ -201,6 +201,20 @@ int brpa_sosex(brpa_ctx *ctx, int precision, double *ec_exchange, double alpha, 
                const double *A, const double *R, int64_t nocc, int64_t nvir, int64_t naux,
                int64_t nvecs_t2, int s2_operator);
 
+/* Exchange correction of many fragments: brpa_t2_compute followed by brpa_sosex
+ * on the device-resident amplitudes, per fragment (the reference's
+ * rpa_CC.f90:884-892 then 963-976 sequence, subsystem after subsystem).
+ * Fragment f uses GuessNVecsT2 = ceil(guess_factor * naux) (the reference's
+ * default factor is 3.0, rpa_definitions.f90:428) and guess seed seed + f.
+ * The fragments run one after another on a worker context; nworkers must be 0
+ * or 1 (several in flight on host threads gave wrong rocBLAS/rocSOLVER results
+ * and is disabled).  ec_exchange[f] receives E_x; nvecs_t2 (may be NULL) the
+ * retained NT2.  precision: BRPA_FP64 only. */
+int brpa_exchange_batch(brpa_ctx *ctx, int precision, int64_t nfrag, const brpa_frag_t *frags,
+                        double lambda, double cutoff, double guess_factor, uint64_t seed,
+                        double alpha, int s2_operator, int nworkers, double *ec_exchange,
+                        int64_t *nvecs_t2);
+
 /* ---- adjoint ----------------------------------------------------------------- */
 /* dE_RPA/dR (C R[ai][P], same shape as R) and dE_RPA/d(eps_occ), dE_RPA/d(eps_vir)
  * of the FP64 energy of brpa_ecrpa, with the frequency grid held fixed.  Pi(u)

@@ -71,3 +71,64 @@ def test_rpt2_chain_matches_oracle(ctx):
     ex = ctx.sosex(V, A, f.R, o, v)
     print("chain Ex", ex, ex_ref, ex - ex_ref)
     assert abs(ex - ex_ref) <= 1e-8
+
+
+def test_exchange_batch_matches_sequential(ctx):
+    """brpa_exchange_batch gives the same E_x, bit for bit, as T2 + SOSEX
+    called one fragment at a time, and matches the oracle; more than one
+    worker is rejected."""
+    import math
+    from oracle import erpa as oe, sosex as osx, t2 as ot
+    shapes = [(3, 12, 40), (4, 20, 90), (2, 15, 50), (5, 18, 100), (3, 25, 70)]
+    frags, seq, refs = [], [], []
+    for i, (o, v, n) in enumerate(shapes):
+        f = synth.make_fragment(o, v, n, 900 + i)
+        x, w, _ = grid.freq_grid(f.eocc, f.evir, 8)
+        frags.append((f.R, f.eocc, f.evir, x, w))
+        A, V = ctx.t2(f.R, f.eocc, f.evir, x, w, nguess=math.ceil(3.0 * n), seed=77 + i)
+        seq.append(ctx.sosex(V, A, f.R, o, v))
+        g = f.gaps()
+        Pis = [oe.gpiug_mo(f.R, g, xx) for xx in x]
+        Ao, Vo = ot.t2_v5(f.R, g, x, w, Pis, nguess=math.ceil(3.0 * n), seed=77 + i)[:2]
+        refs.append(osx.sosex(f.R, Vo, Ao, o, v))
+    ex, nt = ctx.exchange_batch(frags, seed=77)
+    assert np.array_equal(ex, np.array(seq)), ex - np.array(seq)
+    import brpa
+    with pytest.raises(brpa.BrpaError) as ei:
+        ctx.exchange_batch(frags[:2], seed=77, nworkers=3)
+    assert ei.value.code == brpa._lib.ERR_ARG
+    assert np.max(np.abs(ex - np.array(refs))) <= 1e-8
+    assert np.all(nt > 0)
+
+
+@pytest.mark.parametrize("o,v,n", [(3, 12, 40), (1, 5, 130), (4, 70, 200), (6, 33, 97)])
+def test_sosex_split_i8_matches_oracle(ctx, o, v, n):
+    """Fused split-int8 SOSEX (5 int8 slices, 15 products) within 1e-7 Ha of
+    the FP64 oracle, both B = A and the S2 (cumulant) rescaling."""
+    f = synth.make_fragment(o, v, n, 40 + o)
+    x, w, _ = grid.freq_grid(f.eocc, f.evir, 8)
+    A, V = ctx.t2(f.R, f.eocc, f.evir, x, w, seed=3)
+    for s2 in (False, True):
+        ref = sosex.sosex(f.R, V, A, o, v, s2_operator=s2)
+        e64 = ctx.sosex(V, A, f.R, o, v, s2_operator=s2)
+        e8 = ctx.sosex(V, A, f.R, o, v, s2_operator=s2, precision="split_i8")
+        print("o=%d v=%d n=%d s2=%d: ref %.10f fp64 %.2e i8 %.2e" % (o, v, n, s2, ref, e64 - ref,
+                                                                       e8 - ref))
+        assert abs(e64 - ref) <= 1e-8
+        assert abs(e8 - ref) <= 1e-7
+
+
+def test_sosex_split_i8_config5_vs_fp64(ctx):
+    """o=64, v=600, NAux=3200, 16 frequencies, NT2 = 3 NAux: split-int8 SOSEX
+    within 1e-7 Ha of the FP64 GPU contraction of the same amplitudes."""
+    import math
+    torch = pytest.importorskip("torch")
+    o, v, n = 64, 600, 3200
+    f = synth.make_fragment(o, v, n, synth.BASE_SEED)
+    x, w, _ = grid.freq_grid(f.eocc, f.evir, 16)
+    Rd = torch.from_numpy(f.R).cuda()
+    A, V = ctx.t2(Rd, f.eocc, f.evir, x, w, nguess=math.ceil(3.0 * n), seed=1, device=True)
+    e64 = ctx.sosex(V, A, Rd, o, v)
+    e8 = ctx.sosex(V, A, Rd, o, v, precision="split_i8")
+    print("config 5: NT2 %d  E_x fp64 %.10f  split_i8 - fp64 = %.3e" % (A.numel(), e64, e8 - e64))
+    assert abs(e8 - e64) <= 1e-7
