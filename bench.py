@@ -54,6 +54,62 @@ def chol_flops(n, nf):
     return nf * n ** 3 / 3.0
 
 
+def bench_sosex(a):
+    """SOSEX contraction of one large fragment (o=64, v=600, NAux=3200, 16
+    frequencies, amplitudes with NGuess = 3 NAux built once in FP64): FP64
+    (rocBLAS pair GEMMs) vs the fused split-int8 kernel, |dE| between them."""
+    import math
+
+    import numpy as np
+    import torch
+
+    import brpa
+    from brpa import synth
+    if a.gpus != 1:
+        print("bench.py measures one GPU only (--gpus %d requested)" % a.gpus, file=sys.stderr)
+        sys.exit(2)
+    o, v, n = 64, 600, 3200
+    f = synth.make_fragment(o, v, n, synth.BASE_SEED)
+    x, w, _ = brpa.freq_grid(f.eocc, f.evir, 16)
+    Rd = torch.from_numpy(f.R).cuda()
+    ctx = brpa.Context(0)
+    A, V = ctx.t2(Rd, f.eocc, f.evir, x, w, nguess=math.ceil(3.0 * n), seed=1, device=True)
+    nt = A.numel()
+    res = {}
+    for prec in ("split_i8", "fp64"):
+        for _ in range(a.warmup):
+            ctx.sosex(V, A, Rd, o, v, precision=prec)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        kms = 0.0
+        for _ in range(a.steps):
+            e = ctx.sosex(V, A, Rd, o, v, precision=prec)
+            kms += ctx.last_kernel_ms()
+        torch.cuda.synchronize()
+        res[prec] = (e, (time.perf_counter() - t) / a.steps, kms / a.steps)
+        print("# %s: E_x %.10f  %.1f ms/step (kernels %.1f ms)" % (prec, e, res[prec][1] * 1e3,
+                                                                  res[prec][2]), file=sys.stderr)
+    flops = 2.0 * 2 * (o * (o + 1) // 2) * n * nt * v   # two W GEMMs per pair (i >= j)
+    e8, dt8, k8 = res["split_i8"]
+    e64, dt64, k64 = res["fp64"]
+    out = {"metric": "E_SOSEX contraction time, split-int8 vs FP64 at |dE|<=1e-7 Ha",
+           "value": round(dt8 * 1e3, 3), "unit": "ms", "n_gpus": 1, "steps": a.steps,
+           "warmup": a.warmup, "ms_per_step": round(dt8 * 1e3, 3), "higher_is_better": False,
+           "scaling": "strong", "vs_baseline": None,
+           "dtype": "i8 (5-slice split, 15 products, exact i32 acc) + f64 tile sums",
+           "data": "synthetic (seeded PCG64, brpa.synth)",
+           "config": {"workload": "SOSEX of one fragment o=64, v=600, NAux=3200, 16 "
+                                  "frequencies, NT2=%d" % nt},
+           "fp64_ms": round(dt64 * 1e3, 3), "speedup_vs_fp64": round(dt64 / dt8, 3),
+           "abs_dE_vs_fp64_Ha": abs(e8 - e64), "E_x_fp64": e64,
+           "roofline": {"bound": "mfma", "achieved": round(flops / (k8 * 1e-3) / 1e12, 3),
+                        "peak": round(PEAK_I8_TOPS / 15, 3), "unit": "TFLOP/s",
+                        "frac": round(flops / (k8 * 1e-3) / 1e12 / (PEAK_I8_TOPS / 15), 4),
+                        "traffic": None, "kernel": "k_sosex_i8 (+ slicing)",
+                        "fp64_achieved": round(flops / (k64 * 1e-3) / 1e12, 3)}}
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -72,6 +128,10 @@ def main():
                     help="fragments in the T2 + SOSEX sample (0 skips the exchange part)")
     ap.add_argument("--exchange-workers", type=int, default=1,
                     help="worker count passed to brpa_exchange_batch (only 1 is supported)")
+    ap.add_argument("--workload", default="batch", choices=["batch", "sosex"],
+                    help="batch: the 256-fragment E_RPA + exchange batch (default); sosex: "
+                         "the exchange contraction of one o=64, v=600, NAux=3200 fragment, "
+                         "FP64 vs split-int8")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy 5 times (reference for the "
                          "FETCH_SIZE/WRITE_SIZE counter scale)")
@@ -87,6 +147,8 @@ def main():
         print(json.dumps({"copy_bytes_read_per_launch": 8 * n, "copy_bytes_written_per_launch": 8 * n,
                           "launches": 5}))
         return
+    if a.workload == "sosex":
+        return bench_sosex(a)
     if a.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(--gpus %d requested)" % a.gpus, file=sys.stderr)
@@ -163,16 +225,19 @@ def main():
         # T2 + SOSEX of the first fragments of the batch, several in flight
         kx = min(a.exchange_frags, a.nfrag)
         sub = batch[:kx]
-        ctx.exchange_batch(sub[:min(kx, a.exchange_workers)], seed=synth.BASE_SEED,
-                           nworkers=a.exchange_workers)          # warm-up (workspaces)
+        # SOSEX contraction in the headline form's arithmetic where it exists
+        xprec = "split_i8" if a.precision == "split_i8" else "fp64"
+        ctx.exchange_batch(sub[:1], seed=synth.BASE_SEED, nworkers=a.exchange_workers,
+                           precision=xprec)                      # warm-up (workspaces)
         torch.cuda.synchronize()
         t1 = time.perf_counter()
-        exs, nts = ctx.exchange_batch(sub, seed=synth.BASE_SEED, nworkers=a.exchange_workers)
+        exs, nts = ctx.exchange_batch(sub, seed=synth.BASE_SEED, nworkers=a.exchange_workers,
+                                      precision=xprec)
         torch.cuda.synchronize()
         t_ex = (time.perf_counter() - t1) / kx
         exch = {"n_fragments_timed": kx, "workers": a.exchange_workers,
                 "ms_per_fragment": round(t_ex * 1e3, 3),
-                "precision": "fp64 (T2 amplitudes and SOSEX contraction)",
+                "precision": "fp64 T2 amplitudes, %s SOSEX contraction" % xprec,
                 "nguess": "ceil(3.0 N)", "mean_nt2": float(np.mean(nts))}
         for p, d in forms.items():
             d["with_exchange_fps"] = 1.0 / (d["ms_per_step"] * 1e-3 / a.nfrag + t_ex)
@@ -219,8 +284,9 @@ def main():
         "config": {"workload": "batch of %d uneven fragments, o~U{8..40}, v~U{70..350}, "
                                "N=round(4.5(o+v)), %d frequencies" % (a.nfrag, a.nfreqs),
                    "precision": a.precision,
-                   "components": ("E_RPA (batched, %s) + SOSEX exchange (T2 + contraction, fp64)"
-                                  % a.precision) if exch else "E_RPA only",
+                   "components": ("E_RPA (batched, %s) + SOSEX exchange (fp64 T2, %s "
+                                  "contraction)" % (a.precision, exch["precision"].split()[-3]))
+                                 if exch else "E_RPA only",
                    "graphs": bool(a.graphs),
                    "algorithmic_tflop_per_step": round((tot_syrk + tot_chol) / 1e12, 3)},
         "erpa_only_fragments_per_sec": round(a.nfrag / dt, 4),

@@ -433,6 +433,29 @@ int run_ecrpa(brpa_ctx *ctx, int precision, const std::vector<HostFrag> &fr, dou
 
 }  // namespace
 
+int64_t chol_pre_len(int64_t N) { return 3 * cdiv(N, CHOL_NB) * 2; }
+
+int chol_single(brpa_ctx *ctx, const FragDev *d_fr, int64_t *d_pre, int64_t N, int64_t F) {
+    const int nsteps = (int)cdiv(N, CHOL_NB);
+    // prefix arrays [0, count] per launch: pdiag_0..S-1, trsm_0..S-1, update_0..S-1
+    std::vector<int64_t> pre((size_t)chol_pre_len(N), 0);
+    for (int j = 0; j < nsteps; ++j) {
+        const int64_t j0 = (int64_t)j * CHOL_NB;
+        pre[2 * j + 1] = (N > j0) ? F : 0;
+        pre[2 * (nsteps + j) + 1] = (N > j0 + CHOL_NB) ? F * (cdiv(N - j0, CHOL_NB) - 1) : 0;
+        const int64_t t = (N > j0 + CHOL_NB) ? cdiv(N - j0 - CHOL_NB, 128) : 0;
+        pre[2 * (2 * nsteps + j) + 1] = F * (t * (t + 1) / 2);
+    }
+    BRPA_TRY(copy_sync(ctx, d_pre, pre.data(), 8 * pre.size(), hipMemcpyHostToDevice));
+    for (int j = 0; j < nsteps; ++j) {
+        BRPA_TRY(launch_potrf_diag(ctx, d_fr, d_pre + 2 * j, 1, pre[2 * j + 1], j));
+        BRPA_TRY(launch_trsm(ctx, d_fr, d_pre + 2 * (nsteps + j), 1, pre[2 * (nsteps + j) + 1], j));
+        BRPA_TRY(launch_chol_update(ctx, d_fr, d_pre + 2 * (2 * nsteps + j), 1,
+                                    pre[2 * (2 * nsteps + j) + 1], j));
+    }
+    return BRPA_OK;
+}
+
 extern "C" {
 
 int brpa_ecrpa(brpa_ctx *ctx, int precision, const double *R, int64_t naux, const double *eocc,

@@ -41,7 +41,7 @@ int brpa_exchange_batch(brpa_ctx *ctx, int precision, int64_t nfrag, const brpa_
         const int64_t ng = (int64_t)std::ceil(guess_factor * (double)fr.naux);
         int64_t nt = 0;
         double ex = 0.0;
-        int rc = brpa_t2_compute(w, precision, fr.R, fr.naux, fr.eocc, fr.nocc, fr.evir, fr.nvir,
+        int rc = brpa_t2_compute(w, BRPA_FP64, fr.R, fr.naux, fr.eocc, fr.nocc, fr.evir, fr.nvir,
                                  fr.freqs, fr.weights, fr.nfreqs, lambda, cutoff, ng,
                                  seed + (uint64_t)f, &nt);
         if (rc == BRPA_OK && nt > 0)

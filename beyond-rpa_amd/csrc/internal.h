@@ -135,6 +135,10 @@ struct FragDev {
     int8_t *sl8;             // F x NSLICE x N x Kpad8, per column P the Kpad8 rows contiguous
     double *scale8;          // F x N, column scale max_k |X_w[k][P]| / 127
     int Kpad8;
+    // 1: the Cholesky also writes each factored 64x64 diagonal block back into
+    // Pi, so the lower triangle holds the complete factor L (for triangular
+    // solves after the factorisation)
+    int keep_l11;
 };
 
 constexpr int CHOL_NB = 64;   // Cholesky block size
@@ -167,6 +171,12 @@ enum { I8V_T128 = 0, I8V_T128_XCD = 1, I8V_T64 = 2, I8V_COUNT = 3 };
 int syrk_i8_tile(int variant);
 int launch_syrk_i8(brpa_ctx *, const FragDev *, const int64_t *pre, int nfrag, int64_t total,
                    int variant);
+
+// Blocked Cholesky of the F matrices d_fr->Pi (1 + Pi already formed) of one
+// fragment, in place on ctx->stream; status/linv/ldpart of the descriptor
+// must be allocated.  d_pre: device scratch of chol_pre_len(N) int64.
+int64_t chol_pre_len(int64_t N);
+int chol_single(brpa_ctx *ctx, const FragDev *d_fr, int64_t *d_pre, int64_t N, int64_t F);
 
 // Generic single-matrix FP64 SYRK used by brpa_gpiug_mo:
 //   C(N,N) (+)= X^T diag(d) X, X K x N row-major, both triangles written.

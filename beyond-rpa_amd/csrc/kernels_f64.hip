@@ -385,6 +385,12 @@ __global__ __launch_bounds__(64) void k_potrf_diag(const FragDev *__restrict__ f
         const double x = (i < lane) ? 0.0 : (i == lane ? Xd[lane] : L[lane][i]);
         linv[i * CHOL_NB + lane] = x;
     }
+    if (F.keep_l11 && lane < nbj) {
+        // the factored block back into Pi (lower triangle and diagonal); no other
+        // work-group reads this diagonal block
+        double *Aw = F.Pi + (int64_t)w * N * N + (int64_t)(j0 + lane) * N + j0;
+        for (int c = 0; c <= lane; ++c) Aw[c] = L[lane][c];
+    }
     if (lane == 0) {
         dd_t s = {0.0, 0.0};
         for (int c = 0; c < nbj; ++c) s = dd_add(s, log(L[c][c]));

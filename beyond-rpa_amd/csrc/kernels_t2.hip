@@ -123,6 +123,19 @@ int launch_scale(brpa_ctx *ctx, double *x, double a, int64_t n) {
     return BRPA_OK;
 }
 
+// M(r, r) += a for the n x n matrix M
+__global__ void k_add_diag(double *__restrict__ M, int64_t n, double a) {
+    int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (r < n) M[r + n * r] += a;
+}
+
+int launch_add_diag(brpa_ctx *ctx, double *M, int64_t n, double a) {
+    if (n <= 0) return BRPA_OK;
+    hipLaunchKernelGGL(k_add_diag, dim3(nblocks(n)), dim3(256), 0, ctx->stream, M, n, a);
+    LAUNCH_CHECK(ctx);
+    return BRPA_OK;
+}
+
 // SOSEX pair contraction (rpa_CC_EcExchange1, rpa/rpa_CC_Exchange.f90:100-116):
 // part[b * nchunk + c] = sum over chunk c of W1_b(P,mu) W2_b(P,mu) B[mu], for
 // batch b of npair (N x nt) matrix pairs (stride ld between batches).  Double-

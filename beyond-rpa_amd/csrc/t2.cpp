@@ -27,6 +27,7 @@ int launch_row_scale_acc(brpa_ctx *, double *io, const double *in, const double 
 int launch_select_scale(brpa_ctx *, double *W, const double *U, const double *ev, int64_t m,
                         int64_t c0, int64_t nt);
 int launch_scale(brpa_ctx *, double *x, double a, int64_t n);
+int launch_add_diag(brpa_ctx *, double *M, int64_t n, double a);
 int launch_pair_dot(brpa_ctx *, const double *W1, const double *W2, const double *B, int64_t n_rows,
                     int64_t nt, int64_t stride, int npair, double *part);
 int sosex_nchunk(int64_t n_rows, int64_t nt);
@@ -185,6 +186,7 @@ int brpa_t2_compute(brpa_ctx *ctx, int precision, const double *R, int64_t naux,
     FragDev fd{};
     double *dR = nullptr, *dchi, *omega, *tmp, *iom, *Y, *Hb, *Sm, *sev, *sE;
     int *dinfo;
+    int64_t *dchol = nullptr;
     std::vector<int64_t> pre = {0, 0, 0, 0, 0, 0};
     const int64_t ntile = cdiv(N, SYRK_T);
     pre[1] = cdiv(F * K, 256);                      // dtab
@@ -213,6 +215,10 @@ int brpa_t2_compute(brpa_ctx *ctx, int precision, const double *R, int64_t naux,
         sev = ar.take<double>(NG);
         sE = ar.take<double>(NG);
         dinfo = ar.take<int>(F + 2);
+        fd.linv = ar.take<double>(F * CHOL_NB * CHOL_NB);
+        fd.ldpart = ar.take<double>(F * nsteps * 2);
+        fd.status = ar.take<int>(F);
+        dchol = ar.take<int64_t>(chol_pre_len(N));
     };
     Arena count;
     layout(count);
@@ -226,7 +232,9 @@ int brpa_t2_compute(brpa_ctx *ctx, int precision, const double *R, int64_t naux,
     fd.nocc = (int)nocc;
     fd.nvir = (int)nvir;
     fd.nsteps = (int)nsteps;
+    fd.keep_l11 = 1;
     if (!rdev) BRPA_TRY(to_device(ctx, dR, R, 8 * K * N));
+    BRPA_TRY(launch_fill_int(ctx, fd.status, F, 0));
     BRPA_HIP(ctx, hipMemcpyAsync(const_cast<double *>(fd.weights), fw.data(), 8 * F,
                                  hipMemcpyHostToDevice, ctx->stream));
     BRPA_HIP(ctx, hipMemcpyAsync(const_cast<double *>(fd.freqs), fx.data(), 8 * F,
@@ -246,11 +254,11 @@ int brpa_t2_compute(brpa_ctx *ctx, int precision, const double *R, int64_t naux,
     BRPA_TRY(launch_syrk_f64(ctx, dfd, dpre + 2, 1, pre[3]));
     if (lambda != 1.0) BRPA_TRY(launch_scale(ctx, fd.Pi, lambda, F * N * N));
     BRPA_TRY(launch_set_diag(ctx, dfd, dpre + 4, 1, pre[5], 0));
-    for (int64_t k = 0; k < F; ++k)
-        BRPA_TRY(rb_check(ctx,
-                          rocsolver_dpotrf(h, rocblas_fill_lower, (rocblas_int)N, fd.Pi + k * N * N,
-                                           (rocblas_int)N, dinfo + k),
-                          "rocsolver_dpotrf"));
+    // all F factors in one batched blocked Cholesky (the library's own kernels).
+    // The factor L is the lower triangle of the row-major matrix, i.e. the
+    // upper triangle U = L^T of the column-major view the solvers below read
+    // (A = U^T U): hence rocblas_fill_upper there.
+    BRPA_TRY(chol_single(ctx, dfd, dchol, N, F));
 
     // I * Omega (rpa_CC_IM_v5, rpa/rpa_CC_Doubles.f90:369-422)
     BRPA_TRY(launch_guess(ctx, omega, K, NG, seed));
@@ -261,7 +269,7 @@ int brpa_t2_compute(brpa_ctx *ctx, int precision, const double *R, int64_t naux,
         BRPA_TRY(launch_row_scale(ctx, tmp, omega, ck, K, NG));
         BRPA_TRY(gemm(ctx, h, false, false, N, NG, K, 1.0, dR, N, tmp, K, 0.0, Y, N));
         BRPA_TRY(rb_check(ctx,
-                          rocsolver_dpotrs(h, rocblas_fill_lower, (rocblas_int)N, (rocblas_int)NG,
+                          rocsolver_dpotrs(h, rocblas_fill_upper, (rocblas_int)N, (rocblas_int)NG,
                                            fd.Pi + k * N * N, (rocblas_int)N, Y, (rocblas_int)N),
                           "rocsolver_dpotrs"));
         BRPA_TRY(gemm(ctx, h, true, false, K, NG, N, 1.0, dR, N, Y, N, 0.0, tmp, K));
@@ -269,17 +277,62 @@ int brpa_t2_compute(brpa_ctx *ctx, int precision, const double *R, int64_t naux,
     }
     BRPA_TRY(launch_normalize_columns(ctx, iom, K, NG));
 
-    // S = (I Omega)^T (I Omega), eigenpairs ascending (rpa_CC_Doubles.f90:595-609)
+    // S = (I Omega)^T (I Omega) (rpa_CC_Doubles.f90:595-609).  The reference
+    // keeps the eigenvectors of S with eigenvalues above the cutoff and forms
+    // the orthonormal basis T = (I Omega) U_s s^{-1/2}.  When every eigenvalue
+    // exceeds the cutoff -- exactly when S - cutoff*1 has a Cholesky factor --
+    // T spans all of span(I Omega), and any orthonormal basis of that span
+    // gives the same Rayleigh-Ritz pairs below; it is then built by Cholesky-QR
+    // (the shifted factor, then two unshifted passes) instead of the NG x NG
+    // eigensolve.  Otherwise the eigensolve decides which vectors are kept.
     BRPA_TRY(gemm(ctx, h, true, false, NG, NG, K, 1.0, iom, K, iom, K, 0.0, Sm, NG));
-    BRPA_TRY(syevd(ctx, h, NG, Sm, sev, sE, dinfo + F));
-    std::vector<double> hs(NG);
-    BRPA_TRY(copy_sync(ctx, hs.data(), sev, 8 * NG, hipMemcpyDeviceToHost));
+    double *Tkeep = iom;           // K x nt orthonormal basis (in place of I Omega)
     int64_t nt = 0;
-    for (int64_t i = NG - 1; i >= 0; --i) {
-        if (hs[i] > cutoff)
-            ++nt;
-        else
-            break;
+    bool cholqr = false;
+    {
+        BRPA_TRY(launch_add_diag(ctx, Sm, NG, -cutoff));
+        BRPA_TRY(rb_check(ctx, rocsolver_dpotrf(h, rocblas_fill_lower, (rocblas_int)NG, Sm,
+                                                (rocblas_int)NG, dinfo + F),
+                          "rocsolver_dpotrf"));
+        int hinfo = 0;
+        BRPA_TRY(copy_sync(ctx, &hinfo, dinfo + F, sizeof(int), hipMemcpyDeviceToHost));
+        cholqr = (hinfo == 0);
+    }
+    const double one = 1.0;
+    if (cholqr) {
+        nt = NG;
+        for (int pass = 0; pass < 3; ++pass) {
+            if (pass > 0) {
+                BRPA_TRY(gemm(ctx, h, true, false, NG, NG, K, 1.0, iom, K, iom, K, 0.0, Sm, NG));
+                BRPA_TRY(rb_check(ctx, rocsolver_dpotrf(h, rocblas_fill_lower, (rocblas_int)NG,
+                                                        Sm, (rocblas_int)NG, dinfo + F),
+                                  "rocsolver_dpotrf"));
+                int hinfo = 0;
+                BRPA_TRY(copy_sync(ctx, &hinfo, dinfo + F, sizeof(int), hipMemcpyDeviceToHost));
+                if (hinfo != 0)
+                    return brpa_fail(ctx, BRPA_ERR_SOLVER,
+                                     "Cholesky-QR pass %d of the T2 basis broke down", pass);
+            }
+            // X <- X L^{-T}
+            BRPA_TRY(rb_check(ctx,
+                              rocblas_dtrsm(h, rocblas_side_right, rocblas_fill_lower,
+                                            rocblas_operation_transpose,
+                                            rocblas_diagonal_non_unit, (rocblas_int)K,
+                                            (rocblas_int)NG, &one, Sm, (rocblas_int)NG, iom,
+                                            (rocblas_int)K),
+                              "rocblas_dtrsm"));
+        }
+    } else {
+        BRPA_TRY(gemm(ctx, h, true, false, NG, NG, K, 1.0, iom, K, iom, K, 0.0, Sm, NG));
+        BRPA_TRY(syevd(ctx, h, NG, Sm, sev, sE, dinfo + F));
+        std::vector<double> hs(NG);
+        BRPA_TRY(copy_sync(ctx, hs.data(), sev, 8 * NG, hipMemcpyDeviceToHost));
+        for (int64_t i = NG - 1; i >= 0; --i) {
+            if (hs[i] > cutoff)
+                ++nt;
+            else
+                break;
+        }
     }
     *nvecs_t2 = nt;
     ctx->t2_nt = nt;
@@ -287,14 +340,15 @@ int brpa_t2_compute(brpa_ctx *ctx, int precision, const double *R, int64_t naux,
     if (nt == 0) return BRPA_OK;
 
     // T = (I Omega) U_s s^{-1/2}; TIT = T^T I T (rpa_CC_MIM_v5, 425-488)
-    double *Wm = omega;            // NG x nt   (omega no longer needed)
     double *T = tmp;               // K x nt
     double *G = Y;                 // N x nt
     double *H = Hb;                // N x nt
-    BRPA_TRY(launch_select_scale(ctx, Wm, Sm, sev, NG, NG - nt, nt));
-    BRPA_TRY(gemm(ctx, h, false, false, K, nt, NG, 1.0, iom, K, Wm, NG, 0.0, T, K));
-    double *Tkeep = iom;           // K x nt copy of T (iom is free now)
-    BRPA_HIP(ctx, hipMemcpyAsync(Tkeep, T, 8 * K * nt, hipMemcpyDeviceToDevice, ctx->stream));
+    if (!cholqr) {
+        double *Wm = omega;        // NG x nt   (omega no longer needed)
+        BRPA_TRY(launch_select_scale(ctx, Wm, Sm, sev, NG, NG - nt, nt));
+        BRPA_TRY(gemm(ctx, h, false, false, K, nt, NG, 1.0, iom, K, Wm, NG, 0.0, T, K));
+        BRPA_HIP(ctx, hipMemcpyAsync(Tkeep, T, 8 * K * nt, hipMemcpyDeviceToDevice, ctx->stream));
+    }
     double *TIT = Sm;              // nt x nt
     for (int64_t k = 0; k < F; ++k) {
         const double *ck = dchi + k * K;
@@ -302,7 +356,7 @@ int brpa_t2_compute(brpa_ctx *ctx, int precision, const double *R, int64_t naux,
         BRPA_TRY(gemm(ctx, h, false, false, N, nt, K, 1.0, dR, N, T, K, 0.0, G, N));
         BRPA_HIP(ctx, hipMemcpyAsync(H, G, 8 * N * nt, hipMemcpyDeviceToDevice, ctx->stream));
         BRPA_TRY(rb_check(ctx,
-                          rocsolver_dpotrs(h, rocblas_fill_lower, (rocblas_int)N, (rocblas_int)nt,
+                          rocsolver_dpotrs(h, rocblas_fill_upper, (rocblas_int)N, (rocblas_int)nt,
                                            fd.Pi + k * N * N, (rocblas_int)N, H, (rocblas_int)N),
                           "rocsolver_dpotrs"));
         BRPA_TRY(gemm(ctx, h, true, false, nt, nt, N, -fw[k] * lambda / (4.0 * kPi), G, N, H, N,
@@ -319,7 +373,7 @@ int brpa_t2_compute(brpa_ctx *ctx, int precision, const double *R, int64_t naux,
     BRPA_TRY(gemm(ctx, h, true, true, nt, K, nt, 1.0, TIT, nt, Tkeep, K, 0.0, ctx->t2_V, nt));
     BRPA_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
     std::vector<int> info(F);
-    BRPA_HIP(ctx, hipMemcpyAsync(info.data(), dinfo, sizeof(int) * F, hipMemcpyDeviceToHost,
+    BRPA_HIP(ctx, hipMemcpyAsync(info.data(), fd.status, sizeof(int) * F, hipMemcpyDeviceToHost,
                                  ctx->stream));
     BRPA_HIP(ctx, hipStreamSynchronize(ctx->stream));
     float ms = 0.f;

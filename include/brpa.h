@@ -196,7 +196,11 @@ int brpa_t2_copyout(brpa_ctx *ctx, double *A, double *V);
  * B = A (or A/(1-4A^2) when s2_operator), evaluated pair by pair
  * W_ij = R_i V_j^T.  Replaces rpa_CC_Exchange_SOSEX
  * (rpa/rpa_CC_Exchange.f90:36-97).  V: C V[ai][mu] (Fortran Vkai(NVecsT2, v, o)),
- * R: C R[ai][P] (Fortran Rkai(NVecsPiU, v, o)).  precision: BRPA_FP64 only. */
+ * R: C R[ai][P] (Fortran Rkai(NVecsPiU, v, o)).  precision: BRPA_FP64 (the
+ * drop-in: rocBLAS FP64 GEMMs per pair block, double-double pair sums) or
+ * BRPA_SPLIT_I8 (one fused gfx950 kernel: W_ij and W_ji tiles from 5 int8
+ * slices per value, 15 products with exact int32 level sums, FP64
+ * combination, double-double tile sums; W is never stored; nvir <= 65536). */
 int brpa_sosex(brpa_ctx *ctx, int precision, double *ec_exchange, double alpha, const double *V,
                const double *A, const double *R, int64_t nocc, int64_t nvir, int64_t naux,
                int64_t nvecs_t2, int s2_operator);
@@ -209,7 +213,8 @@ int brpa_sosex(brpa_ctx *ctx, int precision, double *ec_exchange, double alpha, 
  * The fragments run one after another on a worker context; nworkers must be 0
  * or 1 (several in flight on host threads gave wrong rocBLAS/rocSOLVER results
  * and is disabled).  ec_exchange[f] receives E_x; nvecs_t2 (may be NULL) the
- * retained NT2.  precision: BRPA_FP64 only. */
+ * retained NT2.  precision: arithmetic of the SOSEX contraction (BRPA_FP64 or
+ * BRPA_SPLIT_I8); the amplitudes are built in FP64. */
 int brpa_exchange_batch(brpa_ctx *ctx, int precision, int64_t nfrag, const brpa_frag_t *frags,
                         double lambda, double cutoff, double guess_factor, uint64_t seed,
                         double alpha, int s2_operator, int nworkers, double *ec_exchange,
