@@ -45,6 +45,22 @@ PI_KERNEL = {"split_i8": ("k_syrk_i8", PEAK_I8_TOPS / SPLIT_PRODUCTS),
              "fp64": ("k_syrk_f64", PEAK_FP64_TFLOPS)}
 
 
+def pmc_traffic(kernel, profile="erpa_batch256_split_i8_kernels.md"):
+    """HBM bytes per launch (read + written) of `kernel` from the committed
+    rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs (copy-calibrated), or None."""
+    path = os.path.join(ROOT, "profiles", profile)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 6 and cells[0] == kernel:
+            try:
+                return round((float(cells[4]) + float(cells[5])) * 1e9)
+            except ValueError:
+                return None
+    return None
+
+
 def syrk_flops(o, v, n, nf):
     """Algorithmic Pi(u) flops of one fragment: F * ov * N (N+1) (SYRK, FMA = 2)."""
     return nf * o * v * n * (n + 1)
@@ -215,8 +231,11 @@ def main():
     else:
         kernel, peak = "k_potrf_diag+k_trsm+k_chol_update", PEAK_FP64_TFLOPS
         achieved = tot_chol / (stage_ms["chol"] * 1e-3) / 1e12
+    traffic = pmc_traffic(kernel + "<128>") if kernel == "k_syrk_i8" else None
     roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": round(peak, 3),
-                "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": None,
+                "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": traffic,
+                "traffic_source": ("profiles/erpa_batch256_split_i8_kernels.md (FETCH_SIZE x 2.0 "
+                                   "+ WRITE_SIZE x 1.0, copy-calibrated)") if traffic else None,
                 "kernel": kernel, "kernel_ms_per_step": round(stage_ms[dom], 3),
                 "launches_per_step": stage_n[dom]}
 

@@ -211,10 +211,11 @@ contains
 
 
       subroutine brpa_ecrpa_fused(EcRPA, Rkai, NVecs, OccEnergies, VirtEnergies, NOcc, NVirt, &
-            Freqs, FreqWeights, NFreqs, SplitBF16)
+            Freqs, FreqWeights, NFreqs, SplitBF16, Precision)
             !
             ! Pi(u) build + E_RPA in one call, Pi kept on the GPU
             ! (rpa_CC_PiU frequency loop, rpa_CC.f90:523-526, then rpa_CC_EcRPA_Analytic).
+            ! Precision (optional, overrides SplitBF16): 0 FP64, 1 split-BF16, 2 split-int8.
             !
             real(F64), intent(out)                             :: EcRPA
             real(F64), dimension(:, :), intent(in), contiguous :: Rkai
@@ -222,10 +223,12 @@ contains
             real(F64), dimension(:), intent(in)                :: OccEnergies, VirtEnergies
             real(F64), dimension(:), intent(in)                :: Freqs, FreqWeights
             logical, intent(in)                                :: SplitBF16
+            integer, intent(in), optional                      :: Precision
             integer(c_int) :: prec
             call brpa_init()
             prec = 0
             if (SplitBF16) prec = 1
+            if (present(Precision)) prec = int(Precision, c_int)
             call check(c_ecrpa(ctx, prec, Rkai, int(NVecs, c_int64_t), OccEnergies, &
                   int(NOcc, c_int64_t), VirtEnergies, int(NVirt, c_int64_t), Freqs, FreqWeights, &
                   int(NFreqs, c_int64_t), EcRPA, c_null_ptr), "brpa_ecrpa_fused")
@@ -277,9 +280,10 @@ contains
 
 
       subroutine rpa_CC_Exchange_SOSEX(EcExchange, Alpha, Vkai, Ak, Rkai, NOcc, NVirt, NVecsPiU, &
-            NVecsT2, S2Operator)
+            NVecsT2, S2Operator, Precision)
             !
             ! EcExchange = -Alpha * Sum(abij) (aj|bi)*Taibj (rpa_CC_Exchange.f90:36-97)
+            ! Precision (optional, default 0 = FP64): 2 selects the fused split-int8 kernel.
             !
             real(F64), intent(out)                                          :: EcExchange
             real(F64), intent(in)                                           :: Alpha
@@ -291,12 +295,15 @@ contains
             integer, intent(in)                                             :: NVecsPiU
             integer, intent(in)                                             :: NVecsT2
             logical, intent(in)                                             :: S2Operator
-            integer(c_int) :: s2
+            integer, intent(in), optional                                   :: Precision
+            integer(c_int) :: s2, prec
 
             call brpa_init()
             s2 = 0
             if (S2Operator) s2 = 1
-            call check(c_sosex(ctx, 0_c_int, EcExchange, Alpha, Vkai, Ak, Rkai, int(NOcc, c_int64_t), &
+            prec = 0
+            if (present(Precision)) prec = int(Precision, c_int)
+            call check(c_sosex(ctx, prec, EcExchange, Alpha, Vkai, Ak, Rkai, int(NOcc, c_int64_t), &
                   int(NVirt, c_int64_t), int(NVecsPiU, c_int64_t), int(NVecsT2, c_int64_t), s2), &
                   "rpa_CC_Exchange_SOSEX")
       end subroutine rpa_CC_Exchange_SOSEX
