@@ -41,9 +41,11 @@ int brpa_exchange_batch(brpa_ctx *ctx, int precision, int64_t nfrag, const brpa_
         const int64_t ng = (int64_t)std::ceil(guess_factor * (double)fr.naux);
         int64_t nt = 0;
         double ex = 0.0;
-        int rc = brpa_t2_compute(w, BRPA_FP64, fr.R, fr.naux, fr.eocc, fr.nocc, fr.evir, fr.nvir,
-                                 fr.freqs, fr.weights, fr.nfreqs, lambda, cutoff, ng,
-                                 seed + (uint64_t)f, &nt);
+        // energy only: the factored amplitude core (no eigensolve) unless the S2
+        // rescaling needs the amplitudes themselves
+        int rc = t2_build(w, BRPA_FP64, fr.R, fr.naux, fr.eocc, fr.nocc, fr.evir, fr.nvir,
+                          fr.freqs, fr.weights, fr.nfreqs, lambda, cutoff, ng, seed + (uint64_t)f,
+                          &nt, !s2_operator);
         if (rc == BRPA_OK && nt > 0)
             rc = brpa_sosex(w, precision, &ex, alpha, w->t2_V, w->t2_A, fr.R, fr.nocc, fr.nvir,
                             fr.naux, nt, s2_operator);

@@ -176,6 +176,14 @@ int launch_syrk_i8(brpa_ctx *, const FragDev *, const int64_t *pre, int nfrag, i
 // fragment, in place on ctx->stream; status/linv/ldpart of the descriptor
 // must be allocated.  d_pre: device scratch of chol_pre_len(N) int64.
 int64_t chol_pre_len(int64_t N);
+
+// brpa_t2_compute with an option for energy-only callers: factored = true
+// leaves (A' = -1, V' = T L) with T2 = -V' V'^T (no eigensolve of T^T I T)
+// when the factorisation exists, the reference's (A, V) otherwise.
+int t2_build(brpa_ctx *ctx, int precision, const double *R, int64_t naux, const double *eocc,
+             int64_t nocc, const double *evir, int64_t nvir, const double *freqs,
+             const double *weights, int64_t nfreqs, double lambda, double cutoff, int64_t nguess,
+             uint64_t seed, int64_t *nvecs_t2, bool factored);
 int chol_single(brpa_ctx *ctx, const FragDev *d_fr, int64_t *d_pre, int64_t N, int64_t F);
 
 // Generic single-matrix FP64 SYRK used by brpa_gpiug_mo:

@@ -136,6 +136,21 @@ int launch_add_diag(brpa_ctx *ctx, double *M, int64_t n, double a) {
     return BRPA_OK;
 }
 
+// M(r, c) = 0 for r < c (n x n, column-major)
+__global__ void k_zero_upper(double *__restrict__ M, int64_t n) {
+    int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= n * n) return;
+    const int64_t c = e / n, r = e - c * n;
+    if (r < c) M[e] = 0.0;
+}
+
+int launch_zero_upper(brpa_ctx *ctx, double *M, int64_t n) {
+    if (n <= 0) return BRPA_OK;
+    hipLaunchKernelGGL(k_zero_upper, dim3(nblocks(n * n)), dim3(256), 0, ctx->stream, M, n);
+    LAUNCH_CHECK(ctx);
+    return BRPA_OK;
+}
+
 // SOSEX pair contraction (rpa_CC_EcExchange1, rpa/rpa_CC_Exchange.f90:100-116):
 // part[b * nchunk + c] = sum over chunk c of W1_b(P,mu) W2_b(P,mu) B[mu], for
 // batch b of npair (N x nt) matrix pairs (stride ld between batches).  Double-

@@ -28,6 +28,8 @@ int launch_select_scale(brpa_ctx *, double *W, const double *U, const double *ev
                         int64_t c0, int64_t nt);
 int launch_scale(brpa_ctx *, double *x, double a, int64_t n);
 int launch_add_diag(brpa_ctx *, double *M, int64_t n, double a);
+int launch_zero_upper(brpa_ctx *, double *M, int64_t n);
+int launch_fill(brpa_ctx *, double *x, int64_t n, double v);
 int launch_pair_dot(brpa_ctx *, const double *W1, const double *W2, const double *B, int64_t n_rows,
                     int64_t nt, int64_t stride, int npair, double *part);
 int sosex_nchunk(int64_t n_rows, int64_t nt);
@@ -140,12 +142,11 @@ void t2_release(brpa_ctx *ctx) {
     ctx->rocblas = nullptr;
 }
 
-extern "C" {
 
-int brpa_t2_compute(brpa_ctx *ctx, int precision, const double *R, int64_t naux,
-                    const double *eocc, int64_t nocc, const double *evir, int64_t nvir,
-                    const double *freqs, const double *weights, int64_t nfreqs, double lambda,
-                    double cutoff, int64_t nguess, uint64_t seed, int64_t *nvecs_t2) {
+int t2_build(brpa_ctx *ctx, int precision, const double *R, int64_t naux, const double *eocc,
+             int64_t nocc, const double *evir, int64_t nvir, const double *freqs,
+             const double *weights, int64_t nfreqs, double lambda, double cutoff, int64_t nguess,
+             uint64_t seed, int64_t *nvecs_t2, bool factored) {
     if (!ctx) return BRPA_ERR_ARG;
     if (precision != BRPA_FP64)
         return brpa_fail(ctx, BRPA_ERR_ARG, "brpa_t2_compute: only BRPA_FP64 is implemented");
@@ -362,15 +363,65 @@ int brpa_t2_compute(brpa_ctx *ctx, int precision, const double *R, int64_t naux,
         BRPA_TRY(gemm(ctx, h, true, false, nt, nt, N, -fw[k] * lambda / (4.0 * kPi), G, N, H, N,
                       k == 0 ? 0.0 : 1.0, TIT, nt));
     }
-    // eigenpairs of T^T I T; A = a / (2 (1 + a)); V = eigvecs^T T^T (629-641)
-    BRPA_TRY(syevd(ctx, h, nt, TIT, sev, sE, dinfo + F + 1));
-    std::vector<double> ha(nt);
-    BRPA_TRY(copy_sync(ctx, ha.data(), sev, 8 * nt, hipMemcpyDeviceToHost));
-    for (int64_t i = 0; i < nt; ++i) ha[i] = 0.5 * (ha[i] / (ha[i] + 1.0));
     BRPA_TRY(grow(ctx, &ctx->t2_A, &ctx->t2_A_cap, nt));
     BRPA_TRY(grow(ctx, &ctx->t2_V, &ctx->t2_V_cap, (size_t)nt * K));
-    BRPA_HIP(ctx, hipMemcpyAsync(ctx->t2_A, ha.data(), 8 * nt, hipMemcpyHostToDevice, ctx->stream));
-    BRPA_TRY(gemm(ctx, h, true, true, nt, K, nt, 1.0, TIT, nt, Tkeep, K, 0.0, ctx->t2_V, nt));
+    bool done = false;
+    if (factored) {
+        // Energy-only callers need T2 = T C T^T, C = (1/2) TIT (1 + TIT)^{-1}, not the
+        // eigenpairs.  The amplitudes A = a/(2(1+a)) are negative (I is negative
+        // semidefinite on the subspace), so -C = ((1 + TIT)^{-1} - 1)/2 has a Cholesky
+        // factor L, and T2 = -V' V'^T with V' = T L: the pair (A' = -1, V') gives
+        // the same exchange energy.  Any breakdown falls back to the eigensolve.
+        double *Msave = omega;   // nt x nt copy of TIT (omega is free here)
+        BRPA_HIP(ctx, hipMemcpyAsync(Msave, TIT, 8 * nt * nt, hipMemcpyDeviceToDevice,
+                                     ctx->stream));
+        auto info_ok = [&](int *d, int *okv) -> int {
+            int hi = 0;
+            BRPA_TRY(copy_sync(ctx, &hi, d, sizeof(int), hipMemcpyDeviceToHost));
+            *okv = (hi == 0);
+            return BRPA_OK;
+        };
+        BRPA_TRY(launch_add_diag(ctx, TIT, nt, 1.0));
+        BRPA_TRY(rb_check(ctx, rocsolver_dpotrf(h, rocblas_fill_lower, (rocblas_int)nt, TIT,
+                                                (rocblas_int)nt, dinfo + F + 1),
+                          "rocsolver_dpotrf"));
+        int ok = 0;
+        BRPA_TRY(info_ok(dinfo + F + 1, &ok));
+        if (ok) {
+            BRPA_TRY(rb_check(ctx, rocsolver_dpotri(h, rocblas_fill_lower, (rocblas_int)nt, TIT,
+                                                    (rocblas_int)nt, dinfo + F + 1),
+                              "rocsolver_dpotri"));
+            BRPA_TRY(info_ok(dinfo + F + 1, &ok));
+        }
+        if (ok) {
+            BRPA_TRY(launch_scale(ctx, TIT, 0.5, nt * nt));
+            BRPA_TRY(launch_add_diag(ctx, TIT, nt, -0.5));
+            BRPA_TRY(rb_check(ctx, rocsolver_dpotrf(h, rocblas_fill_lower, (rocblas_int)nt, TIT,
+                                                    (rocblas_int)nt, dinfo + F + 1),
+                              "rocsolver_dpotrf"));
+            BRPA_TRY(info_ok(dinfo + F + 1, &ok));
+        }
+        if (ok) {
+            BRPA_TRY(launch_zero_upper(ctx, TIT, nt));
+            BRPA_TRY(launch_fill(ctx, ctx->t2_A, nt, -1.0));
+            // V'^T = L^T T^T, stored as V[ai][mu]
+            BRPA_TRY(gemm(ctx, h, true, true, nt, K, nt, 1.0, TIT, nt, Tkeep, K, 0.0, ctx->t2_V,
+                          nt));
+            done = true;
+        } else {
+            BRPA_HIP(ctx, hipMemcpyAsync(TIT, Msave, 8 * nt * nt, hipMemcpyDeviceToDevice,
+                                         ctx->stream));
+        }
+    }
+    if (!done) {
+        // eigenpairs of T^T I T; A = a / (2 (1 + a)); V = eigvecs^T T^T (629-641)
+        BRPA_TRY(syevd(ctx, h, nt, TIT, sev, sE, dinfo + F + 1));
+        std::vector<double> ha(nt);
+        BRPA_TRY(copy_sync(ctx, ha.data(), sev, 8 * nt, hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < nt; ++i) ha[i] = 0.5 * (ha[i] / (ha[i] + 1.0));
+        BRPA_TRY(copy_sync(ctx, ctx->t2_A, ha.data(), 8 * nt, hipMemcpyHostToDevice));
+        BRPA_TRY(gemm(ctx, h, true, true, nt, K, nt, 1.0, TIT, nt, Tkeep, K, 0.0, ctx->t2_V, nt));
+    }
     BRPA_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
     std::vector<int> info(F);
     BRPA_HIP(ctx, hipMemcpyAsync(info.data(), fd.status, sizeof(int) * F, hipMemcpyDeviceToHost,
@@ -384,6 +435,16 @@ int brpa_t2_compute(brpa_ctx *ctx, int precision, const double *R, int64_t naux,
             return brpa_fail(ctx, BRPA_ERR_NOT_SPD, "frequency %lld: 1 + l Pi(u) is not positive definite",
                              (long long)k);
     return BRPA_OK;
+}
+
+extern "C" {
+
+int brpa_t2_compute(brpa_ctx *ctx, int precision, const double *R, int64_t naux,
+                    const double *eocc, int64_t nocc, const double *evir, int64_t nvir,
+                    const double *freqs, const double *weights, int64_t nfreqs, double lambda,
+                    double cutoff, int64_t nguess, uint64_t seed, int64_t *nvecs_t2) {
+    return t2_build(ctx, precision, R, naux, eocc, nocc, evir, nvir, freqs, weights, nfreqs,
+                    lambda, cutoff, nguess, seed, nvecs_t2, false);
 }
 
 int brpa_t2_copyout(brpa_ctx *ctx, double *A, double *V) {
