@@ -138,6 +138,8 @@ def main():
                     help="CPU oracle sample budget (0 disables the CPU baseline)")
     ap.add_argument("--compare-fp64", type=int, default=1,
                     help="also time the other forms (reported under `forms`)")
+    ap.add_argument("--autotune", type=int, default=1,
+                    help="run brpa_autotune on the batch before timing (split_i8)")
     ap.add_argument("--graphs", type=int, default=1,
                     help="replay the batched launch sequence as a hipGraph (1) or launch eagerly")
     ap.add_argument("--exchange-frags", type=int, default=64,
@@ -188,6 +190,17 @@ def main():
     batch = [(Rd[i], f.eocc, f.evir, grids[i][0], grids[i][1]) for i, f in enumerate(frags)]
 
     ctx = brpa.Context(0)
+    tune = None
+    if a.autotune and a.precision == "split_i8":
+        # the project's autotuner picks the product-kernel variant per NAux bucket
+        # on this batch (outside the timed region)
+        t = time.perf_counter()
+        res = ctx.autotune(batch, "split_i8")
+        names = ["T128", "T128_xcd", "T64"]
+        tune = {"seconds": round(time.perf_counter() - t, 2),
+                "buckets": {str(b): {"choice": names[v], "ms": [round(m, 2) for m in ms]}
+                            for b, (v, ms) in res.items()}}
+        print("# autotune: %s" % tune, file=sys.stderr, flush=True)
 
     def run(prec, warmup, steps):
         # one eager pass: per-stage kernel times (HIP events inside the library)
@@ -306,7 +319,7 @@ def main():
                    "components": ("E_RPA (batched, %s) + SOSEX exchange (fp64 T2, %s "
                                   "contraction)" % (a.precision, exch["precision"].split()[-3]))
                                  if exch else "E_RPA only",
-                   "graphs": bool(a.graphs),
+                   "graphs": bool(a.graphs), "autotune": tune,
                    "algorithmic_tflop_per_step": round((tot_syrk + tot_chol) / 1e12, 3)},
         "erpa_only_fragments_per_sec": round(a.nfrag / dt, 4),
         "exchange": exch,

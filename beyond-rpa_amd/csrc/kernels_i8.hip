@@ -43,7 +43,9 @@ constexpr int CS_WG = 16;   // frequencies per register group
 
 __global__ __launch_bounds__(256) void k_colscale_i8(const FragDev *__restrict__ fr,
                                                     const int64_t *__restrict__ pre, int nfrag) {
+    constexpr int CH = 64;   // rows per table chunk staged in LDS
     __shared__ int64_t s_pre[FIND_LDS_MAX + 1];
+    __shared__ double sSD[CS_WG][CH], sCD[CS_WG][CH];
     __shared__ double sM[4][64], sH[4][64], sL[4][64];
     const int64_t b = blockIdx.x;
     const int f = find_frag_lds(pre, nfrag, b, s_pre);
@@ -56,7 +58,6 @@ __global__ __launch_bounds__(256) void k_colscale_i8(const FragDev *__restrict__
     const gdbl_t *R = (const gdbl_t *)F.R;
     const gdbl_t *SD = (const gdbl_t *)F.sdtab;
     const gdbl_t *CD = (const gdbl_t *)F.dtab;
-    constexpr int RU = 4;   // rows in flight per thread
     for (int w0 = 0; w0 < F.F; w0 += CS_WG) {
         const int nw = min(CS_WG, F.F - w0);
         double m[CS_WG];
@@ -66,30 +67,40 @@ __global__ __launch_bounds__(256) void k_colscale_i8(const FragDev *__restrict__
             m[i] = 0.0;
             ds[i] = (dd_t){0.0, 0.0};
         }
-        for (int kb = g; kb < K; kb += 4 * RU) {
-            double rv[RU];
-#pragma unroll
-            for (int u = 0; u < RU; ++u) {
-                const int k = kb + 4 * u;
-                const bool ok = colok && k < K;
-                rv[u] = R[ok ? (int64_t)k * N + P : 0];
-                rv[u] = ok ? rv[u] : 0.0;
+        for (int k0 = 0; k0 < K; k0 += CH) {
+            // this chunk's rows of the two tables (zeros past K or past nw)
+            __syncthreads();
+            for (int e = tid; e < CS_WG * CH; e += 256) {
+                const int i = e / CH, r = e - i * CH;
+                const bool ok = i < nw && k0 + r < K;
+                const int64_t idx = ok ? (int64_t)(w0 + i) * K + k0 + r : 0;
+                const double sd = SD[idx], cd = CD[idx];
+                sSD[i][r] = ok ? sd : 0.0;
+                sCD[i][r] = ok ? cd : 0.0;
             }
+            // this thread's 16 rows of the chunk (rows g + 4 j), loads first
+            double rv[CH / 4];
 #pragma unroll
-            for (int u = 0; u < RU; ++u) {
-                const int k = min(kb + 4 * u, K - 1);   // rows past K carry rv = 0
-                const double r2 = rv[u] * rv[u], r2e = fma(rv[u], rv[u], -r2);
+            for (int j = 0; j < CH / 4; ++j) {
+                const int k = k0 + g + 4 * j;
+                const bool ok = colok && k < K;
+                rv[j] = R[ok ? (int64_t)k * N + P : 0];
+                rv[j] = ok ? rv[j] : 0.0;
+            }
+            __syncthreads();
+#pragma unroll 2
+            for (int j = 0; j < CH / 4; ++j) {
+                const int r = g + 4 * j;
+                const double x = rv[j];
+                const double r2 = x * x, r2e = fma(x, x, -r2);
 #pragma unroll
                 for (int i = 0; i < CS_WG; ++i) {
-                    if (i < nw) {
-                        const int64_t e = (int64_t)(w0 + i) * K + k;
-                        const double y = rv[u] * SD[e];
-                        m[i] = fmax(m[i], fabs(y));
-                        const double cw = CD[e];
-                        const double p = cw * r2, pe = fma(cw, r2, -p) + cw * r2e;
-                        ds[i] = dd_add(ds[i], p);
-                        ds[i].lo += pe;
-                    }
+                    const double y = x * sSD[i][r];
+                    m[i] = fmax(m[i], fabs(y));
+                    const double cw = sCD[i][r];
+                    const double p = cw * r2, pe = fma(cw, r2, -p) + cw * r2e;
+                    ds[i] = dd_add(ds[i], p);
+                    ds[i].lo += pe;
                 }
             }
         }
@@ -138,7 +149,7 @@ int launch_colscale_i8(brpa_ctx *ctx, const FragDev *fr, const int64_t *pre, int
 __global__ __launch_bounds__(256) void k_split_i8(const FragDev *__restrict__ fr,
                                                  const int64_t *__restrict__ pre, int nfrag) {
     __shared__ int64_t s_pre[FIND_LDS_MAX + 1];
-    __shared__ double sX[I8_KB][65];
+    __shared__ double sSD[CS_WG][I8_KB];   // sqrt(c_w) of this work-group's rows
     const int64_t b = blockIdx.x;
     const int f = find_frag_lds(pre, nfrag, b, s_pre);
     const FragDev &F = fr[f];
@@ -149,55 +160,60 @@ __global__ __launch_bounds__(256) void k_split_i8(const FragDev *__restrict__ fr
     const int k0 = kc * I8_KB, P0 = cc * 64;
     const int tid = threadIdx.x;
     const gdbl_t *R = (const gdbl_t *)F.R;
-    {
-        // 128 x 64 doubles, 32 per thread, all loads issued before the LDS stores
-        constexpr int PER = I8_KB * 64 / 256;
-        const int c = tid & 63, r0 = tid >> 6;
-        double v[PER];
-#pragma unroll
-        for (int u = 0; u < PER; ++u) {
-            const int r = r0 + 4 * u;
-            const bool ok = (k0 + r < K) && (P0 + c < N);
-            v[u] = R[ok ? (int64_t)(k0 + r) * N + P0 + c : 0];
-            v[u] = ok ? v[u] : 0.0;
-        }
-#pragma unroll
-        for (int u = 0; u < PER; ++u) sX[r0 + 4 * u][c] = v[u];
-    }
-    __syncthreads();
+    const gdbl_t *SD = (const gdbl_t *)F.sdtab;
+    // thread -> column P0 + tid/4 and its 32 rows 32 (tid%4) .. +32, held in registers
     const int c = tid >> 2, r0 = (tid & 3) * 32;
     const int P = P0 + c;
-    if (P >= N) return;
+    const bool colok = P < N;
+    double x[32];
+#pragma unroll
+    for (int j = 0; j < 32; ++j) {
+        const int k = k0 + r0 + j;
+        const bool ok = colok && k < K;
+        x[j] = R[ok ? (int64_t)k * N + P : 0];
+        x[j] = ok ? x[j] : 0.0;
+    }
     const int64_t Kp = F.Kpad8;
-    for (int w = 0; w < F.F; ++w) {
-        const double sc = F.scale8[(int64_t)w * N + P];
-        const double inv = sc > 0.0 ? 1.0 / sc : 0.0;
-        const double *sd = F.sdtab + (int64_t)w * K;
-        uint32_t pk[NSLICE][8];
+    for (int w0 = 0; w0 < F.F; w0 += CS_WG) {
+        const int nw = min(CS_WG, F.F - w0);
+        __syncthreads();
+        for (int e = tid; e < CS_WG * I8_KB; e += 256) {
+            const int i = e / I8_KB, r = e - i * I8_KB;
+            const bool ok = i < nw && k0 + r < K;
+            const double v = SD[ok ? (int64_t)(w0 + i) * K + k0 + r : 0];
+            sSD[i][r] = ok ? v : 0.0;
+        }
+        __syncthreads();
+        if (!colok) continue;
+        for (int i = 0; i < nw; ++i) {
+            const int w = w0 + i;
+            const double sc = F.scale8[(int64_t)w * N + P];
+            const double inv = sc > 0.0 ? 1.0 / sc : 0.0;
+            uint32_t pk[NSLICE][8];
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            uint32_t word[NSLICE] = {0u, 0u, 0u, 0u};
+            for (int q = 0; q < 8; ++q) {
+                uint32_t word[NSLICE] = {0u, 0u, 0u, 0u};
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const int r = r0 + 4 * q + i, k = k0 + r;
-                const double y = sX[r][c] * (k < K ? sd[k] : 0.0);
-                double t = y * inv;   // |t| <= 127 (up to one rounding)
+                for (int u = 0; u < 4; ++u) {
+                    const int j = 4 * q + u;
+                    double t = x[j] * sSD[i][r0 + j] * inv;   // |t| <= 127 (up to rounding)
 #pragma unroll
-                for (int s = 0; s < NSLICE; ++s) {
-                    const double n = rint(t);
-                    t = (t - n) * 128.0;   // exact
-                    word[s] |= ((uint32_t)(uint8_t)(int8_t)(int)n) << (8 * i);
+                    for (int s = 0; s < NSLICE; ++s) {
+                        const double n = rint(t);
+                        t = (t - n) * 128.0;   // exact
+                        word[s] |= ((uint32_t)(uint8_t)(int8_t)(int)n) << (8 * u);
+                    }
                 }
+#pragma unroll
+                for (int s = 0; s < NSLICE; ++s) pk[s][q] = word[s];
             }
 #pragma unroll
-            for (int s = 0; s < NSLICE; ++s) pk[s][q] = word[s];
-        }
-#pragma unroll
-        for (int s = 0; s < NSLICE; ++s) {
-            int8_t *dst = F.sl8 + (((int64_t)w * NSLICE + s) * N + P) * Kp + k0 + r0;
-            uint4 *d4 = reinterpret_cast<uint4 *>(dst);
-            d4[0] = make_uint4(pk[s][0], pk[s][1], pk[s][2], pk[s][3]);
-            d4[1] = make_uint4(pk[s][4], pk[s][5], pk[s][6], pk[s][7]);
+            for (int s = 0; s < NSLICE; ++s) {
+                int8_t *dst = F.sl8 + (((int64_t)w * NSLICE + s) * N + P) * Kp + k0 + r0;
+                uint4 *d4 = reinterpret_cast<uint4 *>(dst);
+                d4[0] = make_uint4(pk[s][0], pk[s][1], pk[s][2], pk[s][3]);
+                d4[1] = make_uint4(pk[s][4], pk[s][5], pk[s][6], pk[s][7]);
+            }
         }
     }
 }
