@@ -89,3 +89,49 @@ def test_autotune_chooses_and_agrees(ctx):
         assert 0 <= v < 3 and all(m > 0 for m in ms)
     e = ctx.ecrpa_batch(frags, "split_i8")
     assert np.max(np.abs(e - np.array(refs))) <= 1e-7
+
+
+@pytest.mark.skip(reason="the persistent Cholesky launch did not complete on the GPU (twice); "
+                         "the path is switched off and raises an error until the cause is found")
+@pytest.mark.parametrize("prec", ["fp64", "split_i8"])
+def test_persistent_cholesky_matches_per_step(ctx, prec, monkeypatch):
+    """The one-launch persistent Cholesky (dependency counters between work
+    items) gives the same energies, bit for bit, as one launch per step, on
+    uneven fragments with 1..12 Cholesky steps; also under graph replay."""
+    torch = pytest.importorskip("torch")
+    rng = np.random.default_rng(11)
+    frags = []
+    for i in range(24):
+        o, v = int(rng.integers(2, 9)), int(rng.integers(10, 90))
+        n = int(rng.integers(20, 760))
+        f = synth.make_fragment(o, v, n, 1200 + i)
+        x, w, _ = grid.freq_grid(f.eocc, f.evir, 6 + i % 5)
+        frags.append((torch.from_numpy(f.R).cuda(), f.eocc, f.evir, x, w))
+    monkeypatch.setenv("BRPA_CHOL_PERSISTENT", "0")
+    ref = ctx.ecrpa_batch(frags, prec)
+    monkeypatch.setenv("BRPA_CHOL_PERSISTENT", "1")
+    try:
+        e = ctx.ecrpa_batch(frags, prec)
+        assert np.array_equal(e, ref), e - ref
+        ctx.set_graphs(True)
+        for _ in range(3):
+            e = ctx.ecrpa_batch(frags, prec)
+            assert np.array_equal(e, ref), e - ref
+    finally:
+        ctx.set_graphs(False)
+        monkeypatch.setenv("BRPA_CHOL_PERSISTENT", "0")
+        ctx.ecrpa_batch(frags[:1], prec)   # back to the per-step mode for later tests
+
+
+def test_persistent_cholesky_is_switched_off(ctx, monkeypatch):
+    import brpa
+    f = synth.make_fragment(3, 20, 100, 5)
+    x, w, _ = grid.freq_grid(f.eocc, f.evir, 8)
+    monkeypatch.setenv("BRPA_CHOL_PERSISTENT", "1")
+    try:
+        with pytest.raises(brpa.BrpaError) as ei:
+            ctx.ecrpa_batch([(f.R, f.eocc, f.evir, x, w)], "fp64")
+        assert ei.value.code == brpa._lib.ERR_ARG
+    finally:
+        monkeypatch.setenv("BRPA_CHOL_PERSISTENT", "0")
+        ctx.ecrpa_batch([(f.R, f.eocc, f.evir, x, w)], "fp64")
