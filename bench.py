@@ -12,9 +12,10 @@ frequency grids (host code, independent of R) are built once before timing.
 
 Exchange correction: the factorised direct-ring amplitudes (T2) and the SOSEX
 contraction are timed on the first --exchange-frags fragments of the same
-batch (brpa_exchange_batch, one fragment after another), and the
-E_RPA + exchange rate is 1 / (E_RPA time per fragment + exchange time per
-fragment).
+batch (brpa_exchange_batch, one fragment after another), once with the
+split-int8 SOSEX contraction and once in FP64; each form's E_RPA + exchange
+rate is 1 / (its E_RPA time per fragment + the exchange time per fragment in
+its own arithmetic).
 
 Prints one JSON line.  `roofline` is measured for the dominant kernel with
 HIP events inside the library over the timed steps; `cpu_baseline` times the
@@ -81,9 +82,7 @@ def bench_sosex(a):
 
     import brpa
     from brpa import synth
-    if a.gpus != 1:
-        print("bench.py measures one GPU only (--gpus %d requested)" % a.gpus, file=sys.stderr)
-        sys.exit(2)
+    _one_gpu(a)
     o, v, n = 64, 600, 3200
     f = synth.make_fragment(o, v, n, synth.BASE_SEED)
     x, w, _ = brpa.freq_grid(f.eocc, f.evir, 16)
@@ -126,6 +125,179 @@ def bench_sosex(a):
     print(json.dumps(out))
 
 
+def _one_gpu(a):
+    if a.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(--gpus %d requested)" % a.gpus, file=sys.stderr)
+        sys.exit(2)
+
+
+def _cpu_threads():
+    from threadpoolctl import threadpool_info
+    return max([p.get("num_threads", 1) for p in threadpool_info()] or [1])
+
+
+def bench_single(a):
+    """E_RPA of one large fragment (o=36, v=300, NAux=1800, 24 frequencies) in
+    every form on one GPU, each energy against the FP64 CPU oracle; the oracle's
+    own time on the host cores is the CPU baseline."""
+    import torch
+
+    import brpa
+    from brpa import synth
+    from oracle import erpa as oerpa
+    _one_gpu(a)
+    o, v, n, nf = 36, 300, 1800, 24
+    f = synth.make_fragment(o, v, n, synth.BASE_SEED)
+    x, w, _ = brpa.freq_grid(f.eocc, f.evir, nf)
+    Rd = torch.from_numpy(f.R).cuda()
+    torch.cuda.synchronize()
+    ctx = brpa.Context(0)
+    res = {}
+    for prec in (a.precision,) + tuple(p for p in PRECISIONS if p != a.precision):
+        ctx.set_graphs(False)
+        e_eager = ctx.ecrpa(Rd, f.eocc, f.evir, x, w, prec)[0]
+        stage = {k: s[0] for k, s in ctx.last_stage_ms().items()}
+        ctx.set_graphs(bool(a.graphs))
+        for _ in range(a.warmup):
+            ctx.ecrpa(Rd, f.eocc, f.evir, x, w, prec)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(a.steps):
+            e = ctx.ecrpa(Rd, f.eocc, f.evir, x, w, prec)[0]
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t) / a.steps
+        ctx.set_graphs(False)
+        if e != e_eager:
+            raise RuntimeError("%s: replayed energy differs from the eager pass" % prec)
+        res[prec] = {"E": e, "ms": dt * 1e3, "stage_ms": stage}
+        print("# %s: E %.10f  %.2f ms/step stages %s" % (prec, e, dt * 1e3, stage),
+              file=sys.stderr, flush=True)
+    t = time.perf_counter()
+    eref = oerpa.ecrpa(f.R, f.gaps(), x, w)
+    tc = time.perf_counter() - t
+    for d in res.values():
+        d["abs_dE_vs_cpu_oracle_Ha"] = abs(d["E"] - eref)
+    main = res[a.precision]
+    flops = syrk_flops(o, v, n, nf)
+    kernel, peak = PI_KERNEL[a.precision]
+    achieved = flops / (main["stage_ms"]["pi"] * 1e-3) / 1e12
+    out = {"metric": "E_RPA time of one fragment o=36, v=300, NAux=1800, 24 frequencies, "
+                     "split vs FP64 at |dE|<=1e-7 Ha",
+           "value": round(main["ms"], 3), "unit": "ms", "n_gpus": 1, "steps": a.steps,
+           "warmup": a.warmup, "ms_per_step": round(main["ms"], 3), "higher_is_better": False,
+           "scaling": "strong", "vs_baseline": None, "dtype": DTYPES[a.precision],
+           "data": "synthetic (seeded PCG64, brpa.synth)",
+           "config": {"workload": "single fragment o=36, v=300, NAux=1800, 24 frequencies",
+                      "precision": a.precision, "graphs": bool(a.graphs)},
+           "E_cpu_oracle": eref,
+           "speedup_vs_fp64": round(res["fp64"]["ms"] / main["ms"], 3),
+           "forms": {p: {"ms_per_step": round(d["ms"], 3), "E": d["E"],
+                         "abs_dE_vs_cpu_oracle_Ha": d["abs_dE_vs_cpu_oracle_Ha"],
+                         "stage_ms": {k: round(s, 3) for k, s in d["stage_ms"].items()}}
+                     for p, d in res.items()},
+           "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": round(peak, 3),
+                        "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": None,
+                        "kernel": kernel, "kernel_ms_per_step": round(main["stage_ms"]["pi"], 3)},
+           "cpu_baseline": {"value": round(tc * 1e3, 1), "unit": "ms", "cores": _cpu_threads(),
+                            "kind": "port",
+                            "sample": "the same fragment, E_RPA by the numpy FP64 oracle "
+                                      "(dgemm Pi + dgeqrf log-det), one evaluation"}}
+    print(json.dumps(out))
+
+
+def bench_adjoint(a):
+    """dE_RPA/dR and dE_RPA/d eps of one fragment (o=48, v=400, NAux=2400, 16
+    frequencies, Pi recomputed per frequency) on one GPU; checked against
+    central finite differences on --fd-entries random R entries plus 6 orbital
+    energies and against the FP64 closed form on 8 entries."""
+    import numpy as np
+    import torch
+
+    import brpa
+    from brpa import synth
+    from oracle import adjoint
+    _one_gpu(a)
+    o, v, n, nf = 48, 400, 2400, 16
+    f = synth.make_fragment(o, v, n, seed=20261202)
+    x, w, _ = brpa.freq_grid(f.eocc, f.evir, nf)
+    Rd = torch.from_numpy(f.R).cuda()
+    torch.cuda.synchronize()
+    ctx = brpa.Context(0)
+    for _ in range(a.warmup):
+        ctx.ecrpa_grad(Rd, f.eocc, f.evir, x, w)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    kms = 0.0
+    for _ in range(a.steps):
+        dR, deo, dev = ctx.ecrpa_grad(Rd, f.eocc, f.evir, x, w)
+        kms += ctx.last_kernel_ms()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t) / a.steps
+    kms /= a.steps
+    ctx.ecrpa(Rd, f.eocc, f.evir, x, w, "fp64")
+    t_e = ctx.last_kernel_ms()
+    print("# adjoint: %.1f ms/step (kernels %.1f ms), energy alone %.1f ms" % (dt * 1e3, kms, t_e),
+          file=sys.stderr, flush=True)
+    dR = dR.cpu().numpy()
+    # finite differences need (1 + Pi_k)^-1; the Pi_k come from the GPU build
+    # (its parity against the oracle is tested separately), the inverses and
+    # the differences from the CPU
+    t = time.perf_counter()
+    Pis = []
+    for xx in x:
+        P = torch.zeros((n, n), dtype=torch.float64, device="cuda")
+        ctx.gpiug_mo(P, Rd, f.eocc, f.evir, xx, 4.0)
+        Pis.append(P.cpu().numpy())
+    Ms = adjoint.inverses(f.R, f.eocc, f.evir, x, Pis=Pis)
+    rng = np.random.default_rng(2024)
+    ents = [(int(rng.integers(0, o * v)), int(rng.integers(0, n))) for _ in range(a.fd_entries)]
+    fd = adjoint.central_fd_R(f.R, f.eocc, f.evir, x, w, ents, 1e-5, Ms=Ms)
+    worst = max(abs(fd[e] - dR[e]) / abs(fd[e]) for e in ents)
+    for which, idx, g in (("o", 0, deo), ("o", o // 2 - 1, deo), ("o", o - 1, deo),
+                          ("v", 0, dev), ("v", v // 2 - 1, dev), ("v", v - 1, dev)):
+        fv = adjoint.central_fd_eps(f.R, f.eocc, f.evir, x, w, which, idx, 1e-5, Ms=Ms)
+        worst = max(worst, abs(fv - g[idx]) / abs(fv))
+    ref = adjoint.ecrpa_grad_R_entries(f.R, f.eocc, f.evir, x, w, ents[:8], Ms=Ms)
+    worst_cf = max(abs(dR[e] - g) / abs(g) for e, g in ref.items())
+    tcheck = time.perf_counter() - t
+    print("# FD worst relative %.2e, closed form worst relative %.2e (%.1f s)" %
+          (worst, worst_cf, tcheck), file=sys.stderr, flush=True)
+    # per frequency: Pi SYRK (ov N^2), Cholesky (N^3/3), inverse from the factor
+    # (2 N^3/3), dsymm M R^T (2 ov N^2), dR accumulation (2 ov N)
+    flops = nf * (o * v * n * (n + 1) + n ** 3 / 3.0 + 2.0 * n ** 3 / 3.0 + 2.0 * o * v * n * n)
+    achieved = flops / (kms * 1e-3) / 1e12
+    # CPU baseline: the oracle's gradient on a bounded sample of the frequencies
+    thr = _cpu_threads()
+    t = time.perf_counter()
+    k = 0
+    while k < nf and (k == 0 or time.perf_counter() - t < a.cpu_seconds):
+        adjoint.ecrpa_grad(f.R, f.eocc, f.evir, x[k:k + 1], w[k:k + 1])
+        k += 1
+    tc = (time.perf_counter() - t) / k * nf
+    out = {"metric": "dE_RPA/dR + dE_RPA/d eps time of one fragment o=48, v=400, NAux=2400, "
+                     "16 frequencies (FD-checked to 1e-6 relative)",
+           "value": round(dt * 1e3, 3), "unit": "ms", "n_gpus": 1, "steps": a.steps,
+           "warmup": a.warmup, "ms_per_step": round(dt * 1e3, 3), "higher_is_better": False,
+           "scaling": "strong", "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic (seeded PCG64, brpa.synth)",
+           "config": {"workload": "adjoint of one fragment o=48, v=400, NAux=2400, 16 "
+                                  "frequencies, Pi recomputed per frequency"},
+           "kernel_ms_per_step": round(kms, 3), "energy_only_kernel_ms": round(t_e, 3),
+           "fd_check": {"entries_R": len(ents), "orbital_energies": 6, "h": 1e-5,
+                        "worst_relative": worst, "closed_form_entries": len(ref),
+                        "closed_form_worst_relative": worst_cf, "pass": bool(worst <= 1e-6)},
+           "roofline": {"bound": "mfma", "achieved": round(achieved, 3),
+                        "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                        "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": None,
+                        "kernel": "whole adjoint (SYRK + potrf + potri + dsymm + accumulation)"},
+           "cpu_baseline": {"value": round(tc * 1e3, 1), "unit": "ms", "cores": thr,
+                            "kind": "port",
+                            "sample": "numpy FP64 oracle gradient on %d of the %d frequencies, "
+                                      "scaled to all %d" % (k, nf, nf)}}
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -146,10 +318,16 @@ def main():
                     help="fragments in the T2 + SOSEX sample (0 skips the exchange part)")
     ap.add_argument("--exchange-workers", type=int, default=1,
                     help="worker count passed to brpa_exchange_batch (only 1 is supported)")
-    ap.add_argument("--workload", default="batch", choices=["batch", "sosex"],
-                    help="batch: the 256-fragment E_RPA + exchange batch (default); sosex: "
-                         "the exchange contraction of one o=64, v=600, NAux=3200 fragment, "
-                         "FP64 vs split-int8")
+    ap.add_argument("--workload", default="batch",
+                    choices=["batch", "single", "adjoint", "sosex"],
+                    help="batch: the 256-fragment E_RPA + exchange batch (default); single: "
+                         "E_RPA of one o=36, v=300, NAux=1800 fragment in every form; "
+                         "adjoint: dE/dR and dE/d eps of one o=48, v=400, NAux=2400 "
+                         "fragment with a finite-difference check; sosex: the exchange "
+                         "contraction of one o=64, v=600, NAux=3200 fragment, FP64 vs "
+                         "split-int8")
+    ap.add_argument("--fd-entries", type=int, default=64,
+                    help="random R entries in the adjoint workload's finite-difference check")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy 5 times (reference for the "
                          "FETCH_SIZE/WRITE_SIZE counter scale)")
@@ -167,10 +345,11 @@ def main():
         return
     if a.workload == "sosex":
         return bench_sosex(a)
-    if a.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(--gpus %d requested)" % a.gpus, file=sys.stderr)
-        sys.exit(2)
+    if a.workload == "single":
+        return bench_single(a)
+    if a.workload == "adjoint":
+        return bench_adjoint(a)
+    _one_gpu(a)
 
     import numpy as np
     import torch
@@ -244,7 +423,9 @@ def main():
     else:
         kernel, peak = "k_potrf_diag+k_trsm+k_chol_update", PEAK_FP64_TFLOPS
         achieved = tot_chol / (stage_ms["chol"] * 1e-3) / 1e12
-    traffic = pmc_traffic(kernel + "<128>") if kernel == "k_syrk_i8" else None
+    # the committed PMC runs cover the default batch (256 fragments, 16 frequencies)
+    traffic = (pmc_traffic(kernel + "<128>")
+               if kernel == "k_syrk_i8" and a.nfrag == 256 and a.nfreqs == 16 else None)
     roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": round(peak, 3),
                 "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": traffic,
                 "traffic_source": ("profiles/erpa_batch256_split_i8_kernels.md (FETCH_SIZE x 2.0 "
@@ -254,27 +435,37 @@ def main():
 
     exch = None
     if a.exchange_frags > 0:
-        # T2 + SOSEX of the first fragments of the batch, several in flight
+        # T2 + SOSEX of the first fragments of the batch, timed once per SOSEX
+        # arithmetic: each form's exchange runs in that form's own arithmetic
+        # (split-int8 SOSEX for split_i8, FP64 for the others)
         kx = min(a.exchange_frags, a.nfrag)
         sub = batch[:kx]
-        # SOSEX contraction in the headline form's arithmetic where it exists
-        xprec = "split_i8" if a.precision == "split_i8" else "fp64"
-        ctx.exchange_batch(sub[:1], seed=synth.BASE_SEED, nworkers=a.exchange_workers,
-                           precision=xprec)                      # warm-up (workspaces)
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        exs, nts = ctx.exchange_batch(sub, seed=synth.BASE_SEED, nworkers=a.exchange_workers,
-                                      precision=xprec)
-        torch.cuda.synchronize()
-        t_ex = (time.perf_counter() - t1) / kx
+        xprec_of = {p: ("split_i8" if p == "split_i8" else "fp64") for p in forms}
+        t_ex, ex_val = {}, {}
+        for xprec in sorted(set(xprec_of.values()), key=lambda q: q != xprec_of[a.precision]):
+            ctx.exchange_batch(sub[:1], seed=synth.BASE_SEED, nworkers=a.exchange_workers,
+                               precision=xprec)                  # warm-up (workspaces)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            exs, nts = ctx.exchange_batch(sub, seed=synth.BASE_SEED, nworkers=a.exchange_workers,
+                                          precision=xprec)
+            torch.cuda.synchronize()
+            t_ex[xprec] = (time.perf_counter() - t1) / kx
+            ex_val[xprec] = exs
+            print("# exchange (%s SOSEX): %d fragments, %.1f ms per fragment" %
+                  (xprec, kx, t_ex[xprec] * 1e3), file=sys.stderr, flush=True)
+        xprec = xprec_of[a.precision]
         exch = {"n_fragments_timed": kx, "workers": a.exchange_workers,
-                "ms_per_fragment": round(t_ex * 1e3, 3),
+                "ms_per_fragment": round(t_ex[xprec] * 1e3, 3),
+                "ms_per_fragment_by_sosex_precision": {q: round(v * 1e3, 3)
+                                                       for q, v in t_ex.items()},
                 "precision": "fp64 T2 amplitudes, %s SOSEX contraction" % xprec,
                 "nguess": "ceil(3.0 N)", "mean_nt2": float(np.mean(nts))}
+        if "fp64" in ex_val and "split_i8" in ex_val:
+            exch["max_abs_dEx_split_i8_vs_fp64_Ha"] = float(
+                np.max(np.abs(ex_val["split_i8"] - ex_val["fp64"])))
         for p, d in forms.items():
-            d["with_exchange_fps"] = 1.0 / (d["ms_per_step"] * 1e-3 / a.nfrag + t_ex)
-        print("# exchange: %d fragments, %d workers, %.1f ms per fragment" %
-              (kx, a.exchange_workers, t_ex * 1e3), file=sys.stderr, flush=True)
+            d["with_exchange_fps"] = 1.0 / (d["ms_per_step"] * 1e-3 / a.nfrag + t_ex[xprec_of[p]])
 
     cpu = None
     parity = None

@@ -152,9 +152,6 @@ class Context:
         """Capture/replay the batched E_RPA launch sequence as a hipGraph."""
         self._check(self._lib.brpa_ctx_set_graphs(self._h, int(bool(enable))))
 
-    def last_kernel_ms(self):
-        return self._lib.brpa_last_kernel_ms(self._h)
-
     def graph_stats(self):
         """(captures, replays) of the last ecrpa_batch call."""
         c, r = C.c_int(0), C.c_int(0)

@@ -552,16 +552,105 @@ __device__ __forceinline__ int trsm_chunks(int N, int j) {
 
 // ---------------------------------------------------------------------------
 // Per-step kernels (flat work lists over fragments and frequencies)
+
+// lane `l`'s value of v, broadcast to the whole wave (two 32-bit readlanes)
+__device__ __forceinline__ double wave_bcast(double v, int l) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+    return __hiloint2double(hi, lo);
+}
+
+// Diagonal block of step `step` of matrix w with one wave: lane r keeps row r
+// of the 64 x 64 block in registers (right-looking; the pivot is broadcast by
+// readlane, column c's multipliers through a double-buffered LDS column, one
+// barrier per column); then column r of L11^{-1} by forward substitution with
+// L11 broadcast from LDS.
+// Same outputs as potrf_body: linv, log-det partial, status, optional L11
+// write-back.
 __global__ __launch_bounds__(64) void k_potrf_diag(const FragDev *__restrict__ fr,
                                                   const int64_t *__restrict__ pre, int nfrag,
                                                   int step) {
     __shared__ double L[CHOL_NB][CHOL_NB + 1];
-    __shared__ double Xd[CHOL_NB];
+    __shared__ double col[2][CHOL_NB];
     __shared__ int64_t s_pre[FIND_LDS_MAX + 1];
     const int64_t b = blockIdx.x;
     const int f = find_frag_lds(pre, nfrag, b, s_pre);
     const int w = (int)(b - (nfrag <= FIND_LDS_MAX ? s_pre[f] : pre[f]));
-    potrf_body<64>(fr[f], w, step, L, Xd, threadIdx.x);
+    const FragDev &F = fr[f];
+    const int lane = threadIdx.x;
+    const int N = F.N;
+    const int j0 = step * CHOL_NB;
+    const int nbj = min(CHOL_NB, N - j0);
+    double *A = F.Pi + (int64_t)w * N * N;
+    // row `lane` of the block straight into registers, lower triangle only (the
+    // other triangle is never read); rows/cols past nbj padded with the identity
+    double a[CHOL_NB];
+    const gdouble_t *Arow = (const gdouble_t *)(A + (int64_t)(j0 + min(lane, nbj - 1)) * N + j0);
+#pragma unroll
+    for (int c = 0; c < CHOL_NB; ++c) {
+        const bool in = lane < nbj && c < nbj;
+        const double v = (in && c <= lane) ? Arow[min(c, nbj - 1)] : 0.0;
+        a[c] = in ? v : (lane == c ? 1.0 : 0.0);
+    }
+    int bad = 0;
+#pragma unroll
+    for (int c = 0; c < CHOL_NB; ++c) {
+        double piv = wave_bcast(a[c], c);
+        if (!(piv > 0.0) || !isfinite(piv)) {
+            bad = 1;
+            piv = 1.0;
+        }
+        const double d = sqrt(piv);
+        // lanes above row c keep garbage (finite) in their upper triangle,
+        // never read; zeroing it here makes the compiler spill
+        const double q = a[c] / d;
+        a[c] = (lane == c) ? d : q;
+        col[c & 1][lane] = a[c];
+        __syncthreads();
+        const double m = lane > c ? a[c] : 0.0;
+        // the scheduling barriers keep at most 16 broadcast loads in flight
+        // (the fully unrolled loops would otherwise hoist them all and spill)
+#pragma unroll
+        for (int k = c + 1; k < CHOL_NB; ++k) {
+            a[k] -= m * col[c & 1][k];
+            if (k % 16 == 15) __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < CHOL_NB; ++c) L[lane][c] = c <= lane ? a[c] : 0.0;
+    __syncthreads();
+    // column `lane` of X = L11^{-1}: x_i = (delta_i,lane - sum_{k<i} L_ik x_k) / L_ii
+    // (x_i = 0 for i < lane); a[] now holds x
+#pragma unroll
+    for (int i = 0; i < CHOL_NB; ++i) {
+        double s = (i == lane) ? 1.0 : 0.0;
+#pragma unroll
+        for (int k = 0; k < i; ++k) {
+            s -= L[i][k] * a[k];
+            if (k % 16 == 15) __builtin_amdgcn_sched_barrier(0);
+        }
+        a[i] = s / L[i][i];
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    double *linv = F.linv + (int64_t)w * CHOL_NB * CHOL_NB;
+#pragma unroll
+    for (int i = 0; i < CHOL_NB; ++i) linv[i * CHOL_NB + lane] = a[i];
+    if (F.keep_l11) {
+        // the factored block back into Pi (lower triangle and diagonal), one
+        // row per pass with the lanes along the columns; no other work item
+        // reads this diagonal block
+        for (int r = 0; r < nbj; ++r)
+            if (lane <= r) A[(int64_t)(j0 + r) * N + j0 + lane] = L[r][lane];
+    }
+    if (lane == 0) {
+        dd_t sl = {0.0, 0.0};
+        for (int c = 0; c < nbj; ++c) sl = dd_add(sl, log(L[c][c]));
+        double *part = F.ldpart + ((int64_t)w * F.nsteps + step) * 2;
+        part[0] = sl.hi;
+        part[1] = sl.lo;
+    }
+    if (lane == 0 && bad) F.status[w] = 1;
 }
 
 int launch_potrf_diag(brpa_ctx *ctx, const FragDev *fr, const int64_t *pre, int nfrag,

@@ -136,17 +136,17 @@ int launch_add_diag(brpa_ctx *ctx, double *M, int64_t n, double a) {
     return BRPA_OK;
 }
 
-// M(r, c) = 0 for r < c (n x n, column-major)
-__global__ void k_zero_upper(double *__restrict__ M, int64_t n) {
+// M(r, c) = 0 for r > c (n x n, column-major): keeps the upper triangle
+__global__ void k_zero_lower(double *__restrict__ M, int64_t n) {
     int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (e >= n * n) return;
     const int64_t c = e / n, r = e - c * n;
-    if (r < c) M[e] = 0.0;
+    if (r > c) M[e] = 0.0;
 }
 
-int launch_zero_upper(brpa_ctx *ctx, double *M, int64_t n) {
+int launch_zero_lower(brpa_ctx *ctx, double *M, int64_t n) {
     if (n <= 0) return BRPA_OK;
-    hipLaunchKernelGGL(k_zero_upper, dim3(nblocks(n * n)), dim3(256), 0, ctx->stream, M, n);
+    hipLaunchKernelGGL(k_zero_lower, dim3(nblocks(n * n)), dim3(256), 0, ctx->stream, M, n);
     LAUNCH_CHECK(ctx);
     return BRPA_OK;
 }

@@ -5,10 +5,13 @@
 // rpa/rpa_CC.f90:1108-1129) only to apply Q_k diag(l/(1+l W_k)) Q_k^T =
 // l (1 + l Pi_k)^{-1}; here that operator is applied through a Cholesky
 // factor of 1 + l Pi_k.  Pi_k comes from the FP64 MFMA SYRK of the E_RPA
-// path; the plain GEMMs run on rocBLAS, the Cholesky factor/solves and the two
-// dense symmetric eigenproblems (S and T^T I T, which the reference solves
-// with LAPACK dsyevd) on rocSOLVER.  Column-major throughout (Fortran layout):
-//   Rc(P, ai) = R[ai][P] with leading dimension N.
+// path and every Cholesky factorisation (1 + l Pi_k, the Gram matrices of the
+// basis, 1 + T^T I T) runs on the library's own blocked kernels; the plain
+// GEMMs and triangular solves run on rocBLAS, and the inverse and the
+// dense symmetric eigenproblems (S and T^T I T when the Cholesky routes do not
+// apply; the reference solves them with LAPACK dsyevd) on rocSOLVER.
+// Column-major throughout (Fortran layout): Rc(P, ai) = R[ai][P] with leading
+// dimension N.
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 
@@ -28,7 +31,7 @@ int launch_select_scale(brpa_ctx *, double *W, const double *U, const double *ev
                         int64_t c0, int64_t nt);
 int launch_scale(brpa_ctx *, double *x, double a, int64_t n);
 int launch_add_diag(brpa_ctx *, double *M, int64_t n, double a);
-int launch_zero_upper(brpa_ctx *, double *M, int64_t n);
+int launch_zero_lower(brpa_ctx *, double *M, int64_t n);
 int launch_fill(brpa_ctx *, double *x, int64_t n, double v);
 int launch_pair_dot(brpa_ctx *, const double *W1, const double *W2, const double *B, int64_t n_rows,
                     int64_t nt, int64_t stride, int npair, double *part);
@@ -77,9 +80,10 @@ int gemm(brpa_ctx *ctx, rocblas_handle h, bool ta, bool tb, int64_t m, int64_t n
                     "rocblas_dgemm");
 }
 
+// eigenpairs of the symmetric matrix whose column-major upper triangle A holds
 int syevd(brpa_ctx *ctx, rocblas_handle h, int64_t n, double *A, double *D, double *E, int *info) {
     BRPA_TRY(rb_check(ctx,
-                      rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_lower,
+                      rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_upper,
                                        (rocblas_int)n, A, (rocblas_int)n, D, E, info),
                       "rocsolver_dsyevd"));
     int hinfo = 0;
@@ -185,9 +189,11 @@ int t2_build(brpa_ctx *ctx, int precision, const double *R, int64_t naux, const 
     // workspace layout
     const int64_t nsteps = cdiv(N, CHOL_NB);
     FragDev fd{};
-    double *dR = nullptr, *dchi, *omega, *tmp, *iom, *Y, *Hb, *Sm, *sev, *sE;
+    double *dR = nullptr, *dchi, *omega, *tmp, *iom, *Y, *Sm, *sev, *sE;
     int *dinfo;
-    int64_t *dchol = nullptr;
+    int64_t *dchol = nullptr, *dchol2 = nullptr;
+    FragDev fm{};                  // one dense matrix for the library's Cholesky
+    FragDev *dfm = nullptr;
     std::vector<int64_t> pre = {0, 0, 0, 0, 0, 0};
     const int64_t ntile = cdiv(N, SYRK_T);
     pre[1] = cdiv(F * K, 256);                      // dtab
@@ -211,7 +217,6 @@ int t2_build(brpa_ctx *ctx, int precision, const double *R, int64_t naux, const 
         tmp = ar.take<double>(K * NG);
         iom = ar.take<double>(K * NG);
         Y = ar.take<double>(N * NG);
-        Hb = ar.take<double>(N * NG);
         Sm = ar.take<double>(NG * NG);
         sev = ar.take<double>(NG);
         sE = ar.take<double>(NG);
@@ -220,6 +225,11 @@ int t2_build(brpa_ctx *ctx, int precision, const double *R, int64_t naux, const 
         fd.ldpart = ar.take<double>(F * nsteps * 2);
         fd.status = ar.take<int>(F);
         dchol = ar.take<int64_t>(chol_pre_len(N));
+        dfm = ar.take<FragDev>(1);
+        fm.linv = ar.take<double>(CHOL_NB * CHOL_NB);
+        fm.ldpart = ar.take<double>(cdiv(NG, CHOL_NB) * 2);
+        fm.status = ar.take<int>(1);
+        dchol2 = ar.take<int64_t>(chol_pre_len(NG));
     };
     Arena count;
     layout(count);
@@ -261,6 +271,24 @@ int t2_build(brpa_ctx *ctx, int precision, const double *R, int64_t naux, const 
     // (A = U^T U): hence rocblas_fill_upper there.
     BRPA_TRY(chol_single(ctx, dfd, dchol, N, F));
 
+    // Cholesky factor of one dense n x n matrix M in place with the same
+    // kernels: M = U^T U, U in the column-major upper triangle (the row-major
+    // lower one the kernels read and write); *okv = every pivot positive
+    auto chol_dense = [&](double *M, int64_t n, bool *okv) -> int {
+        fm.Pi = M;
+        fm.N = (int)n;
+        fm.F = 1;
+        fm.nsteps = (int)cdiv(n, CHOL_NB);
+        fm.keep_l11 = 1;
+        BRPA_TRY(launch_fill_int(ctx, fm.status, 1, 0));
+        BRPA_TRY(copy_sync(ctx, dfm, &fm, sizeof(FragDev), hipMemcpyHostToDevice));
+        BRPA_TRY(chol_single(ctx, dfm, dchol2, n, 1));
+        int st = 0;
+        BRPA_TRY(copy_sync(ctx, &st, fm.status, sizeof(int), hipMemcpyDeviceToHost));
+        *okv = (st == 0);
+        return BRPA_OK;
+    };
+
     // I * Omega (rpa_CC_IM_v5, rpa/rpa_CC_Doubles.f90:369-422)
     BRPA_TRY(launch_guess(ctx, omega, K, NG, seed));
     BRPA_TRY(launch_normalize_columns(ctx, omega, K, NG));
@@ -286,41 +314,31 @@ int t2_build(brpa_ctx *ctx, int precision, const double *R, int64_t naux, const 
     // gives the same Rayleigh-Ritz pairs below; it is then built by Cholesky-QR
     // (the shifted factor, then two unshifted passes) instead of the NG x NG
     // eigensolve.  Otherwise the eigensolve decides which vectors are kept.
+    // Gram matrices as full GEMMs: rocBLAS dgemm takes a third of dsyrk's time here
     BRPA_TRY(gemm(ctx, h, true, false, NG, NG, K, 1.0, iom, K, iom, K, 0.0, Sm, NG));
     double *Tkeep = iom;           // K x nt orthonormal basis (in place of I Omega)
     int64_t nt = 0;
     bool cholqr = false;
-    {
-        BRPA_TRY(launch_add_diag(ctx, Sm, NG, -cutoff));
-        BRPA_TRY(rb_check(ctx, rocsolver_dpotrf(h, rocblas_fill_lower, (rocblas_int)NG, Sm,
-                                                (rocblas_int)NG, dinfo + F),
-                          "rocsolver_dpotrf"));
-        int hinfo = 0;
-        BRPA_TRY(copy_sync(ctx, &hinfo, dinfo + F, sizeof(int), hipMemcpyDeviceToHost));
-        cholqr = (hinfo == 0);
-    }
+    BRPA_TRY(launch_add_diag(ctx, Sm, NG, -cutoff));
+    BRPA_TRY(chol_dense(Sm, NG, &cholqr));
     const double one = 1.0;
     if (cholqr) {
         nt = NG;
         for (int pass = 0; pass < 3; ++pass) {
             if (pass > 0) {
                 BRPA_TRY(gemm(ctx, h, true, false, NG, NG, K, 1.0, iom, K, iom, K, 0.0, Sm, NG));
-                BRPA_TRY(rb_check(ctx, rocsolver_dpotrf(h, rocblas_fill_lower, (rocblas_int)NG,
-                                                        Sm, (rocblas_int)NG, dinfo + F),
-                                  "rocsolver_dpotrf"));
-                int hinfo = 0;
-                BRPA_TRY(copy_sync(ctx, &hinfo, dinfo + F, sizeof(int), hipMemcpyDeviceToHost));
-                if (hinfo != 0)
+                bool okp = false;
+                BRPA_TRY(chol_dense(Sm, NG, &okp));
+                if (!okp)
                     return brpa_fail(ctx, BRPA_ERR_SOLVER,
                                      "Cholesky-QR pass %d of the T2 basis broke down", pass);
             }
-            // X <- X L^{-T}
+            // X <- X U^{-1}, so that X^T X = 1
             BRPA_TRY(rb_check(ctx,
-                              rocblas_dtrsm(h, rocblas_side_right, rocblas_fill_lower,
-                                            rocblas_operation_transpose,
-                                            rocblas_diagonal_non_unit, (rocblas_int)K,
-                                            (rocblas_int)NG, &one, Sm, (rocblas_int)NG, iom,
-                                            (rocblas_int)K),
+                              rocblas_dtrsm(h, rocblas_side_right, rocblas_fill_upper,
+                                            rocblas_operation_none, rocblas_diagonal_non_unit,
+                                            (rocblas_int)K, (rocblas_int)NG, &one, Sm,
+                                            (rocblas_int)NG, iom, (rocblas_int)K),
                               "rocblas_dtrsm"));
         }
     } else {
@@ -343,7 +361,6 @@ int t2_build(brpa_ctx *ctx, int precision, const double *R, int64_t naux, const 
     // T = (I Omega) U_s s^{-1/2}; TIT = T^T I T (rpa_CC_MIM_v5, 425-488)
     double *T = tmp;               // K x nt
     double *G = Y;                 // N x nt
-    double *H = Hb;                // N x nt
     if (!cholqr) {
         double *Wm = omega;        // NG x nt   (omega no longer needed)
         BRPA_TRY(launch_select_scale(ctx, Wm, Sm, sev, NG, NG - nt, nt));
@@ -355,12 +372,15 @@ int t2_build(brpa_ctx *ctx, int precision, const double *R, int64_t naux, const 
         const double *ck = dchi + k * K;
         BRPA_TRY(launch_row_scale(ctx, T, Tkeep, ck, K, nt));
         BRPA_TRY(gemm(ctx, h, false, false, N, nt, K, 1.0, dR, N, T, K, 0.0, G, N));
-        BRPA_HIP(ctx, hipMemcpyAsync(H, G, 8 * N * nt, hipMemcpyDeviceToDevice, ctx->stream));
+        // G <- U_k^{-T} G with 1 + l Pi_k = U_k^T U_k, so that
+        // G^T (1 + l Pi_k)^{-1} G = G^T G
         BRPA_TRY(rb_check(ctx,
-                          rocsolver_dpotrs(h, rocblas_fill_upper, (rocblas_int)N, (rocblas_int)nt,
-                                           fd.Pi + k * N * N, (rocblas_int)N, H, (rocblas_int)N),
-                          "rocsolver_dpotrs"));
-        BRPA_TRY(gemm(ctx, h, true, false, nt, nt, N, -fw[k] * lambda / (4.0 * kPi), G, N, H, N,
+                          rocblas_dtrsm(h, rocblas_side_left, rocblas_fill_upper,
+                                        rocblas_operation_transpose, rocblas_diagonal_non_unit,
+                                        (rocblas_int)N, (rocblas_int)nt, &one, fd.Pi + k * N * N,
+                                        (rocblas_int)N, G, (rocblas_int)N),
+                          "rocblas_dtrsm"));
+        BRPA_TRY(gemm(ctx, h, true, false, nt, nt, N, -fw[k] * lambda / (4.0 * kPi), G, N, G, N,
                       k == 0 ? 0.0 : 1.0, TIT, nt));
     }
     BRPA_TRY(grow(ctx, &ctx->t2_A, &ctx->t2_A_cap, nt));
@@ -382,13 +402,11 @@ int t2_build(brpa_ctx *ctx, int precision, const double *R, int64_t naux, const 
             return BRPA_OK;
         };
         BRPA_TRY(launch_add_diag(ctx, TIT, nt, 1.0));
-        BRPA_TRY(rb_check(ctx, rocsolver_dpotrf(h, rocblas_fill_lower, (rocblas_int)nt, TIT,
-                                                (rocblas_int)nt, dinfo + F + 1),
-                          "rocsolver_dpotrf"));
-        int ok = 0;
-        BRPA_TRY(info_ok(dinfo + F + 1, &ok));
+        bool okb = false;
+        BRPA_TRY(chol_dense(TIT, nt, &okb));
+        int ok = okb;
         if (ok) {
-            BRPA_TRY(rb_check(ctx, rocsolver_dpotri(h, rocblas_fill_lower, (rocblas_int)nt, TIT,
+            BRPA_TRY(rb_check(ctx, rocsolver_dpotri(h, rocblas_fill_upper, (rocblas_int)nt, TIT,
                                                     (rocblas_int)nt, dinfo + F + 1),
                               "rocsolver_dpotri"));
             BRPA_TRY(info_ok(dinfo + F + 1, &ok));
@@ -396,16 +414,15 @@ int t2_build(brpa_ctx *ctx, int precision, const double *R, int64_t naux, const 
         if (ok) {
             BRPA_TRY(launch_scale(ctx, TIT, 0.5, nt * nt));
             BRPA_TRY(launch_add_diag(ctx, TIT, nt, -0.5));
-            BRPA_TRY(rb_check(ctx, rocsolver_dpotrf(h, rocblas_fill_lower, (rocblas_int)nt, TIT,
-                                                    (rocblas_int)nt, dinfo + F + 1),
-                              "rocsolver_dpotrf"));
-            BRPA_TRY(info_ok(dinfo + F + 1, &ok));
+            BRPA_TRY(chol_dense(TIT, nt, &okb));
+            ok = okb;
         }
         if (ok) {
-            BRPA_TRY(launch_zero_upper(ctx, TIT, nt));
+            // -C = U'^T U' (U' upper): T2 = -V' V'^T with V' = T U'^T
+            BRPA_TRY(launch_zero_lower(ctx, TIT, nt));
             BRPA_TRY(launch_fill(ctx, ctx->t2_A, nt, -1.0));
-            // V'^T = L^T T^T, stored as V[ai][mu]
-            BRPA_TRY(gemm(ctx, h, true, true, nt, K, nt, 1.0, TIT, nt, Tkeep, K, 0.0, ctx->t2_V,
+            // V'^T = U' T^T, stored as V[ai][mu]
+            BRPA_TRY(gemm(ctx, h, false, true, nt, K, nt, 1.0, TIT, nt, Tkeep, K, 0.0, ctx->t2_V,
                           nt));
             done = true;
         } else {
