@@ -441,7 +441,7 @@ def main():
         kx = min(a.exchange_frags, a.nfrag)
         sub = batch[:kx]
         xprec_of = {p: ("split_i8" if p == "split_i8" else "fp64") for p in forms}
-        t_ex, ex_val = {}, {}
+        t_ex, ex_val, ex_stages = {}, {}, {}
         for xprec in sorted(set(xprec_of.values()), key=lambda q: q != xprec_of[a.precision]):
             ctx.exchange_batch(sub[:1], seed=synth.BASE_SEED, nworkers=a.exchange_workers,
                                precision=xprec)                  # warm-up (workspaces)
@@ -452,6 +452,8 @@ def main():
             torch.cuda.synchronize()
             t_ex[xprec] = (time.perf_counter() - t1) / kx
             ex_val[xprec] = exs
+            ex_stages[xprec] = {k: round(v[0] / kx, 3)
+                                for k, v in ctx.last_exchange_stage_ms().items()}
             print("# exchange (%s SOSEX): %d fragments, %.1f ms per fragment" %
                   (xprec, kx, t_ex[xprec] * 1e3), file=sys.stderr, flush=True)
         xprec = xprec_of[a.precision]
@@ -459,6 +461,7 @@ def main():
                 "ms_per_fragment": round(t_ex[xprec] * 1e3, 3),
                 "ms_per_fragment_by_sosex_precision": {q: round(v * 1e3, 3)
                                                        for q, v in t_ex.items()},
+                "stage_ms_per_fragment_by_sosex_precision": ex_stages,
                 "precision": "fp64 T2 amplitudes, %s SOSEX contraction" % xprec,
                 "nguess": "ceil(3.0 N)", "mean_nt2": float(np.mean(nts))}
         if "fp64" in ex_val and "split_i8" in ex_val:

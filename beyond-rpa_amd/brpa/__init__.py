@@ -145,6 +145,15 @@ class Context:
         return {k: (self._lib.brpa_last_stage_ms(self._h, v),
                     self._lib.brpa_last_stage_launches(self._h, v)) for k, v in names.items()}
 
+    def last_exchange_stage_ms(self):
+        """{stage: (ms, calls)} of the last amplitude, SOSEX or exchange-batch
+        call (device time per stage, summed over the fragments of a batch)."""
+        names = {"t2_pi": _lib.STAGE_T2_PI, "t2_iomega": _lib.STAGE_T2_IOMEGA,
+                 "t2_basis": _lib.STAGE_T2_BASIS, "t2_tit": _lib.STAGE_T2_TIT,
+                 "t2_core": _lib.STAGE_T2_CORE, "sosex": _lib.STAGE_SOSEX}
+        return {k: (self._lib.brpa_last_stage_ms(self._h, v),
+                    self._lib.brpa_last_stage_launches(self._h, v)) for k, v in names.items()}
+
     def trim(self):
         self._check(self._lib.brpa_ctx_trim(self._h))
 

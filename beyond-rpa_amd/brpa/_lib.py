@@ -9,6 +9,7 @@ LIB_PATH = os.path.join(_HERE, "libbrpa.so")
  ERR_INTERNAL) = range(10)
 FP64, SPLIT_BF16, SPLIT_I8 = 0, 1, 2
 STAGE_PI, STAGE_SPLIT, STAGE_CHOL, STAGE_AUX = range(4)
+STAGE_T2_PI, STAGE_T2_IOMEGA, STAGE_T2_BASIS, STAGE_T2_TIT, STAGE_T2_CORE, STAGE_SOSEX = range(4, 10)
 
 _i64 = C.c_int64
 _dp = C.POINTER(C.c_double)

@@ -36,6 +36,13 @@ int brpa_exchange_batch(brpa_ctx *ctx, int precision, int64_t nfrag, const brpa_
         ctx->workers.push_back(w);
     }
     brpa_ctx *w = ctx->workers[0];
+    // per-stage device times summed over the fragments, reported on ctx
+    stage_reset(w);
+    w->stage_keep = true;
+    struct KeepOff {
+        brpa_ctx *w;
+        ~KeepOff() { w->stage_keep = false; }
+    } keep_off{w};
     for (int64_t f = 0; f < nfrag; ++f) {
         const brpa_frag_t &fr = frags[f];
         const int64_t ng = (int64_t)std::ceil(guess_factor * (double)fr.naux);
@@ -43,7 +50,7 @@ int brpa_exchange_batch(brpa_ctx *ctx, int precision, int64_t nfrag, const brpa_
         double ex = 0.0;
         // energy only: the factored amplitude core (no eigensolve) unless the S2
         // rescaling needs the amplitudes themselves
-        int rc = t2_build(w, BRPA_FP64, fr.R, fr.naux, fr.eocc, fr.nocc, fr.evir, fr.nvir,
+        int rc = t2_build(w, precision, fr.R, fr.naux, fr.eocc, fr.nocc, fr.evir, fr.nvir,
                           fr.freqs, fr.weights, fr.nfreqs, lambda, cutoff, ng, seed + (uint64_t)f,
                           &nt, !s2_operator);
         if (rc == BRPA_OK && nt > 0)
@@ -54,6 +61,10 @@ int brpa_exchange_batch(brpa_ctx *ctx, int precision, int64_t nfrag, const brpa_
                              w->err.c_str());
         ec_exchange[f] = ex;
         if (nvecs_t2) nvecs_t2[f] = nt;
+    }
+    for (int st = 0; st < BRPA_NSTAGES; ++st) {
+        ctx->stage_ms[st] = w->stage_ms[st];
+        ctx->stage_launches[st] = w->stage_launches[st];
     }
     return BRPA_OK;
 }

@@ -23,6 +23,9 @@ struct brpa_ctx {
     hipEvent_t sev[2 * BRPA_NSTAGES] = {};
     double stage_ms[BRPA_NSTAGES] = {};
     int stage_launches[BRPA_NSTAGES] = {};
+    // set on an exchange-batch worker: the amplitude and SOSEX calls add to the
+    // stage times instead of starting them afresh
+    bool stage_keep = false;
     // amplitude (T2) results kept on the device until brpa_t2_copyout
     void *rocblas = nullptr;   // rocblas_handle, created on first use
     double *t2_A = nullptr, *t2_V = nullptr;

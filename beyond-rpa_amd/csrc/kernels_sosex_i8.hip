@@ -34,10 +34,12 @@ constexpr int SX_NSL = 5;    // slices per value
 constexpr int SX_KB = 128;   // rows (a) per split work-group; vpad is a multiple
 
 // ---------------------------------------------------------------------------
-// Column scales of the blocks: scale[b][c] = max_{a<v} |X[b*v + a][c]| / 127.
+// Column scales of the blocks: scale[b][c] = max_{a<v} |X[b*v + a][c]| / 127,
+// the rows optionally scaled first (X[r][c] * rs[r] when rs is not null).
 // Work-groups: nblk * ceil(ncols/64); thread -> column tid%64, rows tid/64 + 4 i.
 __global__ __launch_bounds__(256) void k_blkscale_i8(const double *__restrict__ X, int64_t ncols,
-                                                    int v, int ncc, double *__restrict__ scale) {
+                                                    int v, int ncc, const double *__restrict__ rs,
+                                                    double *__restrict__ scale) {
     __shared__ double sM[4][64];
     const int b = blockIdx.x / ncc, cc = blockIdx.x - b * ncc;
     const int tid = threadIdx.x, c = tid & 63, g = tid >> 6;
@@ -52,6 +54,7 @@ __global__ __launch_bounds__(256) void k_blkscale_i8(const double *__restrict__ 
             const int a = a0 + 4 * u;
             const bool in = ok && a < v;
             x[u] = G[in ? ((int64_t)b * v + a) * ncols + col : 0];
+            if (rs) x[u] *= rs[in ? (int64_t)b * v + a : 0];
             x[u] = in ? fabs(x[u]) : 0.0;
         }
         m = fmax(m, fmax(fmax(x[0], x[1]), fmax(x[2], x[3])));
@@ -68,6 +71,7 @@ __global__ __launch_bounds__(256) void k_blkscale_i8(const double *__restrict__ 
 // is staged in LDS; thread -> column tid/4, rows 32 (tid%4) .. +32.
 __global__ __launch_bounds__(256) void k_blksplit_i8(const double *__restrict__ X, int64_t ncols,
                                                     int v, int vpad, int ncc,
+                                                    const double *__restrict__ rs,
                                                     const double *__restrict__ scale,
                                                     int8_t *__restrict__ out) {
     __shared__ double sX[SX_KB][65];
@@ -88,6 +92,7 @@ __global__ __launch_bounds__(256) void k_blksplit_i8(const double *__restrict__ 
             const int r = r0 + 4 * u;
             const bool in = (a0 + r < v) && (c0 + c < ncols);
             x[u] = G[in ? ((int64_t)b * v + a0 + r) * ncols + c0 + c : 0];
+            if (rs) x[u] *= rs[in ? (int64_t)b * v + a0 + r : 0];   // the product of the scale pass
             x[u] = in ? x[u] : 0.0;
         }
 #pragma unroll
@@ -340,14 +345,70 @@ size_t sosex_i8_slice_bytes(int64_t nblk, int64_t ncols, int64_t v) {
 }
 
 int launch_sosex_i8_split(brpa_ctx *ctx, const double *X, int64_t nblk, int64_t v, int64_t ncols,
-                          double *scale, int8_t *out) {
+                          double *scale, int8_t *out, const double *rs) {
     const int ncc = (int)((ncols + 63) / 64);
     const int vpad = sosex_i8_vpad(v);
     hipLaunchKernelGGL(k_blkscale_i8, dim3((unsigned)(nblk * ncc)), dim3(256), 0, ctx->stream, X,
-                       ncols, (int)v, ncc, scale);
+                       ncols, (int)v, ncc, rs, scale);
     LAUNCH_CHECK(ctx);
     hipLaunchKernelGGL(k_blksplit_i8, dim3((unsigned)(nblk * (vpad / SX_KB) * ncc)), dim3(256), 0,
-                       ctx->stream, X, ncols, (int)v, vpad, ncc, scale, out);
+                       ctx->stream, X, ncols, (int)v, vpad, ncc, rs, scale, out);
+    LAUNCH_CHECK(ctx);
+    return BRPA_OK;
+}
+
+// Column maxima of one row-major block in parallel over row chunks of 256:
+// pm[rc][c] = max over rows 256 rc .. of |X[r][c] rs[r]|; then scale[c] =
+// max_rc pm[rc][c] / 127 (the same products as k_blksplit_i8 forms).
+__global__ __launch_bounds__(256) void k_colmax_part(const double *__restrict__ X, int64_t ncols,
+                                                    int v, int ncc, const double *__restrict__ rs,
+                                                    double *__restrict__ pm) {
+    __shared__ double sM[4][64];
+    const int rc = blockIdx.x / ncc, cc = blockIdx.x - rc * ncc;
+    const int tid = threadIdx.x, c = tid & 63, g = tid >> 6;
+    const int64_t col = (int64_t)cc * 64 + c;
+    const bool ok = col < ncols;
+    const gdbl_t *G = (const gdbl_t *)X;
+    double m = 0.0;
+#pragma unroll 4
+    for (int u = 0; u < 64; ++u) {
+        const int a = rc * 256 + g + 4 * u;
+        const bool in = ok && a < v;
+        double x = G[in ? (int64_t)a * ncols + col : 0];
+        if (rs) x *= rs[in ? a : 0];
+        m = fmax(m, in ? fabs(x) : 0.0);
+    }
+    sM[g][c] = m;
+    __syncthreads();
+    if (tid < 64 && ok)
+        pm[(int64_t)rc * ncols + col] = fmax(fmax(sM[0][c], sM[1][c]), fmax(sM[2][c], sM[3][c]));
+}
+
+__global__ __launch_bounds__(256) void k_colmax_final(const double *__restrict__ pm, int64_t ncols,
+                                                     int nrc, double *__restrict__ scale) {
+    const int64_t col = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (col >= ncols) return;
+    double m = 0.0;
+    for (int rc = 0; rc < nrc; ++rc) m = fmax(m, pm[(int64_t)rc * ncols + col]);
+    scale[col] = m / 127.0;
+}
+
+int64_t rowslice_i8_pm_len(int64_t v, int64_t ncols) { return (v + 255) / 256 * ncols; }
+
+int launch_rowslice_i8(brpa_ctx *ctx, const double *X, int64_t v, int64_t ncols, const double *rs,
+                       double *pm, double *scale, int8_t *out) {
+    if (v <= 0 || ncols <= 0) return BRPA_OK;
+    const int ncc = (int)((ncols + 63) / 64);
+    const int nrc = (int)((v + 255) / 256);
+    const int vpad = sosex_i8_vpad(v);
+    hipLaunchKernelGGL(k_colmax_part, dim3((unsigned)(nrc * ncc)), dim3(256), 0, ctx->stream, X,
+                       ncols, (int)v, ncc, rs, pm);
+    LAUNCH_CHECK(ctx);
+    hipLaunchKernelGGL(k_colmax_final, dim3((unsigned)((ncols + 255) / 256)), dim3(256), 0,
+                       ctx->stream, pm, ncols, nrc, scale);
+    LAUNCH_CHECK(ctx);
+    hipLaunchKernelGGL(k_blksplit_i8, dim3((unsigned)((vpad / SX_KB) * ncc)), dim3(256), 0,
+                       ctx->stream, X, ncols, (int)v, vpad, ncc, rs, scale, out);
     LAUNCH_CHECK(ctx);
     return BRPA_OK;
 }

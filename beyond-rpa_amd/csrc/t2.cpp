@@ -38,8 +38,19 @@ int launch_pair_dot(brpa_ctx *, const double *W1, const double *W2, const double
 int sosex_nchunk(int64_t n_rows, int64_t nt);
 int sosex_i8_vpad(int64_t v);
 size_t sosex_i8_slice_bytes(int64_t nblk, int64_t ncols, int64_t v);
+// rs: optional row scale (X[r][c] * rs[r] is sliced)
 int launch_sosex_i8_split(brpa_ctx *, const double *X, int64_t nblk, int64_t v, int64_t ncols,
-                          double *scale, int8_t *out);
+                          double *scale, int8_t *out, const double *rs = nullptr);
+int launch_colslice_i8(brpa_ctx *, const double *X, int64_t ld, int64_t len, int64_t ncols,
+                       const double *rs, int64_t kpad, double *scale, int8_t *out);
+// one row-major v x ncols block, per column over the v rows (scaled by rs),
+// column maxima in parallel over row chunks (pm: rowslice_i8_pm_len doubles)
+int64_t rowslice_i8_pm_len(int64_t v, int64_t ncols);
+int launch_rowslice_i8(brpa_ctx *, const double *X, int64_t v, int64_t ncols, const double *rs,
+                       double *pm, double *scale, int8_t *out);
+int launch_gemm_i8(brpa_ctx *, const int8_t *SA, const double *sa, int64_t M, const int8_t *SB,
+                   const double *sb, int64_t N, int64_t kpad, double alpha, const double *rs,
+                   double beta, double *out, int64_t sm, int64_t sn);
 int64_t sosex_i8_nparts(int64_t o, int64_t N, int64_t NT);
 int launch_sosex_i8_pairs(brpa_ctx *, const int8_t *SR, const int8_t *SV, const double *scR,
                           const double *scV, const double *B, int64_t o, int64_t v, int64_t N,
@@ -152,14 +163,21 @@ int t2_build(brpa_ctx *ctx, int precision, const double *R, int64_t naux, const 
              const double *weights, int64_t nfreqs, double lambda, double cutoff, int64_t nguess,
              uint64_t seed, int64_t *nvecs_t2, bool factored) {
     if (!ctx) return BRPA_ERR_ARG;
-    if (precision != BRPA_FP64)
-        return brpa_fail(ctx, BRPA_ERR_ARG, "brpa_t2_compute: only BRPA_FP64 is implemented");
+    if (precision != BRPA_FP64 && precision != BRPA_SPLIT_I8)
+        return brpa_fail(ctx, BRPA_ERR_ARG,
+                         "brpa_t2_compute: precision must be BRPA_FP64 or BRPA_SPLIT_I8");
     if (!R || !eocc || !evir || !freqs || !weights || !nvecs_t2 || naux <= 0 || nocc <= 0 ||
         nvir <= 0 || nfreqs <= 0 || !(lambda > 0.0))
         return brpa_fail(ctx, BRPA_ERR_ARG, "brpa_t2_compute: bad arguments");
     const int64_t K = nocc * nvir, N = naux, F = nfreqs;
     if (K > INT32_MAX || N > 65536)
         return brpa_fail(ctx, BRPA_ERR_ARG, "brpa_t2_compute: sizes beyond the supported range");
+    // split-int8 amplitude contractions: exact int32 level sums need K <= 65536
+    const bool split = precision == BRPA_SPLIT_I8;
+    if (split && K > I8_KMAX)
+        return brpa_fail(ctx, BRPA_ERR_ARG,
+                         "brpa_t2_compute: nocc*nvir = %lld exceeds %d, the largest contraction "
+                         "with exact int32 sums in the split-int8 form", (long long)K, I8_KMAX);
     const int64_t NG = nguess > 0 ? std::min<int64_t>(nguess, K)
                                   : std::min<int64_t>(K, (int64_t)std::ceil(3.0 * (double)N));
     std::vector<double> eo, ev, fx, fw;
@@ -194,6 +212,12 @@ int t2_build(brpa_ctx *ctx, int precision, const double *R, int64_t naux, const 
     int64_t *dchol = nullptr, *dchol2 = nullptr;
     FragDev fm{};                  // one dense matrix for the library's Cholesky
     FragDev *dfm = nullptr;
+    // split-int8 operands, 5 slices each: chi_k R per column P over ai for every
+    // k (SCR), R per row ai over P (SRR), Omega and then T per column over ai
+    // (SOT), Y_k per column over P (SY)
+    const int64_t Kp = split ? sosex_i8_vpad(K) : 0, Np = cdiv(N, 64) * 64;
+    int8_t *SCR = nullptr, *SRR = nullptr, *SOT = nullptr, *SY = nullptr;
+    double *scCR = nullptr, *scRR = nullptr, *scOT = nullptr, *scY = nullptr, *pmx = nullptr;
     std::vector<int64_t> pre = {0, 0, 0, 0, 0, 0};
     const int64_t ntile = cdiv(N, SYRK_T);
     pre[1] = cdiv(F * K, 256);                      // dtab
@@ -230,6 +254,17 @@ int t2_build(brpa_ctx *ctx, int precision, const double *R, int64_t naux, const 
         fm.ldpart = ar.take<double>(cdiv(NG, CHOL_NB) * 2);
         fm.status = ar.take<int>(1);
         dchol2 = ar.take<int64_t>(chol_pre_len(NG));
+        if (split) {
+            SCR = ar.take<int8_t>(F * 5 * N * Kp);
+            SRR = ar.take<int8_t>(5 * K * Np);
+            SOT = ar.take<int8_t>(5 * NG * Kp);
+            SY = ar.take<int8_t>(5 * NG * Np);
+            scCR = ar.take<double>(F * N);
+            scRR = ar.take<double>(K);
+            scOT = ar.take<double>(NG);
+            scY = ar.take<double>(NG);
+            pmx = ar.take<double>(rowslice_i8_pm_len(K, N));
+        }
     };
     Arena count;
     layout(count);
@@ -259,6 +294,14 @@ int t2_build(brpa_ctx *ctx, int precision, const double *R, int64_t naux, const 
     BRPA_HIP(ctx, hipMemcpyAsync(dpre, pre.data(), 8 * pre.size(), hipMemcpyHostToDevice,
                                  ctx->stream));
     BRPA_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    if (!ctx->stage_keep) stage_reset(ctx);
+    // stages whose events were recorded in this call (collected at the end)
+    bool staged[BRPA_NSTAGES] = {};
+    auto begin = [&](int st) -> int {
+        staged[st] = true;
+        return stage_begin(ctx, st);
+    };
+    BRPA_TRY(begin(BRPA_STAGE_T2_PI));
 
     // Pi_k for all k (FP64 MFMA SYRK), then 1 + l Pi_k and its Cholesky factor
     BRPA_TRY(launch_dtab(ctx, dfd, dpre, 1, pre[1]));
@@ -270,6 +313,7 @@ int t2_build(brpa_ctx *ctx, int precision, const double *R, int64_t naux, const 
     // upper triangle U = L^T of the column-major view the solvers below read
     // (A = U^T U): hence rocblas_fill_upper there.
     BRPA_TRY(chol_single(ctx, dfd, dchol, N, F));
+    BRPA_TRY(stage_end(ctx, BRPA_STAGE_T2_PI));
 
     // Cholesky factor of one dense n x n matrix M in place with the same
     // kernels: M = U^T U, U in the column-major upper triangle (the row-major
@@ -290,11 +334,35 @@ int t2_build(brpa_ctx *ctx, int precision, const double *R, int64_t naux, const 
     };
 
     // I * Omega (rpa_CC_IM_v5, rpa/rpa_CC_Doubles.f90:369-422)
+    BRPA_TRY(begin(BRPA_STAGE_T2_IOMEGA));
     BRPA_TRY(launch_guess(ctx, omega, K, NG, seed));
     BRPA_TRY(launch_normalize_columns(ctx, omega, K, NG));
     BRPA_HIP(ctx, hipMemsetAsync(iom, 0, 8 * K * NG, ctx->stream));
+    if (split) {
+        // frequency-independent operands, once per fragment
+        BRPA_TRY(launch_colslice_i8(ctx, dR, N, N, K, nullptr, Np, scRR, SRR));
+        BRPA_TRY(launch_colslice_i8(ctx, omega, K, K, NG, nullptr, Kp, scOT, SOT));
+    }
     for (int64_t k = 0; k < F; ++k) {
         const double *ck = dchi + k * K;
+        const double coef = -fw[k] * lambda / (4.0 * kPi);
+        if (split) {
+            // Y = (chi_k R)^T Omega, Y <- (1 + l Pi_k)^-1 Y, I Omega += coef chi_k R Y;
+            // the chi_k R slices are kept for T^T I T
+            int8_t *SCRk = SCR + k * 5 * N * Kp;
+            double *scCRk = scCR + k * N;
+            BRPA_TRY(launch_rowslice_i8(ctx, dR, K, N, ck, pmx, scCRk, SCRk));
+            BRPA_TRY(launch_gemm_i8(ctx, SCRk, scCRk, N, SOT, scOT, NG, Kp, 1.0, nullptr, 0.0, Y, 1,
+                                    N));
+            BRPA_TRY(rb_check(ctx,
+                              rocsolver_dpotrs(h, rocblas_fill_upper, (rocblas_int)N,
+                                               (rocblas_int)NG, fd.Pi + k * N * N, (rocblas_int)N,
+                                               Y, (rocblas_int)N),
+                              "rocsolver_dpotrs"));
+            BRPA_TRY(launch_colslice_i8(ctx, Y, N, N, NG, nullptr, Np, scY, SY));
+            BRPA_TRY(launch_gemm_i8(ctx, SRR, scRR, K, SY, scY, NG, Np, coef, ck, 1.0, iom, 1, K));
+            continue;
+        }
         BRPA_TRY(launch_row_scale(ctx, tmp, omega, ck, K, NG));
         BRPA_TRY(gemm(ctx, h, false, false, N, NG, K, 1.0, dR, N, tmp, K, 0.0, Y, N));
         BRPA_TRY(rb_check(ctx,
@@ -302,9 +370,10 @@ int t2_build(brpa_ctx *ctx, int precision, const double *R, int64_t naux, const 
                                            fd.Pi + k * N * N, (rocblas_int)N, Y, (rocblas_int)N),
                           "rocsolver_dpotrs"));
         BRPA_TRY(gemm(ctx, h, true, false, K, NG, N, 1.0, dR, N, Y, N, 0.0, tmp, K));
-        BRPA_TRY(launch_row_scale_acc(ctx, iom, tmp, ck, -fw[k] * lambda / (4.0 * kPi), K, NG));
+        BRPA_TRY(launch_row_scale_acc(ctx, iom, tmp, ck, coef, K, NG));
     }
     BRPA_TRY(launch_normalize_columns(ctx, iom, K, NG));
+    BRPA_TRY(stage_end(ctx, BRPA_STAGE_T2_IOMEGA));
 
     // S = (I Omega)^T (I Omega) (rpa_CC_Doubles.f90:595-609).  The reference
     // keeps the eigenvectors of S with eigenvalues above the cutoff and forms
@@ -312,8 +381,14 @@ int t2_build(brpa_ctx *ctx, int precision, const double *R, int64_t naux, const 
     // exceeds the cutoff -- exactly when S - cutoff*1 has a Cholesky factor --
     // T spans all of span(I Omega), and any orthonormal basis of that span
     // gives the same Rayleigh-Ritz pairs below; it is then built by Cholesky-QR
-    // (the shifted factor, then two unshifted passes) instead of the NG x NG
+    // (the shifted factor, then one unshifted pass) instead of the NG x NG
     // eigensolve.  Otherwise the eigensolve decides which vectors are kept.
+    // Two passes suffice: with every eigenvalue above the cutoff, X = I Omega
+    // has kappa(X) <= (NG / cutoff)^(1/2) < 1e8, the first pass leaves
+    // X1^T X1 = 1 + cutoff U1^-T U1^-1 + O(eps kappa^2), and the second pass
+    // restores orthonormality to O(eps kappa(X1)^2) -- never worse than the
+    // eps s_max / s_min of the reference's own eigenvector basis.
+    BRPA_TRY(begin(BRPA_STAGE_T2_BASIS));
     // Gram matrices as full GEMMs: rocBLAS dgemm takes a third of dsyrk's time here
     BRPA_TRY(gemm(ctx, h, true, false, NG, NG, K, 1.0, iom, K, iom, K, 0.0, Sm, NG));
     double *Tkeep = iom;           // K x nt orthonormal basis (in place of I Omega)
@@ -324,7 +399,7 @@ int t2_build(brpa_ctx *ctx, int precision, const double *R, int64_t naux, const 
     const double one = 1.0;
     if (cholqr) {
         nt = NG;
-        for (int pass = 0; pass < 3; ++pass) {
+        for (int pass = 0; pass < 2; ++pass) {
             if (pass > 0) {
                 BRPA_TRY(gemm(ctx, h, true, false, NG, NG, K, 1.0, iom, K, iom, K, 0.0, Sm, NG));
                 bool okp = false;
@@ -356,7 +431,17 @@ int t2_build(brpa_ctx *ctx, int precision, const double *R, int64_t naux, const 
     *nvecs_t2 = nt;
     ctx->t2_nt = nt;
     ctx->t2_nai = K;
-    if (nt == 0) return BRPA_OK;
+    // the stage times of this call, once the stream is idle
+    auto collect = [&]() -> int {
+        BRPA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        for (int st = 0; st < BRPA_NSTAGES; ++st)
+            if (staged[st]) BRPA_TRY(stage_collect(ctx, st));
+        return BRPA_OK;
+    };
+    if (nt == 0) {
+        BRPA_TRY(stage_end(ctx, BRPA_STAGE_T2_BASIS));
+        return collect();
+    }
 
     // T = (I Omega) U_s s^{-1/2}; TIT = T^T I T (rpa_CC_MIM_v5, 425-488)
     double *T = tmp;               // K x nt
@@ -367,11 +452,20 @@ int t2_build(brpa_ctx *ctx, int precision, const double *R, int64_t naux, const 
         BRPA_TRY(gemm(ctx, h, false, false, K, nt, NG, 1.0, iom, K, Wm, NG, 0.0, T, K));
         BRPA_HIP(ctx, hipMemcpyAsync(Tkeep, T, 8 * K * nt, hipMemcpyDeviceToDevice, ctx->stream));
     }
+    BRPA_TRY(stage_end(ctx, BRPA_STAGE_T2_BASIS));
+    BRPA_TRY(begin(BRPA_STAGE_T2_TIT));
     double *TIT = Sm;              // nt x nt
+    if (split) BRPA_TRY(launch_colslice_i8(ctx, Tkeep, K, K, nt, nullptr, Kp, scOT, SOT));
     for (int64_t k = 0; k < F; ++k) {
         const double *ck = dchi + k * K;
-        BRPA_TRY(launch_row_scale(ctx, T, Tkeep, ck, K, nt));
-        BRPA_TRY(gemm(ctx, h, false, false, N, nt, K, 1.0, dR, N, T, K, 0.0, G, N));
+        if (split) {
+            // G = (chi_k R)^T T
+            BRPA_TRY(launch_gemm_i8(ctx, SCR + k * 5 * N * Kp, scCR + k * N, N, SOT, scOT, nt, Kp,
+                                    1.0, nullptr, 0.0, G, 1, N));
+        } else {
+            BRPA_TRY(launch_row_scale(ctx, T, Tkeep, ck, K, nt));
+            BRPA_TRY(gemm(ctx, h, false, false, N, nt, K, 1.0, dR, N, T, K, 0.0, G, N));
+        }
         // G <- U_k^{-T} G with 1 + l Pi_k = U_k^T U_k, so that
         // G^T (1 + l Pi_k)^{-1} G = G^T G
         BRPA_TRY(rb_check(ctx,
@@ -383,8 +477,10 @@ int t2_build(brpa_ctx *ctx, int precision, const double *R, int64_t naux, const 
         BRPA_TRY(gemm(ctx, h, true, false, nt, nt, N, -fw[k] * lambda / (4.0 * kPi), G, N, G, N,
                       k == 0 ? 0.0 : 1.0, TIT, nt));
     }
+    BRPA_TRY(stage_end(ctx, BRPA_STAGE_T2_TIT));
     BRPA_TRY(grow(ctx, &ctx->t2_A, &ctx->t2_A_cap, nt));
     BRPA_TRY(grow(ctx, &ctx->t2_V, &ctx->t2_V_cap, (size_t)nt * K));
+    BRPA_TRY(begin(BRPA_STAGE_T2_CORE));
     bool done = false;
     if (factored) {
         // Energy-only callers need T2 = T C T^T, C = (1/2) TIT (1 + TIT)^{-1}, not the
@@ -439,11 +535,12 @@ int t2_build(brpa_ctx *ctx, int precision, const double *R, int64_t naux, const 
         BRPA_TRY(copy_sync(ctx, ctx->t2_A, ha.data(), 8 * nt, hipMemcpyHostToDevice));
         BRPA_TRY(gemm(ctx, h, true, true, nt, K, nt, 1.0, TIT, nt, Tkeep, K, 0.0, ctx->t2_V, nt));
     }
+    BRPA_TRY(stage_end(ctx, BRPA_STAGE_T2_CORE));
     BRPA_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
     std::vector<int> info(F);
     BRPA_HIP(ctx, hipMemcpyAsync(info.data(), fd.status, sizeof(int) * F, hipMemcpyDeviceToHost,
                                  ctx->stream));
-    BRPA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    BRPA_TRY(collect());
     float ms = 0.f;
     BRPA_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     ctx->last_kernel_ms = ms;
@@ -488,6 +585,7 @@ int brpa_sosex(brpa_ctx *ctx, int precision, double *ec_exchange, double alpha, 
                          "brpa_sosex: nvir = %lld exceeds 65536, the largest K with exact int32 "
                          "sums in the split-int8 form", (long long)nvir);
     *ec_exchange = 0.0;
+    if (!ctx->stage_keep) stage_reset(ctx);
     const int64_t K = nocc * nvir, N = naux, nt = nvecs_t2, o = nocc, v = nvir;
     if (nt == 0) return BRPA_OK;
     if (K > INT32_MAX || N > 65536 || nt > 1000000)
@@ -529,12 +627,15 @@ int brpa_sosex(brpa_ctx *ctx, int precision, double *ec_exchange, double alpha, 
         if (!vdev) BRPA_TRY(to_device(ctx, dV, V, 8 * K * nt));
         BRPA_HIP(ctx, hipMemcpyAsync(dB, hB.data(), 8 * nt, hipMemcpyHostToDevice, ctx->stream));
         BRPA_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+        BRPA_TRY(stage_begin(ctx, BRPA_STAGE_SOSEX));
         BRPA_TRY(launch_sosex_i8_split(ctx, dR, o, v, N, scR, SR));
         BRPA_TRY(launch_sosex_i8_split(ctx, dV, o, v, nt, scV, SV));
         BRPA_TRY(launch_sosex_i8_pairs(ctx, SR, SV, scR, scV, dB, o, v, N, nt, part, out2));
+        BRPA_TRY(stage_end(ctx, BRPA_STAGE_SOSEX));
         BRPA_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
         double h2[2];
         BRPA_TRY(copy_sync(ctx, h2, out2, sizeof(h2), hipMemcpyDeviceToHost));
+        BRPA_TRY(stage_collect(ctx, BRPA_STAGE_SOSEX));
         float ms = 0.f;
         BRPA_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
         ctx->last_kernel_ms = ms;
@@ -553,7 +654,7 @@ int brpa_sosex(brpa_ctx *ctx, int precision, double *ec_exchange, double alpha, 
         dB = ar.take<double>(nt);
         W1 = ar.take<double>(jb * N * nt);
         W2 = ar.take<double>(jb * N * nt);
-        part = ar.take<double>(2 * jb * nchunk);
+        part = ar.take<double>(2 * o * o * nchunk);   // (i, j) pair partials
     };
     Arena count;
     layout(count);
@@ -564,8 +665,7 @@ int brpa_sosex(brpa_ctx *ctx, int precision, double *ec_exchange, double alpha, 
     if (!vdev) BRPA_TRY(to_device(ctx, dV, V, 8 * K * nt));
     BRPA_HIP(ctx, hipMemcpyAsync(dB, hB.data(), 8 * nt, hipMemcpyHostToDevice, ctx->stream));
     BRPA_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-    std::vector<double> hpart(2 * jb * nchunk);
-    long double e = 0.0L;
+    BRPA_TRY(stage_begin(ctx, BRPA_STAGE_SOSEX));
     const double one = 1.0, zero = 0.0;
     for (int64_t i = 0; i < o; ++i) {
         for (int64_t j0 = i; j0 < o; j0 += jb) {
@@ -588,22 +688,28 @@ int brpa_sosex(brpa_ctx *ctx, int precision, double *ec_exchange, double alpha, 
                                   (rocblas_int)nt, 0, &zero, W2, (rocblas_int)N, N * nt,
                                   (rocblas_int)nb),
                               "rocblas_dgemm_strided_batched"));
-            BRPA_TRY(launch_pair_dot(ctx, W1, W2, dB, N, nt, N * nt, (int)nb, part));
-            BRPA_HIP(ctx, hipMemcpyAsync(hpart.data(), part, 8 * 2 * nb * nchunk,
-                                         hipMemcpyDeviceToHost, ctx->stream));
-            BRPA_HIP(ctx, hipStreamSynchronize(ctx->stream));
-            for (int64_t jj = 0; jj < nb; ++jj) {
-                long double s = 0.0L;
-                for (int c = 0; c < nchunk; ++c)
-                    s += (long double)hpart[2 * (jj * nchunk + c)] +
-                         (long double)hpart[2 * (jj * nchunk + c) + 1];
-                // EcExchange1 = -sum W_ji W_ij B; off-diagonal pairs count twice
-                e += (j0 + jj == i ? 1.0L : 2.0L) * (-s);
-            }
+            BRPA_TRY(launch_pair_dot(ctx, W1, W2, dB, N, nt, N * nt, (int)nb,
+                                     part + 2 * (i * o + j0) * nchunk));
+        }
+    }
+    BRPA_TRY(stage_end(ctx, BRPA_STAGE_SOSEX));
+    std::vector<double> hpart(2 * o * o * nchunk);
+    BRPA_HIP(ctx, hipMemcpyAsync(hpart.data(), part, 8 * hpart.size(), hipMemcpyDeviceToHost,
+                                 ctx->stream));
+    long double e = 0.0L;
+    for (int64_t i = 0; i < o; ++i) {
+        for (int64_t j = i; j < o; ++j) {
+            long double sp = 0.0L;
+            for (int c = 0; c < nchunk; ++c)
+                sp += (long double)hpart[2 * ((i * o + j) * nchunk + c)] +
+                      (long double)hpart[2 * ((i * o + j) * nchunk + c) + 1];
+            // EcExchange1 = -sum W_ji W_ij B; off-diagonal pairs count twice
+            e += (j == i ? 1.0L : 2.0L) * (-sp);
         }
     }
     BRPA_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
     BRPA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    BRPA_TRY(stage_collect(ctx, BRPA_STAGE_SOSEX));
     float ms = 0.f;
     BRPA_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     ctx->last_kernel_ms = ms;

@@ -84,15 +84,23 @@ int brpa_last_graph_stats(const brpa_ctx *ctx, int *captures, int *replays);
  * (HIP events recorded on the context's stream around the kernels only). */
 double brpa_last_kernel_ms(const brpa_ctx *ctx);
 
-/* Device time (ms) of one stage of the last E_RPA call, summed over its
- * launches, and the number of launches.  Stages: */
+/* Device time (ms) of one stage of the last E_RPA call (stages 0-3), or of
+ * the last amplitude / SOSEX / exchange-batch call (stages 4-9), summed over
+ * its launches (the exchange stages include the device idle time inside the
+ * stage), and the number of launches.  Stages: */
 enum {
     BRPA_STAGE_PI = 0,     /* Pi(u) SYRK kernel (FP64, split-BF16 or split-int8 MFMA) */
     BRPA_STAGE_SPLIT = 1,  /* slicing passes of the split forms (int8: column scales,
                               exact diagonal and slices) */
     BRPA_STAGE_CHOL = 2,   /* blocked Cholesky log-det (panel + trailing updates) */
     BRPA_STAGE_AUX = 3,    /* Delta table, exact diagonal, quadrature */
-    BRPA_NSTAGES = 4
+    BRPA_STAGE_T2_PI = 4,     /* amplitudes: Pi(u_k) and the Cholesky factors of 1 + l Pi */
+    BRPA_STAGE_T2_IOMEGA = 5, /* amplitudes: I * Omega over all frequencies */
+    BRPA_STAGE_T2_BASIS = 6,  /* amplitudes: orthonormal basis T (Cholesky-QR or eigensolve) */
+    BRPA_STAGE_T2_TIT = 7,    /* amplitudes: T^T I T over all frequencies */
+    BRPA_STAGE_T2_CORE = 8,   /* amplitudes: factored core or eigenpairs, and V */
+    BRPA_STAGE_SOSEX = 9,     /* SOSEX contraction (slicing and pair products) */
+    BRPA_NSTAGES = 10
 };
 double brpa_last_stage_ms(const brpa_ctx *ctx, int stage);
 int brpa_last_stage_launches(const brpa_ctx *ctx, int stage);
@@ -183,7 +191,11 @@ int brpa_autotune_result(const brpa_ctx *ctx, int bucket, int *variant, double *
  * a Cholesky factor of 1 + l Pi(u) built from R.  Guess vectors: uniform
  * (-1/2, 1/2) from a SplitMix64 counter hash of (seed, column, row), columns
  * normalised (the reference draws them from gfortran's RNG, 571-588).
- * precision: BRPA_FP64 only. */
+ * precision: BRPA_FP64 (the drop-in: rocBLAS FP64 GEMMs) or BRPA_SPLIT_I8:
+ * the amplitude contractions R^T chi_k Omega, R Y_k and R^T chi_k T run as
+ * split-int8 GEMMs (5 int8 slices per value, 15 products, exact int32 level
+ * sums, FP64 combination; nocc*nvir <= 65536); Pi(u), the Cholesky factors,
+ * the solves, the Gram matrices and the amplitude core stay FP64. */
 int brpa_t2_compute(brpa_ctx *ctx, int precision, const double *R, int64_t naux,
                     const double *eocc, int64_t nocc, const double *evir, int64_t nvir,
                     const double *freqs, const double *weights, int64_t nfreqs, double lambda,
@@ -213,8 +225,9 @@ int brpa_sosex(brpa_ctx *ctx, int precision, double *ec_exchange, double alpha, 
  * The fragments run one after another on a worker context; nworkers must be 0
  * or 1 (several in flight on host threads gave wrong rocBLAS/rocSOLVER results
  * and is disabled).  ec_exchange[f] receives E_x; nvecs_t2 (may be NULL) the
- * retained NT2.  precision: arithmetic of the SOSEX contraction (BRPA_FP64 or
- * BRPA_SPLIT_I8); the amplitudes are built in FP64. */
+ * retained NT2.  precision: arithmetic of the amplitude contractions and of
+ * the SOSEX contraction (BRPA_FP64 or BRPA_SPLIT_I8, as in brpa_t2_compute
+ * and brpa_sosex). */
 int brpa_exchange_batch(brpa_ctx *ctx, int precision, int64_t nfrag, const brpa_frag_t *frags,
                         double lambda, double cutoff, double guess_factor, uint64_t seed,
                         double alpha, int s2_operator, int nworkers, double *ec_exchange,

@@ -39,6 +39,29 @@ def test_t2_matches_oracle(ctx, o, v, n, nf, ng):
     assert np.max(np.abs(V.T @ V - np.eye(len(A)))) <= 100 * dev_ref + 1e-9
 
 
+@pytest.mark.parametrize("o,v,n,nf", [(3, 12, 40, 10), (6, 40, 90, 12), (12, 60, 324, 12),
+                                      (20, 70, 405, 16)])
+def test_t2_split_i8_matches_oracle(ctx, o, v, n, nf):
+    """Split-int8 amplitude contractions (5 slices, 15 products): the T2 matrix
+    within 1e-7 of the FP64 oracle relative to its largest element, on the same
+    guess vectors, and E_x within 1e-7 Ha (split-int8 SOSEX of those amplitudes).
+    A Gram eigenvalue next to the cutoff may be kept on one side only, so the
+    retained counts may differ by one."""
+    f, x, w, Pis = _setup(o, v, n, nf, 31)
+    A_ref, V_ref = t2.t2_v5(f.R, f.gaps(), x, w, Pis, seed=77)
+    A, V = ctx.t2(f.R, f.eocc, f.evir, x, w, seed=77, precision="split_i8")
+    print("NT2", len(A), len(A_ref))
+    assert abs(len(A) - len(A_ref)) <= 1
+    T_ref = (V_ref * A_ref) @ V_ref.T
+    T = (V * A) @ V.T
+    print("max |dT| / max |T|", np.max(np.abs(T - T_ref)) / np.max(np.abs(T_ref)))
+    assert np.max(np.abs(T - T_ref)) <= 1e-7 * np.max(np.abs(T_ref))
+    ex_ref = sosex.sosex(f.R, V_ref, A_ref, o, v)
+    ex = ctx.sosex(V, A, f.R, o, v, precision="split_i8")
+    print("E_x %.10f  split - oracle %.3e" % (ex_ref, ex - ex_ref))
+    assert abs(ex - ex_ref) <= 1e-7
+
+
 def test_t2_full_space_invariants(ctx):
     """NG = ov and a zero cutoff: T2 = I(1+I)^-1/2 and E_T2 = E_RPA."""
     f, x, w, Pis = _setup(2, 3, 4, 24, 7)
@@ -100,7 +123,9 @@ def test_exchange_batch_matches_sequential(ctx):
     assert np.all(nt > 0)
     ex2, _ = ctx.exchange_batch(frags, seed=77, s2_operator=True)
     assert np.array_equal(ex2, np.array(seq2)), ex2 - np.array(seq2)
+    # split-int8 amplitude contractions and SOSEX
     exi, _ = ctx.exchange_batch(frags, seed=77, precision="split_i8")
+    print("split_i8 - oracle", exi - np.array(refs))
     assert np.max(np.abs(exi - np.array(refs))) <= 1e-7
     import brpa
     with pytest.raises(brpa.BrpaError) as ei:
