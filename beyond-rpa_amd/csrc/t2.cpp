@@ -4,7 +4,8 @@
 // The reference diagonalises every Pi(u_k) (rpa_CC_Diagonalize_PiU,
 // rpa/rpa_CC.f90:1108-1129) only to apply Q_k diag(l/(1+l W_k)) Q_k^T =
 // l (1 + l Pi_k)^{-1}; here that operator is applied through a Cholesky
-// factor of 1 + l Pi_k.  Pi_k comes from the FP64 MFMA SYRK of the E_RPA
+// factor 1 + l Pi_k = U_k^T U_k as Rt_k Rt_k^T with Rt_k = R U_k^{-1}, so the
+// frequency loops are plain GEMMs.  Pi_k comes from the FP64 MFMA SYRK of the E_RPA
 // path and every Cholesky factorisation (1 + l Pi_k, the Gram matrices of the
 // basis, 1 + T^T I T) runs on the library's own blocked kernels; the plain
 // GEMMs and triangular solves run on rocBLAS, and the inverse and the
@@ -207,14 +208,14 @@ int t2_build(brpa_ctx *ctx, int precision, const double *R, int64_t naux, const 
     // workspace layout
     const int64_t nsteps = cdiv(N, CHOL_NB);
     FragDev fd{};
-    double *dR = nullptr, *dchi, *omega, *tmp, *iom, *Y, *Sm, *sev, *sE;
+    double *dR = nullptr, *dchi, *omega, *tmp, *iom, *Y, *Sm, *sev, *sE, *Uinv, *Rt;
     int *dinfo;
     int64_t *dchol = nullptr, *dchol2 = nullptr;
     FragDev fm{};                  // one dense matrix for the library's Cholesky
     FragDev *dfm = nullptr;
-    // split-int8 operands, 5 slices each: chi_k R per column P over ai for every
-    // k (SCR), R per row ai over P (SRR), Omega and then T per column over ai
-    // (SOT), Y_k per column over P (SY)
+    // split-int8 operands, 5 slices each: chi_k Rt_k per column P over ai for
+    // every k (SCR), Rt_k per row ai over P (SRR), Omega and then T per column
+    // over ai (SOT), Y_k per column over P (SY)
     const int64_t Kp = split ? sosex_i8_vpad(K) : 0, Np = cdiv(N, 64) * 64;
     int8_t *SCR = nullptr, *SRR = nullptr, *SOT = nullptr, *SY = nullptr;
     double *scCR = nullptr, *scRR = nullptr, *scOT = nullptr, *scY = nullptr, *pmx = nullptr;
@@ -231,6 +232,8 @@ int t2_build(brpa_ctx *ctx, int precision, const double *R, int64_t naux, const 
         dR = rdev ? const_cast<double *>(R) : ar.take<double>(K * N);
         fd.dtab = ar.take<double>(F * K);
         fd.Pi = ar.take<double>(F * N * N);
+        Uinv = ar.take<double>(F * N * N);
+        Rt = ar.take<double>(F * K * N);
         fd.diag = ar.take<double>(F * N);
         fd.weights = ar.take<double>(F);
         fd.freqs = ar.take<double>(F);
@@ -313,6 +316,21 @@ int t2_build(brpa_ctx *ctx, int precision, const double *R, int64_t naux, const 
     // upper triangle U = L^T of the column-major view the solvers below read
     // (A = U^T U): hence rocblas_fill_upper there.
     BRPA_TRY(chol_single(ctx, dfd, dchol, N, F));
+    // Rt_k = R U_k^-1 (C Rt_k[ai][P], the layout of R), so that
+    // R (1 + l Pi_k)^-1 R^T = Rt_k Rt_k^T: no triangular solves in the
+    // frequency loops.  Column-major Rt_k^T = U_k^-T Rc with U_k^-1 from the
+    // batched triangular inverse, its lower triangle cleared for the GEMM.
+    BRPA_TRY(rb_check(ctx,
+                      rocblas_dtrtri_strided_batched(h, rocblas_fill_upper,
+                                                     rocblas_diagonal_non_unit, (rocblas_int)N,
+                                                     fd.Pi, (rocblas_int)N, N * N, Uinv,
+                                                     (rocblas_int)N, N * N, (rocblas_int)F),
+                      "rocblas_dtrtri_strided_batched"));
+    for (int64_t k = 0; k < F; ++k) {
+        BRPA_TRY(launch_zero_lower(ctx, Uinv + k * N * N, N));
+        BRPA_TRY(gemm(ctx, h, true, false, N, K, N, 1.0, Uinv + k * N * N, N, dR, N, 0.0,
+                      Rt + k * K * N, N));
+    }
     BRPA_TRY(stage_end(ctx, BRPA_STAGE_T2_PI));
 
     // Cholesky factor of one dense n x n matrix M in place with the same
@@ -338,38 +356,28 @@ int t2_build(brpa_ctx *ctx, int precision, const double *R, int64_t naux, const 
     BRPA_TRY(launch_guess(ctx, omega, K, NG, seed));
     BRPA_TRY(launch_normalize_columns(ctx, omega, K, NG));
     BRPA_HIP(ctx, hipMemsetAsync(iom, 0, 8 * K * NG, ctx->stream));
-    if (split) {
-        // frequency-independent operands, once per fragment
-        BRPA_TRY(launch_colslice_i8(ctx, dR, N, N, K, nullptr, Np, scRR, SRR));
-        BRPA_TRY(launch_colslice_i8(ctx, omega, K, K, NG, nullptr, Kp, scOT, SOT));
-    }
+    // frequency-independent operand of the split form, once per fragment
+    if (split) BRPA_TRY(launch_colslice_i8(ctx, omega, K, K, NG, nullptr, Kp, scOT, SOT));
     for (int64_t k = 0; k < F; ++k) {
         const double *ck = dchi + k * K;
         const double coef = -fw[k] * lambda / (4.0 * kPi);
+        const double *Rtk = Rt + k * K * N;
         if (split) {
-            // Y = (chi_k R)^T Omega, Y <- (1 + l Pi_k)^-1 Y, I Omega += coef chi_k R Y;
-            // the chi_k R slices are kept for T^T I T
+            // Y = (chi_k Rt_k)^T Omega, I Omega += coef chi_k Rt_k Y; the chi_k Rt_k
+            // slices are kept for T^T I T
             int8_t *SCRk = SCR + k * 5 * N * Kp;
             double *scCRk = scCR + k * N;
-            BRPA_TRY(launch_rowslice_i8(ctx, dR, K, N, ck, pmx, scCRk, SCRk));
+            BRPA_TRY(launch_rowslice_i8(ctx, Rtk, K, N, ck, pmx, scCRk, SCRk));
             BRPA_TRY(launch_gemm_i8(ctx, SCRk, scCRk, N, SOT, scOT, NG, Kp, 1.0, nullptr, 0.0, Y, 1,
                                     N));
-            BRPA_TRY(rb_check(ctx,
-                              rocsolver_dpotrs(h, rocblas_fill_upper, (rocblas_int)N,
-                                               (rocblas_int)NG, fd.Pi + k * N * N, (rocblas_int)N,
-                                               Y, (rocblas_int)N),
-                              "rocsolver_dpotrs"));
+            BRPA_TRY(launch_colslice_i8(ctx, Rtk, N, N, K, nullptr, Np, scRR, SRR));
             BRPA_TRY(launch_colslice_i8(ctx, Y, N, N, NG, nullptr, Np, scY, SY));
             BRPA_TRY(launch_gemm_i8(ctx, SRR, scRR, K, SY, scY, NG, Np, coef, ck, 1.0, iom, 1, K));
             continue;
         }
         BRPA_TRY(launch_row_scale(ctx, tmp, omega, ck, K, NG));
-        BRPA_TRY(gemm(ctx, h, false, false, N, NG, K, 1.0, dR, N, tmp, K, 0.0, Y, N));
-        BRPA_TRY(rb_check(ctx,
-                          rocsolver_dpotrs(h, rocblas_fill_upper, (rocblas_int)N, (rocblas_int)NG,
-                                           fd.Pi + k * N * N, (rocblas_int)N, Y, (rocblas_int)N),
-                          "rocsolver_dpotrs"));
-        BRPA_TRY(gemm(ctx, h, true, false, K, NG, N, 1.0, dR, N, Y, N, 0.0, tmp, K));
+        BRPA_TRY(gemm(ctx, h, false, false, N, NG, K, 1.0, Rtk, N, tmp, K, 0.0, Y, N));
+        BRPA_TRY(gemm(ctx, h, true, false, K, NG, N, 1.0, Rtk, N, Y, N, 0.0, tmp, K));
         BRPA_TRY(launch_row_scale_acc(ctx, iom, tmp, ck, coef, K, NG));
     }
     BRPA_TRY(launch_normalize_columns(ctx, iom, K, NG));
@@ -459,21 +467,14 @@ int t2_build(brpa_ctx *ctx, int precision, const double *R, int64_t naux, const 
     for (int64_t k = 0; k < F; ++k) {
         const double *ck = dchi + k * K;
         if (split) {
-            // G = (chi_k R)^T T
+            // G = (chi_k Rt_k)^T T
             BRPA_TRY(launch_gemm_i8(ctx, SCR + k * 5 * N * Kp, scCR + k * N, N, SOT, scOT, nt, Kp,
                                     1.0, nullptr, 0.0, G, 1, N));
         } else {
             BRPA_TRY(launch_row_scale(ctx, T, Tkeep, ck, K, nt));
-            BRPA_TRY(gemm(ctx, h, false, false, N, nt, K, 1.0, dR, N, T, K, 0.0, G, N));
+            BRPA_TRY(gemm(ctx, h, false, false, N, nt, K, 1.0, Rt + k * K * N, N, T, K, 0.0, G, N));
         }
-        // G <- U_k^{-T} G with 1 + l Pi_k = U_k^T U_k, so that
-        // G^T (1 + l Pi_k)^{-1} G = G^T G
-        BRPA_TRY(rb_check(ctx,
-                          rocblas_dtrsm(h, rocblas_side_left, rocblas_fill_upper,
-                                        rocblas_operation_transpose, rocblas_diagonal_non_unit,
-                                        (rocblas_int)N, (rocblas_int)nt, &one, fd.Pi + k * N * N,
-                                        (rocblas_int)N, G, (rocblas_int)N),
-                          "rocblas_dtrsm"));
+        // G = Rt_k^T chi_k T = U_k^-T R^T chi_k T, so G^T G = T^T chi R (1 + l Pi_k)^-1 R^T chi T
         BRPA_TRY(gemm(ctx, h, true, false, nt, nt, N, -fw[k] * lambda / (4.0 * kPi), G, N, G, N,
                       k == 0 ? 0.0 : 1.0, TIT, nt));
     }

@@ -94,7 +94,8 @@ enum {
                               exact diagonal and slices) */
     BRPA_STAGE_CHOL = 2,   /* blocked Cholesky log-det (panel + trailing updates) */
     BRPA_STAGE_AUX = 3,    /* Delta table, exact diagonal, quadrature */
-    BRPA_STAGE_T2_PI = 4,     /* amplitudes: Pi(u_k) and the Cholesky factors of 1 + l Pi */
+    BRPA_STAGE_T2_PI = 4,     /* amplitudes: Pi(u_k), the Cholesky factors of 1 + l Pi and
+                                 R U_k^-1 */
     BRPA_STAGE_T2_IOMEGA = 5, /* amplitudes: I * Omega over all frequencies */
     BRPA_STAGE_T2_BASIS = 6,  /* amplitudes: orthonormal basis T (Cholesky-QR or eigensolve) */
     BRPA_STAGE_T2_TIT = 7,    /* amplitudes: T^T I T over all frequencies */
