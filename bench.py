@@ -12,7 +12,7 @@ frequency grids (host code, independent of R) are built once before timing.
 
 Exchange correction: the factorised direct-ring amplitudes (T2) and the SOSEX
 contraction are timed on the first --exchange-frags fragments of the same
-batch (brpa_exchange_batch, one fragment after another), once with the
+batch (brpa_exchange_batch, two fragments in flight on worker threads), once with the
 split-int8 SOSEX contraction and once in FP64; each form's E_RPA + exchange
 rate is 1 / (its E_RPA time per fragment + the exchange time per fragment in
 its own arithmetic).
@@ -316,8 +316,8 @@ def main():
                     help="replay the batched launch sequence as a hipGraph (1) or launch eagerly")
     ap.add_argument("--exchange-frags", type=int, default=64,
                     help="fragments in the T2 + SOSEX sample (0 skips the exchange part)")
-    ap.add_argument("--exchange-workers", type=int, default=1,
-                    help="worker count passed to brpa_exchange_batch (only 1 is supported)")
+    ap.add_argument("--exchange-workers", type=int, default=2,
+                    help="worker threads of brpa_exchange_batch (fragments in flight at once)")
     ap.add_argument("--workload", default="batch",
                     choices=["batch", "single", "adjoint", "sosex"],
                     help="batch: the 256-fragment E_RPA + exchange batch (default); single: "

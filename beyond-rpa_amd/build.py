@@ -30,7 +30,7 @@ CXXFLAGS = [
     "-I", os.path.join(ROOT, "include"), "-I", CSRC,
     "-Wno-unused-result",
 ]
-LDFLAGS = ["-shared", "-fPIC", "--offload-arch=gfx950", "-L", os.path.join(ROCM, "lib"),
+LDFLAGS = ["-shared", "-fPIC", "-pthread", "--offload-arch=gfx950", "-L", os.path.join(ROCM, "lib"),
            "-Wl,-rpath," + os.path.join(ROCM, "lib"), "-lrocsolver", "-lrocblas"]
 
 

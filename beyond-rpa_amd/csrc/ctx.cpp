@@ -29,9 +29,12 @@ bool is_device_ptr(const void *p) {
 
 int to_device(brpa_ctx *ctx, void *dst, const void *src, size_t bytes) {
     if (bytes == 0) return BRPA_OK;
-    hipMemcpyKind kind = is_device_ptr(src) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
-    BRPA_HIP(ctx, hipMemcpyAsync(dst, src, bytes, kind, ctx->stream));
-    return BRPA_OK;
+    if (is_device_ptr(src)) {
+        BRPA_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+        return BRPA_OK;
+    }
+    // a pageable host source is copied to completion before returning
+    return copy_sync(ctx, dst, src, bytes, hipMemcpyHostToDevice);
 }
 
 int from_device(brpa_ctx *ctx, void *dst, const void *src, size_t bytes) {

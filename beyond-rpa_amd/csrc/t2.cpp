@@ -284,18 +284,19 @@ int t2_build(brpa_ctx *ctx, int precision, const double *R, int64_t naux, const 
     fd.keep_l11 = 1;
     if (!rdev) BRPA_TRY(to_device(ctx, dR, R, 8 * K * N));
     BRPA_TRY(launch_fill_int(ctx, fd.status, F, 0));
-    BRPA_HIP(ctx, hipMemcpyAsync(const_cast<double *>(fd.weights), fw.data(), 8 * F,
-                                 hipMemcpyHostToDevice, ctx->stream));
-    BRPA_HIP(ctx, hipMemcpyAsync(const_cast<double *>(fd.freqs), fx.data(), 8 * F,
-                                 hipMemcpyHostToDevice, ctx->stream));
-    BRPA_HIP(ctx, hipMemcpyAsync(const_cast<double *>(fd.eocc), eo.data(), 8 * nocc,
-                                 hipMemcpyHostToDevice, ctx->stream));
-    BRPA_HIP(ctx, hipMemcpyAsync(const_cast<double *>(fd.evir), ev.data(), 8 * nvir,
-                                 hipMemcpyHostToDevice, ctx->stream));
-    BRPA_HIP(ctx, hipMemcpyAsync(dchi, chi.data(), 8 * F * K, hipMemcpyHostToDevice, ctx->stream));
-    BRPA_HIP(ctx, hipMemcpyAsync(dfd, &fd, sizeof(FragDev), hipMemcpyHostToDevice, ctx->stream));
-    BRPA_HIP(ctx, hipMemcpyAsync(dpre, pre.data(), 8 * pre.size(), hipMemcpyHostToDevice,
-                                 ctx->stream));
+    // host inputs go up with completed (stream-ordered, synchronised) copies:
+    // nothing of a pageable host buffer is read after this call returns
+    BRPA_TRY(copy_sync(ctx, const_cast<double *>(fd.weights), fw.data(), 8 * F,
+                       hipMemcpyHostToDevice));
+    BRPA_TRY(copy_sync(ctx, const_cast<double *>(fd.freqs), fx.data(), 8 * F,
+                       hipMemcpyHostToDevice));
+    BRPA_TRY(copy_sync(ctx, const_cast<double *>(fd.eocc), eo.data(), 8 * nocc,
+                       hipMemcpyHostToDevice));
+    BRPA_TRY(copy_sync(ctx, const_cast<double *>(fd.evir), ev.data(), 8 * nvir,
+                       hipMemcpyHostToDevice));
+    BRPA_TRY(copy_sync(ctx, dchi, chi.data(), 8 * F * K, hipMemcpyHostToDevice));
+    BRPA_TRY(copy_sync(ctx, dfd, &fd, sizeof(FragDev), hipMemcpyHostToDevice));
+    BRPA_TRY(copy_sync(ctx, dpre, pre.data(), 8 * pre.size(), hipMemcpyHostToDevice));
     BRPA_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
     if (!ctx->stage_keep) stage_reset(ctx);
     // stages whose events were recorded in this call (collected at the end)
@@ -626,7 +627,7 @@ int brpa_sosex(brpa_ctx *ctx, int precision, double *ec_exchange, double alpha, 
         layout(ar);
         if (!rdev) BRPA_TRY(to_device(ctx, dR, R, 8 * K * N));
         if (!vdev) BRPA_TRY(to_device(ctx, dV, V, 8 * K * nt));
-        BRPA_HIP(ctx, hipMemcpyAsync(dB, hB.data(), 8 * nt, hipMemcpyHostToDevice, ctx->stream));
+        BRPA_TRY(copy_sync(ctx, dB, hB.data(), 8 * nt, hipMemcpyHostToDevice));
         BRPA_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
         BRPA_TRY(stage_begin(ctx, BRPA_STAGE_SOSEX));
         BRPA_TRY(launch_sosex_i8_split(ctx, dR, o, v, N, scR, SR));
@@ -664,7 +665,7 @@ int brpa_sosex(brpa_ctx *ctx, int precision, double *ec_exchange, double alpha, 
     layout(ar);
     if (!rdev) BRPA_TRY(to_device(ctx, dR, R, 8 * K * N));
     if (!vdev) BRPA_TRY(to_device(ctx, dV, V, 8 * K * nt));
-    BRPA_HIP(ctx, hipMemcpyAsync(dB, hB.data(), 8 * nt, hipMemcpyHostToDevice, ctx->stream));
+    BRPA_TRY(copy_sync(ctx, dB, hB.data(), 8 * nt, hipMemcpyHostToDevice));
     BRPA_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
     BRPA_TRY(stage_begin(ctx, BRPA_STAGE_SOSEX));
     const double one = 1.0, zero = 0.0;

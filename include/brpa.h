@@ -223,9 +223,12 @@ int brpa_sosex(brpa_ctx *ctx, int precision, double *ec_exchange, double alpha, 
  * rpa_CC.f90:884-892 then 963-976 sequence, subsystem after subsystem).
  * Fragment f uses GuessNVecsT2 = ceil(guess_factor * naux) (the reference's
  * default factor is 3.0, rpa_definitions.f90:428) and guess seed seed + f.
- * The fragments run one after another on a worker context; nworkers must be 0
- * or 1 (several in flight on host threads gave wrong rocBLAS/rocSOLVER results
- * and is disabled).  ec_exchange[f] receives E_x; nvecs_t2 (may be NULL) the
+ * nworkers (1..8, 0 = 1) worker contexts, each with its own stream, rocBLAS
+ * handle and workspace and driven by its own host thread, take the fragments
+ * one at a time, so that the latency-bound stages of different fragments
+ * overlap; the results do not depend on nworkers.  Per-stage device times
+ * (summed over the workers) are left in the BRPA_STAGE_T2_* / BRPA_STAGE_SOSEX
+ * slots.  ec_exchange[f] receives E_x; nvecs_t2 (may be NULL) the
  * retained NT2.  precision: arithmetic of the amplitude contractions and of
  * the SOSEX contraction (BRPA_FP64 or BRPA_SPLIT_I8, as in brpa_t2_compute
  * and brpa_sosex). */

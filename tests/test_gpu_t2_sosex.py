@@ -100,7 +100,8 @@ def test_exchange_batch_matches_sequential(ctx):
     """brpa_exchange_batch (energy only: factored amplitude core, no eigensolve
     of T^T I T) agrees with T2 + SOSEX called one fragment at a time and with
     the oracle within 1e-8 Ha; with the S2 rescaling (eigensolve route) it is
-    the same bits as the sequential calls; more than one worker is rejected."""
+    the same bits as the sequential calls; three worker threads give the same
+    results; more than eight workers are rejected."""
     import math
     from oracle import erpa as oe, sosex as osx, t2 as ot
     shapes = [(3, 12, 40), (4, 20, 90), (2, 15, 50), (5, 18, 100), (3, 25, 70)]
@@ -127,10 +128,43 @@ def test_exchange_batch_matches_sequential(ctx):
     exi, _ = ctx.exchange_batch(frags, seed=77, precision="split_i8")
     print("split_i8 - oracle", exi - np.array(refs))
     assert np.max(np.abs(exi - np.array(refs))) <= 1e-7
+    # several fragments in flight on worker threads: the same results
+    ex3, nt3 = ctx.exchange_batch(frags, seed=77, nworkers=3)
+    assert np.array_equal(nt3, nt)
+    assert np.max(np.abs(ex3 - ex)) <= 1e-10, ex3 - ex
     import brpa
     with pytest.raises(brpa.BrpaError) as ei:
-        ctx.exchange_batch(frags[:2], seed=77, nworkers=3)
+        ctx.exchange_batch(frags[:2], seed=77, nworkers=9)
     assert ei.value.code == brpa._lib.ERR_ARG
+
+
+@pytest.mark.parametrize("precision", ["fp64", "split_i8"])
+def test_exchange_batch_workers(ctx, precision):
+    """12 uneven fragments (both basis routes) on 1 and 4 worker threads: the same
+    NT2 and E_x to 1e-10 Ha, and both within the form's bar of the oracle on a
+    sample."""
+    shapes = [(3, 12, 40), (8, 40, 180), (2, 15, 50), (10, 30, 160), (6, 22, 120), (4, 60, 200),
+              (12, 25, 150), (5, 18, 100), (7, 45, 210), (3, 25, 70), (9, 35, 190), (6, 50, 230)]
+    frags = []
+    for i, (o, v, n) in enumerate(shapes):
+        f = synth.make_fragment(o, v, n, 700 + i)
+        x, w, _ = grid.freq_grid(f.eocc, f.evir, 10)
+        frags.append((f.R, f.eocc, f.evir, x, w))
+    e1, n1 = ctx.exchange_batch(frags, seed=5, precision=precision)
+    e4, n4 = ctx.exchange_batch(frags, seed=5, precision=precision, nworkers=4)
+    print("4 workers - 1 worker", e4 - e1)
+    assert np.array_equal(n1, n4)
+    assert np.max(np.abs(e4 - e1)) <= 1e-10
+    import math
+    from oracle import erpa as oe, sosex as osx, t2 as ot
+    for i in (0, 3, 6):
+        R, eo, ev, x, w = frags[i]
+        o, v, n = shapes[i]
+        g = (ev[None, :] - eo[:, None]).ravel()
+        Pis = [oe.gpiug_mo(R, g, xx) for xx in x]
+        A, V = ot.t2_v5(R, g, x, w, Pis, nguess=math.ceil(3.0 * n), seed=5 + i)
+        ref = osx.sosex(R, V, A, o, v)
+        assert abs(e4[i] - ref) <= (1e-8 if precision == "fp64" else 1e-7), (i, e4[i] - ref)
 
 
 @pytest.mark.parametrize("o,v,n", [(3, 12, 40), (1, 5, 130), (4, 70, 200), (6, 33, 97)])
