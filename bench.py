@@ -17,8 +17,10 @@ split-int8 SOSEX contraction and once in FP64; each form's E_RPA + exchange
 rate is 1 / (its E_RPA time per fragment + the exchange time per fragment in
 its own arithmetic).
 
-Prints one JSON line.  `roofline` is measured for the dominant kernel with
-HIP events inside the library over the timed steps; `cpu_baseline` times the
+Prints one JSON line.  `roofline` is measured for the dominant kernel of the job,
+the split-int8 GEMM of the amplitude contractions (k_gemm_i8), with HIP events
+inside the library around every launch of a one-worker exchange pass;
+`roofline_erpa` is the E_RPA product kernel's over the timed steps; `cpu_baseline` times the
 CPU oracle (numpy float64 + threaded OpenBLAS) on a bounded sample of the
 same fragments and checks the GPU energies against it.
 """
@@ -426,7 +428,7 @@ def main():
     # the committed PMC runs cover the default batch (256 fragments, 16 frequencies)
     traffic = (pmc_traffic(kernel + "<128>")
                if kernel == "k_syrk_i8" and a.nfrag == 256 and a.nfreqs == 16 else None)
-    roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": round(peak, 3),
+    roofline_erpa = {"bound": "mfma", "achieved": round(achieved, 3), "peak": round(peak, 3),
                 "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": traffic,
                 "traffic_source": ("profiles/erpa_batch256_split_i8_kernels.md (FETCH_SIZE x 2.0 "
                                    "+ WRITE_SIZE x 1.0, copy-calibrated)") if traffic else None,
@@ -434,6 +436,7 @@ def main():
                 "launches_per_step": stage_n[dom]}
 
     exch = None
+    roofline_exch = None
     if a.exchange_frags > 0:
         # T2 + SOSEX of the first fragments of the batch, timed once per SOSEX
         # arithmetic: each form's exchange runs in that form's own arithmetic
@@ -469,6 +472,27 @@ def main():
                 np.max(np.abs(ex_val["split_i8"] - ex_val["fp64"])))
         for p, d in forms.items():
             d["with_exchange_fps"] = 1.0 / (d["ms_per_step"] * 1e-3 / a.nfrag + t_ex[xprec_of[p]])
+        if a.precision == "split_i8":
+            # the dominant kernel of the E_RPA + exchange job is the split-int8 GEMM of
+            # the amplitude contractions: timed in the library (HIP events around every
+            # launch, on the worker's stream) over a one-worker pass of the first 16
+            # fragments, so that its launches do not overlap with another fragment's
+            kr = min(16, kx)
+            ctx.exchange_batch(sub[:kr], seed=synth.BASE_SEED, nworkers=1, precision="split_i8")
+            gms, gfl, gn = ctx.last_gemm_i8_stats()
+            peak_i8 = PEAK_I8_TOPS / 15
+            ach = gfl / (gms * 1e-3) / 1e12
+            roofline_exch = {
+                "bound": "mfma", "achieved": round(ach, 3), "peak": round(peak_i8, 3),
+                "unit": "TFLOP/s", "frac": round(ach / peak_i8, 4), "traffic": None,
+                "traffic_note": "no PMC counts: the FETCH_SIZE run of the exchange path "
+                                "crashed inside rocSOLVER dsyevd under the profiler",
+                "kernel": "k_gemm_i8 (5-slice split-int8 GEMM, 15 products; peak = int8 "
+                          "MFMA / 15, FP64-equivalent)",
+                "kernel_ms_per_launch": round(gms / max(gn, 1), 4), "launches": gn,
+                "sample": "one-worker exchange pass over the first %d fragments" % kr}
+            print("# k_gemm_i8: %d launches, %.3f ms each, %.1f TF/s FP64-equivalent" %
+                  (gn, gms / max(gn, 1), ach), file=sys.stderr, flush=True)
 
     cpu = None
     parity = None
@@ -523,7 +547,8 @@ def main():
                       "ms_per_step": round(d["ms_per_step"], 3),
                       "stage_ms": {k: round(v, 3) for k, v in d["stage_ms"].items()}}
                   for p, d in forms.items()},
-        "roofline": roofline,
+        "roofline": roofline_exch if roofline_exch else roofline_erpa,
+        "roofline_erpa": roofline_erpa,
         "cpu_baseline": cpu,
         "parity": parity,
     }

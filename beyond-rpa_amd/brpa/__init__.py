@@ -154,6 +154,14 @@ class Context:
         return {k: (self._lib.brpa_last_stage_ms(self._h, v),
                     self._lib.brpa_last_stage_launches(self._h, v)) for k, v in names.items()}
 
+    def last_gemm_i8_stats(self):
+        """(ms, FP64-equivalent flop, launches) of the split-int8 GEMM launches of
+        the last amplitude or exchange-batch call (HIP events around each launch)."""
+        ms, fl, n = C.c_double(0.0), C.c_double(0.0), C.c_int64(0)
+        self._check(self._lib.brpa_last_gemm_i8_stats(self._h, C.byref(ms), C.byref(fl),
+                                                      C.byref(n)))
+        return ms.value, fl.value, n.value
+
     def trim(self):
         self._check(self._lib.brpa_ctx_trim(self._h))
 

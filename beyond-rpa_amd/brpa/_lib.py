@@ -33,6 +33,7 @@ SIGNATURES = {
     "brpa_last_kernel_ms": (C.c_double, [_vp]),
     "brpa_last_stage_ms": (C.c_double, [_vp, C.c_int]),
     "brpa_last_stage_launches": (C.c_int, [_vp, C.c_int]),
+    "brpa_last_gemm_i8_stats": (C.c_int, [_vp, _vp, _vp, _vp]),
     "brpa_freq_grid": (C.c_int, [_vp, _i64, _vp, _i64, _i64, _vp, _vp, _vp]),
     "brpa_optimize_freq_grid": (C.c_int, [_vp, _i64, _vp, _i64, C.c_double, C.c_double, _i64,
                                           _i64, _vp, _vp, C.POINTER(_i64), _vp]),

@@ -126,6 +126,8 @@ void stage_reset(brpa_ctx *ctx) {
         ctx->stage_ms[s] = 0.0;
         ctx->stage_launches[s] = 0;
     }
+    ctx->gemm_i8_ms = ctx->gemm_i8_flops = 0.0;
+    ctx->gemm_i8_launches = 0;
 }
 
 int stage_begin(brpa_ctx *ctx, int stage) {
@@ -189,6 +191,7 @@ int brpa_ctx_destroy(brpa_ctx *ctx) {
     if (ctx->pin) (void)hipHostFree(ctx->pin);
     if (ctx->sync) (void)hipFree(ctx->sync);
     t2_release(ctx);
+    for (hipEvent_t e : ctx->gev) (void)hipEventDestroy(e);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     for (int i = 0; i < 2 * BRPA_NSTAGES; ++i)
@@ -236,6 +239,14 @@ int brpa_ctx_trim(brpa_ctx *ctx) {
 }
 
 double brpa_last_kernel_ms(const brpa_ctx *ctx) { return ctx ? ctx->last_kernel_ms : 0.0; }
+
+int brpa_last_gemm_i8_stats(const brpa_ctx *ctx, double *ms, double *flops, int64_t *launches) {
+    if (!ctx) return BRPA_ERR_ARG;
+    if (ms) *ms = ctx->gemm_i8_ms;
+    if (flops) *flops = ctx->gemm_i8_flops;
+    if (launches) *launches = ctx->gemm_i8_launches;
+    return BRPA_OK;
+}
 
 double brpa_last_stage_ms(const brpa_ctx *ctx, int stage) {
     return (ctx && stage >= 0 && stage < BRPA_NSTAGES) ? ctx->stage_ms[stage] : 0.0;

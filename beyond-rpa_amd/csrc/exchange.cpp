@@ -106,6 +106,13 @@ int brpa_exchange_batch(brpa_ctx *ctx, int precision, int64_t nfrag, const brpa_
     if (first_rc != BRPA_OK)
         return brpa_fail(ctx, first_rc, "brpa_exchange_batch: fragment %lld: %s",
                          (long long)first_f, first_msg.c_str());
+    ctx->gemm_i8_ms = ctx->gemm_i8_flops = 0.0;
+    ctx->gemm_i8_launches = 0;
+    for (int i = 0; i < nworkers; ++i) {
+        ctx->gemm_i8_ms += ctx->workers[i]->gemm_i8_ms;
+        ctx->gemm_i8_flops += ctx->workers[i]->gemm_i8_flops;
+        ctx->gemm_i8_launches += ctx->workers[i]->gemm_i8_launches;
+    }
     for (int st = 0; st < BRPA_NSTAGES; ++st) {
         ctx->stage_ms[st] = 0.0;
         ctx->stage_launches[st] = 0;

@@ -26,6 +26,14 @@ struct brpa_ctx {
     // set on an exchange-batch worker: the amplitude and SOSEX calls add to the
     // stage times instead of starting them afresh
     bool stage_keep = false;
+    // device time (HIP events around each launch) and algorithmic FP64-
+    // equivalent flop of the split-int8 GEMM launches since the last
+    // brpa_gemm_i8_stats_reset; gev holds event pairs not yet collected
+    std::vector<hipEvent_t> gev;
+    std::vector<double> gev_flops;
+    size_t gev_used = 0;
+    double gemm_i8_ms = 0.0, gemm_i8_flops = 0.0;
+    int64_t gemm_i8_launches = 0;
     // amplitude (T2) results kept on the device until brpa_t2_copyout
     void *rocblas = nullptr;   // rocblas_handle, created on first use
     double *t2_A = nullptr, *t2_V = nullptr;

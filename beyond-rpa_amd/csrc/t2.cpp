@@ -49,9 +49,12 @@ int launch_colslice_i8(brpa_ctx *, const double *X, int64_t ld, int64_t len, int
 int64_t rowslice_i8_pm_len(int64_t v, int64_t ncols);
 int launch_rowslice_i8(brpa_ctx *, const double *X, int64_t v, int64_t ncols, const double *rs,
                        double *pm, double *scale, int8_t *out);
+// kreal: the contraction length (flop accounting); every launch is timed with
+// an event pair, gemm_i8_collect adds the times once the stream is idle
 int launch_gemm_i8(brpa_ctx *, const int8_t *SA, const double *sa, int64_t M, const int8_t *SB,
                    const double *sb, int64_t N, int64_t kpad, double alpha, const double *rs,
-                   double beta, double *out, int64_t sm, int64_t sn);
+                   double beta, double *out, int64_t sm, int64_t sn, int64_t kreal);
+int gemm_i8_collect(brpa_ctx *);
 int64_t sosex_i8_nparts(int64_t o, int64_t N, int64_t NT);
 int launch_sosex_i8_pairs(brpa_ctx *, const int8_t *SR, const int8_t *SV, const double *scR,
                           const double *scV, const double *B, int64_t o, int64_t v, int64_t N,
@@ -299,6 +302,8 @@ int t2_build(brpa_ctx *ctx, int precision, const double *R, int64_t naux, const 
     BRPA_TRY(copy_sync(ctx, dpre, pre.data(), 8 * pre.size(), hipMemcpyHostToDevice));
     BRPA_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
     if (!ctx->stage_keep) stage_reset(ctx);
+    ctx->gev_used = 0;   // event pairs left by a call that failed are dropped
+    ctx->gev_flops.clear();
     // stages whose events were recorded in this call (collected at the end)
     bool staged[BRPA_NSTAGES] = {};
     auto begin = [&](int st) -> int {
@@ -370,10 +375,11 @@ int t2_build(brpa_ctx *ctx, int precision, const double *R, int64_t naux, const 
             double *scCRk = scCR + k * N;
             BRPA_TRY(launch_rowslice_i8(ctx, Rtk, K, N, ck, pmx, scCRk, SCRk));
             BRPA_TRY(launch_gemm_i8(ctx, SCRk, scCRk, N, SOT, scOT, NG, Kp, 1.0, nullptr, 0.0, Y, 1,
-                                    N));
+                                    N, K));
             BRPA_TRY(launch_colslice_i8(ctx, Rtk, N, N, K, nullptr, Np, scRR, SRR));
             BRPA_TRY(launch_colslice_i8(ctx, Y, N, N, NG, nullptr, Np, scY, SY));
-            BRPA_TRY(launch_gemm_i8(ctx, SRR, scRR, K, SY, scY, NG, Np, coef, ck, 1.0, iom, 1, K));
+            BRPA_TRY(launch_gemm_i8(ctx, SRR, scRR, K, SY, scY, NG, Np, coef, ck, 1.0, iom, 1, K,
+                                    N));
             continue;
         }
         BRPA_TRY(launch_row_scale(ctx, tmp, omega, ck, K, NG));
@@ -445,7 +451,7 @@ int t2_build(brpa_ctx *ctx, int precision, const double *R, int64_t naux, const 
         BRPA_HIP(ctx, hipStreamSynchronize(ctx->stream));
         for (int st = 0; st < BRPA_NSTAGES; ++st)
             if (staged[st]) BRPA_TRY(stage_collect(ctx, st));
-        return BRPA_OK;
+        return gemm_i8_collect(ctx);
     };
     if (nt == 0) {
         BRPA_TRY(stage_end(ctx, BRPA_STAGE_T2_BASIS));
@@ -470,7 +476,7 @@ int t2_build(brpa_ctx *ctx, int precision, const double *R, int64_t naux, const 
         if (split) {
             // G = (chi_k Rt_k)^T T
             BRPA_TRY(launch_gemm_i8(ctx, SCR + k * 5 * N * Kp, scCR + k * N, N, SOT, scOT, nt, Kp,
-                                    1.0, nullptr, 0.0, G, 1, N));
+                                    1.0, nullptr, 0.0, G, 1, N, K));
         } else {
             BRPA_TRY(launch_row_scale(ctx, T, Tkeep, ck, K, nt));
             BRPA_TRY(gemm(ctx, h, false, false, N, nt, K, 1.0, Rt + k * K * N, N, T, K, 0.0, G, N));

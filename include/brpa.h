@@ -105,6 +105,11 @@ enum {
 };
 double brpa_last_stage_ms(const brpa_ctx *ctx, int stage);
 int brpa_last_stage_launches(const brpa_ctx *ctx, int stage);
+/* Split-int8 GEMM launches (the amplitude contractions of the split form) of the
+ * last amplitude or exchange-batch call: summed device time (ms, HIP events
+ * around each launch), algorithmic FP64-equivalent flop (2 m n k per launch)
+ * and the launch count.  No reference counterpart (measurement). */
+int brpa_last_gemm_i8_stats(const brpa_ctx *ctx, double *ms, double *flops, int64_t *launches);
 
 /* ---- frequency grid (host code) ----------------------------------------- */
 /* Fixed-size grid of one fragment: 100-bin gap histogram, 15000-point scan of
