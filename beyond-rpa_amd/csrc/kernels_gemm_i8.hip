@@ -17,8 +17,6 @@
 //                  (global_load_lds_dwordx4, double-buffered, XOR-swizzled)
 // Vectors stored across rows of a row-major matrix use the SOSEX block slicer
 // (k_blkscale_i8 / k_blksplit_i8 with one block).
-#include <cstdlib>
-
 #include "device_common.h"
 #include "internal.h"
 
@@ -34,8 +32,7 @@ typedef __attribute__((address_space(1))) const double gdbl_t;
 typedef int v4i_t __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(3))) void lds_void_t;
 
-constexpr int GM_NSL = 5;     // slices per value
-constexpr int GM_GROUP = 4;   // tile rows per group in the XCD-grouped order
+constexpr int GM_NSL = 5;   // slices per value
 
 // ---------------------------------------------------------------------------
 // One work-group per vector c: x_k = X[c*ld + k] (times rs[k] when rs is not
@@ -117,29 +114,15 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_i8(
     const int8_t *__restrict__ SA, const double *__restrict__ sa, int64_t M,
     const int8_t *__restrict__ SB, const double *__restrict__ sb, int64_t N, int kpad, int ntn,
     double alpha, const double *__restrict__ rs, double beta, double *__restrict__ out, int64_t sm,
-    int64_t sn, int order) {
+    int64_t sn) {
     constexpr int TM = 16 * RI * WM, TN = 16 * RJ * WN, NWV = WM * WN;
     constexpr int LOADS = GM_NSL * (TM + TN) / 16;
     constexpr int LPW = (LOADS + NWV - 1) / NWV;   // wave loads per wave and stage
     // [buffer][A vectors (slice-major) then B vectors][slot]
     __shared__ __attribute__((aligned(16))) uint4 sbuf[2][GM_NSL * (TM + TN)][4];
-    // order 0: tiles row by row; order 1: each XCD takes a contiguous range of
-    // tiles (xcd_remap), walked in groups of GM_GROUP tile rows column by column,
-    // so that the tiles in flight on one XCD share A and B panels in its L2
-    int tm, tn;
-    if (order == 0) {
-        const int64_t bid = blockIdx.x;
-        tm = (int)(bid / ntn);
-        tn = (int)(bid - (int64_t)tm * ntn);
-    } else {
-        const int64_t ntile = gridDim.x, ntm = ntile / ntn;
-        const int64_t bid = xcd_remap(blockIdx.x, ntile);
-        const int64_t gsz = (int64_t)GM_GROUP * ntn;
-        const int64_t g = bid / gsz, in = bid - g * gsz;
-        const int64_t rows = ntm - g * GM_GROUP < GM_GROUP ? ntm - g * GM_GROUP : GM_GROUP;
-        tm = (int)(g * GM_GROUP + in % rows);
-        tn = (int)(in / rows);
-    }
+    // tiles row by row (an XCD-grouped order measured the same)
+    const int64_t bid = blockIdx.x;
+    const int tm = (int)(bid / ntn), tn = (int)(bid - (int64_t)(bid / ntn) * ntn);
     const int64_t M0 = (int64_t)tm * TM, N0 = (int64_t)tn * TN;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int wi = wave / WN, wj = wave % WN;   // (16 RI)-row block, (16 RJ)-col block
@@ -266,10 +249,6 @@ int launch_gemm_i8(brpa_ctx *ctx, const int8_t *SA, const double *sa, int64_t M,
     // 128 x 128 tiles (8 waves of 64 x 32, the whole 160 KB of LDS double-
     // buffered) when both sides fill them, else 128 x 64 (8 waves of 32 x 32)
     const bool big = N >= 128 && M >= 128;
-    static const int order = [] {
-        const char *e = std::getenv("BRPA_GEMM_I8_ORDER");
-        return e ? std::atoi(e) : 0;
-    }();
     const int64_t TM = 128, TN = big ? 128 : 64;
     const int64_t ntm = (M + TM - 1) / TM, ntn = (N + TN - 1) / TN;
     if (ntm * ntn > INT32_MAX || ntn > INT32_MAX)
@@ -277,11 +256,11 @@ int launch_gemm_i8(brpa_ctx *ctx, const int8_t *SA, const double *sa, int64_t M,
     if (big)
         hipLaunchKernelGGL((k_gemm_i8<2, 4, 4, 2>), dim3((unsigned)(ntm * ntn)), dim3(512), 0,
                            ctx->stream, SA, sa, M, SB, sb, N, (int)kpad, (int)ntn, alpha, rs, beta,
-                           out, sm, sn, order);
+                           out, sm, sn);
     else
         hipLaunchKernelGGL((k_gemm_i8<4, 2, 2, 2>), dim3((unsigned)(ntm * ntn)), dim3(512), 0,
                            ctx->stream, SA, sa, M, SB, sb, N, (int)kpad, (int)ntn, alpha, rs, beta,
-                           out, sm, sn, order);
+                           out, sm, sn);
     LAUNCH_CHECK(ctx);
     BRPA_HIP(ctx, hipEventRecord(e1, ctx->stream));
     ctx->gev_used += 2;
