@@ -321,6 +321,48 @@ class Context:
             self._check(self._lib.brpa_t2_copyout(self._h, _ptr(A), _ptr(V)))
         return A, V
 
+    _CUTOFF_TYPES = {"eig": 0, "eig/maxeig": 1, "eig/nelectron": 2, "eig/nocc": 2,
+                     "sum-rejected": 3, "sum_rejected": 3}
+
+    def thc_t2(self, R, eocc, evir, freqs, weights, lam=1.0, cutoff=1e-10, nguess=0, seed=0,
+               t2_cutoff_thresh=1e-4, t2_cutoff_type="eig", smooth_step=False, steepness=0.1,
+               common_thresh=-1.0, precision="fp64"):
+        """Amplitudes of the THC route (rpa_THC_CC_T2, rpa_CC_Doubles.f90:645-984):
+        brpa_t2_compute, the small-eigenvalue cutoff, then EigRPA.
+
+        Returns (A, V, EigRPA, common_out, n_scaled) with V in the route's
+        layout V(ov, NT2), i.e. a C-order (NT2, ov) array V[mu][ai].
+        common_thresh < 0: supersystem (common_out = the K it used); otherwise
+        the supersystem's K is applied and returned unchanged.
+        """
+        ctype = (self._CUTOFF_TYPES[t2_cutoff_type.lower()] if isinstance(t2_cutoff_type, str)
+                 else int(t2_cutoff_type))
+        nai, naux = R.shape
+        eocc, evir = np.ascontiguousarray(eocc, np.float64), np.ascontiguousarray(evir, np.float64)
+        freqs = np.ascontiguousarray(freqs, np.float64)
+        weights = np.ascontiguousarray(weights, np.float64)
+        if nai != eocc.size * evir.size or freqs.size != weights.size:
+            raise ValueError("shape mismatch")
+        nt = C.c_int64(0)
+        self._sync_stream(R)
+        self._check(self._lib.brpa_t2_compute(self._h, _prec(precision), _ptr(R), naux, _ptr(eocc),
+                                              eocc.size, _ptr(evir), evir.size, _ptr(freqs),
+                                              _ptr(weights), freqs.size, float(lam), float(cutoff),
+                                              int(nguess), int(seed) & 0xFFFFFFFFFFFFFFFF,
+                                              C.byref(nt)))
+        if nt.value == 0:
+            raise BrpaError(_lib.ERR_ARG, "no T2 eigenvectors above the Gram cutoff")
+        common = C.c_double(float(common_thresh))
+        nsc = C.c_int64(0)
+        self._check(self._lib.brpa_t2_cutoff(self._h, ctype, float(t2_cutoff_thresh),
+                                             int(bool(smooth_step)), float(steepness), eocc.size,
+                                             C.byref(common), C.byref(nt), C.byref(nsc)))
+        k = nt.value
+        A, V, eig = np.zeros(k), np.zeros((k, nai)), np.zeros(k)
+        self._check(self._lib.brpa_t2_copyout_ov(self._h, _ptr(A), _ptr(V)))
+        self._check(self._lib.brpa_t2_eig_rpa(self._h, _ptr(R), naux, _ptr(eig)))
+        return A, V, eig, common.value, nsc.value
+
     def sosex(self, V, A, R, nocc, nvir, alpha=1.0, s2_operator=False, precision="fp64"):
         """Exchange correction -alpha sum (aj|bi) T(ai,bj) (rpa_CC_Exchange_SOSEX)."""
         nai, naux = R.shape

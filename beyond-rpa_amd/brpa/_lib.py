@@ -9,6 +9,8 @@ LIB_PATH = os.path.join(_HERE, "libbrpa.so")
  ERR_INTERNAL) = range(10)
 FP64, SPLIT_BF16, SPLIT_I8 = 0, 1, 2
 STAGE_PI, STAGE_SPLIT, STAGE_CHOL, STAGE_AUX = range(4)
+# amplitude-eigenvalue cutoff rules (brpa_t2_cutoff)
+T2_CUTOFF_EIG, T2_CUTOFF_EIG_DIV_MAXEIG, T2_CUTOFF_EIG_DIV_NELECTRON, T2_CUTOFF_SUM_REJECTED = range(4)
 STAGE_T2_PI, STAGE_T2_IOMEGA, STAGE_T2_BASIS, STAGE_T2_TIT, STAGE_T2_CORE, STAGE_SOSEX = range(4, 10)
 
 _i64 = C.c_int64
@@ -48,6 +50,10 @@ SIGNATURES = {
     "brpa_t2_compute": (C.c_int, [_vp, C.c_int, _vp, _i64, _vp, _i64, _vp, _i64, _vp, _vp, _i64,
                                   C.c_double, C.c_double, _i64, C.c_uint64, C.POINTER(_i64)]),
     "brpa_t2_copyout": (C.c_int, [_vp, _vp, _vp]),
+    "brpa_t2_cutoff": (C.c_int, [_vp, C.c_int, C.c_double, C.c_int, C.c_double, _i64,
+                                 C.POINTER(C.c_double), C.POINTER(_i64), C.POINTER(_i64)]),
+    "brpa_t2_eig_rpa": (C.c_int, [_vp, _vp, _i64, _vp]),
+    "brpa_t2_copyout_ov": (C.c_int, [_vp, _vp, _vp]),
     "brpa_exchange_batch": (C.c_int, [_vp, C.c_int, _i64, C.POINTER(brpa_frag_t), C.c_double,
                                       C.c_double, C.c_double, C.c_uint64, C.c_double, C.c_int,
                                       C.c_int, _vp, _vp]),

@@ -39,6 +39,8 @@ struct brpa_ctx {
     double *t2_A = nullptr, *t2_V = nullptr;
     size_t t2_A_cap = 0, t2_V_cap = 0;
     int64_t t2_nt = -1, t2_nai = 0;
+    // the kept pair is the factored core (A' = -1, V' = T L), not eigenpairs
+    bool t2_factored = false;
     // pinned host staging of the batched E_RPA path (uploads and results)
     char *pin = nullptr;
     size_t pin_cap = 0;

@@ -251,3 +251,60 @@ int launch_fill(brpa_ctx *ctx, double *p, int64_t n, double v) {
     LAUNCH_CHECK(ctx);
     return BRPA_OK;
 }
+
+// EigRPA_mu = 2 A_mu sum_P RU(P, mu)^2 with RU = R^T V (m x n), one work-group
+// per column (rpa_T2_EigRPA, rpa/rpa_CC_Doubles.f90:987-1009).
+__global__ __launch_bounds__(256) void k_eig_rpa(const double *__restrict__ RU, int64_t m,
+                                                 const double *__restrict__ A,
+                                                 double *__restrict__ out) {
+    __shared__ double sh[256];
+    const double *col = RU + (int64_t)blockIdx.x * m;
+    double s = 0.0;
+    for (int64_t r = threadIdx.x; r < m; r += 256) s = fma(col[r], col[r], s);
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if (threadIdx.x < k) sh[threadIdx.x] += sh[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = 2.0 * sh[0] * A[blockIdx.x];
+}
+
+int launch_eig_rpa(brpa_ctx *ctx, const double *RU, int64_t m, int64_t n, const double *A,
+                   double *out) {
+    if (n <= 0) return BRPA_OK;
+    hipLaunchKernelGGL(k_eig_rpa, dim3((unsigned)n), dim3(256), 0, ctx->stream, RU, m, A, out);
+    LAUNCH_CHECK(ctx);
+    return BRPA_OK;
+}
+
+// out(c, r) = in(r, c): in is m x n column-major, out n x m column-major.  32 x 32
+// tiles through LDS (one padding column against bank conflicts); 256 threads
+// cover a tile in 4 passes of 8 rows.
+__global__ __launch_bounds__(256) void k_transpose(const double *__restrict__ in, int64_t m,
+                                                   int64_t n, double *__restrict__ out) {
+    __shared__ double t[32][33];
+    const int64_t r0 = (int64_t)blockIdx.x * 32, c0 = (int64_t)blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    for (int j = ty; j < 32; j += 8) {
+        const int64_t r = r0 + tx, c = c0 + j;
+        if (r < m && c < n) t[j][tx] = in[r + m * c];
+    }
+    __syncthreads();
+    for (int j = ty; j < 32; j += 8) {
+        const int64_t c = c0 + tx, r = r0 + j;
+        if (r < m && c < n) out[c + n * r] = t[tx][j];
+    }
+}
+
+int launch_transpose(brpa_ctx *ctx, const double *in, int64_t m, int64_t n, double *out) {
+    if (m <= 0 || n <= 0) return BRPA_OK;
+    const int64_t gx = (m + 31) / 32, gy = (n + 31) / 32;
+    if (gy > 65535)
+        return brpa_fail(ctx, BRPA_ERR_ARG, "transpose: %lld columns exceed the grid limit",
+                         (long long)n);
+    hipLaunchKernelGGL(k_transpose, dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, ctx->stream, in,
+                       m, n, out);
+    LAUNCH_CHECK(ctx);
+    return BRPA_OK;
+}

@@ -34,6 +34,8 @@ int launch_scale(brpa_ctx *, double *x, double a, int64_t n);
 int launch_add_diag(brpa_ctx *, double *M, int64_t n, double a);
 int launch_zero_lower(brpa_ctx *, double *M, int64_t n);
 int launch_fill(brpa_ctx *, double *x, int64_t n, double v);
+int launch_eig_rpa(brpa_ctx *, const double *RU, int64_t m, int64_t n, const double *A, double *out);
+int launch_transpose(brpa_ctx *, const double *in, int64_t m, int64_t n, double *out);
 int launch_pair_dot(brpa_ctx *, const double *W1, const double *W2, const double *B, int64_t n_rows,
                     int64_t nt, int64_t stride, int npair, double *part);
 int sosex_nchunk(int64_t n_rows, int64_t nt);
@@ -446,6 +448,7 @@ int t2_build(brpa_ctx *ctx, int precision, const double *R, int64_t naux, const 
     *nvecs_t2 = nt;
     ctx->t2_nt = nt;
     ctx->t2_nai = K;
+    ctx->t2_factored = false;
     // the stage times of this call, once the stream is idle
     auto collect = [&]() -> int {
         BRPA_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -529,6 +532,7 @@ int t2_build(brpa_ctx *ctx, int precision, const double *R, int64_t naux, const 
             BRPA_TRY(gemm(ctx, h, false, true, nt, K, nt, 1.0, TIT, nt, Tkeep, K, 0.0, ctx->t2_V,
                           nt));
             done = true;
+            ctx->t2_factored = true;
         } else {
             BRPA_HIP(ctx, hipMemcpyAsync(TIT, Msave, 8 * nt * nt, hipMemcpyDeviceToDevice,
                                          ctx->stream));
@@ -565,8 +569,10 @@ int brpa_t2_compute(brpa_ctx *ctx, int precision, const double *R, int64_t naux,
                     const double *eocc, int64_t nocc, const double *evir, int64_t nvir,
                     const double *freqs, const double *weights, int64_t nfreqs, double lambda,
                     double cutoff, int64_t nguess, uint64_t seed, int64_t *nvecs_t2) {
-    return t2_build(ctx, precision, R, naux, eocc, nocc, evir, nvir, freqs, weights, nfreqs,
-                    lambda, cutoff, nguess, seed, nvecs_t2, false);
+    int rc = t2_build(ctx, precision, R, naux, eocc, nocc, evir, nvir, freqs, weights, nfreqs,
+                      lambda, cutoff, nguess, seed, nvecs_t2, false);
+    if (rc != BRPA_OK && ctx) ctx->t2_nt = -1;   // no usable amplitudes after a failure
+    return rc;
 }
 
 int brpa_t2_copyout(brpa_ctx *ctx, double *A, double *V) {
@@ -575,6 +581,180 @@ int brpa_t2_copyout(brpa_ctx *ctx, double *A, double *V) {
     if (ctx->t2_nt == 0) return BRPA_OK;
     if (A) BRPA_TRY(from_device(ctx, A, ctx->t2_A, 8 * ctx->t2_nt));
     if (V) BRPA_TRY(from_device(ctx, V, ctx->t2_V, 8 * ctx->t2_nt * ctx->t2_nai));
+    BRPA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return BRPA_OK;
+}
+
+// Amplitude-eigenvalue cutoff of the THC route, rpa_THC_CC_T2
+// (rpa/rpa_CC_Doubles.f90:809-983), applied to the eigenpairs left by
+// brpa_t2_compute.  The threshold rules and the scan over the NT2 eigenvalues
+// are host work on NT2 numbers; V is compacted on the device.
+int brpa_t2_cutoff(brpa_ctx *ctx, int cutoff_type, double thresh, int smooth_step,
+                   double steepness, int64_t nocc, double *common_thresh, int64_t *nvecs_t2,
+                   int64_t *nvecs_scaled) {
+    if (!ctx) return BRPA_ERR_ARG;
+    if (!common_thresh || !nvecs_t2)
+        return brpa_fail(ctx, BRPA_ERR_ARG, "brpa_t2_cutoff: null output argument");
+    if (ctx->t2_nt <= 0)
+        return brpa_fail(ctx, BRPA_ERR_ARG, "brpa_t2_cutoff: no amplitudes (call brpa_t2_compute)");
+    if (ctx->t2_factored)
+        return brpa_fail(ctx, BRPA_ERR_ARG, "brpa_t2_cutoff: the amplitudes are not eigenpairs");
+    const double common = *common_thresh;
+    if (!std::isfinite(common) || !std::isfinite(thresh) ||
+        (smooth_step && !(steepness >= 0.0 && std::isfinite(steepness))))
+        return brpa_fail(ctx, BRPA_ERR_ARG, "brpa_t2_cutoff: non-finite or negative parameter");
+    if (common < 0.0 && cutoff_type == BRPA_T2_CUTOFF_EIG_DIV_NELECTRON && nocc <= 0)
+        return brpa_fail(ctx, BRPA_ERR_ARG, "brpa_t2_cutoff: nocc must be positive");
+    BRPA_HIP(ctx, hipSetDevice(ctx->device));
+    const int64_t nt = ctx->t2_nt, K = ctx->t2_nai;
+    std::vector<double> a(nt);
+    BRPA_TRY(copy_sync(ctx, a.data(), ctx->t2_A, 8 * nt, hipMemcpyDeviceToHost));
+    // threshold K (880-930): the supersystem evaluates it, a subsystem takes the
+    // supersystem's value
+    double kth = 0.0;
+    if (common < 0.0) {
+        switch (cutoff_type) {
+        case BRPA_T2_CUTOFF_EIG:
+            kth = thresh;
+            break;
+        case BRPA_T2_CUTOFF_EIG_DIV_MAXEIG: {
+            double mx = 0.0;
+            for (double x : a) mx = std::max(mx, std::fabs(x));
+            kth = thresh * mx;
+            break;
+        }
+        case BRPA_T2_CUTOFF_EIG_DIV_NELECTRON:
+            kth = thresh / (double)(nocc * 2);
+            break;
+        case BRPA_T2_CUTOFF_SUM_REJECTED: {
+            // the first i with |A(i+1) + ... + A(nt)| < thresh gives K = |A(i+1)|;
+            // tail sums accumulated from the end, in the order the scan visits them
+            std::vector<double> tail(nt + 1, 0.0);
+            for (int64_t i = nt - 1; i >= 0; --i) tail[i] = tail[i + 1] + a[i];
+            kth = 0.0;
+            for (int64_t i = 1; i < nt; ++i)
+                if (std::fabs(tail[i]) < thresh) {
+                    kth = std::fabs(a[i]);
+                    break;
+                }
+            break;
+        }
+        default:
+            return brpa_fail(ctx, BRPA_ERR_ARG, "brpa_t2_cutoff: invalid cutoff type %d",
+                             cutoff_type);
+        }
+    } else {
+        kth = common;
+    }
+    // scan in the eigensolver's order (largest |A| first); the first rejected
+    // eigenvalue ends the kept set (936-959)
+    const double ka = kth, kb = kth * steepness;
+    int64_t nnew = nt, nscaled = 0;
+    for (int64_t i = 0; i < nt; ++i) {
+        const double x = std::fabs(a[i]);
+        if (smooth_step) {
+            if (x >= ka) continue;
+            if (x > kb) {
+                // rpa_CC_T2_SmoothStepFunction (29-40)
+                const double mu = (2.0 * x - (ka + kb)) / (kb - ka);
+                const double mu2 = mu * mu;
+                const double z =
+                    mu * (35.0 + mu2 * (-35.0 + mu2 * (21.0 - 5.0 * mu2))) / 16.0;
+                a[i] *= 0.5 * (1.0 - z);
+                ++nscaled;
+                continue;
+            }
+        } else if (x > kth) {
+            continue;
+        }
+        nnew = i;
+        break;
+    }
+    if (nnew <= 0)
+        return brpa_fail(ctx, BRPA_ERR_ARG,
+                         "brpa_t2_cutoff: threshold %.3e leaves no amplitude eigenvectors", kth);
+    if (nnew < nt) {
+        // V[ai][mu] keeps its first nnew columns: K rows of nt -> K rows of nnew,
+        // through the workspace (the in-place copy would overlap)
+        Arena ar;
+        BRPA_TRY(ws_reserve(ctx, 8 * (size_t)K * nnew + 256, &ar));
+        double *tmp = ar.take<double>((size_t)K * nnew);
+        BRPA_HIP(ctx, hipMemcpy2DAsync(tmp, 8 * nnew, ctx->t2_V, 8 * nt, 8 * nnew, K,
+                                       hipMemcpyDeviceToDevice, ctx->stream));
+        BRPA_HIP(ctx, hipMemcpyAsync(ctx->t2_V, tmp, 8 * (size_t)K * nnew,
+                                     hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    BRPA_TRY(copy_sync(ctx, ctx->t2_A, a.data(), 8 * nnew, hipMemcpyHostToDevice));
+    BRPA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->t2_nt = nnew;
+    *nvecs_t2 = nnew;
+    if (nvecs_scaled) *nvecs_scaled = nscaled;
+    if (common < 0.0) *common_thresh = kth;   // kept for the subsystem calculations (980-982)
+    return BRPA_OK;
+}
+
+// EigRPA_mu = 2 A_mu |R^T V_mu|^2 of the amplitudes kept on the device
+// (rpa_T2_EigRPA, rpa/rpa_CC_Doubles.f90:987-1009); sum_mu EigRPA_mu is the
+// direct-ring energy of the amplitudes (rpa_THC.f90:126).
+int brpa_t2_eig_rpa(brpa_ctx *ctx, const double *R, int64_t naux, double *eig_rpa) {
+    if (!ctx) return BRPA_ERR_ARG;
+    if (!R || !eig_rpa || naux <= 0)
+        return brpa_fail(ctx, BRPA_ERR_ARG, "brpa_t2_eig_rpa: bad arguments");
+    if (ctx->t2_nt < 0)
+        return brpa_fail(ctx, BRPA_ERR_ARG, "brpa_t2_eig_rpa: no amplitudes (call brpa_t2_compute)");
+    if (ctx->t2_factored)
+        return brpa_fail(ctx, BRPA_ERR_ARG, "brpa_t2_eig_rpa: the amplitudes are not eigenpairs");
+    const int64_t nt = ctx->t2_nt, K = ctx->t2_nai, N = naux;
+    if (nt == 0) return BRPA_OK;
+    BRPA_HIP(ctx, hipSetDevice(ctx->device));
+    rocblas_handle h;
+    BRPA_TRY(handle(ctx, &h));
+    const bool rdev = is_device_ptr(R), odev = is_device_ptr(eig_rpa);
+    Arena count, ar;
+    auto layout = [&](Arena &x) {
+        x.take<double>((size_t)N * nt);
+        if (!rdev) x.take<double>((size_t)K * N);
+        if (!odev) x.take<double>(nt);
+    };
+    layout(count);
+    BRPA_TRY(ws_reserve(ctx, count.off, &ar));
+    double *RU = ar.take<double>((size_t)N * nt);
+    const double *dR = R;
+    if (!rdev) {
+        double *t = ar.take<double>((size_t)K * N);
+        BRPA_TRY(copy_sync(ctx, t, R, 8 * (size_t)K * N, hipMemcpyHostToDevice));
+        dR = t;
+    }
+    double *out = odev ? eig_rpa : ar.take<double>(nt);
+    // RU (N x nt) = R^T V: R[ai][P] is column-major N x K, V[ai][mu] column-major nt x K
+    BRPA_TRY(gemm(ctx, h, false, true, N, nt, K, 1.0, dR, N, ctx->t2_V, nt, 0.0, RU, N));
+    BRPA_TRY(launch_eig_rpa(ctx, RU, N, nt, ctx->t2_A, out));
+    if (!odev)
+        BRPA_TRY(copy_sync(ctx, eig_rpa, out, 8 * nt, hipMemcpyDeviceToHost));
+    BRPA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return BRPA_OK;
+}
+
+// The THC route's layout of the amplitudes: Fortran V(ov, NT2), i.e. C
+// V[mu][ai] (rpa/rpa_CC_Doubles.f90:968-975), transposed on the device.
+int brpa_t2_copyout_ov(brpa_ctx *ctx, double *A, double *V) {
+    if (!ctx) return BRPA_ERR_ARG;
+    if (ctx->t2_nt < 0) return brpa_fail(ctx, BRPA_ERR_ARG, "no T2 result to copy");
+    const int64_t nt = ctx->t2_nt, K = ctx->t2_nai;
+    if (nt == 0) return BRPA_OK;
+    BRPA_HIP(ctx, hipSetDevice(ctx->device));
+    if (A) BRPA_TRY(from_device(ctx, A, ctx->t2_A, 8 * nt));
+    if (V) {
+        const bool vdev = is_device_ptr(V);
+        Arena ar;
+        double *dst = V;
+        if (!vdev) {
+            BRPA_TRY(ws_reserve(ctx, 8 * (size_t)K * nt + 256, &ar));
+            dst = ar.take<double>((size_t)K * nt);
+        }
+        BRPA_TRY(launch_transpose(ctx, ctx->t2_V, nt, K, dst));
+        if (!vdev) BRPA_TRY(copy_sync(ctx, V, dst, 8 * (size_t)K * nt, hipMemcpyDeviceToHost));
+    }
     BRPA_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return BRPA_OK;
 }

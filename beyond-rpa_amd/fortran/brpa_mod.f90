@@ -7,6 +7,7 @@
 !   rpa_GPiUG_MO            rpa/rpa_core_MO.f90:347-406
 !   rpa_CC_EcRPA_Analytic   rpa/rpa_CC.f90:412-471
 !   rpa_CC_T2               rpa/rpa_CC_Doubles.f90:43-77
+!   rpa_THC_CC_T2           rpa/rpa_CC_Doubles.f90:645-984 (with rpa_T2_EigRPA 987-1009)
 !   rpa_CC_Exchange_SOSEX   rpa/rpa_CC_Exchange.f90:36-97
 !
 ! Built with -fdefault-integer-8 like every reference build (default INTEGER
@@ -26,6 +27,12 @@ module brpa_mod
 
       public :: brpa_init, brpa_finalize
       public :: rpa_GPiUG_MO, rpa_CC_EcRPA_Analytic, rpa_CC_T2, rpa_CC_Exchange_SOSEX
+      public :: rpa_THC_CC_T2
+      ! T2CutoffType values (rpa/rpa_definitions.f90:139-142)
+      integer, parameter, public :: RPA_T2_CUTOFF_EIG = 0
+      integer, parameter, public :: RPA_T2_CUTOFF_EIG_DIV_MAXEIG = 1
+      integer, parameter, public :: RPA_T2_CUTOFF_EIG_DIV_NELECTRON = 2
+      integer, parameter, public :: RPA_T2_CUTOFF_SUM_REJECTED = 3
       public :: brpa_ecrpa_fused
 
       interface
@@ -82,6 +89,27 @@ module brpa_mod
                   integer(c_int64_t) :: nvecs
             end function
             integer(c_int) function c_t2_copyout(c, a, v) bind(C, name="brpa_t2_copyout")
+                  import :: c_int, c_ptr, c_double
+                  type(c_ptr), value :: c
+                  real(c_double) :: a(*), v(*)
+            end function
+            integer(c_int) function c_t2_cutoff(c, ctype, thresh, smooth, steep, nocc, common, nvecs, &
+                  nscaled) bind(C, name="brpa_t2_cutoff")
+                  import :: c_int, c_ptr, c_int64_t, c_double
+                  type(c_ptr), value :: c
+                  integer(c_int), value :: ctype, smooth
+                  real(c_double), value :: thresh, steep
+                  integer(c_int64_t), value :: nocc
+                  real(c_double) :: common
+                  integer(c_int64_t) :: nvecs, nscaled
+            end function
+            integer(c_int) function c_t2_eig_rpa(c, r, naux, eig) bind(C, name="brpa_t2_eig_rpa")
+                  import :: c_int, c_ptr, c_int64_t, c_double
+                  type(c_ptr), value :: c
+                  real(c_double) :: r(*), eig(*)
+                  integer(c_int64_t), value :: naux
+            end function
+            integer(c_int) function c_t2_copyout_ov(c, a, v) bind(C, name="brpa_t2_copyout_ov")
                   import :: c_int, c_ptr, c_double
                   type(c_ptr), value :: c
                   real(c_double) :: a(*), v(*)
@@ -277,6 +305,65 @@ contains
             if (NVecsT2 > 0) call check(c_t2_copyout(ctx, A, V), "rpa_CC_T2")
             if (.false.) print *, PiUEigenvecs(1,1,1), PiUEigenvals(1,1), OccBounds, VirtBounds, MaxBatchDim
       end subroutine rpa_CC_T2
+
+
+      subroutine rpa_THC_CC_T2(A, V, EigRPA, NVecsT2, PiUEigenvecs, PiUEigenvals, Rkai, NVecsChol, &
+            NOcc, NVirt, Freqs, FreqWeights, NFreqs, Lambda, OccEnergies, VirtEnergies, &
+            SmallEigenvalsCutoffT2, GuessNVecsT2, MaxBatchDim, T2CutoffThresh, T2CutoffType, &
+            T2CutoffSmoothStep, T2CutoffSteepness, T2CutoffCommonThresh, RPAParams)
+            !
+            ! Direct-ring amplitudes with the small-eigenvalue cutoff of the THC
+            ! route (rpa_CC_Doubles.f90:645-984): V(NVirt*NOcc, NVecsT2) and
+            ! EigRPA(NVecsT2) = 2 A |Rkai V|^2.  T2CutoffCommonThresh < 0 on entry:
+            ! supersystem, the threshold used is stored in it for the subsystems.
+            ! RPAParams is not read by the reference routine either.
+            !
+            real(F64), dimension(:), allocatable, intent(out)    :: A
+            real(F64), dimension(:, :), allocatable, intent(out) :: V
+            real(F64), dimension(:), allocatable, intent(out)    :: EigRPA
+            integer, intent(inout)                               :: NVecsT2
+            real(F64), dimension(:, :, :), intent(in)            :: PiUEigenvecs
+            real(F64), dimension(:, :), intent(in)               :: PiUEigenvals
+            real(F64), dimension(:, :), intent(in), contiguous   :: Rkai
+            integer, intent(in)                                  :: NVecsChol
+            integer, intent(in)                                  :: NOcc
+            integer, intent(in)                                  :: NVirt
+            real(F64), dimension(:), intent(in)                  :: Freqs
+            real(F64), dimension(:), intent(in)                  :: FreqWeights
+            integer, intent(in)                                  :: NFreqs
+            real(F64), intent(in)                                :: Lambda
+            real(F64), dimension(:), intent(in)                  :: OccEnergies
+            real(F64), dimension(:), intent(in)                  :: VirtEnergies
+            real(F64), intent(in)                                :: SmallEigenvalsCutoffT2
+            integer, intent(in)                                  :: GuessNVecsT2
+            integer, intent(in)                                  :: MaxBatchDim
+            real(F64), intent(in)                                :: T2CutoffThresh
+            integer, intent(in)                                  :: T2CutoffType
+            logical, intent(in)                                  :: T2CutoffSmoothStep
+            real(F64), intent(in)                                :: T2CutoffSteepness
+            real(F64), intent(inout)                             :: T2CutoffCommonThresh
+            class(*), intent(in), optional                       :: RPAParams
+            integer(c_int64_t) :: nt, nsc
+            integer(c_int) :: smooth
+
+            call brpa_init()
+            call check(c_t2_compute(ctx, 0_c_int, Rkai, int(NVecsChol, c_int64_t), OccEnergies, &
+                  int(NOcc, c_int64_t), VirtEnergies, int(NVirt, c_int64_t), Freqs, FreqWeights, &
+                  int(NFreqs, c_int64_t), Lambda, SmallEigenvalsCutoffT2, int(GuessNVecsT2, c_int64_t), &
+                  0_c_int64_t, nt), "rpa_THC_CC_T2")
+            smooth = 0_c_int
+            if (T2CutoffSmoothStep) smooth = 1_c_int
+            call check(c_t2_cutoff(ctx, int(T2CutoffType, c_int), T2CutoffThresh, smooth, &
+                  T2CutoffSteepness, int(NOcc, c_int64_t), T2CutoffCommonThresh, nt, nsc), &
+                  "rpa_THC_CC_T2")
+            NVecsT2 = int(nt)
+            allocate(A(NVecsT2))
+            allocate(V(NVirt*NOcc, NVecsT2))
+            allocate(EigRPA(NVecsT2))
+            call check(c_t2_copyout_ov(ctx, A, V), "rpa_THC_CC_T2")
+            call check(c_t2_eig_rpa(ctx, Rkai, int(NVecsChol, c_int64_t), EigRPA), "rpa_THC_CC_T2")
+            if (.false.) print *, PiUEigenvecs(1,1,1), PiUEigenvals(1,1), MaxBatchDim, present(RPAParams)
+      end subroutine rpa_THC_CC_T2
 
 
       subroutine rpa_CC_Exchange_SOSEX(EcExchange, Alpha, Vkai, Ak, Rkai, NOcc, NVirt, NVecsPiU, &

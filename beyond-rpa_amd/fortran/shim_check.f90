@@ -1,7 +1,8 @@
 !
 ! shim_check -- calls the Fortran drop-in routines of brpa_mod the way the
 ! reference's rpa_CC_Energy_2 does (rpa/rpa_CC.f90:519-527, 785, 884-892,
-! 965-971) and writes the results.  Used by the GPU test of the Fortran
+! 965-971) and the THC route's amplitude call (rpa/rpa_THC.f90:115-121), and
+! writes the results.  Used by the GPU test of the Fortran
 ! boundary.
 !
 !   shim_check input.bin output.txt
@@ -20,6 +21,9 @@ program shim_check
       real(F64), allocatable :: Rkai(:, :), Rk3(:, :, :), Eo(:), Ev(:), Freqs(:), Weights(:)
       real(F64), allocatable :: PiU(:, :, :), DCRG(:, :), OccE(:, :), VirtE(:, :)
       real(F64), allocatable :: A(:), V(:, :), Q(:, :, :), W(:, :)
+      real(F64), allocatable :: Ath(:), Vth(:, :), EigRPA(:)
+      real(F64) :: CommonThresh
+      integer :: NVecsTHC
       integer :: NOccs(2), NVirts(2), OccBounds(2, 2), VirtBounds(2, 2)
       real(F64) :: EcRPA, EcFused, EcSplit, Ex
       character(len=512) :: fin, fout
@@ -59,6 +63,13 @@ program shim_check
             Freqs, Weights, NFreqs, 1.0_F64, Eo, Ev, 1.0E-10_F64, 0, 100)
       call rpa_CC_Exchange_SOSEX(Ex, 1.0_F64, V, A, Rkai, NOcc, NVirt, NVecs, NVecsT2, .false.)
 
+      ! THC route: supersystem call with the EIG/MAXEIG rule and the smooth step
+      CommonThresh = -1.0_F64
+      NVecsTHC = 0
+      call rpa_THC_CC_T2(Ath, Vth, EigRPA, NVecsTHC, Q, W, Rkai, NVecs, NOcc, NVirt, Freqs, &
+            Weights, NFreqs, 1.0_F64, Eo, Ev, 1.0E-10_F64, 0, 100, 1.0E-3_F64, &
+            RPA_T2_CUTOFF_EIG_DIV_MAXEIG, .true., 0.1_F64, CommonThresh)
+
       open(newunit=io, file=trim(fout), status='replace')
       write(io, '(es25.17)') EcRPA
       write(io, '(es25.17)') EcFused
@@ -68,6 +79,15 @@ program shim_check
       do k = 1, NVecsT2
             write(io, '(es25.17)') A(k)
       end do
+      write(io, '(i0)') NVecsTHC
+      write(io, '(es25.17)') CommonThresh
+      write(io, '(es25.17)') sum(EigRPA)
+      do k = 1, NVecsTHC
+            write(io, '(es25.17)') Ath(k)
+      end do
+      ! layout-sensitive, sign-free: |V(:, 1)|^2 and T2(ai=1, bj=1) = sum_mu A_mu V(1, mu)^2
+      write(io, '(es25.17)') sum(Vth(:, 1)**2)
+      write(io, '(es25.17)') sum(Ath * Vth(1, :)**2)
       close(io)
       call brpa_finalize()
       if (.false.) print *, Seed

@@ -209,6 +209,39 @@ int brpa_t2_compute(brpa_ctx *ctx, int precision, const double *R, int64_t naux,
 /* A: NVecsT2 eigenvalues of T2 (ascending); V: Fortran V(NVecsT2, ov) == C V[ai][mu]. */
 int brpa_t2_copyout(brpa_ctx *ctx, double *A, double *V);
 
+/* Rules for the amplitude-eigenvalue threshold K of the THC route
+ * (rpa/rpa_definitions.f90:139-142). */
+enum {
+    BRPA_T2_CUTOFF_EIG = 0,              /* K = thresh */
+    BRPA_T2_CUTOFF_EIG_DIV_MAXEIG = 1,   /* K = thresh * max |A| */
+    BRPA_T2_CUTOFF_EIG_DIV_NELECTRON = 2,/* K = thresh / (2 nocc) */
+    BRPA_T2_CUTOFF_SUM_REJECTED = 3      /* K = |A(i+1)|, first i with |A(i+1)+...+A(NT2)| < thresh */
+};
+/* Small-eigenvalue cutoff of the amplitudes left on the device by
+ * brpa_t2_compute: the cutoff section of rpa_THC_CC_T2
+ * (rpa/rpa_CC_Doubles.f90:809-983).  *common_thresh < 0: supersystem, K
+ * follows cutoff_type and is written back to *common_thresh; >= 0: a subsystem,
+ * K = *common_thresh (unchanged).  Scanning the eigenvalues in stored order
+ * (largest |A| first), the first with |A| <= K ends the kept set; with
+ * smooth_step, |A| >= K keeps A, K*steepness < |A| < K scales A by
+ * rpa_CC_T2_SmoothStepFunction (29-40), |A| <= K*steepness ends the set.
+ * The device amplitudes become the first *nvecs_t2 pairs (A scaled); then
+ * brpa_t2_copyout / brpa_t2_copyout_ov / brpa_sosex see only those.  No kept
+ * eigenvector is an error (BRPA_ERR_ARG, amplitudes unchanged), as the
+ * reference's error stop (941-944).  nvecs_scaled (may be NULL): eigenvalues
+ * in the step interval. */
+int brpa_t2_cutoff(brpa_ctx *ctx, int cutoff_type, double thresh, int smooth_step,
+                   double steepness, int64_t nocc, double *common_thresh, int64_t *nvecs_t2,
+                   int64_t *nvecs_scaled);
+/* EigRPA(mu) = 2 A_mu |R^T V_mu|^2 of the device amplitudes (rpa_T2_EigRPA,
+ * rpa/rpa_CC_Doubles.f90:987-1009); its sum is the T2 direct-ring energy
+ * (rpa_THC.f90:126).  R: C R[ai][P] (ov x naux), host or device; eig_rpa:
+ * NVecsT2 doubles, host or device. */
+int brpa_t2_eig_rpa(brpa_ctx *ctx, const double *R, int64_t naux, double *eig_rpa);
+/* A as brpa_t2_copyout; V in the THC route's layout, Fortran V(ov, NVecsT2) ==
+ * C V[mu][ai] (rpa/rpa_CC_Doubles.f90:968-975). */
+int brpa_t2_copyout_ov(brpa_ctx *ctx, double *A, double *V);
+
 /* ---- exchange correction ---------------------------------------------------- */
 /* EcExchange = -alpha sum_{aibj} (aj|bi) T(ai,bj), T = V^T diag(B) V with
  * B = A (or A/(1-4A^2) when s2_operator), evaluated pair by pair

@@ -3,7 +3,8 @@
 Reference: src/rpa/rpa_CC_Doubles.f90:491-642 (rpa_CC_T2_v5) with
   rpa_CC_NormalizeColumns 80-91, rpa_CC_Chi 276-292, rpa_CC_RChiM 295-316,
   rpa_CC_ChiRTQWQTRChiM 319-345, rpa_CC_D 348-366, rpa_CC_IM_v5 369-422,
-  rpa_CC_MIM_v5 425-488, rpa_T2_EigRPA 987-1009,
+  rpa_CC_MIM_v5 425-488, rpa_T2_EigRPA 987-1009, the cutoff section of
+  rpa_THC_CC_T2 809-983 and rpa_CC_T2_SmoothStepFunction 29-40,
 and rpa_CC_Diagonalize_PiU (src/rpa/rpa_CC.f90:1108-1129).
 
 Algorithm (lambda = T2CouplingStrength, default 1):
@@ -26,7 +27,12 @@ Pinned by (tests/test_oracle_t2.py): with NG >= ov and a zero cutoff the
 factorisation equals the explicit T = I (1+I)^{-1} / 2 of the full ov x ov
 matrix I; that explicit T matches the Riccati closed form at exact frequency
 integration up to the grid error; E_T2 = 2 sum_mu A_mu |R^T V_mu|^2 reproduces
-E_RPA the way the reference's EigRPA diagnostic does.  The eigenvalue cutoff
+E_RPA the way the reference's EigRPA diagnostic does.  The THC route's
+amplitude-eigenvalue cutoff (t2_cutoff, smooth_step) is pinned in
+tests/test_oracle_t2_cutoff.py: the step polynomial against the integral of
+35/16 (1 - t^2)^3, the four threshold rules against hand-worked answers, the
+SUM-REJECTED rule against its shortest-prefix definition, and the truncation
+against the Frobenius identity of orthonormal eigenvectors.  The eigenvalue cutoff
 and the random subspace make the factorised result depend on the guess
 vectors, so bit-level parity with the reference's own (A, V) is impossible by
 construction; A, V are compared with this oracle on the same guess vectors.
@@ -166,3 +172,91 @@ def eig_rpa(A, V, R):
     """rpa_T2_EigRPA: EigRPA_mu = 2 A_mu |R^T V_mu|^2 (rpa_CC_Doubles.f90:987-1009)."""
     RU = R.T @ V
     return 2.0 * A * np.sum(RU * RU, axis=0)
+
+
+# Rules for the amplitude-eigenvalue threshold K of rpa_THC_CC_T2
+# (src/rpa/rpa_definitions.f90:139-142; parser keywords EIG, EIG/MAXEIG,
+# EIG/NELECTRON, SUM-REJECTED in src/misc/parser.f90:3918-3931)
+T2_CUTOFF_EIG = 0
+T2_CUTOFF_EIG_DIV_MAXEIG = 1
+T2_CUTOFF_EIG_DIV_NELECTRON = 2
+T2_CUTOFF_SUM_REJECTED = 3
+T2_CUTOFF_THRESH = 1.0e-4        # src/rpa/rpa_definitions.f90:480
+T2_CUTOFF_STEEPNESS = 0.1        # src/rpa/rpa_definitions.f90:483
+
+
+def smooth_step(lam, ka, kb):
+    """rpa_CC_T2_SmoothStepFunction (rpa_CC_Doubles.f90:29-40).
+
+    mu = (2 lam - (ka + kb)) / (kb - ka) maps lam = ka -> -1, lam = kb -> +1;
+    alpha = (1 - z(mu)) / 2 with z = (35 mu - 35 mu^3 + 21 mu^5 - 5 mu^7) / 16,
+    so alpha(ka) = 1 and alpha(kb) = 0.
+    """
+    mu = (2.0 * lam - (ka + kb)) / (kb - ka)
+    z = (35.0 * mu - 35.0 * mu ** 3 + 21.0 * mu ** 5 - 5.0 * mu ** 7) / 16.0
+    return 0.5 * (1.0 - z)
+
+
+def t2_cutoff_threshold(A, cutoff_type, thresh, nocc):
+    """Threshold K of the supersystem (rpa_CC_Doubles.f90:880-910), A in stored order."""
+    A = np.asarray(A, dtype=np.float64)
+    n = A.size
+    if cutoff_type == T2_CUTOFF_EIG:
+        return float(thresh)
+    if cutoff_type == T2_CUTOFF_EIG_DIV_MAXEIG:
+        return float(thresh * np.max(np.abs(A)))
+    if cutoff_type == T2_CUTOFF_EIG_DIV_NELECTRON:
+        return float(thresh / (nocc * 2))
+    if cutoff_type == T2_CUTOFF_SUM_REJECTED:
+        # first i (1-based) with |sum(A(i+1:n))| < thresh sets K = |A(i+1)|; none: K = 0
+        for i in range(1, n + 1):
+            if i < n:
+                if abs(float(np.sum(A[i:]))) < thresh:
+                    return float(abs(A[i]))
+            else:
+                return 0.0
+        return 0.0
+    raise ValueError("invalid T2 cutoff type %r" % (cutoff_type,))
+
+
+def t2_cutoff(A, V, cutoff_type=T2_CUTOFF_EIG, thresh=T2_CUTOFF_THRESH, nocc=1,
+              smooth_step_on=False, steepness=T2_CUTOFF_STEEPNESS, common_thresh=-1.0):
+    """Amplitude-eigenvalue cutoff of rpa_THC_CC_T2 (rpa_CC_Doubles.f90:809-983).
+
+    A: (NT2,) in the order of the eigensolver (ascending, i.e. the largest |A|
+    first, since A <= 0); V: V[ai][mu] (ov, NT2).  common_thresh < 0: this is
+    the supersystem, K follows cutoff_type; otherwise K = common_thresh (a
+    subsystem).  Scanning mu in order, the first mu with |A_mu| <= K ends the
+    kept set (with the smooth step: |A_mu| >= Ka = K keeps A_mu, Kb = K *
+    steepness < |A_mu| < Ka scales it by smooth_step, |A_mu| <= Kb ends the set).
+    Returns (A_new, V_new, K, common_out, n_scaled); V_new keeps the first
+    NT2new columns of V.  An empty kept set is an error (941-944).
+    """
+    A = np.asarray(A, dtype=np.float64)
+    if common_thresh < 0.0:
+        K = t2_cutoff_threshold(A, cutoff_type, thresh, nocc)
+    else:
+        K = float(common_thresh)
+    n = A.size
+    nnew, nscaled = n, 0
+    Anew = A.copy()
+    ka, kb = K, K * steepness
+    for i in range(n):
+        a = abs(A[i])
+        if smooth_step_on:
+            if a >= ka:
+                continue
+            if a > kb:
+                Anew[i] = smooth_step(a, ka, kb) * A[i]
+                nscaled += 1
+                continue
+            nnew = i
+            break
+        if a > K:
+            continue
+        nnew = i
+        break
+    if nnew <= 0:
+        raise ValueError("T2 cutoff leaves no amplitude eigenvectors")
+    common_out = K if common_thresh < 0.0 else float(common_thresh)
+    return Anew[:nnew].copy(), np.ascontiguousarray(V[:, :nnew]), K, common_out, nscaled

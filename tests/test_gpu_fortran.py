@@ -40,3 +40,16 @@ def test_fortran_shim_matches_oracle(tmp_path):
     assert nt == len(A_ref)
     assert np.max(np.abs(A_f - A_ref)) <= 1e-9 * np.max(np.abs(A_ref))
     assert abs(ex - sosex.sosex(f.R, V_ref, A_ref, o, v)) <= 1e-8
+    # rpa_THC_CC_T2: EIG/MAXEIG rule (1e-3), smooth step 0.1, supersystem
+    k = 5 + nt
+    nth, common, esum = int(vals[k]), float(vals[k + 1]), float(vals[k + 2])
+    A_th = np.array([float(s) for s in vals[k + 3:k + 3 + nth]])
+    vnorm, t11 = float(vals[k + 3 + nth]), float(vals[k + 4 + nth])
+    A_c, V_c, K, common_ref, _ = t2.t2_cutoff(A_ref, V_ref, t2.T2_CUTOFF_EIG_DIV_MAXEIG, 1e-3, o,
+                                              smooth_step_on=True, steepness=0.1)
+    assert nth == len(A_c)
+    assert abs(common - common_ref) <= 1e-9 * abs(common_ref)
+    assert np.max(np.abs(A_th - A_c)) <= 1e-9 * np.max(np.abs(A_c))
+    assert abs(esum - t2.eig_rpa(A_c, V_c, f.R).sum()) <= 1e-9
+    assert abs(vnorm - 1.0) <= 1e-9
+    assert abs(t11 - np.sum(A_c * V_c[0, :] ** 2)) <= 1e-9 * np.max(np.abs(A_c))
