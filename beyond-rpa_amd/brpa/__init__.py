@@ -321,6 +321,49 @@ class Context:
             self._check(self._lib.brpa_t2_copyout(self._h, _ptr(A), _ptr(V)))
         return A, V
 
+    def thc_piu(self, Xga, Xgi, Zgk, eocc, evir, freqs, prefactor=4.0, PiU=None,
+                want_R=False, device=False):
+        """Pi(u) of every frequency from the THC factors (rpa_THC_PiU_BlockAlgo,
+        rpa_THC.f90:285-339).  Xga (ngrid, nvir), Xgi (ngrid, nocc), Zgk (ngrid,
+        naux) are Fortran-ordered arrays, i.e. C arrays of shape (nvir, ngrid),
+        (nocc, ngrid), (naux, ngrid) holding X(g, a) at [a][g].  PiU: an
+        existing (nfreqs, naux, naux) array to add to (second spin block), else
+        a new one is returned.  want_R: also return R[ai][P] (ov, naux)."""
+        nvir, ngrid = Xga.shape
+        nocc, naux = Xgi.shape[0], Zgk.shape[0]
+        if Xgi.shape[1] != ngrid or Zgk.shape[1] != ngrid:
+            raise ValueError("shape mismatch")
+        eocc, evir = np.ascontiguousarray(eocc, np.float64), np.ascontiguousarray(evir, np.float64)
+        freqs = np.ascontiguousarray(freqs, np.float64)
+        if eocc.size != nocc or evir.size != nvir:
+            raise ValueError("shape mismatch")
+        nf = freqs.size
+        accumulate = PiU is not None
+        torch_out = device or _is_torch(Xga)
+        if torch_out:
+            import torch
+            dev = Xga.device if _is_torch(Xga) else torch.device("cuda", self.device)
+            new = lambda *shape: torch.zeros(shape, dtype=torch.float64, device=dev)  # noqa: E731
+        else:
+            new = lambda *shape: np.zeros(shape)  # noqa: E731
+        if PiU is None:
+            PiU = new(nf, naux, naux)
+        elif tuple(PiU.shape) != (nf, naux, naux):
+            raise ValueError("PiU shape mismatch")
+        R = new(nocc * nvir, naux) if want_R else None
+        self._sync_stream(Xga, Xgi, Zgk, PiU)
+        self._check(self._lib.brpa_thc_piu(self._h, _ptr(PiU), _ptr(R), _ptr(Xga), _ptr(Xgi),
+                                           _ptr(Zgk), ngrid, nocc, nvir, naux, _ptr(eocc),
+                                           _ptr(evir), _ptr(freqs), nf, float(prefactor),
+                                           int(accumulate)))
+        return (PiU, R) if want_R else PiU
+
+    def last_thc_stage_ms(self):
+        """{stage: (ms, launches)} of the last thc_piu call."""
+        names = {"thc_r": _lib.STAGE_THC_R, "thc_pi": _lib.STAGE_THC_PI}
+        return {k: (self._lib.brpa_last_stage_ms(self._h, v),
+                    self._lib.brpa_last_stage_launches(self._h, v)) for k, v in names.items()}
+
     _CUTOFF_TYPES = {"eig": 0, "eig/maxeig": 1, "eig/nelectron": 2, "eig/nocc": 2,
                      "sum-rejected": 3, "sum_rejected": 3}
 

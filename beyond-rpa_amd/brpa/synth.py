@@ -76,3 +76,47 @@ CONFIGS = {
     "sosex_large": (64, 600, 3200, 16),
 }
 BATCH_NFREQS = 16
+
+
+@dataclass
+class ThcFragment:
+    """Tensor-hypercontraction factors of one fragment (the THC route's inputs).
+
+    ``Xga`` (nvir, ngrid), ``Xgi`` (nocc, ngrid), ``Zgk`` (naux, ngrid) are C
+    arrays holding X(g, a) at [a][g]: the bytes of the reference's Fortran
+    Xga(NGridTHC, NVirt), Xgi(NGridTHC, NOcc), Zgk(NGridTHC, NCholesky)
+    (src/rpa/rpa_THC.f90:373-403).  Distributions: X ~ N(0, 1), Z(g, P) =
+    N(0, 1) * 0.3 * exp(-4 P / N) / sqrt(ngrid), so that R(P, ai) = sum_g
+    Z X X has the scale profile of the MO-route fragments.  Draw order from one
+    PCG64 stream: eps_occ, Xgi, Xga, Zgk.
+    """
+    nocc: int
+    nvir: int
+    naux: int
+    ngrid: int
+    eocc: np.ndarray
+    evir: np.ndarray
+    Xga: np.ndarray
+    Xgi: np.ndarray
+    Zgk: np.ndarray
+    seed: int
+
+    def gaps(self):
+        return (self.evir[None, :] - self.eocc[:, None]).ravel()
+
+
+def make_thc_fragment(nocc, nvir, naux, ngrid, seed=BASE_SEED):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    eocc = np.sort(rng.uniform(-1.5, -0.3, nocc))
+    evir = np.geomspace(0.05, 3.0, nvir)
+    Xgi = rng.standard_normal((nocc, ngrid))
+    Xga = rng.standard_normal((nvir, ngrid))
+    scale = 0.3 * np.exp(-4.0 * np.arange(naux, dtype=np.float64) / naux) / np.sqrt(ngrid)
+    Zgk = rng.standard_normal((naux, ngrid)) * scale[:, None]
+    return ThcFragment(nocc, nvir, naux, ngrid, eocc, evir, np.ascontiguousarray(Xga),
+                       np.ascontiguousarray(Xgi), np.ascontiguousarray(Zgk), seed)
+
+
+# THC-route workload (nocc, nvir, naux, ngrid, nfreqs): the single-fragment
+# shape with a THC grid of 12 points per orbital
+THC_CONFIG = (36, 300, 1800, 4032, 16)

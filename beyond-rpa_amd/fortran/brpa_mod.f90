@@ -8,6 +8,7 @@
 !   rpa_CC_EcRPA_Analytic   rpa/rpa_CC.f90:412-471
 !   rpa_CC_T2               rpa/rpa_CC_Doubles.f90:43-77
 !   rpa_THC_CC_T2           rpa/rpa_CC_Doubles.f90:645-984 (with rpa_T2_EigRPA 987-1009)
+!   rpa_THC_PiU             rpa/rpa_THC.f90:215-282
 !   rpa_CC_Exchange_SOSEX   rpa/rpa_CC_Exchange.f90:36-97
 !
 ! Built with -fdefault-integer-8 like every reference build (default INTEGER
@@ -27,7 +28,7 @@ module brpa_mod
 
       public :: brpa_init, brpa_finalize
       public :: rpa_GPiUG_MO, rpa_CC_EcRPA_Analytic, rpa_CC_T2, rpa_CC_Exchange_SOSEX
-      public :: rpa_THC_CC_T2
+      public :: rpa_THC_CC_T2, rpa_THC_PiU
       ! T2CutoffType values (rpa/rpa_definitions.f90:139-142)
       integer, parameter, public :: RPA_T2_CUTOFF_EIG = 0
       integer, parameter, public :: RPA_T2_CUTOFF_EIG_DIV_MAXEIG = 1
@@ -102,6 +103,17 @@ module brpa_mod
                   integer(c_int64_t), value :: nocc
                   real(c_double) :: common
                   integer(c_int64_t) :: nvecs, nscaled
+            end function
+            integer(c_int) function c_thc_piu(c, piu, rkai, xga, xgi, zgk, ngrid, nocc, nvir, naux, &
+                  eocc, evir, freqs, nfreqs, pref, accum) bind(C, name="brpa_thc_piu")
+                  import :: c_int, c_ptr, c_int64_t, c_double
+                  type(c_ptr), value :: c
+                  real(c_double) :: piu(*)
+                  type(c_ptr), value :: rkai
+                  real(c_double) :: xga(*), xgi(*), zgk(*), eocc(*), evir(*), freqs(*)
+                  integer(c_int64_t), value :: ngrid, nocc, nvir, naux, nfreqs
+                  real(c_double), value :: pref
+                  integer(c_int), value :: accum
             end function
             integer(c_int) function c_t2_eig_rpa(c, r, naux, eig) bind(C, name="brpa_t2_eig_rpa")
                   import :: c_int, c_ptr, c_int64_t, c_double
@@ -305,6 +317,63 @@ contains
             if (NVecsT2 > 0) call check(c_t2_copyout(ctx, A, V), "rpa_CC_T2")
             if (.false.) print *, PiUEigenvecs(1,1,1), PiUEigenvals(1,1), OccBounds, VirtBounds, MaxBatchDim
       end subroutine rpa_CC_T2
+
+
+      subroutine rpa_THC_PiU(PiU, Rkai, Xga, Xgi, Zgk, NOcc, NVirt, Freqs, &
+            OccEnergies, VirtEnergies, MaxBlockDim, FullRkai)
+            !
+            ! Pi(u) of every frequency from the THC factors (rpa_THC.f90:215-282).
+            ! As in the reference, the number of spin blocks is size(Rkai, 3) and
+            ! Alpha is 4 for one block, 2 otherwise; FullRkai = .true. writes the
+            ! complete R of block s into Rkai(:, :, s), .false. leaves Rkai alone
+            ! (the reference keeps only its last ai block there).  MaxBlockDim only
+            ! bounded the reference's memory.
+            !
+            real(F64), dimension(:, :, :), intent(out), contiguous         :: PiU
+            real(F64), dimension(:, :, :), intent(out), contiguous, target :: Rkai
+            real(F64), dimension(:, :, :), intent(in), contiguous          :: Xga
+            real(F64), dimension(:, :, :), intent(in), contiguous          :: Xgi
+            real(F64), dimension(:, :), intent(in), contiguous             :: Zgk
+            integer, dimension(2), intent(in)                              :: NOcc
+            integer, dimension(2), intent(in)                              :: NVirt
+            real(F64), dimension(:), intent(in)                            :: Freqs
+            real(F64), dimension(:, :), intent(in), contiguous             :: OccEnergies
+            real(F64), dimension(:, :), intent(in), contiguous             :: VirtEnergies
+            integer, intent(in)                                            :: MaxBlockDim
+            logical, intent(in)                                            :: FullRkai
+            integer :: s, NSpins, NCholesky, NGrid, NFreqs
+            integer(c_int) :: accum
+            real(F64) :: Alpha
+            type(c_ptr) :: rp
+
+            call brpa_init()
+            NSpins = size(Rkai, dim=3)
+            NCholesky = size(Zgk, dim=2)
+            NGrid = size(Zgk, dim=1)
+            NFreqs = size(PiU, dim=3)
+            Alpha = 2.0_F64
+            if (NSpins == 1) Alpha = 4.0_F64
+            PiU = 0.0_F64
+            accum = 1_c_int
+            do s = 1, NSpins
+                  if (NOcc(s) > 0) then
+                        rp = c_null_ptr
+                        if (FullRkai) then
+                              if (size(Rkai, dim=1) /= NCholesky .or. &
+                                    size(Rkai, dim=2) < NOcc(s)*NVirt(s)) then
+                                    print *, "rpa_THC_PiU: Rkai must be (NCholesky, >= NOcc*NVirt, NSpins)"
+                                    error stop
+                              end if
+                              rp = c_loc(Rkai(1, 1, s))
+                        end if
+                        call check(c_thc_piu(ctx, PiU, rp, Xga(:, :, s), Xgi(:, :, s), Zgk, &
+                              int(NGrid, c_int64_t), int(NOcc(s), c_int64_t), int(NVirt(s), c_int64_t), &
+                              int(NCholesky, c_int64_t), OccEnergies(:, s), VirtEnergies(:, s), Freqs, &
+                              int(NFreqs, c_int64_t), Alpha, accum), "rpa_THC_PiU")
+                  end if
+            end do
+            if (.false.) print *, MaxBlockDim
+      end subroutine rpa_THC_PiU
 
 
       subroutine rpa_THC_CC_T2(A, V, EigRPA, NVecsT2, PiUEigenvecs, PiUEigenvals, Rkai, NVecsChol, &

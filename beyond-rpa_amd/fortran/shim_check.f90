@@ -5,7 +5,13 @@
 ! writes the results.  Used by the GPU test of the Fortran
 ! boundary.
 !
-!   shim_check input.bin output.txt
+!   shim_check input.bin output.txt [thc_input.bin thc_output.txt]
+!
+! thc_input.bin (stream): NOcc, NVirt, NCholesky, NGrid, NFreqs,
+!   Xga(NGrid, NVirt), Xgi(NGrid, NOcc), Zgk(NGrid, NCholesky),
+!   OccEnergies(NOcc), VirtEnergies(NVirt), Freqs(NFreqs)
+! thc_output.txt: PiU(NCholesky, NCholesky, NFreqs) then Rkai(NCholesky, NOcc*NVirt)
+! of a closed-shell rpa_THC_PiU call with FullRkai = .true.
 !
 ! input.bin (stream, 64-bit integers and doubles):
 !   NOcc, NVirt, NVecs, NFreqs, Seed
@@ -89,6 +95,37 @@ program shim_check
       write(io, '(es25.17)') sum(Vth(:, 1)**2)
       write(io, '(es25.17)') sum(Ath * Vth(1, :)**2)
       close(io)
+      if (command_argument_count() >= 4) call thc_check()
       call brpa_finalize()
       if (.false.) print *, Seed
+
+contains
+
+      subroutine thc_check()
+            integer :: to, tv, tn, tg, tf, i
+            integer :: NO2(2), NV2(2)
+            real(F64), allocatable :: Xa(:, :, :), Xi(:, :, :), Zg(:, :), Eo2(:, :), Ev2(:, :), Fr(:)
+            real(F64), allocatable :: P3(:, :, :), R3(:, :, :)
+            character(len=512) :: tin, tout
+
+            call get_command_argument(3, tin)
+            call get_command_argument(4, tout)
+            open(newunit=io, file=trim(tin), access='stream', form='unformatted', status='old')
+            read(io) to, tv, tn, tg, tf
+            allocate(Xa(tg, tv, 1), Xi(tg, to, 1), Zg(tg, tn), Eo2(to, 1), Ev2(tv, 1), Fr(tf))
+            read(io) Xa, Xi, Zg, Eo2, Ev2, Fr
+            close(io)
+            allocate(P3(tn, tn, tf), R3(tn, to*tv, 1))
+            NO2 = [to, 0]
+            NV2 = [tv, 0]
+            call rpa_THC_PiU(P3, R3, Xa, Xi, Zg, NO2, NV2, Fr, Eo2, Ev2, 100, .true.)
+            open(newunit=io, file=trim(tout), status='replace')
+            do i = 1, size(P3)
+                  write(io, '(es25.17)') P3(mod(i-1, tn) + 1, mod((i-1)/tn, tn) + 1, (i-1)/(tn*tn) + 1)
+            end do
+            do i = 1, size(R3)
+                  write(io, '(es25.17)') R3(mod(i-1, tn) + 1, (i-1)/tn + 1, 1)
+            end do
+            close(io)
+      end subroutine thc_check
 end program shim_check

@@ -84,8 +84,9 @@ int brpa_last_graph_stats(const brpa_ctx *ctx, int *captures, int *replays);
  * (HIP events recorded on the context's stream around the kernels only). */
 double brpa_last_kernel_ms(const brpa_ctx *ctx);
 
-/* Device time (ms) of one stage of the last E_RPA call (stages 0-3), or of
- * the last amplitude / SOSEX / exchange-batch call (stages 4-9), summed over
+/* Device time (ms) of one stage of the last E_RPA call (stages 0-3), of
+ * the last amplitude / SOSEX / exchange-batch call (stages 4-9) or of the last
+ * THC Pi(u) call (stages 10-11), summed over
  * its launches (the exchange stages include the device idle time inside the
  * stage), and the number of launches.  Stages: */
 enum {
@@ -101,7 +102,9 @@ enum {
     BRPA_STAGE_T2_TIT = 7,    /* amplitudes: T^T I T over all frequencies */
     BRPA_STAGE_T2_CORE = 8,   /* amplitudes: factored core or eigenpairs, and V */
     BRPA_STAGE_SOSEX = 9,     /* SOSEX contraction (slicing and pair products) */
-    BRPA_NSTAGES = 10
+    BRPA_STAGE_THC_R = 10,    /* THC producer of R (brpa_thc_piu) */
+    BRPA_STAGE_THC_PI = 11,   /* Pi(u) of all frequencies from that R (brpa_thc_piu) */
+    BRPA_NSTAGES = 12
 };
 double brpa_last_stage_ms(const brpa_ctx *ctx, int stage);
 int brpa_last_stage_launches(const brpa_ctx *ctx, int stage);
@@ -183,6 +186,29 @@ int brpa_autotune(brpa_ctx *ctx, int precision, int64_t nfrag, const brpa_frag_t
 /* Chosen variant of a bucket (-1: not tuned) and the product-kernel time (ms) of
  * each of the 3 variants measured by the last brpa_autotune (0: not measured). */
 int brpa_autotune_result(const brpa_ctx *ctx, int bucket, int *variant, double *ms_per_variant);
+
+/* ---- THC route: Cholesky vectors and Pi(u) from the THC factors --------- */
+/* Pi(u_w) (w = 0..nfreqs-1) of one spin block from the tensor-hypercontraction
+ * factors, replacing rpa_THC_PiU_BlockAlgo (rpa/rpa_THC.f90:285-339) with
+ * rpa_THC_Rkab (373-403) and rpa_THC_ChiRkai (342-370):
+ *   R(P, ai) = sum_g Zgk(g, P) Xga(g, a) Xgi(g, i)      (ai = a + nvir*i)
+ *   Pi(u_w) (+)= prefactor * R diag(d/(d^2 + u_w^2)) R^T,  d = e_a - e_i.
+ * Xga: Fortran Xga(ngrid, nvir) (column a at Xga + ngrid*a), Xgi(ngrid, nocc),
+ * Zgk(ngrid, naux); PiU: nfreqs x naux x naux (Fortran PiU(naux, naux, nfreqs));
+ * all host or device.  accumulate = 0 overwrites PiU (the reference zeroes it
+ * first, rpa_THC.f90:236), 1 adds (the second spin block).  prefactor: 4 for a
+ * closed shell, 2 per spin block of an open shell (rpa_THC.f90:254-266).
+ * Rkai (may be NULL): receives R as Fortran Rkai(naux, nvir*nocc) == C
+ * R[ai][P], the FullRkai = .true. output that the amplitudes read; NULL keeps R
+ * inside the library (FullRkai = .false.).  R is built once in HBM by one FP64
+ * MFMA kernel that forms Xga(g, a) Xgi(g, i) while loading its operand (no
+ * ngrid x ov intermediate); the block size MaxBlockDim of the reference only
+ * bounded its memory and has no counterpart.  Device times: stages
+ * BRPA_STAGE_THC_R and BRPA_STAGE_THC_PI. */
+int brpa_thc_piu(brpa_ctx *ctx, double *PiU, double *Rkai, const double *Xga, const double *Xgi,
+                 const double *Zgk, int64_t ngrid, int64_t nocc, int64_t nvir, int64_t naux,
+                 const double *eocc, const double *evir, const double *freqs, int64_t nfreqs,
+                 double prefactor, int accumulate);
 
 /* ---- factorised direct-ring amplitudes --------------------------------- */
 /* T2 = sum_mu A_mu V_mu V_mu^T (ov x ov), the direct-ring doubles amplitudes

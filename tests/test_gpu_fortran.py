@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 from brpa import synth
-from oracle import erpa, grid, sosex, t2
+from oracle import erpa, grid, sosex, t2, thc
 
 pytestmark = pytest.mark.gpu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -53,3 +53,33 @@ def test_fortran_shim_matches_oracle(tmp_path):
     assert abs(esum - t2.eig_rpa(A_c, V_c, f.R).sum()) <= 1e-9
     assert abs(vnorm - 1.0) <= 1e-9
     assert abs(t11 - np.sum(A_c * V_c[0, :] ** 2)) <= 1e-9 * np.max(np.abs(A_c))
+
+
+def test_fortran_thc_piu_matches_oracle(tmp_path):
+    """rpa_THC_PiU through the Fortran module (closed shell, FullRkai)."""
+    if not os.path.exists(EXE):
+        pytest.fail("brpa_shim_check was not built (amdflang missing at build time?)")
+    o, v, n, g, nf = 3, 17, 40, 90, 5
+    f = synth.make_fragment(2, 3, 4, 1)
+    fin, fout = tmp_path / "in.bin", tmp_path / "out.txt"
+    with open(fin, "wb") as fh:
+        np.array([f.nocc, f.nvir, f.naux, 4, 0], dtype=np.int64).tofile(fh)
+        x, w, _ = grid.freq_grid(f.eocc, f.evir, 4)
+        for a in (f.R, f.eocc, f.evir, x, w):
+            np.ascontiguousarray(a, np.float64).tofile(fh)
+    t = synth.make_thc_fragment(o, v, n, g, 5)
+    freqs = np.geomspace(0.05, 10.0, nf)
+    tin, tout = tmp_path / "thc.bin", tmp_path / "thc.txt"
+    with open(tin, "wb") as fh:
+        np.array([o, v, n, g, nf], dtype=np.int64).tofile(fh)
+        for a in (t.Xga, t.Xgi, t.Zgk, t.eocc, t.evir, freqs):
+            np.ascontiguousarray(a, np.float64).tofile(fh)
+    r = subprocess.run([EXE, str(fin), str(fout), str(tin), str(tout)], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    vals = np.array([float(s) for s in open(tout).read().split()])
+    Pi = vals[:nf * n * n].reshape(nf, n, n)
+    R = vals[nf * n * n:].reshape(o * v, n)
+    Pis_ref, R_ref = thc.thc_piu(t.Xga.T, t.Xgi.T, t.Zgk.T, t.eocc, t.evir, freqs)
+    assert np.max(np.abs(R - R_ref)) <= 1e-13 * np.max(np.abs(R_ref))
+    assert np.max(np.abs(Pi - Pis_ref)) <= 1e-12 * np.max(np.abs(Pis_ref))
