@@ -17,6 +17,9 @@ split-int8 SOSEX contraction and once in FP64; each form's E_RPA + exchange
 rate is 1 / (its E_RPA time per fragment + the exchange time per fragment in
 its own arithmetic).
 
+Other workloads (--workload): single, adjoint, sosex, and thc (R from
+tensor-hypercontraction factors and Pi(u) of all frequencies, one fragment).
+
 Prints one JSON line.  `roofline` is measured for the dominant kernel of the job,
 the split-int8 GEMM of the amplitude contractions (k_gemm_i8), with HIP events
 inside the library around every launch of a one-worker exchange pass;
@@ -300,6 +303,87 @@ def bench_adjoint(a):
     print(json.dumps(out))
 
 
+def bench_thc(a):
+    """THC route: Cholesky vectors R from the THC factors and Pi(u) of all
+    frequencies (brpa_thc_piu) for one fragment o=36, v=300, NAux=1800 with a
+    4032-point THC grid and 16 frequencies, factors resident in HBM.  The CPU
+    baseline is the numpy oracle of the same job (R, then Pi of every frequency)."""
+    import numpy as np
+    import torch
+
+    import brpa
+    from brpa import synth
+    from oracle import erpa as oerpa, thc as othc
+    _one_gpu(a)
+    o, v, n, g, nf = synth.THC_CONFIG
+    f = synth.make_thc_fragment(o, v, n, g, synth.BASE_SEED)
+    x, w, _ = brpa.freq_grid(f.eocc, f.evir, nf)
+    d = lambda t: torch.from_numpy(t).cuda()  # noqa: E731
+    Xa, Xi, Z = d(f.Xga), d(f.Xgi), d(f.Zgk)
+    Pi = torch.empty((nf, n, n), dtype=torch.float64, device="cuda")
+    R = torch.empty((o * v, n), dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    ctx = brpa.Context(0)
+    run = lambda: ctx.thc_piu(Xa, Xi, Z, f.eocc, f.evir, x, out=Pi, want_R=True,  # noqa: E731
+                              R_out=R)
+    for _ in range(a.warmup):
+        run()
+    torch.cuda.synchronize()
+    st_r = st_pi = 0.0
+    t = time.perf_counter()
+    for _ in range(a.steps):
+        run()
+        st = ctx.last_thc_stage_ms()
+        st_r += st["thc_r"][0]
+        st_pi += st["thc_pi"][0]
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t) / a.steps
+    st_r /= a.steps
+    st_pi /= a.steps
+    e_gpu = ctx.ecrpa_analytic(Pi, x, w)[0]
+    print("# THC: %.3f ms/step, R kernel %.3f ms, Pi SYRK %.3f ms" % (dt * 1e3, st_r, st_pi),
+          file=sys.stderr, flush=True)
+    tc = None
+    dpi = de = None
+    if a.cpu_seconds > 0:
+        t = time.perf_counter()
+        Pis_ref, R_ref = othc.thc_piu(f.Xga.T, f.Xgi.T, f.Zgk.T, f.eocc, f.evir, x)
+        tc = time.perf_counter() - t
+        dpi = float(np.max(np.abs(Pi.cpu().numpy() - Pis_ref)) / np.max(np.abs(Pis_ref)))
+        de = abs(e_gpu - oerpa.ecrpa_analytic(Pis_ref, x, w))
+        del Pis_ref, R_ref
+    r_flops = 2.0 * o * v * n * g
+    pi_flops = syrk_flops(o, v, n, nf)
+    r_tf = r_flops / (st_r * 1e-3) / 1e12
+    pi_tf = pi_flops / (st_pi * 1e-3) / 1e12
+    dom = ("k_syrk_f64", pi_tf, st_pi) if st_pi >= st_r else ("k_thc_r", r_tf, st_r)
+    oth = ("k_thc_r", r_tf, st_r) if dom[0] == "k_syrk_f64" else ("k_syrk_f64", pi_tf, st_pi)
+    out = {"metric": "THC-route Pi(u) build (R from the THC factors, then Pi of all 16 "
+                     "frequencies) of one fragment o=36, v=300, NAux=1800, NGridTHC=4032",
+           "value": round(dt * 1e3, 3), "unit": "ms", "n_gpus": 1, "steps": a.steps,
+           "warmup": a.warmup, "ms_per_step": round(dt * 1e3, 3), "higher_is_better": False,
+           "scaling": "strong", "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic (seeded PCG64, brpa.synth.make_thc_fragment)",
+           "config": {"workload": "THC fragment o=36, v=300, NAux=1800, NGridTHC=4032, "
+                                  "16 frequencies", "precision": "fp64"},
+           "stage_ms": {"thc_r": round(st_r, 3), "thc_pi": round(st_pi, 3)},
+           "E_RPA_from_Pi": e_gpu, "max_rel_dPi_vs_cpu_oracle": dpi,
+           "abs_dE_vs_cpu_oracle_Ha": de,
+           "roofline": {"bound": "mfma", "achieved": round(dom[1], 3), "peak": PEAK_FP64_TFLOPS,
+                        "unit": "TFLOP/s", "frac": round(dom[1] / PEAK_FP64_TFLOPS, 4),
+                        "traffic": None, "kernel": dom[0], "kernel_ms_per_step": round(dom[2], 3)},
+           "roofline_other": {"kernel": oth[0], "achieved": round(oth[1], 3),
+                              "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                              "frac": round(oth[1] / PEAK_FP64_TFLOPS, 4),
+                              "kernel_ms_per_step": round(oth[2], 3)},
+           "algorithmic_flop": {"thc_r": r_flops, "thc_pi": pi_flops},
+           "cpu_baseline": None if tc is None else {
+               "value": round(tc * 1e3, 1), "unit": "ms", "cores": _cpu_threads(), "kind": "port",
+               "sample": "the same fragment, R and Pi of all 16 frequencies by the numpy FP64 "
+                         "oracle, one evaluation"}}
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -321,13 +405,13 @@ def main():
     ap.add_argument("--exchange-workers", type=int, default=2,
                     help="worker threads of brpa_exchange_batch (fragments in flight at once)")
     ap.add_argument("--workload", default="batch",
-                    choices=["batch", "single", "adjoint", "sosex"],
+                    choices=["batch", "single", "adjoint", "sosex", "thc"],
                     help="batch: the 256-fragment E_RPA + exchange batch (default); single: "
                          "E_RPA of one o=36, v=300, NAux=1800 fragment in every form; "
                          "adjoint: dE/dR and dE/d eps of one o=48, v=400, NAux=2400 "
                          "fragment with a finite-difference check; sosex: the exchange "
                          "contraction of one o=64, v=600, NAux=3200 fragment, FP64 vs "
-                         "split-int8")
+                         "split-int8; thc: R and Pi(u) of one fragment from THC factors")
     ap.add_argument("--fd-entries", type=int, default=64,
                     help="random R entries in the adjoint workload's finite-difference check")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -351,6 +435,8 @@ def main():
         return bench_single(a)
     if a.workload == "adjoint":
         return bench_adjoint(a)
+    if a.workload == "thc":
+        return bench_thc(a)
     _one_gpu(a)
 
     import numpy as np

@@ -322,13 +322,14 @@ class Context:
         return A, V
 
     def thc_piu(self, Xga, Xgi, Zgk, eocc, evir, freqs, prefactor=4.0, PiU=None,
-                want_R=False, device=False):
+                want_R=False, device=False, out=None, R_out=None):
         """Pi(u) of every frequency from the THC factors (rpa_THC_PiU_BlockAlgo,
         rpa_THC.f90:285-339).  Xga (ngrid, nvir), Xgi (ngrid, nocc), Zgk (ngrid,
         naux) are Fortran-ordered arrays, i.e. C arrays of shape (nvir, ngrid),
         (nocc, ngrid), (naux, ngrid) holding X(g, a) at [a][g].  PiU: an
         existing (nfreqs, naux, naux) array to add to (second spin block), else
-        a new one is returned.  want_R: also return R[ai][P] (ov, naux)."""
+        a new one is returned (or `out`, overwritten).  want_R: also return
+        R[ai][P] (ov, naux), into `R_out` when given."""
         nvir, ngrid = Xga.shape
         nocc, naux = Xgi.shape[0], Zgk.shape[0]
         if Xgi.shape[1] != ngrid or Zgk.shape[1] != ngrid:
@@ -339,6 +340,8 @@ class Context:
             raise ValueError("shape mismatch")
         nf = freqs.size
         accumulate = PiU is not None
+        if accumulate and out is not None:
+            raise ValueError("give PiU (add to it) or out (overwrite it), not both")
         torch_out = device or _is_torch(Xga)
         if torch_out:
             import torch
@@ -347,10 +350,14 @@ class Context:
         else:
             new = lambda *shape: np.zeros(shape)  # noqa: E731
         if PiU is None:
-            PiU = new(nf, naux, naux)
-        elif tuple(PiU.shape) != (nf, naux, naux):
+            PiU = out if out is not None else new(nf, naux, naux)
+        if tuple(PiU.shape) != (nf, naux, naux):
             raise ValueError("PiU shape mismatch")
-        R = new(nocc * nvir, naux) if want_R else None
+        R = None
+        if want_R:
+            R = R_out if R_out is not None else new(nocc * nvir, naux)
+            if tuple(R.shape) != (nocc * nvir, naux):
+                raise ValueError("R shape mismatch")
         self._sync_stream(Xga, Xgi, Zgk, PiU)
         self._check(self._lib.brpa_thc_piu(self._h, _ptr(PiU), _ptr(R), _ptr(Xga), _ptr(Xgi),
                                            _ptr(Zgk), ngrid, nocc, nvir, naux, _ptr(eocc),
