@@ -428,6 +428,44 @@ class Context:
                                          int(bool(s2_operator))))
         return float(e[0])
 
+    def mbpt3_1(self, V, A, R, eocc, evir, precision="fp64"):
+        """Third-order exchange MBPT3-1 (rpa_CC_Exchange_MBPT3_1): Lyapunov
+        equation for X = V^T (dT/dlambda) V, then -sum (aj|bi) (V X V^T)(ai,bj)."""
+        nai, naux = R.shape
+        eocc, evir = np.ascontiguousarray(eocc, np.float64), np.ascontiguousarray(evir, np.float64)
+        if not _is_torch(A):
+            A = np.ascontiguousarray(A, np.float64)
+        nt = A.numel() if _is_torch(A) else A.size
+        if nai != eocc.size * evir.size or tuple(V.shape) != (nai, nt):
+            raise ValueError("shape mismatch")
+        e = np.zeros(1)
+        self._sync_stream(V, R, A)
+        self._check(self._lib.brpa_mbpt3_1(self._h, _prec(precision), _ptr(e), _ptr(V), _ptr(A),
+                                           nt, _ptr(R), naux, _ptr(eocc), _ptr(evir), eocc.size,
+                                           evir.size))
+        return float(e[0])
+
+    def mbpt3_2(self, R, V, A, nocc, nvir):
+        """Third-order exchange MBPT3-2 (rpa_CC_Exchange_MBPT3_2):
+        2 sum_{c,k,l} (sum_{a,i} R(l, ai) T(ak, ci))^2."""
+        nai, naux = R.shape
+        if not _is_torch(A):
+            A = np.ascontiguousarray(A, np.float64)
+        nt = A.numel() if _is_torch(A) else A.size
+        if nai != nocc * nvir or tuple(V.shape) != (nai, nt):
+            raise ValueError("shape mismatch")
+        e = np.zeros(1)
+        self._sync_stream(V, R, A)
+        self._check(self._lib.brpa_mbpt3_2(self._h, _ptr(e), _ptr(R), _ptr(V), _ptr(A), nocc,
+                                           nvir, naux, nt))
+        return float(e[0])
+
+    def last_mbpt3_stage_ms(self):
+        """{stage: (ms, launches)} of the last mbpt3_1 / mbpt3_2 call."""
+        names = {"mbpt3": _lib.STAGE_MBPT3, "sosex": _lib.STAGE_SOSEX}
+        return {k: (self._lib.brpa_last_stage_ms(self._h, v),
+                    self._lib.brpa_last_stage_launches(self._h, v)) for k, v in names.items()}
+
     def ecrpa_grad(self, R, eocc, evir, freqs, weights):
         """(dE/dR, dE/d eps_occ, dE/d eps_vir) of the FP64 E_RPA, grid held fixed.
 

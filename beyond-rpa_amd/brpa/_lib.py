@@ -12,7 +12,7 @@ STAGE_PI, STAGE_SPLIT, STAGE_CHOL, STAGE_AUX = range(4)
 # amplitude-eigenvalue cutoff rules (brpa_t2_cutoff)
 T2_CUTOFF_EIG, T2_CUTOFF_EIG_DIV_MAXEIG, T2_CUTOFF_EIG_DIV_NELECTRON, T2_CUTOFF_SUM_REJECTED = range(4)
 STAGE_T2_PI, STAGE_T2_IOMEGA, STAGE_T2_BASIS, STAGE_T2_TIT, STAGE_T2_CORE, STAGE_SOSEX = range(4, 10)
-STAGE_THC_R, STAGE_THC_PI = 10, 11
+STAGE_THC_R, STAGE_THC_PI, STAGE_MBPT3 = 10, 11, 12
 
 _i64 = C.c_int64
 _dp = C.POINTER(C.c_double)
@@ -57,6 +57,9 @@ SIGNATURES = {
     "brpa_t2_copyout_ov": (C.c_int, [_vp, _vp, _vp]),
     "brpa_thc_piu": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _i64, _i64, _i64, _vp, _vp,
                                _vp, _i64, C.c_double, C.c_int]),
+    "brpa_mbpt3_1": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, _i64, _vp, _i64, _vp, _vp, _i64,
+                               _i64]),
+    "brpa_mbpt3_2": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _i64, _i64, _i64, _i64]),
     "brpa_exchange_batch": (C.c_int, [_vp, C.c_int, _i64, C.POINTER(brpa_frag_t), C.c_double,
                                       C.c_double, C.c_double, C.c_uint64, C.c_double, C.c_int,
                                       C.c_int, _vp, _vp]),

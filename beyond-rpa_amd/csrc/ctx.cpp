@@ -191,6 +191,7 @@ int brpa_ctx_destroy(brpa_ctx *ctx) {
     if (ctx->pin) (void)hipHostFree(ctx->pin);
     if (ctx->sync) (void)hipFree(ctx->sync);
     t2_release(ctx);
+    mbpt3_release(ctx);
     for (hipEvent_t e : ctx->gev) (void)hipEventDestroy(e);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);

@@ -41,6 +41,10 @@ struct brpa_ctx {
     int64_t t2_nt = -1, t2_nai = 0;
     // the kept pair is the factored core (A' = -1, V' = T L), not eigenpairs
     bool t2_factored = false;
+    // MBPT3-1: the transformed amplitudes V'' and weights B'' handed to the SOSEX
+    // contraction (kept apart from the workspace, which that call may regrow)
+    double *x_V = nullptr, *x_B = nullptr;
+    size_t x_V_cap = 0, x_B_cap = 0;
     // pinned host staging of the batched E_RPA path (uploads and results)
     char *pin = nullptr;
     size_t pin_cap = 0;
@@ -77,6 +81,7 @@ int sync_reserve(brpa_ctx *ctx, size_t n);
 void graphs_release(brpa_ctx *ctx);
 
 void t2_release(brpa_ctx *ctx);
+void mbpt3_release(brpa_ctx *ctx);
 
 // Records the start/end of a stage on the context stream; stage_end adds the
 // elapsed time once the stream has been synchronised (stage_collect).

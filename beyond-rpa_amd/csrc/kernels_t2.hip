@@ -308,3 +308,43 @@ int launch_transpose(brpa_ctx *ctx, const double *in, int64_t m, int64_t n, doub
     LAUNCH_CHECK(ctx);
     return BRPA_OK;
 }
+
+// out(r, c) = s[c] * in(r, c) for an m x n column-major matrix (out may alias in)
+__global__ void k_col_scale(double *out, const double *in, const double *__restrict__ s, int64_t m,
+                            int64_t n) {
+    int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= m * n) return;
+    out[e] = s[e / m] * in[e];
+}
+
+int launch_col_scale(brpa_ctx *ctx, double *out, const double *in, const double *s, int64_t m,
+                     int64_t n) {
+    if (m * n <= 0) return BRPA_OK;
+    hipLaunchKernelGGL(k_col_scale, dim3(nblocks(m * n)), dim3(256), 0, ctx->stream, out, in, s, m,
+                       n);
+    LAUNCH_CHECK(ctx);
+    return BRPA_OK;
+}
+
+// The two matrices of rpa_CC_Exchange_MBPT3_1 (rpa/rpa_CC_Exchange.f90:248-262),
+// n x n column-major, element (m, k) at m + n k:
+//   Am(m, k) = (A_m + A_k) D(m, k),   Bm(m, k) = D(m, k) + 2 U(m, k) + 4 U(m, k) A_k
+__global__ void k_mbpt3_ab(const double *__restrict__ D, const double *__restrict__ U,
+                           const double *__restrict__ A, int64_t n, double *__restrict__ Am,
+                           double *__restrict__ Bm) {
+    int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= n * n) return;
+    const int64_t k = e / n, m = e - k * n;
+    const double d = D[e], u = U[e], ak = A[k];
+    Am[e] = A[m] * d + d * ak;
+    Bm[e] = d + 2.0 * u + 4.0 * u * ak;
+}
+
+int launch_mbpt3_ab(brpa_ctx *ctx, const double *D, const double *U, const double *A, int64_t n,
+                    double *Am, double *Bm) {
+    if (n <= 0) return BRPA_OK;
+    hipLaunchKernelGGL(k_mbpt3_ab, dim3(nblocks(n * n)), dim3(256), 0, ctx->stream, D, U, A, n, Am,
+                       Bm);
+    LAUNCH_CHECK(ctx);
+    return BRPA_OK;
+}

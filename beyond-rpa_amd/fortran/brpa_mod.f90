@@ -9,6 +9,8 @@
 !   rpa_CC_T2               rpa/rpa_CC_Doubles.f90:43-77
 !   rpa_THC_CC_T2           rpa/rpa_CC_Doubles.f90:645-984 (with rpa_T2_EigRPA 987-1009)
 !   rpa_THC_PiU             rpa/rpa_THC.f90:215-282
+!   rpa_CC_Exchange_MBPT3_1 rpa/rpa_CC_Exchange.f90:232-270
+!   rpa_CC_Exchange_MBPT3_2 rpa/rpa_CC_Exchange.f90:381-393
 !   rpa_CC_Exchange_SOSEX   rpa/rpa_CC_Exchange.f90:36-97
 !
 ! Built with -fdefault-integer-8 like every reference build (default INTEGER
@@ -29,6 +31,7 @@ module brpa_mod
       public :: brpa_init, brpa_finalize
       public :: rpa_GPiUG_MO, rpa_CC_EcRPA_Analytic, rpa_CC_T2, rpa_CC_Exchange_SOSEX
       public :: rpa_THC_CC_T2, rpa_THC_PiU
+      public :: rpa_CC_Exchange_MBPT3_1, rpa_CC_Exchange_MBPT3_2
       ! T2CutoffType values (rpa/rpa_definitions.f90:139-142)
       integer, parameter, public :: RPA_T2_CUTOFF_EIG = 0
       integer, parameter, public :: RPA_T2_CUTOFF_EIG_DIV_MAXEIG = 1
@@ -114,6 +117,23 @@ module brpa_mod
                   integer(c_int64_t), value :: ngrid, nocc, nvir, naux, nfreqs
                   real(c_double), value :: pref
                   integer(c_int), value :: accum
+            end function
+            integer(c_int) function c_mbpt3_1(c, prec, e, v, a, nt, r, naux, eocc, evir, nocc, nvir) &
+                  bind(C, name="brpa_mbpt3_1")
+                  import :: c_int, c_ptr, c_int64_t, c_double
+                  type(c_ptr), value :: c
+                  integer(c_int), value :: prec
+                  real(c_double) :: e
+                  real(c_double) :: v(*), a(*), r(*), eocc(*), evir(*)
+                  integer(c_int64_t), value :: nt, naux, nocc, nvir
+            end function
+            integer(c_int) function c_mbpt3_2(c, e, r, v, a, nocc, nvir, naux, nt) &
+                  bind(C, name="brpa_mbpt3_2")
+                  import :: c_int, c_ptr, c_int64_t, c_double
+                  type(c_ptr), value :: c
+                  real(c_double) :: e
+                  real(c_double) :: r(*), v(*), a(*)
+                  integer(c_int64_t), value :: nocc, nvir, naux, nt
             end function
             integer(c_int) function c_t2_eig_rpa(c, r, naux, eig) bind(C, name="brpa_t2_eig_rpa")
                   import :: c_int, c_ptr, c_int64_t, c_double
@@ -317,6 +337,55 @@ contains
             if (NVecsT2 > 0) call check(c_t2_copyout(ctx, A, V), "rpa_CC_T2")
             if (.false.) print *, PiUEigenvecs(1,1,1), PiUEigenvals(1,1), OccBounds, VirtBounds, MaxBatchDim
       end subroutine rpa_CC_T2
+
+
+      subroutine rpa_CC_Exchange_MBPT3_1(EcExchange, Vkai, Ak, NVecsT2, Rkai, NVecsPiU, &
+            OccEnergies, VirtEnergies, NOcc, NVirt, Precision)
+            !
+            ! Third-order exchange MBPT3-1 (rpa_CC_Exchange.f90:232-270), same
+            ! argument list; the optional trailing Precision=2 selects the split-int8
+            ! form of the final contraction (default FP64).
+            !
+            real(F64), intent(out)                                          :: EcExchange
+            real(F64), dimension(NVecsT2, NVirt, NOcc), intent(in)          :: Vkai
+            real(F64), dimension(NVecsT2), intent(in)                       :: Ak
+            integer, intent(in)                                             :: NVecsT2
+            real(F64), dimension(NVecsPiU, NVirt, NOcc), intent(in)         :: Rkai
+            integer, intent(in)                                             :: NVecsPiU
+            real(F64), dimension(:), intent(in), contiguous                 :: OccEnergies
+            real(F64), dimension(:), intent(in), contiguous                 :: VirtEnergies
+            integer, intent(in)                                             :: NOcc
+            integer, intent(in)                                             :: NVirt
+            integer, intent(in), optional                                   :: Precision
+            integer(c_int) :: prec
+
+            call brpa_init()
+            prec = 0_c_int
+            if (present(Precision)) prec = int(Precision, c_int)
+            call check(c_mbpt3_1(ctx, prec, EcExchange, Vkai, Ak, int(NVecsT2, c_int64_t), Rkai, &
+                  int(NVecsPiU, c_int64_t), OccEnergies, VirtEnergies, int(NOcc, c_int64_t), &
+                  int(NVirt, c_int64_t)), "rpa_CC_Exchange_MBPT3_1")
+      end subroutine rpa_CC_Exchange_MBPT3_1
+
+
+      subroutine rpa_CC_Exchange_MBPT3_2(EcExchange, Rkai, Vkai, Ak, NOcc, NVirt, NVecsPiU, NVecsT2)
+            !
+            ! Third-order exchange MBPT3-2 (rpa_CC_Exchange.f90:381-393), same argument list.
+            !
+            real(F64), intent(out)                                  :: EcExchange
+            real(F64), dimension(NVecsPiU, NVirt, NOcc), intent(in) :: Rkai
+            real(F64), dimension(NVecsT2, NVirt, NOcc), intent(in)  :: Vkai
+            real(F64), dimension(NVecsT2), intent(in)               :: Ak
+            integer, intent(in)                                     :: NOcc
+            integer, intent(in)                                     :: NVirt
+            integer, intent(in)                                     :: NVecsPiU
+            integer, intent(in)                                     :: NVecsT2
+
+            call brpa_init()
+            call check(c_mbpt3_2(ctx, EcExchange, Rkai, Vkai, Ak, int(NOcc, c_int64_t), &
+                  int(NVirt, c_int64_t), int(NVecsPiU, c_int64_t), int(NVecsT2, c_int64_t)), &
+                  "rpa_CC_Exchange_MBPT3_2")
+      end subroutine rpa_CC_Exchange_MBPT3_2
 
 
       subroutine rpa_THC_PiU(PiU, Rkai, Xga, Xgi, Zgk, NOcc, NVirt, Freqs, &

@@ -31,7 +31,7 @@ program shim_check
       real(F64) :: CommonThresh
       integer :: NVecsTHC
       integer :: NOccs(2), NVirts(2), OccBounds(2, 2), VirtBounds(2, 2)
-      real(F64) :: EcRPA, EcFused, EcSplit, Ex
+      real(F64) :: EcRPA, EcFused, EcSplit, Ex, E31, E32
       character(len=512) :: fin, fout
 
       call get_command_argument(1, fin)
@@ -68,6 +68,8 @@ program shim_check
       call rpa_CC_T2(A, V, NVecsT2, Q, W, Rkai, NVecs, NOcc, NVirt, [1, NOcc], [1, NVirt], &
             Freqs, Weights, NFreqs, 1.0_F64, Eo, Ev, 1.0E-10_F64, 0, 100)
       call rpa_CC_Exchange_SOSEX(Ex, 1.0_F64, V, A, Rkai, NOcc, NVirt, NVecs, NVecsT2, .false.)
+      call rpa_CC_Exchange_MBPT3_1(E31, V, A, NVecsT2, Rkai, NVecs, Eo, Ev, NOcc, NVirt)
+      call rpa_CC_Exchange_MBPT3_2(E32, Rkai, V, A, NOcc, NVirt, NVecs, NVecsT2)
 
       ! THC route: supersystem call with the EIG/MAXEIG rule and the smooth step
       CommonThresh = -1.0_F64
@@ -85,6 +87,8 @@ program shim_check
       do k = 1, NVecsT2
             write(io, '(es25.17)') A(k)
       end do
+      write(io, '(es25.17)') E31
+      write(io, '(es25.17)') E32
       write(io, '(i0)') NVecsTHC
       write(io, '(es25.17)') CommonThresh
       write(io, '(es25.17)') sum(EigRPA)

@@ -86,7 +86,7 @@ double brpa_last_kernel_ms(const brpa_ctx *ctx);
 
 /* Device time (ms) of one stage of the last E_RPA call (stages 0-3), of
  * the last amplitude / SOSEX / exchange-batch call (stages 4-9) or of the last
- * THC Pi(u) call (stages 10-11), summed over
+ * THC Pi(u) call (stages 10-11) or MBPT3 call (stage 12, with 9), summed over
  * its launches (the exchange stages include the device idle time inside the
  * stage), and the number of launches.  Stages: */
 enum {
@@ -104,7 +104,9 @@ enum {
     BRPA_STAGE_SOSEX = 9,     /* SOSEX contraction (slicing and pair products) */
     BRPA_STAGE_THC_R = 10,    /* THC producer of R (brpa_thc_piu) */
     BRPA_STAGE_THC_PI = 11,   /* Pi(u) of all frequencies from that R (brpa_thc_piu) */
-    BRPA_NSTAGES = 12
+    BRPA_STAGE_MBPT3 = 12,    /* MBPT3-1: U, D, Lyapunov solve and factor of X (its final
+                                 contraction is BRPA_STAGE_SOSEX); MBPT3-2: all products */
+    BRPA_NSTAGES = 13
 };
 double brpa_last_stage_ms(const brpa_ctx *ctx, int stage);
 int brpa_last_stage_launches(const brpa_ctx *ctx, int stage);
@@ -281,6 +283,30 @@ int brpa_t2_copyout_ov(brpa_ctx *ctx, double *A, double *V);
 int brpa_sosex(brpa_ctx *ctx, int precision, double *ec_exchange, double alpha, const double *V,
                const double *A, const double *R, int64_t nocc, int64_t nvir, int64_t naux,
                int64_t nvecs_t2, int s2_operator);
+
+/* Third-order exchange MBPT3-1, replacing rpa_CC_Exchange_MBPT3_1
+ * (rpa/rpa_CC_Exchange.f90:232-270, with rpa_CC_Ukl 178-207, rpa_CC_Dkl
+ * 210-229, rpa_CC_TrXK 273-336 and real_Lyapunov, linear-algebra/
+ * real_linalg.f90:125-171), argument order as there:
+ *   U = (R^T V)^T (R^T V), D = V^T diag(e_a - e_i) V,
+ *   Bm^T X + X Bm = Am with Am = (A_m + A_n) D, Bm = D + 2U + 4U diag(A),
+ *   EcExchange = -sum_{aibj} (aj|bi) (V X V^T)(ai, bj).
+ * V: C V[ai][mu] (Fortran Vkai(NVecsT2, v, o)), R: C R[ai][P]; host or device.
+ * The Lyapunov equation is solved by a scaled matrix-sign iteration (no
+ * Schur form); X is factored (-X = U'^T U', or its eigenpairs) and the trace
+ * is the SOSEX contraction of brpa_sosex in `precision` (BRPA_FP64 or
+ * BRPA_SPLIT_I8).  OccEnergies/VirtEnergies: the orbital energies the
+ * caller's route uses (canonical or semicanonical, rpa_CC.f90:977-987). */
+int brpa_mbpt3_1(brpa_ctx *ctx, int precision, double *ec_exchange, const double *V,
+                 const double *A, int64_t nvecs_t2, const double *R, int64_t naux,
+                 const double *eocc, const double *evir, int64_t nocc, int64_t nvir);
+/* Third-order exchange MBPT3-2, replacing rpa_CC_Exchange_MBPT3_2
+ * (rpa/rpa_CC_Exchange.f90:381-393 with rpa_CC_TrW2 339-378), argument order
+ * as there: EcExchange = 2 sum_{c,k,l} (sum_{a,i} R(l, ai) T(ak, ci))^2 with
+ * T = V diag(A) V^T.  FP64 (rocBLAS strided-batched GEMMs over T~_k(c, ia) =
+ * T(ci, ak), then M_k = R^T T~_k^T). */
+int brpa_mbpt3_2(brpa_ctx *ctx, double *ec_exchange, const double *R, const double *V,
+                 const double *A, int64_t nocc, int64_t nvir, int64_t naux, int64_t nvecs_t2);
 
 /* Exchange correction of many fragments: brpa_t2_compute followed by brpa_sosex
  * on the device-resident amplitudes, per fragment (the reference's

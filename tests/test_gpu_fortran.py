@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 from brpa import synth
-from oracle import erpa, grid, sosex, t2, thc
+from oracle import erpa, grid, mbpt3, sosex, t2, thc
 
 pytestmark = pytest.mark.gpu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -40,8 +40,14 @@ def test_fortran_shim_matches_oracle(tmp_path):
     assert nt == len(A_ref)
     assert np.max(np.abs(A_f - A_ref)) <= 1e-9 * np.max(np.abs(A_ref))
     assert abs(ex - sosex.sosex(f.R, V_ref, A_ref, o, v)) <= 1e-8
+    # rpa_CC_Exchange_MBPT3_1 / _2 on the same amplitudes
+    e31, e32 = float(vals[5 + nt]), float(vals[6 + nt])
+    e31_ref = mbpt3.mbpt3_1(f.R, V_ref, A_ref, o, v, f.eocc, f.evir)
+    e32_ref = mbpt3.mbpt3_2(f.R, V_ref, A_ref, o, v)
+    assert abs(e31 - e31_ref) <= 1e-8 * abs(e31_ref)
+    assert abs(e32 - e32_ref) <= 1e-8 * abs(e32_ref)
     # rpa_THC_CC_T2: EIG/MAXEIG rule (1e-3), smooth step 0.1, supersystem
-    k = 5 + nt
+    k = 7 + nt
     nth, common, esum = int(vals[k]), float(vals[k + 1]), float(vals[k + 2])
     A_th = np.array([float(s) for s in vals[k + 3:k + 3 + nth]])
     vnorm, t11 = float(vals[k + 3 + nth]), float(vals[k + 4 + nth])
