@@ -18,7 +18,8 @@ rate is 1 / (its E_RPA time per fragment + the exchange time per fragment in
 its own arithmetic).
 
 Other workloads (--workload): single, adjoint, sosex, and thc (R from
-tensor-hypercontraction factors and Pi(u) of all frequencies, one fragment).
+tensor-hypercontraction factors and Pi(u) of all frequencies, one fragment),
+mbpt3 (the MBPT3-1 and MBPT3-2 exchange corrections of the large fragment).
 
 Prints one JSON line.  `roofline` is measured for the dominant kernel of the job,
 the split-int8 GEMM of the amplitude contractions (k_gemm_i8), with HIP events
@@ -384,6 +385,128 @@ def bench_thc(a):
     print(json.dumps(out))
 
 
+def bench_mbpt3(a):
+    """Third-order exchange corrections MBPT3-1 and MBPT3-2 of the large SOSEX
+    fragment (o=64, v=600, NAux=3200, 16 frequencies, amplitudes with NGuess =
+    3 NAux built once in FP64 on the GPU).  CPU baseline: the oracle's loops
+    on a bounded sample (a subset of occupied pairs / (c, k) pairs and a
+    smaller Lyapunov solve), scaled to the whole job by operation counts."""
+    import math
+
+    import numpy as np
+    import torch
+
+    import brpa
+    from brpa import synth
+    from oracle import mbpt3 as omb
+    _one_gpu(a)
+    o, v, n = 64, 600, 3200
+    f = synth.make_fragment(o, v, n, synth.BASE_SEED)
+    x, w, _ = brpa.freq_grid(f.eocc, f.evir, 16)
+    Rd = torch.from_numpy(f.R).cuda()
+    ctx = brpa.Context(0)
+    A, V = ctx.t2(Rd, f.eocc, f.evir, x, w, nguess=math.ceil(3.0 * n), seed=1, device=True)
+    nt = A.numel()
+    res = {}
+    for name, fn in (("mbpt3_1_split_i8", lambda: ctx.mbpt3_1(V, A, Rd, f.eocc, f.evir,
+                                                              precision="split_i8")),
+                     ("mbpt3_1_fp64", lambda: ctx.mbpt3_1(V, A, Rd, f.eocc, f.evir)),
+                     ("mbpt3_2_fp64", lambda: ctx.mbpt3_2(Rd, V, A, o, v))):
+        for _ in range(a.warmup):
+            fn()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        st_core = st_sx = 0.0
+        iters = 0
+        for _ in range(a.steps):
+            e = fn()
+            st = ctx.last_mbpt3_stage_ms()
+            st_core += st["mbpt3"][0]
+            st_sx += st["sosex"][0]
+            iters = st["mbpt3"][1]
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t) / a.steps
+        res[name] = {"E": e, "ms": dt * 1e3, "core_ms": st_core / a.steps,
+                     "contraction_ms": st_sx / a.steps, "sign_iterations": iters}
+        print("# %s: E %.10f  %.1f ms/step (core %.1f ms, contraction %.1f ms, iters %d)"
+              % (name, e, dt * 1e3, st_core / a.steps, st_sx / a.steps, iters), file=sys.stderr,
+              flush=True)
+    K = o * v
+    r1 = res["mbpt3_1_split_i8"]
+    # algorithmic FP64 flop of the MBPT3-1 core: W, U, D, then per sign step getrf +
+    # getri (2 n^3) and two GEMMs (4 n^3)
+    core_flops = 2.0 * K * n * nt + 2.0 * n * nt * nt + 2.0 * K * nt * nt \
+        + 6.0 * nt ** 3 * max(r1["sign_iterations"], 1)
+    m2_flops = 2.0 * o * o * v * v * (nt + n)
+    m2 = res["mbpt3_2_fp64"]
+    m2_tf = m2_flops / (m2["core_ms"] * 1e-3) / 1e12
+    core_tf = core_flops / (r1["core_ms"] * 1e-3) / 1e12
+    # CPU baseline on a bounded sample (one thread pool of the host)
+    cpu = None
+    if a.cpu_seconds > 0:
+        Vh = V.cpu().numpy()
+        Ah = A.cpu().numpy()
+        rng = np.random.default_rng(0)
+        t = time.perf_counter()
+        # MBPT3-2: the reference loop over (c, k) on 2 of the 600 virtual indices
+        R3 = f.R
+        V3 = Vh.reshape(o, v, nt)
+        cs = rng.choice(v, 2, replace=False)
+        tr = 0.0
+        for c in cs:
+            Vli = Ah[:, None] * V3[:, c, :].T
+            for k in range(o):
+                Wai = V3[k] @ Vli
+                Wm = R3.T @ Wai.T.ravel()
+                tr += float(np.sum(Wm * Wm))
+        t_m2 = (time.perf_counter() - t) * v / len(cs)
+        # MBPT3-1: TrXK pair loop on 2 of the o(o+1)/2 pairs, Lyapunov at nt/4 (n^3)
+        t = time.perf_counter()
+        Rr = f.R.reshape(o, v, n)
+        npair = o * (o + 1) // 2
+        for (i, j) in ((1, 0), (5, 2)):
+            Wij = Rr[i].T @ V3[j]
+            Wji = Rr[j].T @ V3[i]
+            np.sum(Wji.T @ Wij)
+        t_pairs = (time.perf_counter() - t) * npair / 2
+        m = nt // 4
+        B = rng.standard_normal((m, m)) / math.sqrt(m) + 4.0 * np.eye(m)
+        S = rng.standard_normal((m, m))
+        t = time.perf_counter()
+        omb.lyapunov(B, S + S.T)
+        t_lyap = (time.perf_counter() - t) * 64.0
+        cpu = {"mbpt3_1_ms": round((t_pairs + t_lyap) * 1e3, 1), "mbpt3_2_ms": round(t_m2 * 1e3, 1)}
+    total = r1["ms"] + m2["ms"]
+    out = {"metric": "MBPT3-1 + MBPT3-2 exchange time of one fragment o=64, v=600, NAux=3200, "
+                     "NT2=%d (split-int8 final contraction of MBPT3-1)" % nt,
+           "value": round(total, 3), "unit": "ms", "n_gpus": 1, "steps": a.steps,
+           "warmup": a.warmup, "ms_per_step": round(total, 3), "higher_is_better": False,
+           "scaling": "strong", "vs_baseline": None,
+           "dtype": "f64 (Lyapunov, MBPT3-2) + i8 split (MBPT3-1 contraction)",
+           "data": "synthetic (seeded PCG64, brpa.synth)",
+           "config": {"workload": "MBPT3 exchange of one fragment o=64, v=600, NAux=3200, 16 "
+                                  "frequencies, NT2=%d" % nt},
+           "forms": {k: {kk: (round(vv, 3) if isinstance(vv, float) and kk != "E" else vv)
+                         for kk, vv in d.items()} for k, d in res.items()},
+           "abs_dE_mbpt3_1_split_vs_fp64_Ha": abs(res["mbpt3_1_split_i8"]["E"] -
+                                                  res["mbpt3_1_fp64"]["E"]),
+           "roofline": {"bound": "mfma", "achieved": round(m2_tf, 3), "peak": PEAK_FP64_TFLOPS,
+                        "unit": "TFLOP/s", "frac": round(m2_tf / PEAK_FP64_TFLOPS, 4),
+                        "traffic": None, "kernel": "MBPT3-2 products (rocBLAS dgemm)",
+                        "kernel_ms_per_step": round(m2["core_ms"], 3)},
+           "roofline_mbpt3_1_core": {"achieved": round(core_tf, 3), "peak": PEAK_FP64_TFLOPS,
+                                     "unit": "TFLOP/s",
+                                     "frac": round(core_tf / PEAK_FP64_TFLOPS, 4),
+                                     "ms": round(r1["core_ms"], 3)},
+           "cpu_baseline": None if cpu is None else {
+               "value": round(cpu["mbpt3_1_ms"] + cpu["mbpt3_2_ms"], 1), "unit": "ms",
+               "cores": _cpu_threads(), "kind": "port", "parts": cpu,
+               "sample": "numpy oracle loops: MBPT3-2 over 2 of 600 virtual indices (x300), "
+                         "the MBPT3-1 pair loop over 2 of %d pairs (x%d/2), the Lyapunov "
+                         "solve at NT2/4 (x64, n^3)" % (o * (o + 1) // 2, o * (o + 1) // 2)}}
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -405,7 +528,7 @@ def main():
     ap.add_argument("--exchange-workers", type=int, default=2,
                     help="worker threads of brpa_exchange_batch (fragments in flight at once)")
     ap.add_argument("--workload", default="batch",
-                    choices=["batch", "single", "adjoint", "sosex", "thc"],
+                    choices=["batch", "single", "adjoint", "sosex", "thc", "mbpt3"],
                     help="batch: the 256-fragment E_RPA + exchange batch (default); single: "
                          "E_RPA of one o=36, v=300, NAux=1800 fragment in every form; "
                          "adjoint: dE/dR and dE/d eps of one o=48, v=400, NAux=2400 "
@@ -437,6 +560,8 @@ def main():
         return bench_adjoint(a)
     if a.workload == "thc":
         return bench_thc(a)
+    if a.workload == "mbpt3":
+        return bench_mbpt3(a)
     _one_gpu(a)
 
     import numpy as np

@@ -45,6 +45,7 @@ struct brpa_ctx {
     // contraction (kept apart from the workspace, which that call may regrow)
     double *x_V = nullptr, *x_B = nullptr;
     size_t x_V_cap = 0, x_B_cap = 0;
+    bool mbpt3_chol = false;   // the last MBPT3-1 factored -X by Cholesky (else eigenpairs)
     // pinned host staging of the batched E_RPA path (uploads and results)
     char *pin = nullptr;
     size_t pin_cap = 0;

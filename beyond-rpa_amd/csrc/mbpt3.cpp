@@ -208,6 +208,7 @@ int brpa_mbpt3_1(brpa_ctx *ctx, int precision, double *ec_exchange, const double
     // Bm^T X + X Bm = Am by the scaled sign iteration; Am -> 2X, Bm -> I
     const double tol = 1e-12 * std::sqrt((double)n);
     bool conv = false;
+    int iters = 0;
     for (int it = 0; it < 100 && !conv; ++it) {
         BRPA_HIP(ctx, hipMemcpyAsync(Bi, Bm, 8 * n * n, hipMemcpyDeviceToDevice, ctx->stream));
         BRPA_TRY(rbc(ctx, rocsolver_dgetrf(h, (rocblas_int)n, (rocblas_int)n, Bi, (rocblas_int)n,
@@ -238,6 +239,7 @@ int brpa_mbpt3_1(brpa_ctx *ctx, int precision, double *ec_exchange, const double
         double dev = 0.0;
         BRPA_TRY(nrm2(ctx, h, T1, n * n, &dev));
         conv = dev <= tol;
+        iters = it + 1;
     }
     if (!conv)
         return brpa_fail(ctx, BRPA_ERR_SOLVER, "brpa_mbpt3_1: Lyapunov iteration did not converge");
@@ -278,6 +280,8 @@ int brpa_mbpt3_1(brpa_ctx *ctx, int precision, double *ec_exchange, const double
     BRPA_TRY(stage_end(ctx, BRPA_STAGE_MBPT3));
     BRPA_HIP(ctx, hipStreamSynchronize(ctx->stream));
     BRPA_TRY(stage_collect(ctx, BRPA_STAGE_MBPT3));
+    ctx->stage_launches[BRPA_STAGE_MBPT3] = iters;   // reported as the sign-iteration steps
+    ctx->mbpt3_chol = (st == 0);
     // E = -sum (aj|bi) (V'' diag(B'') V''^T)(ai, bj): the SOSEX contraction with alpha = 1
     const bool keep = ctx->stage_keep;
     ctx->stage_keep = true;

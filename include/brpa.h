@@ -108,6 +108,8 @@ enum {
                                  contraction is BRPA_STAGE_SOSEX); MBPT3-2: all products */
     BRPA_NSTAGES = 13
 };
+/* For BRPA_STAGE_MBPT3 after brpa_mbpt3_1 the count is the number of
+ * sign-iteration steps of the Lyapunov solve. */
 double brpa_last_stage_ms(const brpa_ctx *ctx, int stage);
 int brpa_last_stage_launches(const brpa_ctx *ctx, int stage);
 /* Split-int8 GEMM launches (the amplitude contractions of the split form) of the
