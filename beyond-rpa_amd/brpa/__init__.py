@@ -428,6 +428,35 @@ class Context:
                                          int(bool(s2_operator))))
         return float(e[0])
 
+    def ao_mo(self, RG, pairs, Co, Cv, out=None, accumulate=False):
+        """AO -> MO transform of the Cholesky vectors (rpa_CRG / rpa_C).
+
+        RG (npq, nvecs) C array = Fortran RG(nvecs, npq); pairs (npq, 2) int32
+        AO indices of each packed column; Co (nocc, nao) = Fortran (nao, nocc),
+        Cv (nvir, nao).  Returns R[ai][P] (ov, nvecs): `out` when given (added
+        to when accumulate), else a new array on the inputs' side."""
+        npq, nvecs = RG.shape
+        nocc, nao = Co.shape
+        nvir = Cv.shape[0]
+        pairs = np.ascontiguousarray(pairs, np.int32)
+        if pairs.shape != (npq, 2) or Cv.shape[1] != nao:
+            raise ValueError("shape mismatch")
+        if out is None:
+            if accumulate:
+                raise ValueError("accumulate needs out")
+            if _is_torch(RG):
+                import torch
+                out = torch.zeros((nocc * nvir, nvecs), dtype=torch.float64, device=RG.device)
+            else:
+                out = np.zeros((nocc * nvir, nvecs))
+        if tuple(out.shape) != (nocc * nvir, nvecs):
+            raise ValueError("out shape mismatch")
+        self._sync_stream(RG, Co, Cv, out)
+        self._check(self._lib.brpa_ao_mo(self._h, _ptr(out), _ptr(RG), nvecs, npq,
+                                         C.c_void_p(pairs.ctypes.data), _ptr(Co), _ptr(Cv), nao,
+                                         nocc, nvir, int(bool(accumulate))))
+        return out
+
     def mbpt3_1(self, V, A, R, eocc, evir, precision="fp64"):
         """Third-order exchange MBPT3-1 (rpa_CC_Exchange_MBPT3_1): Lyapunov
         equation for X = V^T (dT/dlambda) V, then -sum (aj|bi) (V X V^T)(ai,bj)."""

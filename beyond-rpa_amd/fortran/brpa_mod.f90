@@ -9,6 +9,7 @@
 !   rpa_CC_T2               rpa/rpa_CC_Doubles.f90:43-77
 !   rpa_THC_CC_T2           rpa/rpa_CC_Doubles.f90:645-984 (with rpa_T2_EigRPA 987-1009)
 !   rpa_THC_PiU             rpa/rpa_THC.f90:215-282
+!   rpa_CRG                 rpa/rpa_core_MO.f90:233-301 (one image: NSubsets(2) = 1)
 !   rpa_CC_Exchange_MBPT3_1 rpa/rpa_CC_Exchange.f90:232-270
 !   rpa_CC_Exchange_MBPT3_2 rpa/rpa_CC_Exchange.f90:381-393
 !   rpa_CC_Exchange_SOSEX   rpa/rpa_CC_Exchange.f90:36-97
@@ -31,7 +32,7 @@ module brpa_mod
       public :: brpa_init, brpa_finalize
       public :: rpa_GPiUG_MO, rpa_CC_EcRPA_Analytic, rpa_CC_T2, rpa_CC_Exchange_SOSEX
       public :: rpa_THC_CC_T2, rpa_THC_PiU
-      public :: rpa_CC_Exchange_MBPT3_1, rpa_CC_Exchange_MBPT3_2
+      public :: rpa_CC_Exchange_MBPT3_1, rpa_CC_Exchange_MBPT3_2, rpa_CRG
       ! T2CutoffType values (rpa/rpa_definitions.f90:139-142)
       integer, parameter, public :: RPA_T2_CUTOFF_EIG = 0
       integer, parameter, public :: RPA_T2_CUTOFF_EIG_DIV_MAXEIG = 1
@@ -134,6 +135,15 @@ module brpa_mod
                   real(c_double) :: e
                   real(c_double) :: r(*), v(*), a(*)
                   integer(c_int64_t), value :: nocc, nvir, naux, nt
+            end function
+            integer(c_int) function c_ao_mo(c, crg, rg, nvecs, npq, pairs, occ, vir, nao, nocc, nvir, &
+                  accum) bind(C, name="brpa_ao_mo")
+                  import :: c_int, c_ptr, c_int64_t, c_double, c_int32_t
+                  type(c_ptr), value :: c
+                  real(c_double) :: crg(*), rg(*), occ(*), vir(*)
+                  integer(c_int64_t), value :: nvecs, npq, nao, nocc, nvir
+                  integer(c_int32_t) :: pairs(*)
+                  integer(c_int), value :: accum
             end function
             integer(c_int) function c_t2_eig_rpa(c, r, naux, eig) bind(C, name="brpa_t2_eig_rpa")
                   import :: c_int, c_ptr, c_int64_t, c_double
@@ -337,6 +347,102 @@ contains
             if (NVecsT2 > 0) call check(c_t2_copyout(ctx, A, V), "rpa_CC_T2")
             if (.false.) print *, PiUEigenvecs(1,1,1), PiUEigenvals(1,1), OccBounds, VirtBounds, MaxBatchDim
       end subroutine rpa_CC_T2
+
+
+      subroutine rpa_CRG(CRG, RG, OccCoeffs, VirtCoeffs, NOcc, NVirt, OccBounds, VirtBounds, &
+            NSpins, ShellPairs, ShellPairLoc, ShellPairDim, ShellLoc, ShellParamsIdx, SubsetBounds, &
+            SubsetDim, NSubsets, NAngFunc, NAO, NVecs)
+            !
+            ! AO -> MO transform of the Cholesky vectors (rpa_core_MO.f90:233-301) on
+            ! one image: the AO-pair subsets X = 1..NSubsets(1) of RG(:, :, X) are
+            ! accumulated into CRG(:, :, s).  The AO pair of each packed column is
+            ! taken from the shell-pair tables in rpa_C's order (rpa_Cab: p fastest
+            ! over shell A, rpa_Caa: the lower triangle column by column) and the
+            ! product runs in brpa_ao_mo.
+            !
+            real(F64), dimension(:, :, :), intent(out), contiguous :: CRG
+            real(F64), dimension(:, :, :), intent(in)              :: RG
+            real(F64), dimension(:, :, :), intent(in)              :: OccCoeffs
+            real(F64), dimension(:, :, :), intent(in)              :: VirtCoeffs
+            integer, dimension(2), intent(in)                      :: NOcc
+            integer, dimension(2), intent(in)                      :: NVirt
+            integer, dimension(2, 2), intent(in)                   :: OccBounds
+            integer, dimension(2, 2), intent(in)                   :: VirtBounds
+            integer, intent(in)                                    :: NSpins
+            integer, dimension(:, :), intent(in)                   :: ShellPairs
+            integer, dimension(:, :), intent(in)                   :: ShellPairLoc
+            integer, dimension(:), intent(in)                      :: ShellPairDim
+            integer, dimension(:), intent(in)                      :: ShellLoc
+            integer, dimension(:), intent(in)                      :: ShellParamsIdx
+            integer, dimension(:, :), intent(in)                   :: SubsetBounds
+            integer, dimension(:), intent(in)                      :: SubsetDim
+            integer, dimension(2), intent(in)                      :: NSubsets
+            integer, dimension(:), intent(in)                      :: NAngFunc
+            integer, intent(in)                                    :: NAO
+            integer, intent(in)                                    :: NVecs
+            integer, parameter :: SUBSET_STORAGE = 2   ! ParallelCholesky.f90:45
+            integer(c_int32_t), dimension(:, :), allocatable :: pairs
+            real(F64), dimension(:, :), allocatable :: RGxy, Co, Cv
+            integer :: X, s, ShAB, LocAB, ShA, ShB, Na, Nb, LocA, LocB, p, q, pq, Npq
+            integer(c_int), dimension(2) :: accum
+
+            call brpa_init()
+            if (NSubsets(2) /= 1) then
+                  print *, "rpa_CRG: the GPU module runs on one image (NSubsets(2) must be 1)"
+                  error stop
+            end if
+            CRG = 0.0_F64
+            accum = 0_c_int
+            do X = 1, NSubsets(1)
+                  Npq = SubsetDim(X)
+                  if (Npq <= 0) cycle
+                  allocate(pairs(2, Npq), RGxy(NVecs, Npq))
+                  do ShAB = SubsetBounds(1, X), SubsetBounds(2, X)
+                        LocAB = ShellPairLoc(SUBSET_STORAGE, ShAB)
+                        ShA = ShellPairs(1, ShAB)
+                        ShB = ShellPairs(2, ShAB)
+                        Na = NAngFunc(ShellParamsIdx(ShA))
+                        Nb = NAngFunc(ShellParamsIdx(ShB))
+                        LocA = ShellLoc(ShA)
+                        LocB = ShellLoc(ShB)
+                        if (ShA /= ShB) then
+                              do q = 1, Nb
+                                    do p = 1, Na
+                                          pq = LocAB + p - 1 + (q - 1) * Na
+                                          pairs(1, pq) = int(LocA + p - 2, c_int32_t)
+                                          pairs(2, pq) = int(LocB + q - 2, c_int32_t)
+                                    end do
+                              end do
+                        else
+                              pq = LocAB
+                              do q = 1, Na
+                                    pairs(:, pq) = int(LocA + q - 2, c_int32_t)
+                                    pq = pq + 1
+                                    do p = q + 1, Na
+                                          pairs(1, pq) = int(LocA + p - 2, c_int32_t)
+                                          pairs(2, pq) = int(LocA + q - 2, c_int32_t)
+                                          pq = pq + 1
+                                    end do
+                              end do
+                        end if
+                  end do
+                  RGxy(:, :) = RG(1:NVecs, 1:Npq, X)
+                  do s = 1, NSpins
+                        if (NOcc(s) * NVirt(s) > 0) then
+                              allocate(Co(NAO, NOcc(s)), Cv(NAO, NVirt(s)))
+                              Co = OccCoeffs(:, OccBounds(1, s):OccBounds(2, s), s)
+                              Cv = VirtCoeffs(:, VirtBounds(1, s):VirtBounds(2, s), s)
+                              call check(c_ao_mo(ctx, CRG(:, :, s), RGxy, int(NVecs, c_int64_t), &
+                                    int(Npq, c_int64_t), pairs, Co, Cv, int(NAO, c_int64_t), &
+                                    int(NOcc(s), c_int64_t), int(NVirt(s), c_int64_t), accum(s)), "rpa_CRG")
+                              accum(s) = 1_c_int
+                              deallocate(Co, Cv)
+                        end if
+                  end do
+                  deallocate(pairs, RGxy)
+            end do
+            if (.false.) print *, ShellPairDim(1)
+      end subroutine rpa_CRG
 
 
       subroutine rpa_CC_Exchange_MBPT3_1(EcExchange, Vkai, Ak, NVecsT2, Rkai, NVecsPiU, &

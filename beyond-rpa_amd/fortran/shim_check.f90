@@ -5,7 +5,13 @@
 ! writes the results.  Used by the GPU test of the Fortran
 ! boundary.
 !
-!   shim_check input.bin output.txt [thc_input.bin thc_output.txt]
+!   shim_check input.bin output.txt [thc_input.bin thc_output.txt [aomo_input.bin aomo_output.txt]]
+!
+! aomo_input.bin (stream): NAO, NShells, NVecs, NOcc, NVirt, ShellSize(NShells),
+!   RG(NVecs, Npq) (all shell pairs A >= B in order, rpa_C packing),
+!   OccCoeffs(NAO, NOcc), VirtCoeffs(NAO, NVirt)
+! aomo_output.txt: CRG(NVecs, NVirt*NOcc) of rpa_CRG with the shell pairs split
+! into two subsets
 !
 ! thc_input.bin (stream): NOcc, NVirt, NCholesky, NGrid, NFreqs,
 !   Xga(NGrid, NVirt), Xgi(NGrid, NOcc), Zgk(NGrid, NCholesky),
@@ -100,6 +106,7 @@ program shim_check
       write(io, '(es25.17)') sum(Ath * Vth(1, :)**2)
       close(io)
       if (command_argument_count() >= 4) call thc_check()
+      if (command_argument_count() >= 6) call aomo_check()
       call brpa_finalize()
       if (.false.) print *, Seed
 
@@ -132,4 +139,79 @@ contains
             end do
             close(io)
       end subroutine thc_check
+
+      subroutine aomo_check()
+            integer :: nao_, nsh, nv, no_, nvr, a, b, k, h, npq, npq1, i
+            integer, allocatable :: sz(:), sloc(:), sp(:, :), sploc(:, :), spdim(:), spidx(:)
+            integer :: sb(2, 2), sdim(2)
+            real(F64), allocatable :: RGfull(:, :), RG3(:, :, :), Oc(:, :, :), Vc(:, :, :), CR(:, :, :)
+            character(len=512) :: ain, aout
+
+            call get_command_argument(5, ain)
+            call get_command_argument(6, aout)
+            open(newunit=io, file=trim(ain), access='stream', form='unformatted', status='old')
+            read(io) nao_, nsh, nv, no_, nvr
+            allocate(sz(nsh), sloc(nsh), spidx(nsh))
+            read(io) sz
+            npq = 0
+            k = 0
+            do a = 1, nsh
+                  do b = 1, a
+                        k = k + 1
+                        if (a /= b) then
+                              npq = npq + sz(a) * sz(b)
+                        else
+                              npq = npq + sz(a) * (sz(a) + 1) / 2
+                        end if
+                  end do
+            end do
+            allocate(RGfull(nv, npq), Oc(nao_, no_, 1), Vc(nao_, nvr, 1))
+            read(io) RGfull, Oc, Vc
+            close(io)
+            sloc(1) = 1
+            do a = 2, nsh
+                  sloc(a) = sloc(a - 1) + sz(a - 1)
+            end do
+            do a = 1, nsh
+                  spidx(a) = a
+            end do
+            allocate(sp(2, k), sploc(2, k), spdim(k))
+            h = k / 2
+            k = 0
+            npq1 = 0
+            do a = 1, nsh
+                  do b = 1, a
+                        k = k + 1
+                        sp(:, k) = [a, b]
+                        if (a /= b) then
+                              spdim(k) = sz(a) * sz(b)
+                        else
+                              spdim(k) = sz(a) * (sz(a) + 1) / 2
+                        end if
+                        if (k == 1 .or. k == h + 1) then
+                              sploc(2, k) = 1
+                        else
+                              sploc(2, k) = sploc(2, k - 1) + spdim(k - 1)
+                        end if
+                        if (k <= h) npq1 = npq1 + spdim(k)
+                  end do
+            end do
+            sploc(1, :) = 0
+            sb(:, 1) = [1, h]
+            sb(:, 2) = [h + 1, k]
+            sdim = [npq1, npq - npq1]
+            allocate(RG3(nv, max(sdim(1), sdim(2)), 2))
+            RG3 = 0.0_F64
+            RG3(:, 1:sdim(1), 1) = RGfull(:, 1:npq1)
+            RG3(:, 1:sdim(2), 2) = RGfull(:, npq1 + 1:npq)
+            allocate(CR(nv, no_ * nvr, 1))
+            call rpa_CRG(CR, RG3, Oc, Vc, [no_, 0], [nvr, 0], reshape([1, no_, 1, 0], [2, 2]), &
+                  reshape([1, nvr, 1, 0], [2, 2]), 1, sp, sploc, spdim, sloc, spidx, sb, sdim, [2, 1], &
+                  sz, nao_, nv)
+            open(newunit=io, file=trim(aout), status='replace')
+            do i = 1, size(CR)
+                  write(io, '(es25.17)') CR(mod(i - 1, nv) + 1, (i - 1) / nv + 1, 1)
+            end do
+            close(io)
+      end subroutine aomo_check
 end program shim_check

@@ -214,6 +214,23 @@ int brpa_thc_piu(brpa_ctx *ctx, double *PiU, double *Rkai, const double *Xga, co
                  const double *eocc, const double *evir, const double *freqs, int64_t nfreqs,
                  double prefactor, int accumulate);
 
+/* ---- AO -> MO transform of the Cholesky vectors -------------------------- */
+/* CRG(P, ai) (+)= sum_pq RG(P, pq) C(ai, pq) for one spin block and one subset
+ * of packed AO pairs, replacing rpa_CRGxy(rpa_C(...)) of rpa_CRG
+ * (rpa/rpa_core_MO.f90:233-301, rpa_C 137-190, rpa_Cab 40-82, rpa_Caa
+ * 85-134): C(ai, pq) = Co(p, i) Cv(q, a) + Co(q, i) Cv(p, a) for p != q and
+ * Co(p, i) Cv(p, a) for p == q.  pairs: host array of npq (p, q) 0-based AO
+ * index pairs, the AO pair of each packed column in the caller's order (the
+ * Fortran module builds it from the shell-pair tables as rpa_C enumerates
+ * them).  RG: Fortran RG(nvecs, npq); occ_coeffs / virt_coeffs: the active
+ * orbitals' AO coefficients, Fortran (nao, nocc) / (nao, nvir); CRG: C
+ * R[ai][P] (Fortran CRG(nvecs, nvir*nocc)), ai = i*nvir + a.  accumulate: 0
+ * overwrites CRG, 1 adds (the next subset; the reference zeroes CRG once,
+ * 269).  One FP64 MFMA kernel; C is formed while the operand is loaded. */
+int brpa_ao_mo(brpa_ctx *ctx, double *CRG, const double *RG, int64_t nvecs, int64_t npq,
+               const int32_t *pairs, const double *occ_coeffs, const double *virt_coeffs,
+               int64_t nao, int64_t nocc, int64_t nvir, int accumulate);
+
 /* ---- factorised direct-ring amplitudes --------------------------------- */
 /* T2 = sum_mu A_mu V_mu V_mu^T (ov x ov), the direct-ring doubles amplitudes
  * at coupling strength lambda from nguess random guess vectors (0: the

@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 from brpa import synth
-from oracle import erpa, grid, mbpt3, sosex, t2, thc
+from oracle import aomo, erpa, grid, mbpt3, sosex, t2, thc
 
 pytestmark = pytest.mark.gpu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -80,8 +80,23 @@ def test_fortran_thc_piu_matches_oracle(tmp_path):
         np.array([o, v, n, g, nf], dtype=np.int64).tofile(fh)
         for a in (t.Xga, t.Xgi, t.Zgk, t.eocc, t.evir, freqs):
             np.ascontiguousarray(a, np.float64).tofile(fh)
-    r = subprocess.run([EXE, str(fin), str(fout), str(tin), str(tout)], capture_output=True,
-                       text=True, timeout=120)
+    # rpa_CRG: shells s, p, d, f-like, two AO-pair subsets
+    sizes = [1, 3, 5, 1, 3, 7, 1, 3]
+    nao = sum(sizes)
+    loc = np.concatenate([[0], np.cumsum(sizes)[:-1]])
+    pairs = aomo.pair_table([(A, B) for A in range(len(sizes)) for B in range(A + 1)], loc, sizes)
+    rng = np.random.default_rng(11)
+    nv, no_, nvr = 37, 3, 11
+    RG = rng.standard_normal((len(pairs), nv))
+    Co = rng.standard_normal((no_, nao))
+    Cv = rng.standard_normal((nvr, nao))
+    ain, aout = tmp_path / "aomo.bin", tmp_path / "aomo.txt"
+    with open(ain, "wb") as fh:
+        np.array([nao, len(sizes), nv, no_, nvr] + sizes, dtype=np.int64).tofile(fh)
+        for a in (RG, Co, Cv):
+            np.ascontiguousarray(a, np.float64).tofile(fh)
+    r = subprocess.run([EXE, str(fin), str(fout), str(tin), str(tout), str(ain), str(aout)],
+                       capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     vals = np.array([float(s) for s in open(tout).read().split()])
     Pi = vals[:nf * n * n].reshape(nf, n, n)
@@ -89,3 +104,6 @@ def test_fortran_thc_piu_matches_oracle(tmp_path):
     Pis_ref, R_ref = thc.thc_piu(t.Xga.T, t.Xgi.T, t.Zgk.T, t.eocc, t.evir, freqs)
     assert np.max(np.abs(R - R_ref)) <= 1e-13 * np.max(np.abs(R_ref))
     assert np.max(np.abs(Pi - Pis_ref)) <= 1e-12 * np.max(np.abs(Pis_ref))
+    CR = np.array([float(x) for x in open(aout).read().split()]).reshape(no_ * nvr, nv)
+    ref = aomo.crg(RG, pairs, Co.T, Cv.T)
+    assert np.max(np.abs(CR - ref)) <= 1e-13 * np.max(np.abs(ref))
