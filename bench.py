@@ -19,7 +19,8 @@ its own arithmetic).
 
 Other workloads (--workload): single, adjoint, sosex, and thc (R from
 tensor-hypercontraction factors and Pi(u) of all frequencies, one fragment),
-mbpt3 (the MBPT3-1 and MBPT3-2 exchange corrections of the large fragment).
+mbpt3 (the MBPT3-1 and MBPT3-2 exchange corrections of the large fragment),
+aomo (the AO->MO transform of the Cholesky vectors, one fragment).
 
 Prints one JSON line.  `roofline` is measured for the dominant kernel of the job,
 the split-int8 GEMM of the amplitude contractions (k_gemm_i8), with HIP events
@@ -507,6 +508,73 @@ def bench_mbpt3(a):
     print(json.dumps(out))
 
 
+def bench_aomo(a):
+    """AO -> MO transform of the Cholesky vectors (brpa_ao_mo) of one fragment:
+    o=36, v=300 active orbitals over NAO=336 basis functions in s/p/d/f-like
+    shells, NVecs=1800, all NAO(NAO+1)/2 packed AO pairs in one subset; inputs
+    resident in HBM.  CPU baseline: the numpy oracle (C matrix, then one GEMM)
+    on 2 of the 36 occupied orbitals, scaled by 18."""
+    import numpy as np
+    import torch
+
+    import brpa
+    from oracle import aomo
+    _one_gpu(a)
+    o, v, nvecs = 36, 300, 1800
+    sizes = ([1, 1, 3, 3, 5, 7] * 17)[:-1]
+    sizes = sizes + [336 - sum(sizes)] if sum(sizes) < 336 else sizes
+    nao = sum(sizes)
+    loc = np.concatenate([[0], np.cumsum(sizes)[:-1]])
+    pairs = aomo.pair_table([(A, B) for A in range(len(sizes)) for B in range(A + 1)], loc, sizes)
+    rng = np.random.default_rng(7)
+    RG = rng.standard_normal((len(pairs), nvecs)) * 0.01
+    Co = rng.standard_normal((o, nao)) / np.sqrt(nao)
+    Cv = rng.standard_normal((v, nao)) / np.sqrt(nao)
+    d = lambda t: torch.from_numpy(t).cuda()  # noqa: E731
+    RGd, Cod, Cvd = d(RG), d(Co), d(Cv)
+    out = torch.empty((o * v, nvecs), dtype=torch.float64, device="cuda")
+    ctx = brpa.Context(0)
+    for _ in range(a.warmup):
+        ctx.ao_mo(RGd, pairs, Cod, Cvd, out=out)
+    torch.cuda.synchronize()
+    kms = 0.0
+    t = time.perf_counter()
+    for _ in range(a.steps):
+        ctx.ao_mo(RGd, pairs, Cod, Cvd, out=out)
+        kms += ctx.last_kernel_ms()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t) / a.steps
+    kms /= a.steps
+    flops = 2.0 * o * v * len(pairs) * nvecs
+    tf = flops / (kms * 1e-3) / 1e12
+    cpu = None
+    if a.cpu_seconds > 0:
+        t = time.perf_counter()
+        ref = aomo.crg(RG, pairs, Co[:2].T, Cv.T)
+        tc = (time.perf_counter() - t) * (o / 2)
+        got = out[: 2 * v].cpu().numpy()
+        dmax = float(np.max(np.abs(got - ref)) / np.max(np.abs(ref)))
+        cpu = (tc, dmax)
+    print("# AO->MO: %.3f ms/step, kernel %.3f ms, %.1f TF/s" % (dt * 1e3, kms, tf),
+          file=sys.stderr, flush=True)
+    res = {"metric": "AO->MO transform of the Cholesky vectors, one fragment o=36, v=300, "
+                     "NAO=%d, NVecs=1800, %d packed AO pairs" % (nao, len(pairs)),
+           "value": round(dt * 1e3, 3), "unit": "ms", "n_gpus": 1, "steps": a.steps,
+           "warmup": a.warmup, "ms_per_step": round(dt * 1e3, 3), "higher_is_better": False,
+           "scaling": "strong", "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic (seeded PCG64 coefficients and Cholesky vectors)",
+           "config": {"workload": "AO->MO transform o=36, v=300, NAO=%d, NVecs=1800" % nao},
+           "max_rel_dCRG_vs_cpu_oracle_2_occupied": None if cpu is None else cpu[1],
+           "roofline": {"bound": "mfma", "achieved": round(tf, 3), "peak": PEAK_FP64_TFLOPS,
+                        "unit": "TFLOP/s", "frac": round(tf / PEAK_FP64_TFLOPS, 4),
+                        "traffic": None, "kernel": "k_ao_mo", "kernel_ms_per_step": round(kms, 3)},
+           "cpu_baseline": None if cpu is None else {
+               "value": round(cpu[0] * 1e3, 1), "unit": "ms", "cores": _cpu_threads(),
+               "kind": "port", "sample": "numpy oracle (C matrix + GEMM) on 2 of 36 occupied "
+                                         "orbitals, x18"}}
+    print(json.dumps(res))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -528,7 +596,7 @@ def main():
     ap.add_argument("--exchange-workers", type=int, default=2,
                     help="worker threads of brpa_exchange_batch (fragments in flight at once)")
     ap.add_argument("--workload", default="batch",
-                    choices=["batch", "single", "adjoint", "sosex", "thc", "mbpt3"],
+                    choices=["batch", "single", "adjoint", "sosex", "thc", "mbpt3", "aomo"],
                     help="batch: the 256-fragment E_RPA + exchange batch (default); single: "
                          "E_RPA of one o=36, v=300, NAux=1800 fragment in every form; "
                          "adjoint: dE/dR and dE/d eps of one o=48, v=400, NAux=2400 "
@@ -562,6 +630,8 @@ def main():
         return bench_thc(a)
     if a.workload == "mbpt3":
         return bench_mbpt3(a)
+    if a.workload == "aomo":
+        return bench_aomo(a)
     _one_gpu(a)
 
     import numpy as np
