@@ -202,6 +202,38 @@ class Context:
                                                   _ptr(weights), nf, n, _ptr(eu)))
         return float(e[0]), eu
 
+    def pack_pi(self, piu, packed=None, accumulate=False):
+        """Lower-packed storage (F, N(N+1)/2) of Pi(u) (F, N, N) (rpa_Pack)."""
+        nf, n, _ = piu.shape
+        if packed is None:
+            packed = np.zeros((nf, n * (n + 1) // 2))
+        self._sync_stream(piu, packed)
+        self._check(self._lib.brpa_pack_pi(self._h, _ptr(packed), _ptr(piu), nf, n,
+                                           int(bool(accumulate))))
+        return packed
+
+    def unpack_pi(self, packed, n, out=None):
+        """Full symmetric Pi(u) (F, N, N) from lower-packed storage (rpa_Unpack)."""
+        nf = packed.shape[0]
+        if out is None:
+            out = np.zeros((nf, n, n))
+        self._sync_stream(packed, out)
+        self._check(self._lib.brpa_unpack_pi(self._h, _ptr(out), _ptr(packed), nf, n))
+        return out
+
+    def ecrpa_packed(self, packed, n, freqs, weights):
+        """E_RPA from lower-packed Pi(u) (the energy loop of rpa_FreqIntegral_MO).
+        Returns (E, E(u))."""
+        nf = packed.shape[0]
+        freqs = np.ascontiguousarray(freqs, np.float64)
+        weights = np.ascontiguousarray(weights, np.float64)
+        e = np.zeros(1)
+        eu = np.zeros(nf)
+        self._sync_stream(packed)
+        self._check(self._lib.brpa_ecrpa_packed(self._h, _ptr(e), _ptr(packed), _ptr(freqs),
+                                                _ptr(weights), nf, n, _ptr(eu)))
+        return float(e[0]), eu
+
     def ecrpa(self, R, eocc, evir, freqs, weights, precision="fp64"):
         """Fused E_RPA of one fragment.  Returns (E, E(u))."""
         nai, naux = R.shape

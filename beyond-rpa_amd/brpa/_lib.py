@@ -62,6 +62,9 @@ SIGNATURES = {
     "brpa_mbpt3_2": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _i64, _i64, _i64, _i64]),
     "brpa_ao_mo": (C.c_int, [_vp, _vp, _vp, _i64, _i64, _vp, _vp, _vp, _i64, _i64, _i64,
                              C.c_int]),
+    "brpa_pack_pi": (C.c_int, [_vp, _vp, _vp, _i64, _i64, C.c_int]),
+    "brpa_unpack_pi": (C.c_int, [_vp, _vp, _vp, _i64, _i64]),
+    "brpa_ecrpa_packed": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _i64, _i64, _vp]),
     "brpa_exchange_batch": (C.c_int, [_vp, C.c_int, _i64, C.POINTER(brpa_frag_t), C.c_double,
                                       C.c_double, C.c_double, C.c_uint64, C.c_double, C.c_int,
                                       C.c_int, _vp, _vp]),

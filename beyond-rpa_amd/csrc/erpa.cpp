@@ -637,12 +637,17 @@ int brpa_gpiug_mo(brpa_ctx *ctx, double *gpiug, const double *crg, int64_t naux,
     return BRPA_OK;
 }
 
-int brpa_ecrpa_analytic(brpa_ctx *ctx, double *ecrpa, const double *piu, const double *freqs,
-                        const double *weights, int64_t nfreqs, int64_t naux, double *eu) {
+}  // extern "C"
+
+int launch_unpack_lower(brpa_ctx *, const double *AP, int64_t n, int64_t F, double *A);
+
+// E_RPA from stored Pi(u): full (F x N x N) or lower-packed (F x N(N+1)/2) input
+static int ecrpa_stored(brpa_ctx *ctx, double *ecrpa, const double *piu, bool packed,
+                        const double *weights, int64_t nfreqs, int64_t naux, double *eu,
+                        const char *who) {
     if (!ctx) return BRPA_ERR_ARG;
     if (!ecrpa || !piu || !weights || nfreqs <= 0 || naux <= 0 || naux > 65536)
-        return brpa_fail(ctx, BRPA_ERR_ARG, "brpa_ecrpa_analytic: bad arguments");
-    (void)freqs;   // printed by the reference only (rpa_CC.f90:462-463)
+        return brpa_fail(ctx, BRPA_ERR_ARG, "%s: bad arguments", who);
     BRPA_HIP(ctx, hipSetDevice(ctx->device));
     const int64_t N = naux, F = nfreqs;
     const int nsteps = (int)cdiv(N, CHOL_NB);
@@ -671,16 +676,30 @@ int brpa_ecrpa_analytic(brpa_ctx *ctx, double *ecrpa, const double *piu, const d
         hd.status = ar.take<int>(F);
         hd.linv = ar.take<double>(F * CHOL_NB * CHOL_NB);
     };
+    const int64_t np = N * (N + 1) / 2;
+    const bool pk_host = packed && !is_device_ptr(piu);
     Arena count;
     count.take<FragDev>(1);
     count.take<int64_t>(pre.size());
+    if (pk_host) count.take<double>(F * np);
     layout(count);
     Arena ar;
     BRPA_TRY(ws_reserve(ctx, count.off, &ar));
     FragDev *d_fr = ar.take<FragDev>(1);
     int64_t *d_pre = ar.take<int64_t>(pre.size());
+    double *d_pk = pk_host ? ar.take<double>(F * np) : nullptr;
     layout(ar);
-    BRPA_TRY(to_device(ctx, hd.Pi, piu, 8 * F * N * N));
+    if (packed) {
+        // rpa_Unpack (rpa.f90:390-404) on the device, into the Cholesky's buffer
+        const double *src = piu;
+        if (pk_host) {
+            BRPA_TRY(copy_sync(ctx, d_pk, piu, 8 * F * np, hipMemcpyHostToDevice));
+            src = d_pk;
+        }
+        BRPA_TRY(launch_unpack_lower(ctx, src, N, F, hd.Pi));
+    } else {
+        BRPA_TRY(to_device(ctx, hd.Pi, piu, 8 * F * N * N));
+    }
     BRPA_TRY(to_device(ctx, const_cast<double *>(hd.weights), weights, 8 * F));
     BRPA_TRY(launch_fill_int(ctx, hd.status, F, 0));
     BRPA_HIP(ctx, hipMemcpyAsync(d_fr, &hd, sizeof(FragDev), hipMemcpyHostToDevice, ctx->stream));
@@ -711,6 +730,21 @@ int brpa_ecrpa_analytic(brpa_ctx *ctx, double *ecrpa, const double *piu, const d
             return brpa_fail(ctx, BRPA_ERR_NOT_SPD,
                              "frequency %lld: 1 + Pi(u) is not positive definite", (long long)w);
     return BRPA_OK;
+}
+
+extern "C" {
+
+int brpa_ecrpa_analytic(brpa_ctx *ctx, double *ecrpa, const double *piu, const double *freqs,
+                        const double *weights, int64_t nfreqs, int64_t naux, double *eu) {
+    (void)freqs;   // printed by the reference only (rpa_CC.f90:462-463)
+    return ecrpa_stored(ctx, ecrpa, piu, false, weights, nfreqs, naux, eu, "brpa_ecrpa_analytic");
+}
+
+int brpa_ecrpa_packed(brpa_ctx *ctx, double *ecrpa, const double *piu_packed, const double *freqs,
+                      const double *weights, int64_t nfreqs, int64_t naux, double *eu) {
+    (void)freqs;   // printed by the reference only (rpa.f90:1368-1372)
+    return ecrpa_stored(ctx, ecrpa, piu_packed, true, weights, nfreqs, naux, eu,
+                        "brpa_ecrpa_packed");
 }
 
 }  // extern "C"

@@ -348,3 +348,54 @@ int launch_mbpt3_ab(brpa_ctx *ctx, const double *D, const double *U, const doubl
     LAUNCH_CHECK(ctx);
     return BRPA_OK;
 }
+
+// LAPACK lower-packed storage of symmetric n x n matrices (dtrttp / dtpttr with
+// uplo 'L', the reference's real_Pack / real_Unpack, linear-algebra/
+// real_linalg.f90:737-773): column j holds rows i >= j at s_j + (i - j),
+// s_j = j n - j (j - 1) / 2.  One work-group per (column j, matrix w).
+__device__ __forceinline__ int64_t packed_col_start(int64_t j, int64_t n) {
+    return j * n - j * (j - 1) / 2;
+}
+
+// AP (+)= lower triangle of A (column-major, element (i, j) at i + n j)
+__global__ __launch_bounds__(256) void k_pack_lower(const double *__restrict__ A, int64_t n,
+                                                    double *__restrict__ AP, int accumulate) {
+    const int64_t j = blockIdx.x, w = blockIdx.y;
+    const int64_t np = n * (n + 1) / 2;
+    const double *a = A + w * n * n + j * n;
+    double *ap = AP + w * np + packed_col_start(j, n) - j;
+    for (int64_t i = j + threadIdx.x; i < n; i += 256) ap[i] = accumulate ? ap[i] + a[i] : a[i];
+}
+
+// A = symmetric matrix of the lower-packed AP (both triangles written)
+__global__ __launch_bounds__(256) void k_unpack_lower(const double *__restrict__ AP, int64_t n,
+                                                      double *__restrict__ A) {
+    const int64_t j = blockIdx.x, w = blockIdx.y;
+    const int64_t np = n * (n + 1) / 2;
+    const double *ap = AP + w * np + packed_col_start(j, n) - j;
+    double *a = A + w * n * n;
+    for (int64_t i = j + threadIdx.x; i < n; i += 256) {
+        const double v = ap[i];
+        a[i + n * j] = v;
+        a[j + n * i] = v;
+    }
+}
+
+int launch_pack_lower(brpa_ctx *ctx, const double *A, int64_t n, int64_t F, double *AP,
+                      int accumulate) {
+    if (n <= 0 || F <= 0) return BRPA_OK;
+    if (F > 65535) return brpa_fail(ctx, BRPA_ERR_ARG, "pack: more than 65535 matrices");
+    hipLaunchKernelGGL(k_pack_lower, dim3((unsigned)n, (unsigned)F), dim3(256), 0, ctx->stream, A,
+                       n, AP, accumulate);
+    LAUNCH_CHECK(ctx);
+    return BRPA_OK;
+}
+
+int launch_unpack_lower(brpa_ctx *ctx, const double *AP, int64_t n, int64_t F, double *A) {
+    if (n <= 0 || F <= 0) return BRPA_OK;
+    if (F > 65535) return brpa_fail(ctx, BRPA_ERR_ARG, "unpack: more than 65535 matrices");
+    hipLaunchKernelGGL(k_unpack_lower, dim3((unsigned)n, (unsigned)F), dim3(256), 0, ctx->stream,
+                       AP, n, A);
+    LAUNCH_CHECK(ctx);
+    return BRPA_OK;
+}

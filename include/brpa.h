@@ -191,6 +191,25 @@ int brpa_autotune(brpa_ctx *ctx, int precision, int64_t nfrag, const brpa_frag_t
  * each of the 3 variants measured by the last brpa_autotune (0: not measured). */
 int brpa_autotune_result(const brpa_ctx *ctx, int bucket, int *variant, double *ms_per_variant);
 
+/* ---- packed storage of Pi(u) ----------------------------------------------- */
+/* Lower-packed storage of symmetric N x N matrices, LAPACK order (uplo 'L'):
+ * column j holds rows i >= j at j*N - j*(j-1)/2 + (i - j), N(N+1)/2 doubles per
+ * matrix, the reference's GPiUG(:, u) (rpa/rpa.f90:379-404 rpa_Pack /
+ * rpa_Unpack on real_Pack / real_Unpack = dtrttp / dtpttr,
+ * linear-algebra/real_linalg.f90:737-773).  Full matrices: nfreqs x N x N,
+ * Fortran PiU(N, N, nfreqs), element (i, j) at i + N j.  Host or device.
+ * brpa_pack_pi: packed (+)= lower triangle (accumulate = 1 as rpa_Pack, 0
+ * overwrites).  brpa_unpack_pi: the full symmetric matrices (rpa_Unpack).
+ * brpa_ecrpa_packed: E_RPA from packed Pi(u) as the energy loop of
+ * rpa_FreqIntegral_MO (rpa/rpa.f90:1336-1380): unpacked on the device, then the
+ * Cholesky log-det of brpa_ecrpa_analytic; eu (may be NULL) receives E(u). */
+int brpa_pack_pi(brpa_ctx *ctx, double *packed, const double *piu, int64_t nfreqs, int64_t naux,
+                 int accumulate);
+int brpa_unpack_pi(brpa_ctx *ctx, double *piu, const double *packed, int64_t nfreqs,
+                   int64_t naux);
+int brpa_ecrpa_packed(brpa_ctx *ctx, double *ecrpa, const double *piu_packed, const double *freqs,
+                      const double *weights, int64_t nfreqs, int64_t naux, double *eu);
+
 /* ---- THC route: Cholesky vectors and Pi(u) from the THC factors --------- */
 /* Pi(u_w) (w = 0..nfreqs-1) of one spin block from the tensor-hypercontraction
  * factors, replacing rpa_THC_PiU_BlockAlgo (rpa/rpa_THC.f90:285-339) with

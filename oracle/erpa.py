@@ -99,3 +99,32 @@ def ecrpa(R, gaps, freqs, weights, prefactor=4.0, logdet=log_det_qr, return_term
     if return_terms:
         return float(e), terms
     return float(e)
+
+
+def pack_lower(A):
+    """LAPACK lower-packed storage (dtrttp, uplo 'L') of each N x N matrix in A
+    (F, N, N) read column-major, i.e. A[w] holds Fortran A(i, j) at [j][i]
+    (rpa_Pack / real_Pack, src/rpa/rpa.f90:379-387, linear-algebra/real_linalg.f90:737-750)."""
+    A = np.asarray(A)
+    F, n, _ = A.shape
+    out = np.empty((F, n * (n + 1) // 2))
+    for w in range(F):
+        k = 0
+        for j in range(n):
+            out[w, k:k + n - j] = A[w, j, j:]      # column j, rows i >= j
+            k += n - j
+    return out
+
+
+def unpack_lower(AP, n):
+    """Full symmetric matrices of lower-packed AP (F, N(N+1)/2) (rpa_Unpack, rpa.f90:390-404)."""
+    AP = np.asarray(AP)
+    F = AP.shape[0]
+    out = np.zeros((F, n, n))
+    for w in range(F):
+        k = 0
+        for j in range(n):
+            out[w, j, j:] = AP[w, k:k + n - j]
+            out[w, j:, j] = AP[w, k:k + n - j]
+            k += n - j
+    return out
