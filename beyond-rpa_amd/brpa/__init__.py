@@ -202,6 +202,28 @@ class Context:
                                                   _ptr(weights), nf, n, _ptr(eu)))
         return float(e[0]), eu
 
+    _BASES = {"random": 1, "eigen": 2, "full": 3, "full_cholesky": 3}
+
+    def rpa_g(self, RWR, basis="eigen", target_error=10 ** -2.5, seed=0):
+        """Basis of the effective dielectric matrix (rpa_G).  RWR: (N, N) Pi(u) at
+        the lowest frequency, Fortran order (symmetric, so either).  Returns
+        (G, RWR', NVecs, AbsError) with G, RWR' as Fortran-ordered (N, N) arrays,
+        i.e. C arrays whose [j][i] is element (i, j)."""
+        b = self._BASES[basis] if isinstance(basis, str) else int(basis)
+        n = RWR.shape[0]
+        R = RWR.clone() if _is_torch(RWR) else np.array(RWR, dtype=np.float64, copy=True)
+        if _is_torch(R):
+            import torch
+            G = torch.zeros((n, n), dtype=torch.float64, device=R.device)
+        else:
+            G = np.zeros((n, n))
+        nv, err = C.c_int64(0), C.c_double(0.0)
+        self._sync_stream(R, G)
+        self._check(self._lib.brpa_rpa_g(self._h, b, _ptr(G), _ptr(R), n, float(target_error),
+                                         int(seed) & 0xFFFFFFFFFFFFFFFF, C.byref(nv),
+                                         C.byref(err)))
+        return G, R, nv.value, err.value
+
     def pack_pi(self, piu, packed=None, accumulate=False):
         """Lower-packed storage (F, N(N+1)/2) of Pi(u) (F, N, N) (rpa_Pack)."""
         nf, n, _ = piu.shape

@@ -399,3 +399,43 @@ int launch_unpack_lower(brpa_ctx *ctx, const double *AP, int64_t n, int64_t F, d
     LAUNCH_CHECK(ctx);
     return BRPA_OK;
 }
+
+// Standard normal numbers for the random basis of rpa_G: Box-Muller on two
+// SplitMix64 counter draws per element, u1 = (z1 >> 11 + 1) / 2^53 in (0, 1],
+// u2 = (z2 >> 11) / 2^53; g = sqrt(-2 ln u1) cos(2 pi u2).
+__device__ __forceinline__ uint64_t splitmix(uint64_t seed, uint64_t ctr) {
+    uint64_t z = seed + ctr * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+__global__ void k_gauss(double *__restrict__ G, int64_t n, uint64_t seed) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= n) return;
+    const double u1 = (double)((splitmix(seed, 2 * e + 1) >> 11) + 1) * (1.0 / 9007199254740992.0);
+    const double u2 = (double)(splitmix(seed, 2 * e + 2) >> 11) * (1.0 / 9007199254740992.0);
+    G[e] = sqrt(-2.0 * log(u1)) * cos(6.283185307179586476925 * u2);
+}
+
+int launch_gauss(brpa_ctx *ctx, double *G, int64_t n, uint64_t seed) {
+    if (n <= 0) return BRPA_OK;
+    hipLaunchKernelGGL(k_gauss, dim3(nblocks(n)), dim3(256), 0, ctx->stream, G, n, seed);
+    LAUNCH_CHECK(ctx);
+    return BRPA_OK;
+}
+
+// M = diag(d) (n x n, column-major)
+__global__ void k_set_diag_only(double *__restrict__ M, const double *__restrict__ d, int64_t n) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= n * n) return;
+    const int64_t j = e / n, i = e - j * n;
+    M[e] = (i == j) ? d[i] : 0.0;
+}
+
+int launch_set_diag_only(brpa_ctx *ctx, double *M, const double *d, int64_t n) {
+    if (n <= 0) return BRPA_OK;
+    hipLaunchKernelGGL(k_set_diag_only, dim3(nblocks(n * n)), dim3(256), 0, ctx->stream, M, d, n);
+    LAUNCH_CHECK(ctx);
+    return BRPA_OK;
+}

@@ -10,6 +10,7 @@
 !   rpa_THC_CC_T2           rpa/rpa_CC_Doubles.f90:645-984 (with rpa_T2_EigRPA 987-1009)
 !   rpa_THC_PiU             rpa/rpa_THC.f90:215-282
 !   rpa_CRG                 rpa/rpa_core_MO.f90:233-301 (one image: NSubsets(2) = 1)
+!   rpa_G                   rpa/rpa.f90:31-173
 !   rpa_CC_Exchange_MBPT3_1 rpa/rpa_CC_Exchange.f90:232-270
 !   rpa_CC_Exchange_MBPT3_2 rpa/rpa_CC_Exchange.f90:381-393
 !   rpa_CC_Exchange_SOSEX   rpa/rpa_CC_Exchange.f90:36-97
@@ -32,7 +33,7 @@ module brpa_mod
       public :: brpa_init, brpa_finalize
       public :: rpa_GPiUG_MO, rpa_CC_EcRPA_Analytic, rpa_CC_T2, rpa_CC_Exchange_SOSEX
       public :: rpa_THC_CC_T2, rpa_THC_PiU
-      public :: rpa_CC_Exchange_MBPT3_1, rpa_CC_Exchange_MBPT3_2, rpa_CRG
+      public :: rpa_CC_Exchange_MBPT3_1, rpa_CC_Exchange_MBPT3_2, rpa_CRG, rpa_G
       ! T2CutoffType values (rpa/rpa_definitions.f90:139-142)
       integer, parameter, public :: RPA_T2_CUTOFF_EIG = 0
       integer, parameter, public :: RPA_T2_CUTOFF_EIG_DIV_MAXEIG = 1
@@ -144,6 +145,17 @@ module brpa_mod
                   integer(c_int64_t), value :: nvecs, npq, nao, nocc, nvir
                   integer(c_int32_t) :: pairs(*)
                   integer(c_int), value :: accum
+            end function
+            integer(c_int) function c_rpa_g(c, basis, g, rwr, n, target, seed, nvecs, abserr) &
+                  bind(C, name="brpa_rpa_g")
+                  import :: c_int, c_ptr, c_int64_t, c_double
+                  type(c_ptr), value :: c
+                  integer(c_int), value :: basis
+                  real(c_double) :: g(*), rwr(*)
+                  integer(c_int64_t), value :: n, seed
+                  real(c_double), value :: target
+                  integer(c_int64_t) :: nvecs
+                  real(c_double) :: abserr
             end function
             integer(c_int) function c_t2_eig_rpa(c, r, naux, eig) bind(C, name="brpa_t2_eig_rpa")
                   import :: c_int, c_ptr, c_int64_t, c_double
@@ -347,6 +359,30 @@ contains
             if (NVecsT2 > 0) call check(c_t2_copyout(ctx, A, V), "rpa_CC_T2")
             if (.false.) print *, PiUEigenvecs(1,1,1), PiUEigenvals(1,1), OccBounds, VirtBounds, MaxBatchDim
       end subroutine rpa_CC_T2
+
+
+      subroutine rpa_G(G, RWR, NVecs, TargetError, NCholesky, RWRBasis, NAO)
+            !
+            ! Basis of the effective dielectric matrix (rpa.f90:31-173), same argument
+            ! list.  RWRBasis = 1 (random: the library's own Gaussian draws), 2
+            ! (eigenvectors, the reference default) or 3 (full Cholesky space).
+            !
+            real(F64), dimension(NCholesky, NCholesky), intent(out)   :: G
+            real(F64), dimension(NCholesky, NCholesky), intent(inout) :: RWR
+            integer, intent(out)                                      :: NVecs
+            real(F64), intent(in)                                     :: TargetError
+            integer, intent(in)                                       :: NCholesky
+            integer, intent(in)                                       :: RWRBasis
+            integer, intent(in)                                       :: NAO
+            integer(c_int64_t) :: nv
+            real(c_double) :: err
+
+            call brpa_init()
+            call check(c_rpa_g(ctx, int(RWRBasis, c_int), G, RWR, int(NCholesky, c_int64_t), &
+                  TargetError, 0_c_int64_t, nv, err), "rpa_G")
+            NVecs = int(nv)
+            if (.false.) print *, NAO
+      end subroutine rpa_G
 
 
       subroutine rpa_CRG(CRG, RG, OccCoeffs, VirtCoeffs, NOcc, NVirt, OccBounds, VirtBounds, &

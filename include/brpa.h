@@ -191,6 +191,24 @@ int brpa_autotune(brpa_ctx *ctx, int precision, int64_t nfrag, const brpa_frag_t
  * each of the 3 variants measured by the last brpa_autotune (0: not measured). */
 int brpa_autotune_result(const brpa_ctx *ctx, int bucket, int *variant, double *ms_per_variant);
 
+/* ---- basis of the effective dielectric matrix ------------------------------ */
+/* Basis types of rpa_G (rpa/rpa_definitions.f90:92-104). */
+enum { BRPA_BASIS_RANDOM = 1, BRPA_BASIS_EIGEN = 2, BRPA_BASIS_FULL_CHOLESKY = 3 };
+/* G and NVecs of rpa_G (rpa/rpa.f90:31-173) from RWR = Pi(u) at the lowest
+ * frequency (Fortran RWR(N, N), G(N, N); host or device):
+ *   BRPA_BASIS_EIGEN (the reference default): G = eigenvectors of RWR, the
+ *     largest eigenvalue first; RWR <- diag(eigenvalues);
+ *   BRPA_BASIS_RANDOM: G Gaussian (Box-Muller on a SplitMix64 counter hash of
+ *     `seed`; the reference draws from its own generator), two subspace
+ *     iterations G <- Q(RWR G) (rocSOLVER geqrf + orgqr), RWR <- G^T RWR G;
+ *   BRPA_BASIS_FULL_CHOLESKY: G = I, NVecs = N, RWR unchanged.
+ * NVecs: scanning p = N..1, the diagonal of the transformed RWR is summed while
+ * the sum stays below target_error (extended-precision sum); NVecs is the last
+ * p (1-based) added, N when none is (rpa.f90:141-152).  abs_error (may be
+ * NULL): that sum, Tr(Pi(exact) - Pi(eff)). */
+int brpa_rpa_g(brpa_ctx *ctx, int basis, double *G, double *RWR, int64_t ncholesky,
+               double target_error, uint64_t seed, int64_t *nvecs, double *abs_error);
+
 /* ---- packed storage of Pi(u) ----------------------------------------------- */
 /* Lower-packed storage of symmetric N x N matrices, LAPACK order (uplo 'L'):
  * column j holds rows i >= j at j*N - j*(j-1)/2 + (i - j), N(N+1)/2 doubles per
