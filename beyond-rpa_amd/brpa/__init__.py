@@ -425,6 +425,27 @@ class Context:
         return {k: (self._lib.brpa_last_stage_ms(self._h, v),
                     self._lib.brpa_last_stage_launches(self._h, v)) for k, v in names.items()}
 
+    def thc_1b2g(self, Zgh, Xgi, Yga, Uaim, Am, want_diag=False):
+        """THC SOSEX + 2g cumulant terms (rpa_CCD_corrections_1b2g).  Fortran-
+        ordered inputs as C arrays: Zgh (G, G) symmetric, Xgi (o, G), Yga (v, G),
+        Uaim (NT2, o, v) = the (NT2, ov) V of thc_t2, Am (NT2,).  Returns
+        (Ec1b, Ec2g, EigSOSEX, Eig2g, YXUggm or None); Ec1b before the 1/2."""
+        G = Zgh.shape[0]
+        o, v = Xgi.shape[0], Yga.shape[0]
+        Am = np.ascontiguousarray(Am, np.float64) if not _is_torch(Am) else Am
+        nt = Am.numel() if _is_torch(Am) else Am.size
+        nu = Uaim.numel() if _is_torch(Uaim) else Uaim.size
+        if nu != nt * o * v or Xgi.shape[1] != G or Yga.shape[1] != G:
+            raise ValueError("shape mismatch")
+        e1, e2 = np.zeros(1), np.zeros(1)
+        es, eg = np.zeros(nt), np.zeros(nt)
+        diag = np.zeros((nt, G)) if want_diag else None
+        self._sync_stream(Zgh, Xgi, Yga, Uaim, Am)
+        self._check(self._lib.brpa_thc_1b2g(self._h, _ptr(Zgh), _ptr(Xgi), _ptr(Yga), _ptr(Uaim),
+                                            _ptr(Am), G, o, v, nt, _ptr(e1), _ptr(e2), _ptr(es),
+                                            _ptr(eg), _ptr(diag)))
+        return float(e1[0]), float(e2[0]), es, eg, diag
+
     _CUTOFF_TYPES = {"eig": 0, "eig/maxeig": 1, "eig/nelectron": 2, "eig/nocc": 2,
                      "sum-rejected": 3, "sum_rejected": 3}
 

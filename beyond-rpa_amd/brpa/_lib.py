@@ -67,6 +67,8 @@ SIGNATURES = {
     "brpa_ecrpa_packed": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _i64, _i64, _vp]),
     "brpa_rpa_g": (C.c_int, [_vp, C.c_int, _vp, _vp, _i64, C.c_double, C.c_uint64,
                              C.POINTER(_i64), C.POINTER(C.c_double)]),
+    "brpa_thc_1b2g": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _i64, _i64, _i64, _vp, _vp,
+                                _vp, _vp, _vp]),
     "brpa_exchange_batch": (C.c_int, [_vp, C.c_int, _i64, C.POINTER(brpa_frag_t), C.c_double,
                                       C.c_double, C.c_double, C.c_uint64, C.c_double, C.c_int,
                                       C.c_int, _vp, _vp]),

@@ -251,6 +251,26 @@ int brpa_thc_piu(brpa_ctx *ctx, double *PiU, double *Rkai, const double *Xga, co
                  const double *eocc, const double *evir, const double *freqs, int64_t nfreqs,
                  double prefactor, int accumulate);
 
+/* ---- THC SOSEX + 2g cumulant terms ---------------------------------------- */
+/* The 1b (SOSEX-type) and 2g cumulant energies of the THC route, replacing
+ * rpa_CCD_corrections_1b2g (rpa/rpa_CCD_Corrections.f90:79-290):
+ *   M_mu(g, h) = sum_{a,i} Y(g, a) U(a, i, mu) X(h, i),
+ *   S2g_mu = sum_{g,h} Z(g, h) M_mu(g, h)^2,  S1b_mu = sum_{g,h} Z(g, h) M_mu(g, h) M_mu(h, g),
+ *   Ec2g = 2 sum_mu (-2 S2g_mu A_mu^2),  Ec1b = 2 sum_mu (-S1b_mu A_mu)
+ * (before the caller's 1/2 on Ec1b, rpa_CCD_Corrections.f90:71, after which
+ * it equals SOSEX).  Zgh: Fortran Zgh(ngrid, ngrid), symmetric (Zgk Zgk^T);
+ * Xgi(ngrid, nocc), Yga(ngrid, nvir); Uaim: Fortran Uaim(nvir, nocc, NVecsT2),
+ * i.e. the V(ov, NVecsT2) of brpa_t2_copyout_ov; Am(NVecsT2).  Host or
+ * device.  eig_sosex / eig_2g (host, may be NULL): RPAOutput%EigSOSEX(mu) =
+ * -S1b A and Eig2g(mu) = -4 S2g A^2; yxu_diag (may be NULL, host or device,
+ * ngrid x NVecsT2): YXUggm(g, mu) = M_mu(g, g), the 2b/2c/2d intermediate.
+ * One fused FP64 MFMA kernel per chunk of vectors: M is built and contracted
+ * in registers, never stored. */
+int brpa_thc_1b2g(brpa_ctx *ctx, const double *Zgh, const double *Xgi, const double *Yga,
+                  const double *Uaim, const double *Am, int64_t ngrid, int64_t nocc, int64_t nvir,
+                  int64_t nvecs_t2, double *ec1b, double *ec2g, double *eig_sosex, double *eig_2g,
+                  double *yxu_diag);
+
 /* ---- AO -> MO transform of the Cholesky vectors -------------------------- */
 /* CRG(P, ai) (+)= sum_pq RG(P, pq) C(ai, pq) for one spin block and one subset
  * of packed AO pairs, replacing rpa_CRGxy(rpa_C(...)) of rpa_CRG
