@@ -20,7 +20,8 @@ its own arithmetic).
 Other workloads (--workload): single, adjoint, sosex, and thc (R from
 tensor-hypercontraction factors and Pi(u) of all frequencies, one fragment),
 mbpt3 (the MBPT3-1 and MBPT3-2 exchange corrections of the large fragment),
-aomo (the AO->MO transform of the Cholesky vectors, one fragment).
+aomo (the AO->MO transform of the Cholesky vectors, one fragment), thc1b2g
+(the THC SOSEX + 2g cumulant terms, one fragment).
 
 Prints one JSON line.  `roofline` is measured for the dominant kernel of the job,
 the split-int8 GEMM of the amplitude contractions (k_gemm_i8), with HIP events
@@ -575,6 +576,73 @@ def bench_aomo(a):
     print(json.dumps(res))
 
 
+def bench_thc1b2g(a):
+    """THC SOSEX + 2g cumulant terms (brpa_thc_1b2g) of the THC fragment o=36,
+    v=300, NAux=1800, NGridTHC=4032: R from the THC factors, amplitudes
+    (NGuess = 3 NAux, EIG cutoff 1e-12) on the GPU, then the timed 1b2g call
+    with Zgh = Zgk Zgk^T, the factors and the amplitudes in HBM.  CPU baseline:
+    the numpy oracle on 16 of the amplitude vectors, scaled to NT2."""
+    import math
+
+    import numpy as np
+    import torch
+
+    import brpa
+    from brpa import synth
+    from oracle import thc as othc
+    _one_gpu(a)
+    o, v, n, g, nf = synth.THC_CONFIG
+    f = synth.make_thc_fragment(o, v, n, g, synth.BASE_SEED)
+    x, w, _ = brpa.freq_grid(f.eocc, f.evir, nf)
+    d = lambda t: torch.from_numpy(np.ascontiguousarray(t)).cuda()  # noqa: E731
+    ctx = brpa.Context(0)
+    _, R = ctx.thc_piu(d(f.Xga), d(f.Xgi), d(f.Zgk), f.eocc, f.evir, x, want_R=True)
+    A, Vt, _, _, _ = ctx.thc_t2(R, f.eocc, f.evir, x, w, nguess=math.ceil(3.0 * n), seed=1,
+                                t2_cutoff_thresh=1e-12)
+    nt = len(A)
+    Zgh = d(f.Zgk.T @ f.Zgk)            # Zgk (G, N) as math: Zgh = Zgk Zgk^T
+    Xi, Ya, U, Ad = d(f.Xgi), d(f.Xga), d(Vt), d(A)
+    for _ in range(a.warmup):
+        ctx.thc_1b2g(Zgh, Xi, Ya, U, Ad)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    kms = 0.0
+    for _ in range(a.steps):
+        e1, e2, _, _, _ = ctx.thc_1b2g(Zgh, Xi, Ya, U, Ad)
+        kms += ctx.last_kernel_ms()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t) / a.steps
+    kms /= a.steps
+    flops = 2.0 * g * g * o * nt + 2.0 * g * v * o * nt      # M tiles (both orders) + YU
+    tf = flops / (kms * 1e-3) / 1e12
+    cpu = None
+    if a.cpu_seconds > 0:
+        m = 16
+        Uh = Vt[:m].reshape(m, o, v).transpose(2, 1, 0)
+        Zh = f.Zgk.T @ f.Zgk
+        t = time.perf_counter()
+        othc.thc_1b2g(Zh, f.Xgi.T, f.Xga.T, Uh, A[:m])
+        cpu = (time.perf_counter() - t) * nt / m
+    print("# THC 1b2g: NT2 %d, %.1f ms/step, device %.1f ms, %.1f TF/s, Ec1b/2 %.8f Ec2g %.8f"
+          % (nt, dt * 1e3, kms, tf, 0.5 * e1, e2), file=sys.stderr, flush=True)
+    out = {"metric": "THC SOSEX + 2g cumulant terms of one fragment o=36, v=300, NAux=1800, "
+                     "NGridTHC=4032, NT2=%d" % nt,
+           "value": round(dt * 1e3, 3), "unit": "ms", "n_gpus": 1, "steps": a.steps,
+           "warmup": a.warmup, "ms_per_step": round(dt * 1e3, 3), "higher_is_better": False,
+           "scaling": "strong", "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic (seeded PCG64, brpa.synth.make_thc_fragment)",
+           "config": {"workload": "THC 1b2g o=36, v=300, NAux=1800, NGridTHC=4032, NT2=%d" % nt},
+           "Ec1b_half": 0.5 * e1, "Ec2g": e2,
+           "roofline": {"bound": "mfma", "achieved": round(tf, 3), "peak": PEAK_FP64_TFLOPS,
+                        "unit": "TFLOP/s", "frac": round(tf / PEAK_FP64_TFLOPS, 4),
+                        "traffic": None, "kernel": "k_thc_1b2g (+ YU dgemm)",
+                        "kernel_ms_per_step": round(kms, 3)},
+           "cpu_baseline": None if cpu is None else {
+               "value": round(cpu * 1e3, 1), "unit": "ms", "cores": _cpu_threads(),
+               "kind": "port", "sample": "numpy oracle on 16 of %d amplitude vectors, scaled" % nt}}
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -596,7 +664,8 @@ def main():
     ap.add_argument("--exchange-workers", type=int, default=2,
                     help="worker threads of brpa_exchange_batch (fragments in flight at once)")
     ap.add_argument("--workload", default="batch",
-                    choices=["batch", "single", "adjoint", "sosex", "thc", "mbpt3", "aomo"],
+                    choices=["batch", "single", "adjoint", "sosex", "thc", "mbpt3", "aomo",
+                             "thc1b2g"],
                     help="batch: the 256-fragment E_RPA + exchange batch (default); single: "
                          "E_RPA of one o=36, v=300, NAux=1800 fragment in every form; "
                          "adjoint: dE/dR and dE/d eps of one o=48, v=400, NAux=2400 "
@@ -632,6 +701,8 @@ def main():
         return bench_mbpt3(a)
     if a.workload == "aomo":
         return bench_aomo(a)
+    if a.workload == "thc1b2g":
+        return bench_thc1b2g(a)
     _one_gpu(a)
 
     import numpy as np
