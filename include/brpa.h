@@ -283,7 +283,11 @@ int brpa_thc_1b2g(brpa_ctx *ctx, const double *Zgh, const double *Xgi, const dou
  * orbitals' AO coefficients, Fortran (nao, nocc) / (nao, nvir); CRG: C
  * R[ai][P] (Fortran CRG(nvecs, nvir*nocc)), ai = i*nvir + a.  accumulate: 0
  * overwrites CRG, 1 adds (the next subset; the reference zeroes CRG once,
- * 269).  One FP64 MFMA kernel; C is formed while the operand is loaded. */
+ * 269).  The packed columns are unpacked on the device into the full symmetric
+ * AO-pair matrices of each Cholesky vector, then two quarter transforms (H =
+ * Co^T RGfull, CRG = H Cv) run as strided-batched dgemm: 2 nvecs nao nocc (nao +
+ * nvir) flop instead of the packed form's 2 nocc nvir npq nvecs.  Device
+ * scratch: up to ~2 GB (vectors in chunks). */
 int brpa_ao_mo(brpa_ctx *ctx, double *CRG, const double *RG, int64_t nvecs, int64_t npq,
                const int32_t *pairs, const double *occ_coeffs, const double *virt_coeffs,
                int64_t nao, int64_t nocc, int64_t nvir, int accumulate);
