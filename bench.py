@@ -797,9 +797,10 @@ def main():
     exch = None
     roofline_exch = None
     if a.exchange_frags > 0:
-        # T2 + SOSEX of the first fragments of the batch, timed once per SOSEX
+        # T2 + SOSEX of the first fragments of the batch, timed once per exchange
         # arithmetic: each form's exchange runs in that form's own arithmetic
-        # (split-int8 SOSEX for split_i8, FP64 for the others)
+        # (split-int8 amplitude and SOSEX contractions for split_i8, FP64 for the
+        # others)
         kx = min(a.exchange_frags, a.nfrag)
         sub = batch[:kx]
         xprec_of = {p: ("split_i8" if p == "split_i8" else "fp64") for p in forms}
@@ -824,7 +825,8 @@ def main():
                 "ms_per_fragment_by_sosex_precision": {q: round(v * 1e3, 3)
                                                        for q, v in t_ex.items()},
                 "stage_ms_per_fragment_by_sosex_precision": ex_stages,
-                "precision": "fp64 T2 amplitudes, %s SOSEX contraction" % xprec,
+                "precision": ("%s amplitude and SOSEX contractions (Pi, Cholesky factors, "
+                              "Gram matrices and amplitude core in FP64)" % xprec),
                 "nguess": "ceil(3.0 N)", "mean_nt2": float(np.mean(nts))}
         if "fp64" in ex_val and "split_i8" in ex_val:
             exch["max_abs_dEx_split_i8_vs_fp64_Ha"] = float(
@@ -893,8 +895,8 @@ def main():
         "config": {"workload": "batch of %d uneven fragments, o~U{8..40}, v~U{70..350}, "
                                "N=round(4.5(o+v)), %d frequencies" % (a.nfrag, a.nfreqs),
                    "precision": a.precision,
-                   "components": ("E_RPA (batched, %s) + SOSEX exchange (fp64 T2, %s "
-                                  "contraction)" % (a.precision, exch["precision"].split()[-3]))
+                   "components": ("E_RPA (batched, %s) + exchange (T2 amplitudes + SOSEX, "
+                                  "%s contractions)" % (a.precision, xprec_of[a.precision]))
                                  if exch else "E_RPA only",
                    "graphs": bool(a.graphs), "autotune": tune,
                    "algorithmic_tflop_per_step": round((tot_syrk + tot_chol) / 1e12, 3)},
