@@ -74,3 +74,23 @@ def test_mbpt3_empty_amplitudes(ctx):
     A = np.zeros(0)
     assert ctx.mbpt3_1(V, A, f.R, f.eocc, f.evir) == 0.0
     assert ctx.mbpt3_2(f.R, V, A, 2, 3) == 0.0
+
+
+def test_mbpt3_1_indefinite_x_takes_the_eigen_route(ctx):
+    """Amplitudes of both signs (orthonormal V, A in (-0.3, 0.3)) give an
+    indefinite X, so -X has no Cholesky factor and the eigenpairs of X carry the
+    contraction; the energy still matches the oracle."""
+    o, v, n = 4, 20, 30
+    f = synth.make_fragment(o, v, n, 12)
+    rng = np.random.default_rng(5)
+    V, _ = np.linalg.qr(rng.standard_normal((o * v, 25)))
+    V = np.ascontiguousarray(V)
+    A = rng.uniform(-0.3, 0.3, 25)
+    Am, Bm = mbpt3.mbpt3_1_matrices(f.R, V, A, f.eocc, f.evir)
+    X = mbpt3.lyapunov(Bm, Am)
+    ev = np.linalg.eigvalsh(0.5 * (X + X.T))
+    assert ev.min() < 0.0 < ev.max()
+    e_ref = -mbpt3.trxk(f.R, V, X, o, v)
+    e = ctx.mbpt3_1(V, A, f.R, f.eocc, f.evir)
+    print("indefinite X: E", e_ref, e - e_ref)
+    assert abs(e - e_ref) <= 1e-10 * abs(e_ref)
