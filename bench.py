@@ -271,8 +271,8 @@ def bench_adjoint(a):
     tcheck = time.perf_counter() - t
     print("# FD worst relative %.2e, closed form worst relative %.2e (%.1f s)" %
           (worst, worst_cf, tcheck), file=sys.stderr, flush=True)
-    # per frequency: Pi SYRK (ov N^2), Cholesky (N^3/3), inverse from the factor
-    # (2 N^3/3), dsymm M R^T (2 ov N^2), dR accumulation (2 ov N)
+    # per frequency, algorithmic: Pi SYRK (ov N^2), Cholesky (N^3/3), inverse from
+    # the factor (2 N^3/3), M R^T (2 ov N^2), dR accumulation (2 ov N)
     flops = nf * (o * v * n * (n + 1) + n ** 3 / 3.0 + 2.0 * n ** 3 / 3.0 + 2.0 * o * v * n * n)
     achieved = flops / (kms * 1e-3) / 1e12
     # CPU baseline: the oracle's gradient on a bounded sample of the frequencies
@@ -298,7 +298,7 @@ def bench_adjoint(a):
            "roofline": {"bound": "mfma", "achieved": round(achieved, 3),
                         "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                         "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": None,
-                        "kernel": "whole adjoint (SYRK + potrf + potri + dsymm + accumulation)"},
+                        "kernel": "whole adjoint (SYRK + batched Cholesky + triangular inverse + dgemm + accumulation)"},
            "cpu_baseline": {"value": round(tc * 1e3, 1), "unit": "ms", "cores": thr,
                             "kind": "port",
                             "sample": "numpy FP64 oracle gradient on %d of the %d frequencies, "

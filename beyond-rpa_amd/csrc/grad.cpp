@@ -7,10 +7,11 @@
 //   dE/dd  = sum_k (R G_k R^T)_{ai,ai} dc_k/dd,   dc/dd = 4 (u^2 - d^2)/(d^2 + u^2)^2
 //   dE/de_a = sum_i dE/dd_ai,  dE/de_i = -sum_a dE/dd_ai
 // The frequency grid is held fixed.  Pi_k is recomputed here (FP64 MFMA SYRK),
-// not kept from the forward pass; (1 + Pi_k)^{-1} comes from rocSOLVER
-// potrf/potri and R (1 + Pi_k)^{-1} from rocBLAS dsymm.
+// not kept from the forward pass.  (1 + Pi_k)^{-1} = U_k^{-1} U_k^{-T} for all k
+// at once: the library's batched blocked Cholesky (1 + Pi_k = U_k^T U_k, U_k the
+// column-major upper triangle), rocBLAS's batched triangular inverse and one
+// strided-batched dgemm; then R (1 + Pi_k)^{-1} by one dgemm per frequency.
 #include <rocblas/rocblas.h>
-#include <rocsolver/rocsolver.h>
 
 #include <algorithm>
 #include <cmath>
@@ -23,6 +24,7 @@ int launch_grad_acc(brpa_ctx *, const double *X, const double *Rc, int64_t N, in
                     const double *c, const double *dc, double coef, double *dR, double *dd);
 int launch_fill(brpa_ctx *, double *p, int64_t n, double v);
 int rocblas_handle_of(brpa_ctx *ctx, void **h);
+int launch_zero_lower(brpa_ctx *, double *M, int64_t n);
 
 namespace {
 constexpr double kPi = 3.14159265358979323846;
@@ -78,14 +80,15 @@ extern "C" int brpa_ecrpa_grad(brpa_ctx *ctx, const double *R, int64_t naux, con
     const bool rdev = is_device_ptr(R), gdev = is_device_ptr(dR);
     FragDev fd{};
     std::vector<int64_t> pre = {0, 0, 0, 0, 0, 0};
-    const int64_t nt = cdiv(N, SYRK_T);
+    const int64_t nt = cdiv(N, SYRK_T), nsteps = cdiv(N, CHOL_NB);
     pre[1] = cdiv(F * K, 256);
     pre[3] = nt * (nt + 1) / 2 * cdiv(F, 4);
     pre[5] = cdiv(F * N, 256);
     FragDev *dfd;
     int64_t *dpre;
     double *dRd, *ddR, *dc, *ddc, *X, *dd;
-    int *dinfo;
+    double *Uinv;
+    int64_t *dchol;
     auto layout = [&](Arena &ar) {
         dfd = ar.take<FragDev>(1);
         dpre = ar.take<int64_t>(pre.size());
@@ -102,7 +105,11 @@ extern "C" int brpa_ecrpa_grad(brpa_ctx *ctx, const double *R, int64_t naux, con
         ddc = ar.take<double>(F * K);
         X = ar.take<double>(K * N);
         dd = ar.take<double>(K);
-        dinfo = ar.take<int>(2 * F);
+        fd.linv = ar.take<double>(F * CHOL_NB * CHOL_NB);
+        fd.ldpart = ar.take<double>(F * nsteps * 2);
+        fd.status = ar.take<int>(F);
+        dchol = ar.take<int64_t>(chol_pre_len(N));
+        Uinv = ar.take<double>(F * N * N);
     };
     Arena count;
     layout(count);
@@ -115,6 +122,8 @@ extern "C" int brpa_ecrpa_grad(brpa_ctx *ctx, const double *R, int64_t naux, con
     fd.F = (int)F;
     fd.nocc = (int)nocc;
     fd.nvir = (int)nvir;
+    fd.nsteps = (int)nsteps;
+    fd.keep_l11 = 1;
     if (!rdev) BRPA_TRY(to_device(ctx, dRd, R, 8 * K * N));
     hipStream_t s = ctx->stream;
     BRPA_HIP(ctx, hipMemcpyAsync(const_cast<double *>(fd.freqs), fx.data(), 8 * F, hipMemcpyHostToDevice, s));
@@ -126,40 +135,53 @@ extern "C" int brpa_ecrpa_grad(brpa_ctx *ctx, const double *R, int64_t naux, con
     BRPA_HIP(ctx, hipMemcpyAsync(dfd, &fd, sizeof(FragDev), hipMemcpyHostToDevice, s));
     BRPA_HIP(ctx, hipMemcpyAsync(dpre, pre.data(), 8 * pre.size(), hipMemcpyHostToDevice, s));
     BRPA_HIP(ctx, hipEventRecord(ctx->ev0, s));
-    // Pi_k (recomputed), 1 + Pi_k, Cholesky, inverse (lower triangle)
+    // Pi_k (recomputed), 1 + Pi_k, its Cholesky factor (all k batched)
     BRPA_TRY(launch_dtab(ctx, dfd, dpre, 1, pre[1]));
     BRPA_TRY(launch_syrk_f64(ctx, dfd, dpre + 2, 1, pre[3]));
     BRPA_TRY(launch_set_diag(ctx, dfd, dpre + 4, 1, pre[5], 0));
     BRPA_TRY(launch_fill(ctx, ddR, K * N, 0.0));
     BRPA_TRY(launch_fill(ctx, dd, K, 0.0));
+    BRPA_TRY(launch_fill_int(ctx, fd.status, F, 0));
+    BRPA_TRY(chol_single(ctx, dfd, dchol, N, F));
+    BRPA_TRY(rb(ctx,
+                rocblas_dtrtri_strided_batched(h, rocblas_fill_upper, rocblas_diagonal_non_unit,
+                                               (rocblas_int)N, fd.Pi, (rocblas_int)N, N * N, Uinv,
+                                               (rocblas_int)N, N * N, (rocblas_int)F),
+                "rocblas_dtrtri_strided_batched"));
+    for (int64_t k = 0; k < F; ++k) BRPA_TRY(launch_zero_lower(ctx, Uinv + k * N * N, N));
+    // M_k = U_k^{-1} U_k^{-T} (full, symmetric) in place of Pi_k
     const double one = 1.0, zero = 0.0;
+    BRPA_TRY(rb(ctx,
+                rocblas_dgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_transpose,
+                                              (rocblas_int)N, (rocblas_int)N, (rocblas_int)N, &one,
+                                              Uinv, (rocblas_int)N, N * N, Uinv, (rocblas_int)N,
+                                              N * N, &zero, fd.Pi, (rocblas_int)N, N * N,
+                                              (rocblas_int)F),
+                "rocblas_dgemm_strided_batched"));
     for (int64_t k = 0; k < F; ++k) {
-        double *A = fd.Pi + k * N * N;
-        BRPA_TRY(rb(ctx, rocsolver_dpotrf(h, rocblas_fill_lower, (rocblas_int)N, A, (rocblas_int)N,
-                                          dinfo + k), "rocsolver_dpotrf"));
-        BRPA_TRY(rb(ctx, rocsolver_dpotri(h, rocblas_fill_lower, (rocblas_int)N, A, (rocblas_int)N,
-                                          dinfo + F + k), "rocsolver_dpotri"));
-        // X(N x K) = M Rc  (M symmetric, lower triangle stored)
-        BRPA_TRY(rb(ctx, rocblas_dsymm(h, rocblas_side_left, rocblas_fill_lower, (rocblas_int)N,
-                                       (rocblas_int)K, &one, A, (rocblas_int)N, dRd, (rocblas_int)N,
-                                       &zero, X, (rocblas_int)N), "rocblas_dsymm"));
+        // X(N x K) = M_k Rc
+        BRPA_TRY(rb(ctx,
+                    rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)N,
+                                  (rocblas_int)K, (rocblas_int)N, &one, fd.Pi + k * N * N,
+                                  (rocblas_int)N, dRd, (rocblas_int)N, &zero, X, (rocblas_int)N),
+                    "rocblas_dgemm"));
         BRPA_TRY(launch_grad_acc(ctx, X, dRd, N, K, dc + k * K, ddc + k * K,
                                  fw[k] / (2.0 * kPi), ddR, dd));
     }
     BRPA_HIP(ctx, hipEventRecord(ctx->ev1, s));
     std::vector<double> hdd(K);
-    std::vector<int> info(2 * F);
+    std::vector<int> info(F);
     BRPA_HIP(ctx, hipMemcpyAsync(hdd.data(), dd, 8 * K, hipMemcpyDeviceToHost, s));
-    BRPA_HIP(ctx, hipMemcpyAsync(info.data(), dinfo, sizeof(int) * 2 * F, hipMemcpyDeviceToHost, s));
+    BRPA_HIP(ctx, hipMemcpyAsync(info.data(), fd.status, sizeof(int) * F, hipMemcpyDeviceToHost, s));
     if (!gdev) BRPA_TRY(from_device(ctx, dR, ddR, 8 * K * N));
     BRPA_HIP(ctx, hipStreamSynchronize(s));
     float ms = 0.f;
     BRPA_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     ctx->last_kernel_ms = ms;
-    for (int64_t k = 0; k < 2 * F; ++k)
+    for (int64_t k = 0; k < F; ++k)
         if (info[k] != 0)
             return brpa_fail(ctx, BRPA_ERR_NOT_SPD, "frequency %lld: 1 + Pi(u) is not positive definite",
-                             (long long)(k % F));
+                             (long long)k);
     // dE/de from dE/dd (small, host)
     std::vector<long double> go(nocc, 0.0L), gv(nvir, 0.0L);
     for (int64_t i = 0; i < nocc; ++i)

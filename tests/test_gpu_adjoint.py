@@ -15,8 +15,10 @@ from oracle import adjoint, erpa, grid
 pytestmark = pytest.mark.gpu
 
 
-def test_grad_matches_oracle_closed_form(ctx):
-    f = synth.make_fragment(4, 20, 60, seed=21)
+@pytest.mark.parametrize("o,v,n", [(4, 20, 60), (5, 30, 150), (3, 50, 200)])
+def test_grad_matches_oracle_closed_form(ctx, o, v, n):
+    """One Cholesky block (N < 64) and several with a ragged last block."""
+    f = synth.make_fragment(o, v, n, seed=21)
     x, w, _ = grid.freq_grid(f.eocc, f.evir, 12)
     dR, deo, dev = ctx.ecrpa_grad(f.R, f.eocc, f.evir, x, w)
     rR, ro, rv = adjoint.ecrpa_grad(f.R, f.eocc, f.evir, x, w)
