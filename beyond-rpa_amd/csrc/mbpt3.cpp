@@ -12,7 +12,8 @@
 // Schur form, so here the equation is solved by the scaled matrix-sign
 // (Newton) iteration, which keeps Bk^T X + X Bk = Ak at every step:
 //   B <- (cB + (cB)^-1)/2,  A <- (cA + c^-1 B^-T A B^-1)/2,  c = sqrt(|B^-1|_F/|B|_F),
-// until B = I, when X = A/2 (getrf/getri and dgemm on the device).  X is the
+// until B = I, when X = A/2 (getrf/getri and dgemm on the device; once
+// |B - 1|_F <= 1/4 the inverse-free Newton-Schulz step, all dgemm).  X is the
 // derivative of the amplitudes with the coupling strength in the V basis and
 // is negative definite in practice: -X = U'^T U' by the library's Cholesky
 // gives V X V^T = -(V U'^T)(V U'^T)^T, so the trace is the SOSEX contraction
@@ -209,7 +210,34 @@ int brpa_mbpt3_1(brpa_ctx *ctx, int precision, double *ec_exchange, const double
     const double tol = 1e-12 * std::sqrt((double)n);
     bool conv = false;
     int iters = 0;
+    double dev = HUGE_VAL;   // |B - 1|_F after the last step
     for (int it = 0; it < 100 && !conv; ++it) {
+        if (dev <= 0.25) {
+            // Newton-Schulz step on the same block matrix Z = [[B, 0], [A, -B^T]]:
+            // Z <- Z (3 - Z^2)/2, i.e. B <- B (3 - B^2)/2 and
+            // A <- (3A + B^T A B - A B^2 - (B^2)^T A)/2.  The map is odd, so
+            // Bk^T X + X Bk = Ak still holds; it needs no inverse and converges
+            // quadratically once |1 - B^2|_2 < 1, which |B - 1|_F <= 1/4 ensures.
+            BRPA_TRY(gemm(ctx, h, false, false, n, n, n, 1.0, Bm, n, Bm, n, 0.0, Bi, n));
+            BRPA_TRY(gemm(ctx, h, false, false, n, n, n, 1.0, Am, n, Bm, n, 0.0, T1, n));
+            BRPA_HIP(ctx, hipMemcpyAsync(Dm, Am, 8 * n * n, hipMemcpyDeviceToDevice, ctx->stream));
+            BRPA_TRY(gemm(ctx, h, true, false, n, n, n, 0.5, Bm, n, T1, n, 1.5, Dm, n));
+            BRPA_TRY(gemm(ctx, h, false, false, n, n, n, -0.5, Am, n, Bi, n, 1.0, Dm, n));
+            BRPA_TRY(gemm(ctx, h, true, false, n, n, n, -0.5, Bi, n, Am, n, 1.0, Dm, n));
+            std::swap(Am, Dm);
+            BRPA_TRY(launch_scale(ctx, Bi, -1.0, n * n));
+            BRPA_TRY(launch_add_diag(ctx, Bi, n, 3.0));
+            BRPA_TRY(gemm(ctx, h, false, false, n, n, n, 0.5, Bm, n, Bi, n, 0.0, T1, n));
+            std::swap(Bm, T1);
+            BRPA_HIP(ctx, hipMemcpyAsync(T1, Bm, 8 * n * n, hipMemcpyDeviceToDevice, ctx->stream));
+            BRPA_TRY(launch_add_diag(ctx, T1, n, -1.0));
+            BRPA_TRY(nrm2(ctx, h, T1, n * n, &dev));
+            if (!std::isfinite(dev))
+                return brpa_fail(ctx, BRPA_ERR_SOLVER, "brpa_mbpt3_1: Lyapunov iteration diverged");
+            conv = dev <= tol;
+            iters = it + 1;
+            continue;
+        }
         BRPA_HIP(ctx, hipMemcpyAsync(Bi, Bm, 8 * n * n, hipMemcpyDeviceToDevice, ctx->stream));
         BRPA_TRY(rbc(ctx, rocsolver_dgetrf(h, (rocblas_int)n, (rocblas_int)n, Bi, (rocblas_int)n,
                                            ipiv, info),
@@ -236,7 +264,6 @@ int brpa_mbpt3_1(brpa_ctx *ctx, int precision, double *ec_exchange, const double
         BRPA_TRY(launch_axpy(ctx, Bm, Bi, n * n));
         BRPA_HIP(ctx, hipMemcpyAsync(T1, Bm, 8 * n * n, hipMemcpyDeviceToDevice, ctx->stream));
         BRPA_TRY(launch_add_diag(ctx, T1, n, -1.0));
-        double dev = 0.0;
         BRPA_TRY(nrm2(ctx, h, T1, n * n, &dev));
         conv = dev <= tol;
         iters = it + 1;
