@@ -1,6 +1,8 @@
 // Host orchestration of the E_RPA hot path: Pi(u) build, exact diagonal,
 // batched blocked Cholesky log-det and the quadrature, for one fragment or a
 // batch of uneven fragments in the same launches.
+#include <rocblas/rocblas.h>
+
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
@@ -8,6 +10,10 @@
 #include <vector>
 
 #include "internal.h"
+
+int rocblas_handle_of(brpa_ctx *ctx, void **h);
+int launch_col_scale(brpa_ctx *, double *out, const double *in, const double *s, int64_t m,
+                     int64_t n);
 
 namespace {
 
@@ -614,20 +620,34 @@ int brpa_gpiug_mo(brpa_ctx *ctx, double *gpiug, const double *crg, int64_t naux,
             c[i * nvir + a] = prefactor * (d / (d * d + freq * freq));
         }
     const bool rdev = is_device_ptr(crg), pdev = is_device_ptr(gpiug);
+    void *hv = nullptr;
+    BRPA_TRY(rocblas_handle_of(ctx, &hv));
+    rocblas_handle h = static_cast<rocblas_handle>(hv);
     Arena count;
     count.take<double>(K);
+    count.take<double>(K * N);
     if (!rdev) count.take<double>(K * N);
     if (!pdev) count.take<double>(N * N);
     Arena ar;
     BRPA_TRY(ws_reserve(ctx, count.off, &ar));
     double *dc = ar.take<double>(K);
+    double *dW = ar.take<double>(K * N);
     const double *dR = rdev ? crg : ar.take<double>(K * N);
     double *dP = pdev ? gpiug : ar.take<double>(N * N);
-    BRPA_HIP(ctx, hipMemcpyAsync(dc, c.data(), 8 * K, hipMemcpyHostToDevice, ctx->stream));
+    BRPA_TRY(copy_sync(ctx, dc, c.data(), 8 * K, hipMemcpyHostToDevice));
     if (!rdev) BRPA_TRY(to_device(ctx, const_cast<double *>(dR), crg, 8 * K * N));
     if (!pdev) BRPA_TRY(to_device(ctx, dP, gpiug, 8 * N * N));
     BRPA_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-    BRPA_TRY(launch_syrk_single(ctx, dR, dc, (int)K, (int)N, dP, 1));
+    // the reference's own shape (rpa_core_MO.f90:398-402): DCRG = CRG diag(W),
+    // then GPiUG += CRG DCRG^T as one plain dgemm (column-major N x K operands)
+    BRPA_TRY(launch_col_scale(ctx, dW, dR, dc, N, K));
+    const double one = 1.0;
+    const rocblas_status st =
+        rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_transpose, (rocblas_int)N,
+                      (rocblas_int)N, (rocblas_int)K, &one, dR, (rocblas_int)N, dW, (rocblas_int)N,
+                      &one, dP, (rocblas_int)N);
+    if (st != rocblas_status_success)
+        return brpa_fail(ctx, BRPA_ERR_SOLVER, "rocblas_dgemm failed: rocblas status %d", (int)st);
     BRPA_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
     if (!pdev) BRPA_TRY(from_device(ctx, gpiug, dP, 8 * N * N));
     BRPA_HIP(ctx, hipStreamSynchronize(ctx->stream));

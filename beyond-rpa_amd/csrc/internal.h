@@ -218,7 +218,3 @@ int t2_build(brpa_ctx *ctx, int precision, const double *R, int64_t naux, const 
              uint64_t seed, int64_t *nvecs_t2, bool factored);
 int chol_single(brpa_ctx *ctx, const FragDev *d_fr, int64_t *d_pre, int64_t N, int64_t F);
 
-// Generic single-matrix FP64 SYRK used by brpa_gpiug_mo:
-//   C(N,N) (+)= X^T diag(d) X, X K x N row-major, both triangles written.
-int launch_syrk_single(brpa_ctx *, const double *X, const double *d, int K, int N, double *C,
-                       int accumulate);

@@ -62,11 +62,9 @@ int launch_dtab(brpa_ctx *ctx, const FragDev *fr, const int64_t *pre, int nfrag,
 constexpr int KB = 16;        // rows of R per LDS stage
 constexpr int LDS_PAD = 16;   // +16 doubles per row: rows r and r+1 land 32 banks apart
 
-template <bool kSingle>
 __device__ __forceinline__ void syrk_f64_tile(const double *__restrict__ R, int K, int N,
                                               const double *__restrict__ dtab, int F, int w0,
-                                              double *__restrict__ Pi, int tp, int tq,
-                                              int accumulate) {
+                                              double *__restrict__ Pi, int tp, int tq) {
     __shared__ double sA[KB][SYRK_T + LDS_PAD];
     __shared__ double sB[KB][SYRK_T + LDS_PAD];
     __shared__ double sD[4][KB];
@@ -94,10 +92,7 @@ __device__ __forceinline__ void syrk_f64_tile(const double *__restrict__ R, int 
         }
         if (tid < 4 * KB) {
             int ww = w0 + (tid / KB), k = k0 + (tid % KB);
-            if (kSingle)
-                rd = (tid < KB && k < K) ? dtab[k] : 0.0;
-            else
-                rd = (ww < F && k < K) ? dtab[(int64_t)ww * K + k] : 0.0;
+            rd = (ww < F && k < K) ? dtab[(int64_t)ww * K + k] : 0.0;
         }
     };
     auto lstore = [&]() {
@@ -120,7 +115,7 @@ __device__ __forceinline__ void syrk_f64_tile(const double *__restrict__ R, int 
 #pragma unroll
         for (int kk = 0; kk < KB; kk += 4) {
             const int kr = kk + (lane >> 4);
-            const double dv = sD[kSingle ? 0 : wave][kr];
+            const double dv = sD[wave][kr];
             double a[4], bv[4];
 #pragma unroll
             for (int i = 0; i < 4; ++i) a[i] = sA[kr][16 * i + (lane & 15)] * dv;
@@ -134,9 +129,8 @@ __device__ __forceinline__ void syrk_f64_tile(const double *__restrict__ R, int 
         }
     }
 
-    if (!kSingle && w >= F) return;
-    if (kSingle && wave != 0) return;
-    double *C = kSingle ? Pi : Pi + (int64_t)w * N * N;
+    if (w >= F) return;
+    double *C = Pi + (int64_t)w * N * N;
     const bool diag = (tp == tq);
 #pragma unroll
     for (int i = 0; i < 4; ++i)
@@ -148,13 +142,8 @@ __device__ __forceinline__ void syrk_f64_tile(const double *__restrict__ R, int 
                 int q = q0 + 16 * j + (lane & 15);
                 if (p < N && q < N) {
                     double v = acc[i][j][r];
-                    if (accumulate) {
-                        C[(int64_t)p * N + q] += v;
-                        if (!diag) C[(int64_t)q * N + p] += v;
-                    } else {
-                        C[(int64_t)p * N + q] = v;
-                        if (!diag) C[(int64_t)q * N + p] = v;
-                    }
+                    C[(int64_t)p * N + q] = v;
+                    if (!diag) C[(int64_t)q * N + p] = v;
                 }
             }
 }
@@ -172,33 +161,12 @@ __global__ __launch_bounds__(256) void k_syrk_f64(const FragDev *__restrict__ fr
     int g = (int)(lb / npairs), pair = (int)(lb - (int64_t)g * npairs);
     int tp, tq;
     tri_decode(pair, tp, tq);
-    syrk_f64_tile<false>(F.R, F.K, F.N, F.dtab, F.F, 4 * g, F.Pi, tp, tq, 0);
+    syrk_f64_tile(F.R, F.K, F.N, F.dtab, F.F, 4 * g, F.Pi, tp, tq);
 }
 
 int launch_syrk_f64(brpa_ctx *ctx, const FragDev *fr, const int64_t *pre, int nfrag, int64_t total) {
     if (total <= 0) return BRPA_OK;
     hipLaunchKernelGGL(k_syrk_f64, dim3(grid1(total)), dim3(256), 0, ctx->stream, fr, pre, nfrag);
-    LAUNCH_CHECK(ctx);
-    return BRPA_OK;
-}
-
-__global__ __launch_bounds__(256) void k_syrk_single(const double *__restrict__ X,
-                                                    const double *__restrict__ d, int K, int N,
-                                                    double *__restrict__ C, int accumulate) {
-    int nt = (N + SYRK_T - 1) / SYRK_T;
-    (void)nt;
-    int tp, tq;
-    tri_decode((int)blockIdx.x, tp, tq);
-    syrk_f64_tile<true>(X, K, N, d, 1, 0, C, tp, tq, accumulate);
-}
-
-int launch_syrk_single(brpa_ctx *ctx, const double *X, const double *d, int K, int N, double *C,
-                       int accumulate) {
-    int nt = (N + SYRK_T - 1) / SYRK_T;
-    int npairs = nt * (nt + 1) / 2;
-    if (npairs <= 0) return BRPA_OK;
-    hipLaunchKernelGGL(k_syrk_single, dim3(npairs), dim3(256), 0, ctx->stream, X, d, K, N, C,
-                       accumulate);
     LAUNCH_CHECK(ctx);
     return BRPA_OK;
 }

@@ -138,7 +138,8 @@ int brpa_optimize_freq_grid(const double *eocc, int64_t nocc, const double *evir
 
 /* ---- Pi(u) build --------------------------------------------------------- */
 /* GPiUG(N,N) += prefactor * CRG * diag(W) * CRG^T, W_ai = d_ai/(d_ai^2+u^2),
- * d_ai = evir[a] - eocc[i], ai = i*nvir + a.  Full symmetric matrix updated.
+ * d_ai = evir[a] - eocc[i], ai = i*nvir + a.  Every entry of GPiUG is updated
+ * (one dgemm CRG * DCRG^T, the shape of the reference's real_abT_x call).
  * Replaces rpa_GPiUG_MO (rpa/rpa_core_MO.f90:347-406) for one spin block;
  * prefactor is 4 for closed shells and 2 per spin otherwise (379-383).
  * crg is the Fortran CRG(N, nocc*nvir) (C: R[ai][P]). */
