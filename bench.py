@@ -435,8 +435,9 @@ def bench_mbpt3(a):
               flush=True)
     K = o * v
     r1 = res["mbpt3_1_split_i8"]
-    # algorithmic FP64 flop of the MBPT3-1 core: W, U, D, then per sign step getrf +
-    # getri (2 n^3) and two GEMMs (4 n^3)
+    # algorithmic FP64 flop of the MBPT3-1 core: W, U, D, then per sign step the
+    # Newton step's getrf + getri (2 n^3) and two GEMMs (4 n^3); the final
+    # Newton-Schulz steps (six GEMMs) are counted at the same 6 n^3
     core_flops = 2.0 * K * n * nt + 2.0 * n * nt * nt + 2.0 * K * nt * nt \
         + 6.0 * nt ** 3 * max(r1["sign_iterations"], 1)
     m2_flops = 2.0 * o * o * v * v * (nt + n)
