@@ -589,16 +589,17 @@ __global__ __launch_bounds__(64) void k_potrf_diag(const FragDev *__restrict__ f
     for (int c = 0; c < CHOL_NB; ++c) L[lane][c] = c <= lane ? a[c] : 0.0;
     __syncthreads();
     // column `lane` of X = L11^{-1}: x_i = (delta_i,lane - sum_{k<i} L_ik x_k) / L_ii
-    // (x_i = 0 for i < lane); a[] now holds x
+    // (x_i = 0 for i < lane); a[] now holds x.  The sum runs as four interleaved
+    // partial sums (k mod 4), so the dependent FMA chain per row is a quarter as long.
 #pragma unroll
     for (int i = 0; i < CHOL_NB; ++i) {
-        double s = (i == lane) ? 1.0 : 0.0;
+        double s[4] = {(i == lane) ? 1.0 : 0.0, 0.0, 0.0, 0.0};
 #pragma unroll
         for (int k = 0; k < i; ++k) {
-            s -= L[i][k] * a[k];
+            s[k & 3] -= L[i][k] * a[k];
             if (k % 16 == 15) __builtin_amdgcn_sched_barrier(0);
         }
-        a[i] = s / L[i][i];
+        a[i] = ((s[0] + s[1]) + (s[2] + s[3])) / L[i][i];
         __builtin_amdgcn_sched_barrier(0);
     }
     double *linv = F.linv + (int64_t)w * CHOL_NB * CHOL_NB;
@@ -611,9 +612,14 @@ __global__ __launch_bounds__(64) void k_potrf_diag(const FragDev *__restrict__ f
         for (int r = 0; r < nbj; ++r)
             if (lane <= r) A[(int64_t)(j0 + r) * N + j0 + lane] = L[r][lane];
     }
+    // log-det partial: one log per lane, then a fixed butterfly in double-double
+    dd_t sl = {lane < nbj ? log(L[lane][lane]) : 0.0, 0.0};
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const dd_t o = {__shfl_xor(sl.hi, off), __shfl_xor(sl.lo, off)};
+        sl = dd_add(sl, o);
+    }
     if (lane == 0) {
-        dd_t sl = {0.0, 0.0};
-        for (int c = 0; c < nbj; ++c) sl = dd_add(sl, log(L[c][c]));
         double *part = F.ldpart + ((int64_t)w * F.nsteps + step) * 2;
         part[0] = sl.hi;
         part[1] = sl.lo;
