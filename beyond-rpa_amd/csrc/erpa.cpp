@@ -110,10 +110,7 @@ Counts counts_of(const HostFrag &f, int nsteps_max, int precision, int i8_varian
         int64_t j0 = (int64_t)j * CHOL_NB;
         if (N > j0) c.pdiag[j] = F;
         if (N > j0 + CHOL_NB) c.trsm[j] = F * (cdiv(N - j0, CHOL_NB) - 1);
-        if (N > j0 + CHOL_NB) {
-            int64_t t = cdiv(N - j0 - CHOL_NB, 128);   // 128x128 update super-tiles
-            c.update[j] = F * (t * (t + 1) / 2);
-        }
+        c.update[j] = F * chol_upd_items(N, j);
     }
     return c;
 }
@@ -494,8 +491,7 @@ int chol_single(brpa_ctx *ctx, const FragDev *d_fr, int64_t *d_pre, int64_t N, i
         const int64_t j0 = (int64_t)j * CHOL_NB;
         pre[2 * j + 1] = (N > j0) ? F : 0;
         pre[2 * (nsteps + j) + 1] = (N > j0 + CHOL_NB) ? F * (cdiv(N - j0, CHOL_NB) - 1) : 0;
-        const int64_t t = (N > j0 + CHOL_NB) ? cdiv(N - j0 - CHOL_NB, 128) : 0;
-        pre[2 * (2 * nsteps + j) + 1] = F * (t * (t + 1) / 2);
+        pre[2 * (2 * nsteps + j) + 1] = F * chol_upd_items(N, j);
     }
     BRPA_TRY(copy_sync(ctx, d_pre, pre.data(), 8 * pre.size(), hipMemcpyHostToDevice));
     for (int j = 0; j < nsteps; ++j) {
@@ -678,8 +674,7 @@ static int ecrpa_stored(brpa_ctx *ctx, double *ecrpa, const double *piu, bool pa
         int64_t j0 = (int64_t)j * CHOL_NB;
         pre[2 * (1 + j) + 1] = (N > j0) ? F : 0;
         pre[2 * (1 + nsteps + j) + 1] = (N > j0 + CHOL_NB) ? F * (cdiv(N - j0, CHOL_NB) - 1) : 0;
-        int64_t t = (N > j0 + CHOL_NB) ? cdiv(N - j0 - CHOL_NB, 128) : 0;   // update super-tiles
-        pre[2 * (1 + 2 * nsteps + j) + 1] = F * (t * (t + 1) / 2);
+        pre[2 * (1 + 2 * nsteps + j) + 1] = F * chol_upd_items(N, j);
     }
     FragDev hd{};
     auto layout = [&](Arena &ar) {

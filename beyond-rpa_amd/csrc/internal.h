@@ -187,6 +187,9 @@ int launch_set_diag(brpa_ctx *, const FragDev *, const int64_t *pre, int nfrag, 
 int launch_potrf_diag(brpa_ctx *, const FragDev *, const int64_t *pre, int nfrag, int64_t total, int step);
 int launch_trsm(brpa_ctx *, const FragDev *, const int64_t *pre, int nfrag, int64_t total, int step);
 int launch_chol_update(brpa_ctx *, const FragDev *, const int64_t *pre, int nfrag, int64_t total, int step);
+// Work items per matrix of the trailing update after Cholesky step j (N x N):
+// the next 64-column strip after an even step, the rest after an odd one.
+int64_t chol_upd_items(int64_t N, int64_t j);
 // all Cholesky steps of the batch in one persistent launch (see kernels_f64.hip)
 int launch_chol_persistent(brpa_ctx *, const FragDev *, const int64_t *pre, int nfrag, int nfix,
                            int nsteps, const int64_t *poff, int *cnt, int *next, int *err, int nwg);

@@ -672,11 +672,25 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     const int j0 = step * CHOL_NB;
     const int t0 = j0 + CHOL_NB;
     const int nt = (N - t0 + UPD_T - 1) / UPD_T;
-    const int npairs = nt * (nt + 1) / 2;
+    // Outer panels of 128 columns: an even step updates only the next 64-column
+    // strip (rows >= t0, columns t0 .. t0+63) with its own panel (rank 64); an
+    // odd step updates the rest of the trailing matrix with the two panels of
+    // steps step-1 and step (rank 128), so A22 is read and written once per two
+    // steps.  Work items: nt row tiles (even) or nt(nt+1)/2 tile pairs (odd).
+    const bool strip = (step % 2) == 0;
+    const int kbase = strip ? j0 : j0 - CHOL_NB;    // first panel column
+    const int nph = (strip ? CHOL_NB : 2 * CHOL_NB) / UPD_KH;
+    const int colmax = strip ? min(N, t0 + CHOL_NB) : N;   // C columns < colmax
+    const int nitems = strip ? nt : nt * (nt + 1) / 2;
     int64_t lb = b - (nfrag <= FIND_LDS_MAX ? s_pre[f] : pre[f]);
-    const int w = (int)(lb / npairs), pair = (int)(lb - (int64_t)w * npairs);
+    const int w = (int)(lb / nitems), item = (int)(lb - (int64_t)w * nitems);
     int ti, tj;
-    tri_decode(pair, ti, tj);
+    if (strip) {
+        ti = item;
+        tj = 0;
+    } else {
+        tri_decode(item, ti, tj);
+    }
     double *A = F.Pi + (int64_t)w * N * N;
     const gdouble_t *G = (const gdouble_t *)A;
     const int Ri = t0 + UPD_T * ti, Rj = t0 + UPD_T * tj;
@@ -684,7 +698,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     const int wave = tid >> 6, lane = tid & 63;
     const int wa = wave >> 2, wq = wave & 3;   // rows 64*wa.., cols 32*wq..
     const int r0 = 64 * wa, c0 = 32 * wq;
-    const bool active = !(ti == tj && c0 > r0 + 63) && Ri + r0 < N && Rj + c0 < N;
+    const bool active = !(ti == tj && c0 > r0 + 63) && Ri + r0 < N && Rj + c0 < colmax;
 
     // panel phase h: rows Ri.. / Rj.. (128 each) x columns j0 + 16 h .. + 16;
     // thread -> column tid%16, rows tid/16 + 32 u (u < 4) of both panels
@@ -696,8 +710,8 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
         for (int u = 0; u < PH; ++u) {
             const int r = lr + RS * u;
             const bool oki = Ri + r < N, okj = Rj + r < N;
-            v[0][u] = G[oki ? (int64_t)(Ri + r) * N + j0 + UPD_KH * h + lc : 0];
-            v[1][u] = G[okj ? (int64_t)(Rj + r) * N + j0 + UPD_KH * h + lc : 0];
+            v[0][u] = G[oki ? (int64_t)(Ri + r) * N + kbase + UPD_KH * h + lc : 0];
+            v[1][u] = G[okj ? (int64_t)(Rj + r) * N + kbase + UPD_KH * h + lc : 0];
             v[0][u] = oki ? v[0][u] : 0.0;
             v[1][u] = okj ? v[1][u] : 0.0;
         }
@@ -724,7 +738,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
             for (int q = 0; q < 4; ++q) {
                 const int row = Ri + r0 + 16 * i + (lane >> 4) + 4 * q;
                 const int col = Rj + c0 + 16 * j + (lane & 15);
-                const bool ok = active && row < N && col < N && col <= row;
+                const bool ok = active && row < N && col < colmax && col <= row;
                 const double v = G[ok ? (int64_t)row * N + col : 0];
                 acc[i][j][q] = ok ? v : 0.0;
             }
@@ -746,7 +760,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
         }
     };
     mult();
-    for (int h = 1; h < CHOL_NB / UPD_KH; ++h) {
+    for (int h = 1; h < nph; ++h) {
         load_half(h, v0);
         __syncthreads();   // everyone is done with the previous phase
         store_half(v0);
@@ -762,9 +776,16 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
             for (int q = 0; q < 4; ++q) {
                 const int row = Ri + r0 + 16 * i + (lane >> 4) + 4 * q;
                 const int col = Rj + c0 + 16 * j + (lane & 15);
-                if (row < N && col < N && col <= row)
+                if (row < N && col < colmax && col <= row)
                     ((gdouble_mut_t *)A)[(int64_t)row * N + col] = acc[i][j][q];
             }
+}
+
+int64_t chol_upd_items(int64_t N, int64_t j) {
+    const int64_t t0 = (j + 1) * CHOL_NB;
+    if (N <= t0) return 0;
+    const int64_t t = (N - t0 + UPD_T - 1) / UPD_T;
+    return (j % 2 == 0) ? t : t * (t + 1) / 2;
 }
 
 int launch_chol_update(brpa_ctx *ctx, const FragDev *fr, const int64_t *pre, int nfrag,
