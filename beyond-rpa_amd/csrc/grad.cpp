@@ -126,14 +126,17 @@ extern "C" int brpa_ecrpa_grad(brpa_ctx *ctx, const double *R, int64_t naux, con
     fd.keep_l11 = 1;
     if (!rdev) BRPA_TRY(to_device(ctx, dRd, R, 8 * K * N));
     hipStream_t s = ctx->stream;
-    BRPA_HIP(ctx, hipMemcpyAsync(const_cast<double *>(fd.freqs), fx.data(), 8 * F, hipMemcpyHostToDevice, s));
-    BRPA_HIP(ctx, hipMemcpyAsync(const_cast<double *>(fd.weights), fw.data(), 8 * F, hipMemcpyHostToDevice, s));
-    BRPA_HIP(ctx, hipMemcpyAsync(const_cast<double *>(fd.eocc), eo.data(), 8 * nocc, hipMemcpyHostToDevice, s));
-    BRPA_HIP(ctx, hipMemcpyAsync(const_cast<double *>(fd.evir), ev.data(), 8 * nvir, hipMemcpyHostToDevice, s));
-    BRPA_HIP(ctx, hipMemcpyAsync(dc, cc.data(), 8 * F * K, hipMemcpyHostToDevice, s));
-    BRPA_HIP(ctx, hipMemcpyAsync(ddc, dcc.data(), 8 * F * K, hipMemcpyHostToDevice, s));
-    BRPA_HIP(ctx, hipMemcpyAsync(dfd, &fd, sizeof(FragDev), hipMemcpyHostToDevice, s));
-    BRPA_HIP(ctx, hipMemcpyAsync(dpre, pre.data(), 8 * pre.size(), hipMemcpyHostToDevice, s));
+    // host inputs go up with completed copies: no pageable host buffer is read
+    // after this point, also when a later step fails and the call returns early
+    const auto h2d = hipMemcpyHostToDevice;
+    BRPA_TRY(copy_sync(ctx, const_cast<double *>(fd.freqs), fx.data(), 8 * F, h2d));
+    BRPA_TRY(copy_sync(ctx, const_cast<double *>(fd.weights), fw.data(), 8 * F, h2d));
+    BRPA_TRY(copy_sync(ctx, const_cast<double *>(fd.eocc), eo.data(), 8 * nocc, h2d));
+    BRPA_TRY(copy_sync(ctx, const_cast<double *>(fd.evir), ev.data(), 8 * nvir, h2d));
+    BRPA_TRY(copy_sync(ctx, dc, cc.data(), 8 * F * K, h2d));
+    BRPA_TRY(copy_sync(ctx, ddc, dcc.data(), 8 * F * K, h2d));
+    BRPA_TRY(copy_sync(ctx, dfd, &fd, sizeof(FragDev), h2d));
+    BRPA_TRY(copy_sync(ctx, dpre, pre.data(), 8 * pre.size(), h2d));
     BRPA_HIP(ctx, hipEventRecord(ctx->ev0, s));
     // Pi_k (recomputed), 1 + Pi_k, its Cholesky factor (all k batched)
     BRPA_TRY(launch_dtab(ctx, dfd, dpre, 1, pre[1]));
@@ -171,8 +174,8 @@ extern "C" int brpa_ecrpa_grad(brpa_ctx *ctx, const double *R, int64_t naux, con
     BRPA_HIP(ctx, hipEventRecord(ctx->ev1, s));
     std::vector<double> hdd(K);
     std::vector<int> info(F);
-    BRPA_HIP(ctx, hipMemcpyAsync(hdd.data(), dd, 8 * K, hipMemcpyDeviceToHost, s));
-    BRPA_HIP(ctx, hipMemcpyAsync(info.data(), fd.status, sizeof(int) * F, hipMemcpyDeviceToHost, s));
+    BRPA_TRY(copy_sync(ctx, hdd.data(), dd, 8 * K, hipMemcpyDeviceToHost));
+    BRPA_TRY(copy_sync(ctx, info.data(), fd.status, sizeof(int) * F, hipMemcpyDeviceToHost));
     if (!gdev) BRPA_TRY(from_device(ctx, dR, ddR, 8 * K * N));
     BRPA_HIP(ctx, hipStreamSynchronize(s));
     float ms = 0.f;
