@@ -717,9 +717,9 @@ static int ecrpa_stored(brpa_ctx *ctx, double *ecrpa, const double *piu, bool pa
     }
     BRPA_TRY(to_device(ctx, const_cast<double *>(hd.weights), weights, 8 * F));
     BRPA_TRY(launch_fill_int(ctx, hd.status, F, 0));
-    BRPA_HIP(ctx, hipMemcpyAsync(d_fr, &hd, sizeof(FragDev), hipMemcpyHostToDevice, ctx->stream));
-    BRPA_HIP(ctx, hipMemcpyAsync(d_pre, pre.data(), 8 * pre.size(), hipMemcpyHostToDevice,
-                                 ctx->stream));
+    // completed copies: the descriptor and prefix sums live on this call's stack
+    BRPA_TRY(copy_sync(ctx, d_fr, &hd, sizeof(FragDev), hipMemcpyHostToDevice));
+    BRPA_TRY(copy_sync(ctx, d_pre, pre.data(), 8 * pre.size(), hipMemcpyHostToDevice));
     BRPA_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
     BRPA_TRY(launch_set_diag(ctx, d_fr, d_pre, 1, pre[1], 0));
     for (int j = 0; j < nsteps; ++j) {
@@ -734,9 +734,7 @@ static int ecrpa_stored(brpa_ctx *ctx, double *ecrpa, const double *piu, bool pa
     std::vector<int> st(F);
     BRPA_TRY(from_device(ctx, ecrpa, hd.energy, 8));
     if (eu) BRPA_TRY(from_device(ctx, eu, hd.eu, 8 * F));
-    BRPA_HIP(ctx, hipMemcpyAsync(st.data(), hd.status, sizeof(int) * F, hipMemcpyDeviceToHost,
-                                 ctx->stream));
-    BRPA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    BRPA_TRY(copy_sync(ctx, st.data(), hd.status, sizeof(int) * F, hipMemcpyDeviceToHost));
     float ms = 0.f;
     BRPA_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     ctx->last_kernel_ms = ms;
