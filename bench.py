@@ -216,7 +216,7 @@ def bench_single(a):
 
 def bench_adjoint(a):
     """dE_RPA/dR and dE_RPA/d eps of one fragment (o=48, v=400, NAux=2400, 16
-    frequencies, Pi recomputed per frequency) on one GPU; checked against
+    frequencies, Pi recomputed in the gradient call) on one GPU; checked against
     central finite differences on --fd-entries random R entries plus 6 orbital
     energies and against the FP64 closed form on 8 entries."""
     import numpy as np
@@ -290,7 +290,7 @@ def bench_adjoint(a):
            "scaling": "strong", "vs_baseline": None, "dtype": "f64",
            "data": "synthetic (seeded PCG64, brpa.synth)",
            "config": {"workload": "adjoint of one fragment o=48, v=400, NAux=2400, 16 "
-                                  "frequencies, Pi recomputed per frequency"},
+                                  "frequencies, Pi recomputed in the gradient call"},
            "kernel_ms_per_step": round(kms, 3), "energy_only_kernel_ms": round(t_e, 3),
            "fd_check": {"entries_R": len(ents), "orbital_energies": 6, "h": 1e-5,
                         "worst_relative": worst, "closed_form_entries": len(ref),

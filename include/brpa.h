@@ -411,9 +411,11 @@ int brpa_exchange_batch(brpa_ctx *ctx, int precision, int64_t nfrag, const brpa_
 /* ---- adjoint ----------------------------------------------------------------- */
 /* dE_RPA/dR (C R[ai][P], same shape as R) and dE_RPA/d(eps_occ), dE_RPA/d(eps_vir)
  * of the FP64 energy of brpa_ecrpa, with the frequency grid held fixed.  Pi(u)
- * is recomputed per frequency, never stored from the forward pass.  No
- * reference counterpart: the reference computes no RPA gradients.  deocc/devir
- * may be NULL. */
+ * of every frequency is recomputed in this call, never kept from the forward
+ * pass; the call's device workspace holds 2 F N^2 + N K + O(F K) doubles,
+ * plus N K for each of R and dR given as a host buffer (F = nfreqs,
+ * N = naux, K = nocc*nvir).  No reference counterpart: the
+ * reference computes no RPA gradients.  deocc/devir may be NULL. */
 int brpa_ecrpa_grad(brpa_ctx *ctx, const double *R, int64_t naux, const double *eocc,
                     int64_t nocc, const double *evir, int64_t nvir, const double *freqs,
                     const double *weights, int64_t nfreqs, double *dR, double *deocc,
