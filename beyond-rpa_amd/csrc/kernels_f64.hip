@@ -700,7 +700,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     const int r0 = 64 * wa, c0 = 32 * wq;
     const bool active = !(ti == tj && c0 > r0 + 63) && Ri + r0 < N && Rj + c0 < colmax;
 
-    // panel phase h: rows Ri.. / Rj.. (128 each) x columns j0 + 16 h .. + 16;
+    // panel phase h (h < nph): rows Ri.. / Rj.. (128 each) x columns kbase + 16 h .. + 16;
     // thread -> column tid%16, rows tid/16 + 32 u (u < 4) of both panels
     constexpr int RS = 512 / UPD_KH;        // rows per load sweep
     constexpr int PH = UPD_T / RS;
