@@ -84,6 +84,7 @@ int brpa_rpa_g(brpa_ctx *ctx, int basis, double *G, double *RWR, int64_t ncholes
         BRPA_TRY(launch_add_diag(ctx, dG, n, 1.0));
     } else if (basis == BRPA_BASIS_EIGEN) {
         // eigenpairs of -RWR (ascending), so that the largest eigenvalue of RWR comes first
+        std::lock_guard<std::mutex> lk(solver_mutex());
         BRPA_HIP(ctx, hipMemcpyAsync(dG, dR, 8 * n * n, hipMemcpyDeviceToDevice, ctx->stream));
         BRPA_TRY(launch_scale(ctx, dG, -1.0, n * n));
         BRPA_TRY(rbc(ctx, rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_upper,
@@ -98,6 +99,7 @@ int brpa_rpa_g(brpa_ctx *ctx, int basis, double *G, double *RWR, int64_t ncholes
         // Gaussian G, two subspace iterations with QR, then RWR <- G^T RWR G
         BRPA_TRY(launch_gauss(ctx, dG, n * n, seed));
         for (int q = 0; q < 2; ++q) {
+            std::lock_guard<std::mutex> lk(solver_mutex());
             BRPA_TRY(dgemm(ctx, h, false, n, dR, dG, T));
             BRPA_TRY(rbc(ctx, rocsolver_dgeqrf(h, (rocblas_int)n, (rocblas_int)n, T,
                                                (rocblas_int)n, ev),

@@ -5,6 +5,11 @@
 
 #include "internal.h"
 
+std::mutex &solver_mutex() {
+    static std::mutex mu;
+    return mu;
+}
+
 int brpa_fail(brpa_ctx *ctx, int code, const char *fmt, ...) {
     char buf[1024];
     va_list ap;

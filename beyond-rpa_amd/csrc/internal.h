@@ -5,6 +5,7 @@
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -92,6 +93,10 @@ int stage_collect(brpa_ctx *ctx, int stage);
 void stage_reset(brpa_ctx *ctx);
 
 int brpa_fail(brpa_ctx *ctx, int code, const char *fmt, ...);
+
+// One lock for every rocSOLVER call of the library: contexts driven from
+// several host threads (the exchange batch's workers) take turns in it.
+std::mutex &solver_mutex();
 
 #define BRPA_HIP(ctx, expr)                                                              \
     do {                                                                                 \

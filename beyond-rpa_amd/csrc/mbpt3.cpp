@@ -239,13 +239,17 @@ int brpa_mbpt3_1(brpa_ctx *ctx, int precision, double *ec_exchange, const double
             continue;
         }
         BRPA_HIP(ctx, hipMemcpyAsync(Bi, Bm, 8 * n * n, hipMemcpyDeviceToDevice, ctx->stream));
-        BRPA_TRY(rbc(ctx, rocsolver_dgetrf(h, (rocblas_int)n, (rocblas_int)n, Bi, (rocblas_int)n,
-                                           ipiv, info),
-                     "rocsolver_dgetrf"));
-        BRPA_TRY(rbc(ctx, rocsolver_dgetri(h, (rocblas_int)n, Bi, (rocblas_int)n, ipiv, info + 1),
-                     "rocsolver_dgetri"));
         int hinfo[2] = {0, 0};
-        BRPA_TRY(copy_sync(ctx, hinfo, info, sizeof(hinfo), hipMemcpyDeviceToHost));
+        {
+            std::lock_guard<std::mutex> lk(solver_mutex());
+            BRPA_TRY(rbc(ctx, rocsolver_dgetrf(h, (rocblas_int)n, (rocblas_int)n, Bi,
+                                               (rocblas_int)n, ipiv, info),
+                         "rocsolver_dgetrf"));
+            BRPA_TRY(rbc(ctx, rocsolver_dgetri(h, (rocblas_int)n, Bi, (rocblas_int)n, ipiv,
+                                               info + 1),
+                         "rocsolver_dgetri"));
+            BRPA_TRY(copy_sync(ctx, hinfo, info, sizeof(hinfo), hipMemcpyDeviceToHost));
+        }
         if (hinfo[0] != 0 || hinfo[1] != 0)
             return brpa_fail(ctx, BRPA_ERR_SOLVER,
                              "brpa_mbpt3_1: singular matrix in the Lyapunov iteration (step %d)", it);
@@ -292,6 +296,7 @@ int brpa_mbpt3_1(brpa_ctx *ctx, int precision, double *ec_exchange, const double
         BRPA_TRY(launch_fill(ctx, ctx->x_B, n, -1.0));
     } else {
         // eigenpairs X = E diag(lambda) E^T; V''^T = E^T V^T
+        std::lock_guard<std::mutex> lk(solver_mutex());
         BRPA_HIP(ctx, hipMemcpyAsync(T1, Am, 8 * n * n, hipMemcpyDeviceToDevice, ctx->stream));
         BRPA_TRY(launch_scale(ctx, T1, 0.5, n * n));
         BRPA_TRY(rbc(ctx, rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_upper,

@@ -19,6 +19,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "internal.h"
@@ -30,6 +31,8 @@ int launch_row_scale_acc(brpa_ctx *, double *io, const double *in, const double 
                          int64_t m, int64_t n);
 int launch_select_scale(brpa_ctx *, double *W, const double *U, const double *ev, int64_t m,
                         int64_t c0, int64_t nt);
+int launch_col_scale(brpa_ctx *, double *out, const double *in, const double *s, int64_t m,
+                     int64_t n);
 int launch_scale(brpa_ctx *, double *x, double a, int64_t n);
 int launch_add_diag(brpa_ctx *, double *M, int64_t n, double a);
 int launch_zero_lower(brpa_ctx *, double *M, int64_t n);
@@ -97,18 +100,32 @@ int gemm(brpa_ctx *ctx, rocblas_handle h, bool ta, bool tb, int64_t m, int64_t n
                     "rocblas_dgemm");
 }
 
-// eigenpairs of the symmetric matrix whose column-major upper triangle A holds
+// eigenpairs of the symmetric matrix whose column-major upper triangle A holds.
+// rocSOLVER calls are serialised over the whole library (solver_mutex): worker
+// contexts of the exchange batch run them from several host threads.
 int syevd(brpa_ctx *ctx, rocblas_handle h, int64_t n, double *A, double *D, double *E, int *info) {
-    BRPA_TRY(rb_check(ctx,
-                      rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_upper,
-                                       (rocblas_int)n, A, (rocblas_int)n, D, E, info),
-                      "rocsolver_dsyevd"));
     int hinfo = 0;
-    BRPA_HIP(ctx, hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-    BRPA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    {
+        std::lock_guard<std::mutex> lk(solver_mutex());
+        BRPA_TRY(rb_check(ctx,
+                          rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_upper,
+                                           (rocblas_int)n, A, (rocblas_int)n, D, E, info),
+                          "rocsolver_dsyevd"));
+        BRPA_TRY(copy_sync(ctx, &hinfo, info, sizeof(int), hipMemcpyDeviceToHost));
+    }
     if (hinfo != 0)
         return brpa_fail(ctx, BRPA_ERR_SOLVER, "dsyevd did not converge (info=%d, n=%lld)", hinfo,
                          (long long)n);
+    return BRPA_OK;
+}
+
+// inverse of A = U^T U from its factor U (column-major upper triangle), in place
+int potri(brpa_ctx *ctx, rocblas_handle h, int64_t n, double *U, int *info) {
+    std::lock_guard<std::mutex> lk(solver_mutex());
+    BRPA_TRY(rb_check(ctx, rocsolver_dpotri(h, rocblas_fill_upper, (rocblas_int)n, U,
+                                            (rocblas_int)n, info),
+                      "rocsolver_dpotri"));
+    BRPA_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return BRPA_OK;
 }
 
@@ -184,8 +201,16 @@ int t2_build(brpa_ctx *ctx, int precision, const double *R, int64_t naux, const 
         return brpa_fail(ctx, BRPA_ERR_ARG,
                          "brpa_t2_compute: nocc*nvir = %lld exceeds %d, the largest contraction "
                          "with exact int32 sums in the split-int8 form", (long long)K, I8_KMAX);
-    const int64_t NG = nguess > 0 ? std::min<int64_t>(nguess, K)
-                                  : std::min<int64_t>(K, (int64_t)std::ceil(3.0 * (double)N));
+    // GuessNVecsT2 as given, ceiling(3 NVecsPiU) by default (rpa.f90:1696), with no
+    // cap at o*v: the reference's I*Omega has GuessNVecsT2 columns and its Gram
+    // matrix is NG x NG whatever o*v is (rpa_CC_Doubles.f90:543, 595-597)
+    const int64_t NG = nguess > 0 ? nguess : (int64_t)std::ceil(3.0 * (double)N);
+    if (NG > INT32_MAX / 2)
+        return brpa_fail(ctx, BRPA_ERR_ARG, "brpa_t2_compute: %lld guess vectors are beyond the "
+                         "supported range", (long long)NG);
+    // NG >= o*v: the guess space oversamples the whole ov space, S has rank <= o*v,
+    // and the basis is found in the ov space itself (see the oversampled route below)
+    const bool over = NG >= K;
     std::vector<double> eo, ev, fx, fw;
     BRPA_TRY(host_vec(ctx, eo, eocc, nocc));
     BRPA_TRY(host_vec(ctx, ev, evir, nvir));
@@ -210,6 +235,20 @@ int t2_build(brpa_ctx *ctx, int precision, const double *R, int64_t naux, const 
             }
 
     const bool rdev = is_device_ptr(R);
+    // order of the dense symmetric matrices of the basis step: S (NG x NG), or
+    // in the oversampled route the ov x ov Gram matrix of the other side
+    const int64_t nd = over ? K : NG;
+    // oversampled route: chi_k(ai) (w_k l / (4 pi))^(1/2), so that
+    // I = -sum_k Y_k^T Y_k with Y_k = Rt_k^T diag(chis_k)
+    double *chis = nullptr;
+    std::vector<double> hchis;
+    if (over) {
+        hchis.resize(F * K);
+        for (int64_t k = 0; k < F; ++k) {
+            const double sk = std::sqrt(std::fabs(fw[k] * lambda) / (4.0 * kPi));
+            for (int64_t i = 0; i < K; ++i) hchis[k * K + i] = chi[k * K + i] * sk;
+        }
+    }
     // workspace layout
     const int64_t nsteps = cdiv(N, CHOL_NB);
     FragDev fd{};
@@ -246,12 +285,14 @@ int t2_build(brpa_ctx *ctx, int precision, const double *R, int64_t naux, const 
         fd.evir = ar.take<double>(nvir);
         dchi = ar.take<double>(F * K);
         omega = ar.take<double>(K * NG);
-        tmp = ar.take<double>(K * NG);
+        // oversampled route: tmp holds the explicit ov x ov I and Sm the ov x ov
+        // Gram matrix of the other side; no Y and no split operands
+        tmp = ar.take<double>(over ? K * K : K * NG);
         iom = ar.take<double>(K * NG);
-        Y = ar.take<double>(N * NG);
-        Sm = ar.take<double>(NG * NG);
-        sev = ar.take<double>(NG);
-        sE = ar.take<double>(NG);
+        Y = over ? nullptr : ar.take<double>(N * NG);
+        Sm = ar.take<double>(nd * nd);
+        sev = ar.take<double>(nd);
+        sE = ar.take<double>(nd);
         dinfo = ar.take<int>(F + 2);
         fd.linv = ar.take<double>(F * CHOL_NB * CHOL_NB);
         fd.ldpart = ar.take<double>(F * nsteps * 2);
@@ -259,10 +300,11 @@ int t2_build(brpa_ctx *ctx, int precision, const double *R, int64_t naux, const 
         dchol = ar.take<int64_t>(chol_pre_len(N));
         dfm = ar.take<FragDev>(1);
         fm.linv = ar.take<double>(CHOL_NB * CHOL_NB);
-        fm.ldpart = ar.take<double>(cdiv(NG, CHOL_NB) * 2);
+        fm.ldpart = ar.take<double>(cdiv(nd, CHOL_NB) * 2);
         fm.status = ar.take<int>(1);
-        dchol2 = ar.take<int64_t>(chol_pre_len(NG));
-        if (split) {
+        dchol2 = ar.take<int64_t>(chol_pre_len(nd));
+        if (over) chis = ar.take<double>(F * K);
+        if (split && !over) {
             SCR = ar.take<int8_t>(F * 5 * N * Kp);
             SRR = ar.take<int8_t>(5 * K * Np);
             SOT = ar.take<int8_t>(5 * NG * Kp);
@@ -300,6 +342,7 @@ int t2_build(brpa_ctx *ctx, int precision, const double *R, int64_t naux, const 
     BRPA_TRY(copy_sync(ctx, const_cast<double *>(fd.evir), ev.data(), 8 * nvir,
                        hipMemcpyHostToDevice));
     BRPA_TRY(copy_sync(ctx, dchi, chi.data(), 8 * F * K, hipMemcpyHostToDevice));
+    if (over) BRPA_TRY(copy_sync(ctx, chis, hchis.data(), 8 * F * K, hipMemcpyHostToDevice));
     BRPA_TRY(copy_sync(ctx, dfd, &fd, sizeof(FragDev), hipMemcpyHostToDevice));
     BRPA_TRY(copy_sync(ctx, dpre, pre.data(), 8 * pre.size(), hipMemcpyHostToDevice));
     BRPA_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
@@ -359,96 +402,6 @@ int t2_build(brpa_ctx *ctx, int precision, const double *R, int64_t naux, const 
         return BRPA_OK;
     };
 
-    // I * Omega (rpa_CC_IM_v5, rpa/rpa_CC_Doubles.f90:369-422)
-    BRPA_TRY(begin(BRPA_STAGE_T2_IOMEGA));
-    BRPA_TRY(launch_guess(ctx, omega, K, NG, seed));
-    BRPA_TRY(launch_normalize_columns(ctx, omega, K, NG));
-    BRPA_HIP(ctx, hipMemsetAsync(iom, 0, 8 * K * NG, ctx->stream));
-    // frequency-independent operand of the split form, once per fragment
-    if (split) BRPA_TRY(launch_colslice_i8(ctx, omega, K, K, NG, nullptr, Kp, scOT, SOT));
-    for (int64_t k = 0; k < F; ++k) {
-        const double *ck = dchi + k * K;
-        const double coef = -fw[k] * lambda / (4.0 * kPi);
-        const double *Rtk = Rt + k * K * N;
-        if (split) {
-            // Y = (chi_k Rt_k)^T Omega, I Omega += coef chi_k Rt_k Y; the chi_k Rt_k
-            // slices are kept for T^T I T
-            int8_t *SCRk = SCR + k * 5 * N * Kp;
-            double *scCRk = scCR + k * N;
-            BRPA_TRY(launch_rowslice_i8(ctx, Rtk, K, N, ck, pmx, scCRk, SCRk));
-            BRPA_TRY(launch_gemm_i8(ctx, SCRk, scCRk, N, SOT, scOT, NG, Kp, 1.0, nullptr, 0.0, Y, 1,
-                                    N, K));
-            BRPA_TRY(launch_colslice_i8(ctx, Rtk, N, N, K, nullptr, Np, scRR, SRR));
-            BRPA_TRY(launch_colslice_i8(ctx, Y, N, N, NG, nullptr, Np, scY, SY));
-            BRPA_TRY(launch_gemm_i8(ctx, SRR, scRR, K, SY, scY, NG, Np, coef, ck, 1.0, iom, 1, K,
-                                    N));
-            continue;
-        }
-        BRPA_TRY(launch_row_scale(ctx, tmp, omega, ck, K, NG));
-        BRPA_TRY(gemm(ctx, h, false, false, N, NG, K, 1.0, Rtk, N, tmp, K, 0.0, Y, N));
-        BRPA_TRY(gemm(ctx, h, true, false, K, NG, N, 1.0, Rtk, N, Y, N, 0.0, tmp, K));
-        BRPA_TRY(launch_row_scale_acc(ctx, iom, tmp, ck, coef, K, NG));
-    }
-    BRPA_TRY(launch_normalize_columns(ctx, iom, K, NG));
-    BRPA_TRY(stage_end(ctx, BRPA_STAGE_T2_IOMEGA));
-
-    // S = (I Omega)^T (I Omega) (rpa_CC_Doubles.f90:595-609).  The reference
-    // keeps the eigenvectors of S with eigenvalues above the cutoff and forms
-    // the orthonormal basis T = (I Omega) U_s s^{-1/2}.  When every eigenvalue
-    // exceeds the cutoff -- exactly when S - cutoff*1 has a Cholesky factor --
-    // T spans all of span(I Omega), and any orthonormal basis of that span
-    // gives the same Rayleigh-Ritz pairs below; it is then built by Cholesky-QR
-    // (the shifted factor, then one unshifted pass) instead of the NG x NG
-    // eigensolve.  Otherwise the eigensolve decides which vectors are kept.
-    // Two passes suffice: with every eigenvalue above the cutoff, X = I Omega
-    // has kappa(X) <= (NG / cutoff)^(1/2) < 1e8, the first pass leaves
-    // X1^T X1 = 1 + cutoff U1^-T U1^-1 + O(eps kappa^2), and the second pass
-    // restores orthonormality to O(eps kappa(X1)^2) -- never worse than the
-    // eps s_max / s_min of the reference's own eigenvector basis.
-    BRPA_TRY(begin(BRPA_STAGE_T2_BASIS));
-    // Gram matrices as full GEMMs: rocBLAS dgemm takes a third of dsyrk's time here
-    BRPA_TRY(gemm(ctx, h, true, false, NG, NG, K, 1.0, iom, K, iom, K, 0.0, Sm, NG));
-    double *Tkeep = iom;           // K x nt orthonormal basis (in place of I Omega)
-    int64_t nt = 0;
-    bool cholqr = false;
-    BRPA_TRY(launch_add_diag(ctx, Sm, NG, -cutoff));
-    BRPA_TRY(chol_dense(Sm, NG, &cholqr));
-    const double one = 1.0;
-    if (cholqr) {
-        nt = NG;
-        for (int pass = 0; pass < 2; ++pass) {
-            if (pass > 0) {
-                BRPA_TRY(gemm(ctx, h, true, false, NG, NG, K, 1.0, iom, K, iom, K, 0.0, Sm, NG));
-                bool okp = false;
-                BRPA_TRY(chol_dense(Sm, NG, &okp));
-                if (!okp)
-                    return brpa_fail(ctx, BRPA_ERR_SOLVER,
-                                     "Cholesky-QR pass %d of the T2 basis broke down", pass);
-            }
-            // X <- X U^{-1}, so that X^T X = 1
-            BRPA_TRY(rb_check(ctx,
-                              rocblas_dtrsm(h, rocblas_side_right, rocblas_fill_upper,
-                                            rocblas_operation_none, rocblas_diagonal_non_unit,
-                                            (rocblas_int)K, (rocblas_int)NG, &one, Sm,
-                                            (rocblas_int)NG, iom, (rocblas_int)K),
-                              "rocblas_dtrsm"));
-        }
-    } else {
-        BRPA_TRY(gemm(ctx, h, true, false, NG, NG, K, 1.0, iom, K, iom, K, 0.0, Sm, NG));
-        BRPA_TRY(syevd(ctx, h, NG, Sm, sev, sE, dinfo + F));
-        std::vector<double> hs(NG);
-        BRPA_TRY(copy_sync(ctx, hs.data(), sev, 8 * NG, hipMemcpyDeviceToHost));
-        for (int64_t i = NG - 1; i >= 0; --i) {
-            if (hs[i] > cutoff)
-                ++nt;
-            else
-                break;
-        }
-    }
-    *nvecs_t2 = nt;
-    ctx->t2_nt = nt;
-    ctx->t2_nai = K;
-    ctx->t2_factored = false;
     // the stage times of this call, once the stream is idle
     auto collect = [&]() -> int {
         BRPA_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -456,39 +409,198 @@ int t2_build(brpa_ctx *ctx, int precision, const double *R, int64_t naux, const 
             if (staged[st]) BRPA_TRY(stage_collect(ctx, st));
         return gemm_i8_collect(ctx);
     };
-    if (nt == 0) {
-        BRPA_TRY(stage_end(ctx, BRPA_STAGE_T2_BASIS));
-        return collect();
-    }
-
-    // T = (I Omega) U_s s^{-1/2}; TIT = T^T I T (rpa_CC_MIM_v5, 425-488)
-    double *T = tmp;               // K x nt
-    double *G = Y;                 // N x nt
-    if (!cholqr) {
-        double *Wm = omega;        // NG x nt   (omega no longer needed)
-        BRPA_TRY(launch_select_scale(ctx, Wm, Sm, sev, NG, NG - nt, nt));
-        BRPA_TRY(gemm(ctx, h, false, false, K, nt, NG, 1.0, iom, K, Wm, NG, 0.0, T, K));
-        BRPA_HIP(ctx, hipMemcpyAsync(Tkeep, T, 8 * K * nt, hipMemcpyDeviceToDevice, ctx->stream));
-    }
-    BRPA_TRY(stage_end(ctx, BRPA_STAGE_T2_BASIS));
-    BRPA_TRY(begin(BRPA_STAGE_T2_TIT));
-    double *TIT = Sm;              // nt x nt
-    if (split) BRPA_TRY(launch_colslice_i8(ctx, Tkeep, K, K, nt, nullptr, Kp, scOT, SOT));
-    for (int64_t k = 0; k < F; ++k) {
-        const double *ck = dchi + k * K;
-        if (split) {
-            // G = (chi_k Rt_k)^T T
-            BRPA_TRY(launch_gemm_i8(ctx, SCR + k * 5 * N * Kp, scCR + k * N, N, SOT, scOT, nt, Kp,
-                                    1.0, nullptr, 0.0, G, 1, N, K));
-        } else {
-            BRPA_TRY(launch_row_scale(ctx, T, Tkeep, ck, K, nt));
-            BRPA_TRY(gemm(ctx, h, false, false, N, nt, K, 1.0, Rt + k * K * N, N, T, K, 0.0, G, N));
+    const double one = 1.0;
+    int64_t nt = 0;
+    double *Tkeep = iom;    // K x nt orthonormal basis of the kept subspace
+    bool ident = false;     // the kept subspace is the whole ov space, T = 1
+    double *TIT = Sm;       // nt x nt, T^T I T
+    if (over) {
+        // Oversampled route, NG >= o*v.  With X = I Omega (columns normalised,
+        // K x NG), S = X^T X (NG x NG) and M = X X^T (K x K) have the same
+        // nonzero eigenvalues, and S has NG - K more that are zero; the kept
+        // vectors T = X U_s s^{-1/2} of the reference (rpa_CC_Doubles.f90:
+        // 595-624) are the eigenvectors of M with eigenvalues above the cutoff.
+        // When all K exceed it -- exactly when M - cutoff*1 has a Cholesky
+        // factor -- T spans the whole ov space, NT2 = K, and T = 1 gives the
+        // same Rayleigh-Ritz pairs; otherwise the K x K eigensolve of M decides.
+        // I is formed explicitly: I = -sum_k Y_k^T Y_k with Y_k = Rt_k^T
+        // diag(chi_k) (w_k l/(4 pi))^(1/2) costs 2 F K^2 N flop, less than I*Omega
+        // through Rt_k (4 F K N NG) whenever K <= NG.
+        double *Im = tmp;   // K x K
+        BRPA_TRY(begin(BRPA_STAGE_T2_IOMEGA));
+        for (int64_t k = 0; k < F; ++k) {
+            double *Yk = Rt + k * K * N;   // column-major N x K, scaled in place
+            BRPA_TRY(launch_col_scale(ctx, Yk, Yk, chis + k * K, N, K));
+            const double a = fw[k] * lambda >= 0.0 ? -1.0 : 1.0;
+            BRPA_TRY(gemm(ctx, h, true, false, K, K, N, a, Yk, N, Yk, N, k == 0 ? 0.0 : 1.0, Im,
+                          K));
         }
-        // G = Rt_k^T chi_k T = U_k^-T R^T chi_k T, so G^T G = T^T chi R (1 + l Pi_k)^-1 R^T chi T
-        BRPA_TRY(gemm(ctx, h, true, false, nt, nt, N, -fw[k] * lambda / (4.0 * kPi), G, N, G, N,
-                      k == 0 ? 0.0 : 1.0, TIT, nt));
+        // I Omega, columns normalised (rpa_CC_Doubles.f90:572-588)
+        BRPA_TRY(launch_guess(ctx, omega, K, NG, seed));
+        BRPA_TRY(launch_normalize_columns(ctx, omega, K, NG));
+        BRPA_TRY(gemm(ctx, h, false, false, K, NG, K, 1.0, Im, K, omega, K, 0.0, iom, K));
+        BRPA_TRY(launch_normalize_columns(ctx, iom, K, NG));
+        BRPA_TRY(stage_end(ctx, BRPA_STAGE_T2_IOMEGA));
+        BRPA_TRY(begin(BRPA_STAGE_T2_BASIS));
+        double *Mm = Sm;    // K x K
+        BRPA_TRY(gemm(ctx, h, false, true, K, K, NG, 1.0, iom, K, iom, K, 0.0, Mm, K));
+        BRPA_TRY(launch_add_diag(ctx, Mm, K, -cutoff));
+        bool full = false;
+        BRPA_TRY(chol_dense(Mm, K, &full));
+        if (full) {
+            nt = K;
+            ident = true;
+            TIT = Im;       // T^T I T = I
+        } else {
+            BRPA_TRY(gemm(ctx, h, false, true, K, K, NG, 1.0, iom, K, iom, K, 0.0, Mm, K));
+            BRPA_TRY(syevd(ctx, h, K, Mm, sev, sE, dinfo + F));
+            std::vector<double> hs(K);
+            BRPA_TRY(copy_sync(ctx, hs.data(), sev, 8 * K, hipMemcpyDeviceToHost));
+            // counted from the largest down, stopping at the first one below (600-607)
+            for (int64_t i = K - 1; i >= 0; --i) {
+                if (hs[i] > cutoff)
+                    ++nt;
+                else
+                    break;
+            }
+            Tkeep = Mm + K * (K - nt);   // the eigenvectors of the nt largest eigenvalues
+        }
+        BRPA_TRY(stage_end(ctx, BRPA_STAGE_T2_BASIS));
+        if (nt > 0 && !ident) {
+            // T^T I T (rpa_CC_MIM_v5, 425-488) from the explicit I
+            BRPA_TRY(begin(BRPA_STAGE_T2_TIT));
+            double *Z = omega;   // K x nt
+            TIT = iom;           // nt x nt
+            BRPA_TRY(gemm(ctx, h, false, false, K, nt, K, 1.0, Im, K, Tkeep, K, 0.0, Z, K));
+            BRPA_TRY(gemm(ctx, h, true, false, nt, nt, K, 1.0, Tkeep, K, Z, K, 0.0, TIT, nt));
+            BRPA_TRY(stage_end(ctx, BRPA_STAGE_T2_TIT));
+        }
+    } else {
+        // I * Omega (rpa_CC_IM_v5, rpa/rpa_CC_Doubles.f90:369-422)
+        BRPA_TRY(begin(BRPA_STAGE_T2_IOMEGA));
+        BRPA_TRY(launch_guess(ctx, omega, K, NG, seed));
+        BRPA_TRY(launch_normalize_columns(ctx, omega, K, NG));
+        BRPA_HIP(ctx, hipMemsetAsync(iom, 0, 8 * K * NG, ctx->stream));
+        // frequency-independent operand of the split form, once per fragment
+        if (split) BRPA_TRY(launch_colslice_i8(ctx, omega, K, K, NG, nullptr, Kp, scOT, SOT));
+        for (int64_t k = 0; k < F; ++k) {
+            const double *ck = dchi + k * K;
+            const double coef = -fw[k] * lambda / (4.0 * kPi);
+            const double *Rtk = Rt + k * K * N;
+            if (split) {
+                // Y = (chi_k Rt_k)^T Omega, I Omega += coef chi_k Rt_k Y; the chi_k Rt_k
+                // slices are kept for T^T I T
+                int8_t *SCRk = SCR + k * 5 * N * Kp;
+                double *scCRk = scCR + k * N;
+                BRPA_TRY(launch_rowslice_i8(ctx, Rtk, K, N, ck, pmx, scCRk, SCRk));
+                BRPA_TRY(launch_gemm_i8(ctx, SCRk, scCRk, N, SOT, scOT, NG, Kp, 1.0, nullptr, 0.0,
+                                        Y, 1, N, K));
+                BRPA_TRY(launch_colslice_i8(ctx, Rtk, N, N, K, nullptr, Np, scRR, SRR));
+                BRPA_TRY(launch_colslice_i8(ctx, Y, N, N, NG, nullptr, Np, scY, SY));
+                BRPA_TRY(launch_gemm_i8(ctx, SRR, scRR, K, SY, scY, NG, Np, coef, ck, 1.0, iom, 1,
+                                        K, N));
+                continue;
+            }
+            BRPA_TRY(launch_row_scale(ctx, tmp, omega, ck, K, NG));
+            BRPA_TRY(gemm(ctx, h, false, false, N, NG, K, 1.0, Rtk, N, tmp, K, 0.0, Y, N));
+            BRPA_TRY(gemm(ctx, h, true, false, K, NG, N, 1.0, Rtk, N, Y, N, 0.0, tmp, K));
+            BRPA_TRY(launch_row_scale_acc(ctx, iom, tmp, ck, coef, K, NG));
+        }
+        BRPA_TRY(launch_normalize_columns(ctx, iom, K, NG));
+        BRPA_TRY(stage_end(ctx, BRPA_STAGE_T2_IOMEGA));
+
+        // S = (I Omega)^T (I Omega) (rpa_CC_Doubles.f90:595-609).  The reference
+        // keeps the eigenvectors of S with eigenvalues above the cutoff and forms
+        // the orthonormal basis T = (I Omega) U_s s^{-1/2}.  When every eigenvalue
+        // exceeds the cutoff -- exactly when S - cutoff*1 has a Cholesky factor --
+        // T spans all of span(I Omega), and any orthonormal basis of that span
+        // gives the same Rayleigh-Ritz pairs below; it is then built by Cholesky-QR
+        // (the shifted factor, then one unshifted pass) instead of the NG x NG
+        // eigensolve.  Otherwise the eigensolve decides which vectors are kept.
+        // Two passes suffice: with every eigenvalue above the cutoff, X = I Omega
+        // has kappa(X) <= (NG / cutoff)^(1/2) < 1e8, the first pass leaves
+        // X1^T X1 = 1 + cutoff U1^-T U1^-1 + O(eps kappa^2), and the second pass
+        // restores orthonormality to O(eps kappa(X1)^2) -- never worse than the
+        // eps s_max / s_min of the reference's own eigenvector basis.
+        BRPA_TRY(begin(BRPA_STAGE_T2_BASIS));
+        // Gram matrices as full GEMMs: rocBLAS dgemm takes a third of dsyrk's time here
+        BRPA_TRY(gemm(ctx, h, true, false, NG, NG, K, 1.0, iom, K, iom, K, 0.0, Sm, NG));
+        bool cholqr = false;
+        BRPA_TRY(launch_add_diag(ctx, Sm, NG, -cutoff));
+        BRPA_TRY(chol_dense(Sm, NG, &cholqr));
+        if (cholqr) {
+            nt = NG;
+            for (int pass = 0; pass < 2; ++pass) {
+                if (pass > 0) {
+                    BRPA_TRY(gemm(ctx, h, true, false, NG, NG, K, 1.0, iom, K, iom, K, 0.0, Sm,
+                                  NG));
+                    bool okp = false;
+                    BRPA_TRY(chol_dense(Sm, NG, &okp));
+                    if (!okp)
+                        return brpa_fail(ctx, BRPA_ERR_SOLVER,
+                                         "Cholesky-QR pass %d of the T2 basis broke down", pass);
+                }
+                // X <- X U^{-1}, so that X^T X = 1
+                BRPA_TRY(rb_check(ctx,
+                                  rocblas_dtrsm(h, rocblas_side_right, rocblas_fill_upper,
+                                                rocblas_operation_none, rocblas_diagonal_non_unit,
+                                                (rocblas_int)K, (rocblas_int)NG, &one, Sm,
+                                                (rocblas_int)NG, iom, (rocblas_int)K),
+                                  "rocblas_dtrsm"));
+            }
+        } else {
+            BRPA_TRY(gemm(ctx, h, true, false, NG, NG, K, 1.0, iom, K, iom, K, 0.0, Sm, NG));
+            BRPA_TRY(syevd(ctx, h, NG, Sm, sev, sE, dinfo + F));
+            std::vector<double> hs(NG);
+            BRPA_TRY(copy_sync(ctx, hs.data(), sev, 8 * NG, hipMemcpyDeviceToHost));
+            for (int64_t i = NG - 1; i >= 0; --i) {
+                if (hs[i] > cutoff)
+                    ++nt;
+                else
+                    break;
+            }
+        }
+        if (nt > 0) {
+            // T = (I Omega) U_s s^{-1/2}; TIT = T^T I T (rpa_CC_MIM_v5, 425-488)
+            double *T = tmp;               // K x nt
+            double *G = Y;                 // N x nt
+            if (!cholqr) {
+                double *Wm = omega;        // NG x nt   (omega no longer needed)
+                BRPA_TRY(launch_select_scale(ctx, Wm, Sm, sev, NG, NG - nt, nt));
+                BRPA_TRY(gemm(ctx, h, false, false, K, nt, NG, 1.0, iom, K, Wm, NG, 0.0, T, K));
+                BRPA_HIP(ctx, hipMemcpyAsync(Tkeep, T, 8 * K * nt, hipMemcpyDeviceToDevice,
+                                             ctx->stream));
+            }
+            BRPA_TRY(stage_end(ctx, BRPA_STAGE_T2_BASIS));
+            BRPA_TRY(begin(BRPA_STAGE_T2_TIT));
+            if (split)
+                BRPA_TRY(launch_colslice_i8(ctx, Tkeep, K, K, nt, nullptr, Kp, scOT, SOT));
+            for (int64_t k = 0; k < F; ++k) {
+                const double *ck = dchi + k * K;
+                if (split) {
+                    // G = (chi_k Rt_k)^T T
+                    BRPA_TRY(launch_gemm_i8(ctx, SCR + k * 5 * N * Kp, scCR + k * N, N, SOT, scOT,
+                                            nt, Kp, 1.0, nullptr, 0.0, G, 1, N, K));
+                } else {
+                    BRPA_TRY(launch_row_scale(ctx, T, Tkeep, ck, K, nt));
+                    BRPA_TRY(gemm(ctx, h, false, false, N, nt, K, 1.0, Rt + k * K * N, N, T, K,
+                                  0.0, G, N));
+                }
+                // G = Rt_k^T chi_k T = U_k^-T R^T chi_k T, so G^T G = T^T chi R (1 + l Pi_k)^-1 R^T chi T
+                BRPA_TRY(gemm(ctx, h, true, false, nt, nt, N, -fw[k] * lambda / (4.0 * kPi), G, N,
+                              G, N, k == 0 ? 0.0 : 1.0, TIT, nt));
+            }
+            BRPA_TRY(stage_end(ctx, BRPA_STAGE_T2_TIT));
+        } else {
+            BRPA_TRY(stage_end(ctx, BRPA_STAGE_T2_BASIS));
+        }
     }
-    BRPA_TRY(stage_end(ctx, BRPA_STAGE_T2_TIT));
+    *nvecs_t2 = nt;
+    ctx->t2_nt = nt;
+    ctx->t2_nai = K;
+    ctx->t2_factored = false;
+    if (nt == 0) return collect();
+
     BRPA_TRY(grow(ctx, &ctx->t2_A, &ctx->t2_A_cap, nt));
     BRPA_TRY(grow(ctx, &ctx->t2_V, &ctx->t2_V_cap, (size_t)nt * K));
     BRPA_TRY(begin(BRPA_STAGE_T2_CORE));
@@ -513,9 +625,7 @@ int t2_build(brpa_ctx *ctx, int precision, const double *R, int64_t naux, const 
         BRPA_TRY(chol_dense(TIT, nt, &okb));
         int ok = okb;
         if (ok) {
-            BRPA_TRY(rb_check(ctx, rocsolver_dpotri(h, rocblas_fill_upper, (rocblas_int)nt, TIT,
-                                                    (rocblas_int)nt, dinfo + F + 1),
-                              "rocsolver_dpotri"));
+            BRPA_TRY(potri(ctx, h, nt, TIT, dinfo + F + 1));
             BRPA_TRY(info_ok(dinfo + F + 1, &ok));
         }
         if (ok) {
@@ -528,9 +638,13 @@ int t2_build(brpa_ctx *ctx, int precision, const double *R, int64_t naux, const 
             // -C = U'^T U' (U' upper): T2 = -V' V'^T with V' = T U'^T
             BRPA_TRY(launch_zero_lower(ctx, TIT, nt));
             BRPA_TRY(launch_fill(ctx, ctx->t2_A, nt, -1.0));
-            // V'^T = U' T^T, stored as V[ai][mu]
-            BRPA_TRY(gemm(ctx, h, false, true, nt, K, nt, 1.0, TIT, nt, Tkeep, K, 0.0, ctx->t2_V,
-                          nt));
+            // V'^T = U' T^T, stored as V[ai][mu]; with T = 1 it is U' itself
+            if (ident)
+                BRPA_HIP(ctx, hipMemcpyAsync(ctx->t2_V, TIT, 8 * nt * nt, hipMemcpyDeviceToDevice,
+                                             ctx->stream));
+            else
+                BRPA_TRY(gemm(ctx, h, false, true, nt, K, nt, 1.0, TIT, nt, Tkeep, K, 0.0,
+                              ctx->t2_V, nt));
             done = true;
             ctx->t2_factored = true;
         } else {
@@ -545,7 +659,11 @@ int t2_build(brpa_ctx *ctx, int precision, const double *R, int64_t naux, const 
         BRPA_TRY(copy_sync(ctx, ha.data(), sev, 8 * nt, hipMemcpyDeviceToHost));
         for (int64_t i = 0; i < nt; ++i) ha[i] = 0.5 * (ha[i] / (ha[i] + 1.0));
         BRPA_TRY(copy_sync(ctx, ctx->t2_A, ha.data(), 8 * nt, hipMemcpyHostToDevice));
-        BRPA_TRY(gemm(ctx, h, true, true, nt, K, nt, 1.0, TIT, nt, Tkeep, K, 0.0, ctx->t2_V, nt));
+        if (ident)
+            BRPA_TRY(launch_transpose(ctx, TIT, nt, nt, ctx->t2_V));
+        else
+            BRPA_TRY(gemm(ctx, h, true, true, nt, K, nt, 1.0, TIT, nt, Tkeep, K, 0.0, ctx->t2_V,
+                          nt));
     }
     BRPA_TRY(stage_end(ctx, BRPA_STAGE_T2_CORE));
     BRPA_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
@@ -881,9 +999,10 @@ int brpa_sosex(brpa_ctx *ctx, int precision, double *ec_exchange, double alpha, 
         }
     }
     BRPA_TRY(stage_end(ctx, BRPA_STAGE_SOSEX));
+    BRPA_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    // a completed copy: the partials are read on the host right after it
     std::vector<double> hpart(2 * o * o * nchunk);
-    BRPA_HIP(ctx, hipMemcpyAsync(hpart.data(), part, 8 * hpart.size(), hipMemcpyDeviceToHost,
-                                 ctx->stream));
+    BRPA_TRY(copy_sync(ctx, hpart.data(), part, 8 * hpart.size(), hipMemcpyDeviceToHost));
     long double e = 0.0L;
     for (int64_t i = 0; i < o; ++i) {
         for (int64_t j = i; j < o; ++j) {
@@ -895,8 +1014,6 @@ int brpa_sosex(brpa_ctx *ctx, int precision, double *ec_exchange, double alpha, 
             e += (j == i ? 1.0L : 2.0L) * (-sp);
         }
     }
-    BRPA_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
-    BRPA_HIP(ctx, hipStreamSynchronize(ctx->stream));
     BRPA_TRY(stage_collect(ctx, BRPA_STAGE_SOSEX));
     float ms = 0.f;
     BRPA_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));

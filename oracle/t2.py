@@ -23,9 +23,11 @@ they come from ``guess_vectors``, a SplitMix64 counter hash of
 (seed, column, row) that the HIP library evaluates bit-identically; the
 uniform-(-1/2,1/2) distribution and column normalisation are the reference's.
 
-Pinned by (tests/test_oracle_t2.py): with NG >= ov and a zero cutoff the
+Pinned by (tests/test_oracle_t2_sosex_adjoint.py): with NG >= ov and a zero cutoff the
 factorisation equals the explicit T = I (1+I)^{-1} / 2 of the full ov x ov
-matrix I; that explicit T matches the Riccati closed form at exact frequency
+matrix I; with the uncapped guess count NG = ceil(3 NAux) > ov on a batch
+shape (o=8, v=70, NAux=351) every seed keeps the whole ov space and E_x equals
+the SOSEX of that explicit T, while a count capped at ov misses it; that explicit T matches the Riccati closed form at exact frequency
 integration up to the grid error; E_T2 = 2 sum_mu A_mu |R^T V_mu|^2 reproduces
 E_RPA the way the reference's EigRPA diagnostic does.  The THC route's
 amplitude-eigenvalue cutoff (t2_cutoff, smooth_step) is pinned in
@@ -49,9 +51,16 @@ _M64 = (1 << 64) - 1
 _GOLDEN = 0x9E3779B97F4A7C15
 
 
-def guess_count(naux, nai, factor=GUESS_NVECS_T2):
-    """ceiling(GuessNVecsT2 * NVecsPiU) (rpa.f90:1696), capped at the ov dimension."""
-    return int(min(nai, math.ceil(factor * naux)))
+def guess_count(naux, nai=None, factor=GUESS_NVECS_T2):
+    """ceiling(GuessNVecsT2 * NVecsPiU) (rpa.f90:1696), with no cap at the ov dimension.
+
+    The reference allocates I*Omega with GuessNVecsT2 columns and the Gram
+    matrix S as GuessNVecsT2 x GuessNVecsT2 whatever o*v is
+    (rpa_CC_Doubles.f90:543, 595-597); when NG > o*v, S has rank <= o*v and
+    its NG - o*v null eigenvalues fall below the cutoff.  ``nai`` is accepted
+    for call compatibility and does not enter.
+    """
+    return int(math.ceil(factor * naux))
 
 
 def guess_vectors(seed, nrows, ncols):

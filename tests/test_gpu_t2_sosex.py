@@ -62,6 +62,44 @@ def test_t2_split_i8_matches_oracle(ctx, o, v, n, nf):
     assert abs(ex - ex_ref) <= 1e-7
 
 
+@pytest.mark.parametrize("o,v,n,seed,route", [
+    (8, 70, 351, 11, "oversampled, whole ov space kept"),
+    (10, 40, 140, 103, "oversampled, ov x ov eigensolve decides"),
+    (8, 40, 80, 103, "NG < ov, NG x NG eigensolve decides")])
+def test_t2_uncapped_guess_count(ctx, o, v, n, seed, route):
+    """NG = ceil(3 NAux) as the reference passes it, with no cap at o*v, on 16
+    frequencies.  The library's basis (the ov x ov route when NG >= o*v) keeps
+    the same number of directions as the oracle's NG x NG Gram eigensolve, the
+    T2 matrix agrees to 1e-9 relative, and E_x to 1e-8 Ha (FP64) and 1e-7 Ha
+    (split-int8 contractions and SOSEX).  The first shape is one of the
+    256-fragment batch (o*v = 560 < NG = 1053): there every seed keeps the
+    whole ov space and E_x is the SOSEX of the explicit full-space amplitudes."""
+    f, x, w, Pis = _setup(o, v, n, 16, seed)
+    g = f.gaps()
+    A_ref, V_ref = t2.t2_v5(f.R, g, x, w, Pis, seed=5)
+    ex_ref = sosex.sosex(f.R, V_ref, A_ref, o, v)
+    A, V = ctx.t2(f.R, f.eocc, f.evir, x, w, seed=5)
+    print("%s: NT2 %d oracle %d" % (route, len(A), len(A_ref)))
+    assert len(A) == len(A_ref)
+    T_ref = (V_ref * A_ref) @ V_ref.T
+    T = (V * A) @ V.T
+    assert np.max(np.abs(T - T_ref)) <= 1e-9 * np.max(np.abs(T_ref))
+    ex = ctx.sosex(V, A, f.R, o, v)
+    print("E_x %.10f fp64 - oracle %.3e" % (ex_ref, ex - ex_ref))
+    assert abs(ex - ex_ref) <= 1e-8
+    Ai, Vi = ctx.t2(f.R, f.eocc, f.evir, x, w, seed=5, precision="split_i8")
+    exi = ctx.sosex(Vi, Ai, f.R, o, v, precision="split_i8")
+    print("split_i8 - oracle %.3e" % (exi - ex_ref))
+    assert abs(exi - ex_ref) <= 1e-7
+    # the energy-only batch path (factored core) on the same guess vectors
+    eb, nb = ctx.exchange_batch([(f.R, f.eocc, f.evir, x, w)], seed=5)
+    assert nb[0] == len(A_ref) and abs(eb[0] - ex_ref) <= 1e-8
+    if o * v == len(A_ref):
+        a, E = np.linalg.eigh(t2.i_matrix_explicit(f.R, g, x, w, Pis))
+        ex_full = sosex.sosex(f.R, E, 0.5 * a / (1.0 + a), o, v)
+        assert abs(ex - ex_full) <= 1e-8
+
+
 def test_t2_full_space_invariants(ctx):
     """NG = ov and a zero cutoff: T2 = I(1+I)^-1/2 and E_T2 = E_RPA."""
     f, x, w, Pis = _setup(2, 3, 4, 24, 7)

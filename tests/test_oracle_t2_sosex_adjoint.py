@@ -52,6 +52,37 @@ def test_eig_rpa_reproduces_erpa(small):
     assert abs(e_t2 - cf.plasmon_energy(f.R, f.gaps())) < 1e-8
 
 
+def test_guess_count_is_not_capped():
+    """ceiling(3 NAux) as rpa.f90:1696 passes it, also where it exceeds o*v."""
+    assert t2.guess_count(351) == 1053
+    assert t2.guess_count(351, 560) == 1053
+    assert t2.guess_count(8, 36) == 24
+
+
+def test_t2_oversampled_guess_space_is_seed_independent():
+    """o=8, v=70, NAux=351, 16 frequencies (a shape of the 256-fragment batch):
+    NG = 1053 > o*v = 560.  Every seed keeps all 560 directions, so E_x is the
+    SOSEX of the explicit full-space amplitudes to 1e-9 Ha whatever the guess
+    vectors.  A guess count capped at o*v (a square Gram matrix whose smallest
+    eigenvalues fall below the cutoff) drops directions and misses that energy by
+    more than 1e-5 Ha: the cap is a planted mistake this test catches."""
+    o, v, n = 8, 70, 351
+    f = synth.make_fragment(o, v, n, seed=11)
+    x, w, _ = grid.freq_grid(f.eocc, f.evir, 16)
+    g = f.gaps()
+    Pis = [erpa.gpiug_mo(f.R, g, xx) for xx in x]
+    a, E = np.linalg.eigh(t2.i_matrix_explicit(f.R, g, x, w, Pis))
+    ex_full = sosex.sosex(f.R, E, 0.5 * a / (1.0 + a), o, v)
+    worst_capped = 0.0
+    for seed in (1, 2, 3):
+        A, V = t2.t2_v5(f.R, g, x, w, Pis, seed=seed)
+        assert len(A) == o * v
+        assert abs(sosex.sosex(f.R, V, A, o, v) - ex_full) <= 1e-9
+        Ac, Vc = t2.t2_v5(f.R, g, x, w, Pis, seed=seed, nguess=o * v)
+        worst_capped = max(worst_capped, abs(sosex.sosex(f.R, Vc, Ac, o, v) - ex_full))
+    assert worst_capped > 1e-5
+
+
 def test_t2_randomized_subspace():
     """With NG < ov the range finder is approximate but E_T2 stays close to E_RPA."""
     f = synth.make_fragment(3, 12, 8, seed=2)
