@@ -54,10 +54,12 @@ def _prec(p):
     return int(p)
 
 
-def freq_grid(eocc, evir, nfreqs):
+def freq_grid(eocc, evir, nfreqs, core_orb_thresh=_lib.CORE_ORB_THRESH, dai_max_thresh=None):
     """Fixed-size Casimir-Polder grid of one fragment (host code in the library).
 
-    Returns (freqs, weights, info) with info = dict(alpha, rmsd, max_rel_error).
+    eocc: all occupied energies ascending (those below core_orb_thresh are
+    frozen); dai_max_thresh: cap of the gap histogram (None: none).  Returns
+    (freqs, weights, info) with info = dict(alpha, rmsd, max_rel_error).
     """
     lib = _lib.load()
     eocc = np.ascontiguousarray(eocc, np.float64)
@@ -65,10 +67,12 @@ def freq_grid(eocc, evir, nfreqs):
     x = np.zeros(nfreqs)
     w = np.zeros(nfreqs)
     info = np.zeros(3)
-    rc = lib.brpa_freq_grid(_ptr(eocc), eocc.size, _ptr(evir), evir.size, nfreqs, _ptr(x),
-                            _ptr(w), _ptr(info))
+    rc = lib.brpa_freq_grid_ex(_ptr(eocc), eocc.size, _ptr(evir), evir.size, nfreqs,
+                               float(core_orb_thresh),
+                               0.0 if dai_max_thresh is None else float(dai_max_thresh),
+                               _ptr(x), _ptr(w), _ptr(info))
     if rc != 0:
-        raise BrpaError(rc, "brpa_freq_grid failed")
+        raise BrpaError(rc, "brpa_freq_grid_ex failed")
     return x, w, dict(alpha=info[0], rmsd=info[1], max_rel_error=info[2])
 
 
@@ -89,6 +93,80 @@ def optimize_freq_grid(eocc, evir, target_error=1e-6, target_rel_error=1e-3, min
     k = n.value
     return x[:k].copy(), w[:k].copy(), dict(alpha=info[0], rmsd=info[1], max_rel_error=info[2],
                                             nfreqs=k)
+
+
+def dai_histogram(eocc, evir, core_orb_thresh=_lib.CORE_ORB_THRESH, dai_max_thresh=None):
+    """Gap histogram of one spin of one system (rpa_DaiHistogram): (values, weights)."""
+    lib = _lib.load()
+    eocc = np.ascontiguousarray(eocc, np.float64)
+    evir = np.ascontiguousarray(evir, np.float64)
+    v = np.zeros(_lib.HISTOGRAM_NBINS)
+    w = np.zeros(_lib.HISTOGRAM_NBINS)
+    rc = lib.brpa_dai_histogram(_ptr(eocc), eocc.size, _ptr(evir), evir.size,
+                                float(core_orb_thresh),
+                                0.0 if dai_max_thresh is None else float(dai_max_thresh),
+                                _ptr(v), _ptr(w))
+    if rc != 0:
+        raise BrpaError(rc, "brpa_dai_histogram failed")
+    return v, w
+
+
+def dai_max_thresh(spins, core_orb_thresh=_lib.CORE_ORB_THRESH):
+    """Largest active gap of a whole complex over its spins [(eocc, evir), ...]
+    (rpa_DaiMaxThresh), the cap for its subsystems' histograms."""
+    lib = _lib.load()
+    eo = np.ascontiguousarray(np.concatenate([np.asarray(a, np.float64) for a, _ in spins]))
+    ev = np.ascontiguousarray(np.concatenate([np.asarray(b, np.float64) for _, b in spins]))
+    no = np.array([len(a) for a, _ in spins], dtype=np.int64)
+    nv = np.array([len(b) for _, b in spins], dtype=np.int64)
+    t = C.c_double(0.0)
+    rc = lib.brpa_dai_max_thresh(len(spins), _ptr(eo), C.c_void_p(no.ctypes.data), _ptr(ev),
+                                 C.c_void_p(nv.ctypes.data), float(core_orb_thresh), C.byref(t))
+    if rc != 0:
+        raise BrpaError(rc, "brpa_dai_max_thresh failed")
+    return t.value
+
+
+def _hist_sets(values, dweights):
+    v = np.ascontiguousarray(np.atleast_2d(values), np.float64)
+    w = np.ascontiguousarray(np.atleast_2d(dweights), np.float64)
+    if v.shape != w.shape or v.shape[1] != _lib.HISTOGRAM_NBINS:
+        raise ValueError("histograms must be (nsets, %d) arrays" % _lib.HISTOGRAM_NBINS)
+    return v, w
+
+
+def freq_grid_sets(values, dweights, nfreqs):
+    """Grid of nfreqs points on one or several histograms (one per row), the
+    worst set deciding the error (rpa_OptQuadScaling)."""
+    lib = _lib.load()
+    v, w = _hist_sets(values, dweights)
+    x = np.zeros(nfreqs)
+    fw = np.zeros(nfreqs)
+    info = np.zeros(3)
+    rc = lib.brpa_opt_quad_scaling(nfreqs, v.shape[0], _ptr(v), _ptr(w), _ptr(x), _ptr(fw),
+                                   _ptr(info))
+    if rc != 0:
+        raise BrpaError(rc, "brpa_opt_quad_scaling failed")
+    return x, fw, dict(alpha=info[0], rmsd=info[1], max_rel_error=info[2])
+
+
+def optimize_freq_quad(values, dweights, target_error=1e-6, target_rel_error=1e-3, min_nfreqs=8,
+                       max_nfreqs=64):
+    """Adaptive grid on one or several histograms (rpa_OptimizeFreqQuad)."""
+    lib = _lib.load()
+    v, w = _hist_sets(values, dweights)
+    x = np.zeros(max_nfreqs)
+    fw = np.zeros(max_nfreqs)
+    info = np.zeros(3)
+    n = C.c_int64(0)
+    rc = lib.brpa_optimize_freq_quad(v.shape[0], _ptr(v), _ptr(w), target_error, target_rel_error,
+                                     min_nfreqs, max_nfreqs, _ptr(x), _ptr(fw), C.byref(n),
+                                     _ptr(info))
+    if rc != 0:
+        raise BrpaError(rc, "brpa_optimize_freq_quad failed")
+    k = n.value
+    return x[:k].copy(), fw[:k].copy(), dict(alpha=info[0], rmsd=info[1],
+                                             max_rel_error=info[2], nfreqs=k)
 
 
 class Context:
@@ -150,7 +228,8 @@ class Context:
         call (device time per stage, summed over the fragments of a batch)."""
         names = {"t2_pi": _lib.STAGE_T2_PI, "t2_iomega": _lib.STAGE_T2_IOMEGA,
                  "t2_basis": _lib.STAGE_T2_BASIS, "t2_tit": _lib.STAGE_T2_TIT,
-                 "t2_core": _lib.STAGE_T2_CORE, "sosex": _lib.STAGE_SOSEX}
+                 "t2_core": _lib.STAGE_T2_CORE, "sosex": _lib.STAGE_SOSEX,
+                 "t2_basis_eig": _lib.STAGE_T2_EIG}
         return {k: (self._lib.brpa_last_stage_ms(self._h, v),
                     self._lib.brpa_last_stage_launches(self._h, v)) for k, v in names.items()}
 

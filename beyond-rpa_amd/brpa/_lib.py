@@ -12,7 +12,9 @@ STAGE_PI, STAGE_SPLIT, STAGE_CHOL, STAGE_AUX = range(4)
 # amplitude-eigenvalue cutoff rules (brpa_t2_cutoff)
 T2_CUTOFF_EIG, T2_CUTOFF_EIG_DIV_MAXEIG, T2_CUTOFF_EIG_DIV_NELECTRON, T2_CUTOFF_SUM_REJECTED = range(4)
 STAGE_T2_PI, STAGE_T2_IOMEGA, STAGE_T2_BASIS, STAGE_T2_TIT, STAGE_T2_CORE, STAGE_SOSEX = range(4, 10)
-STAGE_THC_R, STAGE_THC_PI, STAGE_MBPT3 = 10, 11, 12
+STAGE_THC_R, STAGE_THC_PI, STAGE_MBPT3, STAGE_T2_EIG = 10, 11, 12, 13
+HISTOGRAM_NBINS = 100          # BRPA_HISTOGRAM_NBINS
+CORE_ORB_THRESH = -3.0         # BRPA_CORE_ORB_THRESH
 
 _i64 = C.c_int64
 _dp = C.POINTER(C.c_double)
@@ -38,6 +40,14 @@ SIGNATURES = {
     "brpa_last_stage_launches": (C.c_int, [_vp, C.c_int]),
     "brpa_last_gemm_i8_stats": (C.c_int, [_vp, _vp, _vp, _vp]),
     "brpa_freq_grid": (C.c_int, [_vp, _i64, _vp, _i64, _i64, _vp, _vp, _vp]),
+    "brpa_freq_grid_ex": (C.c_int, [_vp, _i64, _vp, _i64, _i64, C.c_double, C.c_double, _vp, _vp,
+                                    _vp]),
+    "brpa_dai_histogram": (C.c_int, [_vp, _i64, _vp, _i64, C.c_double, C.c_double, _vp, _vp]),
+    "brpa_dai_max_thresh": (C.c_int, [_i64, _vp, _vp, _vp, _vp, C.c_double,
+                                      C.POINTER(C.c_double)]),
+    "brpa_opt_quad_scaling": (C.c_int, [_i64, _i64, _vp, _vp, _vp, _vp, _vp]),
+    "brpa_optimize_freq_quad": (C.c_int, [_i64, _vp, _vp, C.c_double, C.c_double, _i64, _i64,
+                                          _vp, _vp, C.POINTER(_i64), _vp]),
     "brpa_optimize_freq_grid": (C.c_int, [_vp, _i64, _vp, _i64, C.c_double, C.c_double, _i64,
                                           _i64, _vp, _vp, C.POINTER(_i64), _vp]),
     "brpa_gpiug_mo": (C.c_int, [_vp, _vp, _vp, _i64, _vp, _i64, _vp, _i64, C.c_double,

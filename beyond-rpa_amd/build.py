@@ -63,13 +63,14 @@ def build_fortran():
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("amdflang failed:\n" + r.stdout + r.stderr)
-    drv = os.path.join(HERE, "fortran", "shim_check.f90")
-    exe = os.path.join(OUT_DIR, "brpa_shim_check")
-    cmd = [flang, "-fdefault-integer-8", "-O2", "-I", mod_dir, drv, "-o", exe, "-L", OUT_DIR,
-           "-lbrpa_fortran", "-lbrpa", "-Wl,-rpath,$ORIGIN"]
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError("amdflang failed on shim_check:\n" + r.stdout + r.stderr)
+    for name in ("shim_check", "grid_check"):
+        drv = os.path.join(HERE, "fortran", name + ".f90")
+        exe = os.path.join(OUT_DIR, "brpa_" + name)
+        cmd = [flang, "-fdefault-integer-8", "-O2", "-I", mod_dir, drv, "-o", exe, "-L", OUT_DIR,
+               "-lbrpa_fortran", "-lbrpa", "-Wl,-rpath,$ORIGIN"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("amdflang failed on %s:\n%s%s" % (name, r.stdout, r.stderr))
     return out
 
 

@@ -452,8 +452,10 @@ int t2_build(brpa_ctx *ctx, int precision, const double *R, int64_t naux, const 
             ident = true;
             TIT = Im;       // T^T I T = I
         } else {
+            BRPA_TRY(begin(BRPA_STAGE_T2_EIG));
             BRPA_TRY(gemm(ctx, h, false, true, K, K, NG, 1.0, iom, K, iom, K, 0.0, Mm, K));
             BRPA_TRY(syevd(ctx, h, K, Mm, sev, sE, dinfo + F));
+            BRPA_TRY(stage_end(ctx, BRPA_STAGE_T2_EIG));
             std::vector<double> hs(K);
             BRPA_TRY(copy_sync(ctx, hs.data(), sev, 8 * K, hipMemcpyDeviceToHost));
             // counted from the largest down, stopping at the first one below (600-607)
@@ -549,8 +551,10 @@ int t2_build(brpa_ctx *ctx, int precision, const double *R, int64_t naux, const 
                                   "rocblas_dtrsm"));
             }
         } else {
+            BRPA_TRY(begin(BRPA_STAGE_T2_EIG));
             BRPA_TRY(gemm(ctx, h, true, false, NG, NG, K, 1.0, iom, K, iom, K, 0.0, Sm, NG));
             BRPA_TRY(syevd(ctx, h, NG, Sm, sev, sE, dinfo + F));
+            BRPA_TRY(stage_end(ctx, BRPA_STAGE_T2_EIG));
             std::vector<double> hs(NG);
             BRPA_TRY(copy_sync(ctx, hs.data(), sev, 8 * NG, hipMemcpyDeviceToHost));
             for (int64_t i = NG - 1; i >= 0; --i) {

@@ -14,6 +14,10 @@
 !   rpa_CC_Exchange_MBPT3_1 rpa/rpa_CC_Exchange.f90:232-270
 !   rpa_CC_Exchange_MBPT3_2 rpa/rpa_CC_Exchange.f90:381-393
 !   rpa_CC_Exchange_SOSEX   rpa/rpa_CC_Exchange.f90:36-97
+!   rpa_DaiHistogram        rpa/OrbDiffHist.f90:42-123 (with rpa_NCore 126-146)
+!   rpa_DaiMaxThresh        rpa/OrbDiffHist.f90:10-39
+!   rpa_OptQuadScaling      rpa/OptFreqQuad.f90:75-122
+!   rpa_OptimizeFreqQuad    rpa/OptFreqQuad.f90:12-72
 !
 ! Built with -fdefault-integer-8 like every reference build (default INTEGER
 ! is 64-bit, matching int64_t of the C ABI).  Arrays are passed as contiguous
@@ -34,6 +38,7 @@ module brpa_mod
       public :: rpa_GPiUG_MO, rpa_CC_EcRPA_Analytic, rpa_CC_T2, rpa_CC_Exchange_SOSEX
       public :: rpa_THC_CC_T2, rpa_THC_PiU
       public :: rpa_CC_Exchange_MBPT3_1, rpa_CC_Exchange_MBPT3_2, rpa_CRG, rpa_G
+      public :: rpa_DaiHistogram, rpa_DaiMaxThresh, rpa_OptQuadScaling, rpa_OptimizeFreqQuad
       ! T2CutoffType values (rpa/rpa_definitions.f90:139-142)
       integer, parameter, public :: RPA_T2_CUTOFF_EIG = 0
       integer, parameter, public :: RPA_T2_CUTOFF_EIG_DIV_MAXEIG = 1
@@ -179,6 +184,39 @@ module brpa_mod
                   integer(c_int64_t), value :: nocc, nvir, naux, nvecs
                   integer(c_int), value :: s2
             end function
+            integer(c_int) function c_dai_histogram(eocc, nocc, evir, nvir, core, cap, vals, wts) &
+                  bind(C, name="brpa_dai_histogram")
+                  import :: c_int, c_int64_t, c_double
+                  real(c_double), intent(in) :: eocc(*), evir(*)
+                  integer(c_int64_t), value :: nocc, nvir
+                  real(c_double), value :: core, cap
+                  real(c_double), intent(out) :: vals(*), wts(*)
+            end function
+            integer(c_int) function c_dai_max_thresh(nspins, eocc, nocc, evir, nvir, core, out) &
+                  bind(C, name="brpa_dai_max_thresh")
+                  import :: c_int, c_int64_t, c_double
+                  integer(c_int64_t), value :: nspins
+                  real(c_double), intent(in) :: eocc(*), evir(*)
+                  integer(c_int64_t), intent(in) :: nocc(*), nvir(*)
+                  real(c_double), value :: core
+                  real(c_double), intent(out) :: out
+            end function
+            integer(c_int) function c_opt_quad_scaling(nfreqs, nsets, vals, wts, freqs, weights, &
+                  info) bind(C, name="brpa_opt_quad_scaling")
+                  import :: c_int, c_int64_t, c_double
+                  integer(c_int64_t), value :: nfreqs, nsets
+                  real(c_double), intent(in) :: vals(*), wts(*)
+                  real(c_double), intent(out) :: freqs(*), weights(*), info(3)
+            end function
+            integer(c_int) function c_optimize_freq_quad(nsets, vals, wts, terr, trel, nmin, nmax, &
+                  freqs, weights, nfreqs, info) bind(C, name="brpa_optimize_freq_quad")
+                  import :: c_int, c_int64_t, c_double
+                  integer(c_int64_t), value :: nsets, nmin, nmax
+                  real(c_double), intent(in) :: vals(*), wts(*)
+                  real(c_double), value :: terr, trel
+                  real(c_double), intent(out) :: freqs(*), weights(*), info(3)
+                  integer(c_int64_t), intent(out) :: nfreqs
+            end function
             integer(c_size_t) function c_strlen(s) bind(C, name="strlen")
                   import :: c_size_t, c_ptr
                   type(c_ptr), value :: s
@@ -207,6 +245,23 @@ contains
             rc = c_ctx_destroy(ctx)
             ctx = c_null_ptr
       end subroutine brpa_finalize
+
+
+      subroutine check_grid(rc, where)
+            ! the grid routines are host code and need no context: their only
+            ! failures are the reference's own error stops
+            integer(c_int), intent(in) :: rc
+            character(*), intent(in) :: where
+            if (rc == 0) return
+            if (rc == 4) then
+                  print '(a,a)', where, ": Found invalid orbital energy gap. Cannot decompose energy denominators."
+            else if (rc == 7) then
+                  print '(a,a)', where, ": Quadrature did not converge to specified accuracy"
+            else
+                  print '(a,a,i0)', where, ": error ", rc
+            end if
+            error stop
+      end subroutine check_grid
 
 
       subroutine check(rc, where)
@@ -674,5 +729,115 @@ contains
                   int(NVirt, c_int64_t), int(NVecsPiU, c_int64_t), int(NVecsT2, c_int64_t), s2), &
                   "rpa_CC_Exchange_SOSEX")
       end subroutine rpa_CC_Exchange_SOSEX
+
+      subroutine rpa_DaiHistogram(daiValues, daiWeights, OrbEnergies, NOcc, NVirt, &
+            CoreOrbThresh, DaiMaxThresh)
+            !
+            ! Histogram of the gaps of one spin (rpa/OrbDiffHist.f90:42-123).
+            ! OrbEnergies holds the NOcc occupied (core included) then the NVirt
+            ! virtual energies; size(daiValues) must be RPA_HISTOGRAM_NBINS = 100.
+            !
+            real(F64), dimension(:), intent(out) :: daiValues
+            real(F64), dimension(:), intent(out) :: daiWeights
+            real(F64), dimension(:), intent(in)  :: OrbEnergies
+            integer, intent(in)                  :: NOcc
+            integer, intent(in)                  :: NVirt
+            real(F64), intent(in)                :: CoreOrbThresh
+            real(F64), intent(in)                :: DaiMaxThresh
+            real(F64), dimension(100) :: v, w
+            if (size(daiValues) /= 100 .or. size(daiWeights) /= 100) then
+                  print '(a)', "rpa_DaiHistogram: the library's histograms have 100 bins"
+                  error stop
+            end if
+            call check_grid(c_dai_histogram(OrbEnergies(1:NOcc), int(NOcc, c_int64_t), &
+                  OrbEnergies(NOcc+1:NOcc+NVirt), int(NVirt, c_int64_t), CoreOrbThresh, &
+                  DaiMaxThresh, v, w), "rpa_DaiHistogram")
+            daiValues = v
+            daiWeights = w
+      end subroutine rpa_DaiHistogram
+
+
+      subroutine rpa_DaiMaxThresh(daiMaxThresh, OrbEnergies, NOcc, NVirt, NSpins, CoreOrbThresh)
+            !
+            ! Largest active gap of the full complex (rpa/OrbDiffHist.f90:10-39).
+            !
+            real(F64), intent(out)                 :: daiMaxThresh
+            real(F64), dimension(:, :), intent(in) :: OrbEnergies
+            integer, dimension(2), intent(in)      :: NOcc
+            integer, dimension(2), intent(in)      :: NVirt
+            integer, intent(in)                    :: NSpins
+            real(F64), intent(in)                  :: CoreOrbThresh
+            real(F64), dimension(:), allocatable :: eo, ev
+            integer(c_int64_t), dimension(2) :: no, nv
+            integer :: s
+            allocate(eo(0), ev(0))
+            do s = 1, NSpins
+                  eo = [eo, OrbEnergies(1:NOcc(s), s)]
+                  ev = [ev, OrbEnergies(NOcc(s)+1:NOcc(s)+NVirt(s), s)]
+                  no(s) = int(NOcc(s), c_int64_t)
+                  nv(s) = int(NVirt(s), c_int64_t)
+            end do
+            call check_grid(c_dai_max_thresh(int(NSpins, c_int64_t), eo, no, ev, nv, &
+                  CoreOrbThresh, daiMaxThresh), "rpa_DaiMaxThresh")
+      end subroutine rpa_DaiMaxThresh
+
+
+      subroutine rpa_OptQuadScaling(Freqs, FreqWeights, OptAlpha, OptRelError, OptRMSD, NFreqs, &
+            daiValues, daiWeights)
+            !
+            ! Best Casimir-Polder scale on the histogram sets (rpa/OptFreqQuad.f90:75-122).
+            ! The nodes come back as quad_CasimirPolder(NFreqs, OptAlpha) builds them,
+            ! which may differ from OptAlpha * (unscaled nodes) in the last bit.
+            !
+            real(F64), dimension(NFreqs), intent(out) :: Freqs
+            real(F64), dimension(NFreqs), intent(out) :: FreqWeights
+            real(F64), intent(out)                    :: OptAlpha
+            real(F64), intent(out)                    :: OptRelError
+            real(F64), intent(out)                    :: OptRMSD
+            integer, intent(in)                       :: NFreqs
+            real(F64), dimension(:, :), intent(in), contiguous :: daiValues
+            real(F64), dimension(:, :), intent(in), contiguous :: daiWeights
+            real(F64) :: info(3)
+            if (size(daiValues, 1) /= 100) then
+                  print '(a)', "rpa_OptQuadScaling: the library's histograms have 100 bins"
+                  error stop
+            end if
+            call check_grid(c_opt_quad_scaling(int(NFreqs, c_int64_t), &
+                  int(size(daiValues, 2), c_int64_t), daiValues, daiWeights, Freqs, FreqWeights, &
+                  info), "rpa_OptQuadScaling")
+            OptAlpha = info(1)
+            OptRMSD = info(2)
+            OptRelError = info(3)
+      end subroutine rpa_OptQuadScaling
+
+
+      subroutine rpa_OptimizeFreqQuad(Freqs, FreqWeights, NFreqs, daiValues, daiWeights, &
+            TargetError, TargetRelError, MinNFreqs, MaxNFreqs, GridLimitDai)
+            !
+            ! Smallest grid meeting both targets on the histogram sets
+            ! (rpa/OptFreqQuad.f90:12-72).  GridLimitDai only changed the
+            ! reference's log: the cap is in the histograms.
+            !
+            real(F64), dimension(MaxNFreqs), intent(out) :: Freqs
+            real(F64), dimension(MaxNFreqs), intent(out) :: FreqWeights
+            integer, intent(out)                         :: NFreqs
+            real(F64), dimension(:, :), intent(in), contiguous :: daiValues
+            real(F64), dimension(:, :), intent(in), contiguous :: daiWeights
+            real(F64), intent(in)                        :: TargetError
+            real(F64), intent(in)                        :: TargetRelError
+            integer, intent(in)                          :: MinNFreqs, MaxNFreqs
+            logical, intent(in)                          :: GridLimitDai
+            real(F64) :: info(3)
+            integer(c_int64_t) :: n
+            if (size(daiValues, 1) /= 100) then
+                  print '(a)', "rpa_OptimizeFreqQuad: the library's histograms have 100 bins"
+                  error stop
+            end if
+            call check_grid(c_optimize_freq_quad(int(size(daiValues, 2), c_int64_t), daiValues, &
+                  daiWeights, TargetError, TargetRelError, int(MinNFreqs, c_int64_t), &
+                  int(MaxNFreqs, c_int64_t), Freqs, FreqWeights, n, info), "rpa_OptimizeFreqQuad")
+            NFreqs = int(n)
+            if (.false. .and. GridLimitDai) NFreqs = 0
+      end subroutine rpa_OptimizeFreqQuad
 
 end module brpa_mod

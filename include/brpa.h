@@ -106,7 +106,10 @@ enum {
     BRPA_STAGE_THC_PI = 11,   /* Pi(u) of all frequencies from that R (brpa_thc_piu) */
     BRPA_STAGE_MBPT3 = 12,    /* MBPT3-1: U, D, Lyapunov solve and factor of X (its final
                                  contraction is BRPA_STAGE_SOSEX); MBPT3-2: all products */
-    BRPA_NSTAGES = 13
+    BRPA_STAGE_T2_EIG = 13,   /* amplitudes: the basis eigensolve, inside BRPA_STAGE_T2_BASIS,
+                                 when the Gram matrix fails the Cholesky test (its count is
+                                 the number of amplitude builds that needed it) */
+    BRPA_NSTAGES = 14
 };
 /* For BRPA_STAGE_MBPT3 after brpa_mbpt3_1 the count is the number of
  * sign-iteration steps of the Lyapunov solve. */
@@ -119,17 +122,65 @@ int brpa_last_stage_launches(const brpa_ctx *ctx, int stage);
 int brpa_last_gemm_i8_stats(const brpa_ctx *ctx, double *ms, double *flops, int64_t *launches);
 
 /* ---- frequency grid (host code) ----------------------------------------- */
-/* Fixed-size grid of one fragment: 100-bin gap histogram, 15000-point scan of
- * the Casimir-Polder scale alpha in [0.1, 15], transformed Gauss-Legendre rule.
- * Replaces rpa_DaiHistogram (rpa/OrbDiffHist.f90:42-123) + rpa_OptQuadScaling
- * (rpa/OptFreqQuad.f90:75-122) + quad_CasimirPolder (rpa/quadratures.f90:570-643)
- * at a given NFreqs.  eocc: active occupied energies ascending; evir ascending.
- * info (may be NULL): {alpha, rmsd, max_rel_error}. */
+#define BRPA_HISTOGRAM_NBINS 100      /* RPA_HISTOGRAM_NBINS, rpa/rpa_definitions.f90:113 */
+#define BRPA_CORE_ORB_THRESH (-3.0)   /* default CoreOrbThresh (Ha), rpa/rpa_definitions.f90:301 */
+
+/* Gap histogram of one spin of one system.  Replaces rpa_DaiHistogram
+ * (rpa/OrbDiffHist.f90:42-123) with rpa_NCore (126-146): eocc holds all nocc
+ * occupied energies of the spin, ascending, core included; the leading ones
+ * below core_orb_thresh are frozen.  The range is [evir[0] - eocc[nocc-1],
+ * min(dai_max_thresh, evir[nvir-1] - eocc[ncore])], bin k's value is its left
+ * edge, gaps above dai_max_thresh are not counted (dai_max_thresh <= 0: no cap,
+ * the reference's huge(ONE) when GridLimitDai is off).  values/weights hold
+ * BRPA_HISTOGRAM_NBINS entries; with every occupied orbital frozen they get the
+ * reference's special values (weights -1, left out of every error sum). */
+int brpa_dai_histogram(const double *eocc, int64_t nocc, const double *evir, int64_t nvir,
+                       double core_orb_thresh, double dai_max_thresh, double *values,
+                       double *weights);
+
+/* Largest active gap evir[last] - eocc[ncore] over the spins of the whole
+ * complex, the cap of its subsystems' histograms.  Replaces rpa_DaiMaxThresh
+ * (rpa/OrbDiffHist.f90:10-39).  eocc/evir: the spins' energies one after the
+ * other (nocc[s], nvir[s] each). */
+int brpa_dai_max_thresh(int64_t nspins, const double *eocc, const int64_t *nocc,
+                        const double *evir, const int64_t *nvir, double core_orb_thresh,
+                        double *dai_max_thresh);
+
+/* Grid of nfreqs points on nsets histograms (values/dweights: nsets x
+ * BRPA_HISTOGRAM_NBINS, the bytes of the reference's daiValues(NBins, NSets)):
+ * the 15000-point scan of the Casimir-Polder scale alpha in [0.1, 15] keeping
+ * the first strict minimum of sqrt(max over sets of the weighted squared
+ * error), then the transformed Gauss-Legendre rule.  Replaces
+ * rpa_OptQuadScaling + rpa_FreqQuadError (rpa/OptFreqQuad.f90:75-169) and
+ * quad_CasimirPolder (rpa/quadratures.f90:570-643).  info (may be NULL):
+ * {alpha, rmsd, max_rel_error}. */
+int brpa_opt_quad_scaling(int64_t nfreqs, int64_t nsets, const double *values,
+                          const double *dweights, double *freqs, double *weights, double *info);
+
+/* Adaptive grid on nsets histograms: the smallest n in [min_nfreqs,
+ * max_nfreqs] meeting both targets (BRPA_ERR_GRID if none does).  Replaces
+ * rpa_OptimizeFreqQuad (rpa/OptFreqQuad.f90:12-72).  freqs/weights hold
+ * max_nfreqs entries. */
+int brpa_optimize_freq_quad(int64_t nsets, const double *values, const double *dweights,
+                            double target_error, double target_rel_error, int64_t min_nfreqs,
+                            int64_t max_nfreqs, double *freqs, double *weights, int64_t *nfreqs,
+                            double *info);
+
+/* Fixed-size grid of one fragment on its own histogram: brpa_dai_histogram
+ * (core_orb_thresh, dai_max_thresh as there) then brpa_opt_quad_scaling.
+ * eocc: all occupied energies ascending (core included), evir ascending. */
+int brpa_freq_grid_ex(const double *eocc, int64_t nocc, const double *evir, int64_t nvir,
+                      int64_t nfreqs, double core_orb_thresh, double dai_max_thresh,
+                      double *freqs, double *weights, double *info);
+
+/* brpa_freq_grid_ex with the default CoreOrbThresh (BRPA_CORE_ORB_THRESH) and
+ * no cap: a fragment computed on its own (rpa_DaiMaxThresh of a single system
+ * is its own largest active gap, so the cap changes nothing). */
 int brpa_freq_grid(const double *eocc, int64_t nocc, const double *evir, int64_t nvir,
                    int64_t nfreqs, double *freqs, double *weights, double *info);
 
-/* Adaptive grid: the smallest n in [min_nfreqs, max_nfreqs] meeting both
- * targets.  Replaces rpa_OptimizeFreqQuad (rpa/OptFreqQuad.f90:12-72).
+/* Adaptive grid of one fragment (default CoreOrbThresh, no cap).  Replaces
+ * rpa_OptimizeFreqQuad (rpa/OptFreqQuad.f90:12-72) on a single histogram.
  * freqs/weights must hold max_nfreqs entries. */
 int brpa_optimize_freq_grid(const double *eocc, int64_t nocc, const double *evir, int64_t nvir,
                             double target_error, double target_rel_error,

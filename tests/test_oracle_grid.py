@@ -98,3 +98,137 @@ def test_library_adaptive_grid_matches_oracle():
     xr, wr, ir = grid.optimize_freq_grid(f.eocc, f.evir)
     assert info["nfreqs"] == ir["nfreqs"]
     assert np.allclose(x, xr, rtol=1e-13) and np.allclose(w, wr, rtol=1e-13)
+
+
+def _valence(seed, o=6, v=40):
+    rng = np.random.default_rng(seed)
+    return np.sort(rng.uniform(-1.5, -0.3, o)), np.geomspace(0.05, 3.0, v)
+
+
+def test_core_freezing_equals_dropping_the_core():
+    """Occupied orbitals below CoreOrbThresh = -3 Ha leave the histogram exactly
+    as if they were absent (rpa_NCore, OrbDiffHist.f90:69-74); the library agrees."""
+    import brpa
+    eo, ev = _valence(1)
+    full = np.concatenate([[-20.0, -10.0, -3.5], eo])
+    v0, w0 = grid.dai_histogram(eo, ev)
+    v1, w1 = grid.dai_histogram(full, ev)
+    assert np.array_equal(v0, v1) and np.array_equal(w0, w1)
+    assert grid.ncore(full) == 3
+    # a different threshold freezes fewer orbitals and changes the histogram
+    v2, w2 = grid.dai_histogram(full, ev, core_orb_thresh=-15.0)
+    assert not np.array_equal(v2, v0)
+    for thr in (grid.CORE_ORB_THRESH, -15.0):
+        vl, wl = brpa.dai_histogram(full, ev, core_orb_thresh=thr)
+        vr, wr = grid.dai_histogram(full, ev, core_orb_thresh=thr)
+        assert np.array_equal(vl, vr) and np.array_equal(wl, wr)
+    x, w, _ = brpa.freq_grid(full, ev, 16)
+    xr, wr, _ = grid.freq_grid(eo, ev, 16)
+    assert np.max(np.abs(x - xr) / xr) <= 1e-13 and np.max(np.abs(w - wr) / wr) <= 1e-13
+
+
+def test_histogram_cap_against_brute_force():
+    """A DaiMaxThresh below the fragment's own largest gap: the range ends at the
+    cap and the gaps above it are not counted (OrbDiffHist.f90:75, 87)."""
+    import brpa
+    eo, ev = _valence(2)
+    d = (ev[None, :] - eo[:, None]).ravel()
+    cap = 0.5 * (d.min() + d.max())
+    vals, wts = grid.dai_histogram(eo, ev, dai_max_thresh=cap)
+    dmin = ev[0] - eo[-1]
+    width = (cap - dmin) / grid.NBINS
+    assert np.allclose(vals, dmin + width * np.arange(grid.NBINS), rtol=0, atol=1e-15)
+    kept = d[d <= cap]
+    cnt = np.zeros(grid.NBINS)
+    for g in kept:                       # one gap at a time, the reference's loop
+        t = (g - (dmin + width / 2)) / width
+        k = 1 + int(math.copysign(math.floor(abs(t) + 0.5), t))
+        cnt[min(max(k, 1), grid.NBINS) - 1] += 1
+    assert kept.size < d.size and np.array_equal(wts, np.abs(cnt / kept.size))
+    vl, wl = brpa.dai_histogram(eo, ev, dai_max_thresh=cap)
+    assert np.array_equal(vl, vals) and np.array_equal(wl, wts)
+
+
+def test_all_core_set_has_the_special_values_and_drops_out():
+    """Every occupied orbital frozen: weights -1, values(1) = huge, values(NBins) = 0
+    (OrbDiffHist.f90:106-121); such a set adds nothing to the grid error."""
+    import brpa
+    eo, ev = _valence(3)
+    vs, ws = grid.dai_histogram([-9.0, -4.0], ev)
+    assert np.all(ws == -1.0) and vs[0] == np.finfo(np.float64).max and vs[-1] == 0.0
+    vl, wl = brpa.dai_histogram([-9.0, -4.0], ev)
+    assert np.array_equal(vl, vs) and np.array_equal(wl, ws)
+    v0, w0 = grid.dai_histogram(eo, ev)
+    x, w = grid.casimir_polder(12, 0.9)
+    assert grid.freq_quad_error(x, w, np.stack([v0, vs]), np.stack([w0, ws])) == \
+        grid.freq_quad_error(x, w, v0, w0)
+
+
+def test_dai_max_thresh_brute_force():
+    """rpa_DaiMaxThresh: the largest active gap over the spins of the complex."""
+    import brpa
+    a_occ = np.array([-12.0, -1.2, -0.6])
+    a_vir = np.array([0.1, 0.9, 2.5])
+    b_occ = np.array([-1.4, -0.5])
+    b_vir = np.array([0.2, 3.1])
+    spins = [(a_occ, a_vir), (b_occ, b_vir)]
+    want = max(2.5 - (-1.2), 3.1 - (-1.4))
+    assert grid.dai_max_thresh(spins) == want
+    assert brpa.dai_max_thresh(spins) == want
+    assert grid.dai_max_thresh([(np.array([-5.0]), a_vir)]) == 0.0
+
+
+def test_several_sets_take_the_worst_set():
+    """Two histograms (say two monomers of a dimer): the grid error is the worse
+    of the two at every alpha (OptFreqQuad.f90:163-167), the scan keeps a minimum
+    of that maximum, and it differs from either set's own optimum, so a planted
+    'first set only' rule fails; the library gives the same grid."""
+    import brpa
+    eo1, ev1 = _valence(4, o=5, v=30)
+    eo2 = eo1 - 0.8
+    ev2 = np.geomspace(0.3, 9.0, 30)
+    h1, h2 = grid.dai_histogram(eo1, ev1), grid.dai_histogram(eo2, ev2)
+    V, W = np.stack([h1[0], h2[0]]), np.stack([h1[1], h2[1]])
+    x, w = grid.casimir_polder(10, 1.3)
+    r1, m1 = grid.freq_quad_error(x, w, *h1)
+    r2, m2 = grid.freq_quad_error(x, w, *h2)
+    assert grid.freq_quad_error(x, w, V, W) == (max(r1, r2), max(m1, m2))
+    xs, ws, info = grid.freq_grid_sets(V, W, 10)
+    a = info["alpha"]
+    step = (grid.ALPHA_MAX - grid.ALPHA_MIN) / (grid.NSCAN - 1)
+    xu, wu = grid.casimir_polder(10, 1.0)
+    for b in (a - step, a + step):
+        assert grid.freq_quad_error(b * xu, b * wu, V, W)[1] >= info["rmsd"] * (1 - 1e-12)
+    a1 = grid.freq_grid_sets(*h1, 10)[2]["alpha"]
+    a2 = grid.freq_grid_sets(*h2, 10)[2]["alpha"]
+    assert abs(a - a1) > 10 * step and abs(a - a2) > 10 * step
+    xl, wl, il = brpa.freq_grid_sets(V, W, 10)
+    assert np.max(np.abs(xl - xs) / xs) <= 1e-13 and np.max(np.abs(wl - ws) / ws) <= 1e-13
+    xo, wo, io = grid.optimize_freq_quad(V, W)
+    xl, wl, il = brpa.optimize_freq_quad(V, W)
+    assert il["nfreqs"] == io["nfreqs"]
+    assert np.allclose(xl, xo, rtol=1e-13) and np.allclose(wl, wo, rtol=1e-13)
+
+
+def test_subsystem_grid_capped_at_the_complex():
+    """GridLimitDai: the complex's largest gap caps its subsystems' histograms
+    (rpa_driver.f90:123-147); one grid on [complex, A, B] from the library
+    equals the oracle's."""
+    import brpa
+    eoa, eva = _valence(5, o=4, v=25)
+    eob = np.sort(np.random.default_rng(6).uniform(-1.8, -0.4, 3))
+    evb = np.geomspace(0.08, 6.0, 25)      # a ghost-basis monomer with high virtuals
+    eoc = np.sort(np.concatenate([eoa, eob]))
+    evc = np.sort(np.concatenate([eva, evb[:10]]))
+    cap = grid.dai_max_thresh([(eoc, evc)])
+    assert cap == brpa.dai_max_thresh([(eoc, evc)])
+    hs = [grid.dai_histogram(e, v, dai_max_thresh=cap) for e, v in
+          ((eoc, evc), (eoa, eva), (eob, evb))]
+    hl = [brpa.dai_histogram(e, v, dai_max_thresh=cap) for e, v in
+          ((eoc, evc), (eoa, eva), (eob, evb))]
+    for (v0, w0), (v1, w1) in zip(hs, hl):
+        assert np.array_equal(v0, v1) and np.array_equal(w0, w1)
+    V, W = np.stack([h[0] for h in hs]), np.stack([h[1] for h in hs])
+    x, w, _ = grid.freq_grid_sets(V, W, 16)
+    xl, wl, _ = brpa.freq_grid_sets(V, W, 16)
+    assert np.max(np.abs(xl - x) / x) <= 1e-13 and np.max(np.abs(wl - w) / w) <= 1e-13
