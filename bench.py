@@ -762,7 +762,9 @@ def main():
                             for b, (v, ms) in res.items()}}
         print("# autotune: %s" % tune, file=sys.stderr, flush=True)
 
-    xprec_of = {p: ("split_i8" if p == "split_i8" else "fp64") for p in PRECISIONS}
+    # the exchange runs in the explicit ov space, FP64, in every form (the split
+    # forms cover the Pi products of E_RPA)
+    xprec_of = {p: "fp64" for p in PRECISIONS}
     kx = a.nfrag if a.exchange_frags < 0 else min(a.exchange_frags, a.nfrag)
     xsub = batch[:kx]
     nw = a.exchange_workers
@@ -872,8 +874,7 @@ def main():
                 "stage_ms_per_fragment_by_precision": {
                     q: {k: round(v / kx, 3) for k, v in r["stages"].items()}
                     for q, r in xres.items()},
-                "precision": ("%s amplitude and SOSEX contractions (Pi, Cholesky factors, "
-                              "Gram matrices and amplitude core in FP64)" % xp),
+                "precision": "fp64 (explicit ov-space amplitudes and exchange contraction)",
                 "nguess": "ceil(3.0 N), not capped at o*v",
                 "n_oversampled": int(sum(1 for (o, v, n) in shapes[:kx]
                                          if o * v <= int(np.ceil(3.0 * n)))),
@@ -883,25 +884,29 @@ def main():
                 np.max(np.abs(xres["split_i8"]["ex"] - xres["fp64"]["ex"])))
             exch["n_nt2_differ_split_i8_vs_fp64"] = int(
                 np.sum(xres["split_i8"]["nt"] != xres["fp64"]["nt"]))
-        if p0 == "split_i8":
-            # the split-int8 GEMM of the amplitude contractions, timed in the library
-            # (HIP events around every launch, on the worker's stream) over a
-            # one-worker pass of the first 16 fragments, so that its launches do
-            # not overlap with another fragment's
-            kr = min(16, kx)
-            ctx.exchange_batch(xsub[:kr], seed=synth.BASE_SEED, nworkers=1, precision="split_i8")
-            gms, gfl, gn = ctx.last_gemm_i8_stats()
-            peak_i8 = PEAK_I8_TOPS / 15
-            ach = gfl / (gms * 1e-3) / 1e12
-            roofline_exch = {
-                "bound": "mfma", "achieved": round(ach, 3), "peak": round(peak_i8, 3),
-                "unit": "TFLOP/s", "frac": round(ach / peak_i8, 4), "traffic": None,
-                "kernel": "k_gemm_i8 (5-slice split-int8 GEMM, 15 products; peak = int8 "
-                          "MFMA / 15, FP64-equivalent)",
-                "kernel_ms_per_launch": round(gms / max(gn, 1), 4), "launches": gn,
-                "sample": "one-worker exchange pass over the first %d fragments" % kr}
-            print("# k_gemm_i8: %d launches, %.3f ms each, %.1f TF/s FP64-equivalent" %
-                  (gn, gms / max(gn, 1), ach), file=sys.stderr, flush=True)
+        # the FP64 MFMA GEMM of the explicit-space exchange (I, I Omega, Gram
+        # matrices, T^T I T, T2), timed in the library (HIP events around every
+        # launch, on the worker's stream) over a one-worker pass of the first 16
+        # fragments, so that its launches do not overlap with another fragment's
+        from brpa import _lib as blib
+        kr = min(16, kx)
+        ctx.exchange_batch(xsub[:kr], seed=synth.BASE_SEED, nworkers=1, precision="fp64")
+        kst = {name: ctx.last_kernel_stats(kid) for name, kid in
+               (("k_gemm_f64", blib.KERNEL_GEMM_F64), ("k_trsm_f64", blib.KERNEL_TRSM_F64),
+                ("k_sosex_dot", blib.KERNEL_SOSEX_DOT))}
+        gms, gfl, gn = kst["k_gemm_f64"]
+        ach = gfl / (gms * 1e-3) / 1e12
+        roofline_exch = {
+            "bound": "mfma", "achieved": round(ach, 3), "peak": PEAK_FP64_TFLOPS,
+            "unit": "TFLOP/s", "frac": round(ach / PEAK_FP64_TFLOPS, 4), "traffic": None,
+            "kernel": "k_gemm_f64 (FP64 MFMA GEMM/SYRK of the explicit-space exchange)",
+            "kernel_ms_per_launch": round(gms / max(gn, 1), 4), "launches": gn,
+            "sample": "one-worker exchange pass over the first %d fragments" % kr,
+            "other_kernels": {name: {"ms": round(ms, 3), "launches": n,
+                                     "tflops": round(fl / max(ms, 1e-9) / 1e9, 3)}
+                              for name, (ms, fl, n) in kst.items()}}
+        print("# exchange kernels (one worker, %d fragments): %s" % (kr, roofline_exch["other_kernels"]),
+              file=sys.stderr, flush=True)
 
     cpu = None
     parity = None

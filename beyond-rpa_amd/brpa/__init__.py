@@ -241,6 +241,14 @@ class Context:
                                                       C.byref(n)))
         return ms.value, fl.value, n.value
 
+    def last_kernel_stats(self, kernel):
+        """(ms, algorithmic flop, launches) of one measured kernel (brpa._lib.KERNEL_*)
+        over the last measured call (HIP events around each launch)."""
+        ms, fl, n = C.c_double(0.0), C.c_double(0.0), C.c_int64(0)
+        self._check(self._lib.brpa_last_kernel_stats(self._h, int(kernel), C.byref(ms),
+                                                     C.byref(fl), C.byref(n)))
+        return ms.value, fl.value, n.value
+
     def trim(self):
         self._check(self._lib.brpa_ctx_trim(self._h))
 
@@ -581,6 +589,56 @@ class Context:
                                          _ptr(A), _ptr(R), nocc, nvir, naux, nt,
                                          int(bool(s2_operator))))
         return float(e[0])
+
+    # ---- dense FP64 building blocks (column-major, device tensors) ----------
+    @staticmethod
+    def _fmat(x, name):
+        """(pointer, rows, cols, leading dimension) of a column-major (Fortran-
+        ordered) 2-D float64 CUDA tensor, e.g. torch.empty(n, m).T for an m x n
+        matrix."""
+        if not _is_torch(x) or x.dim() != 2 or x.dtype.itemsize != 8 or not x.is_cuda:
+            raise ValueError("%s: a 2-D float64 CUDA tensor is required" % name)
+        m, n = x.shape
+        if x.stride(0) != 1 or (n > 1 and x.stride(1) < max(1, m)):
+            raise ValueError("%s: column-major storage is required" % name)
+        return C.c_void_p(x.data_ptr()), m, n, max(1, x.stride(1) if n > 1 else m)
+
+    def dgemm(self, A, B, Cm, transa=False, transb=False, alpha=1.0, beta=0.0):
+        """Cm = alpha op(A) op(B) + beta Cm (brpa_dgemm), column-major tensors."""
+        pa, ra, ca, lda = self._fmat(A, "A")
+        pb, rb, cb, ldb = self._fmat(B, "B")
+        pc, m, n, ldc = self._fmat(Cm, "C")
+        k = ra if transa else ca
+        if (ca if transa else ra) != m or (rb if transb else cb) != n or (cb if transb else rb) != k:
+            raise ValueError("shape mismatch")
+        self._sync_stream(A, B, Cm)
+        self._check(self._lib.brpa_dgemm(self._h, int(bool(transa)), int(bool(transb)), m, n, k,
+                                         float(alpha), pa, lda, pb, ldb, float(beta), pc, ldc))
+        return Cm
+
+    def dsyrk(self, A, Cm, trans=False, alpha=1.0, beta=0.0, fill_lower=True):
+        """Upper (and with fill_lower the lower) triangle of Cm = alpha op(A) op(A)^T
+        + beta Cm (brpa_dsyrk), column-major tensors."""
+        pa, ra, ca, lda = self._fmat(A, "A")
+        pc, n, n2, ldc = self._fmat(Cm, "C")
+        k = ra if trans else ca
+        if n != n2 or (ca if trans else ra) != n:
+            raise ValueError("shape mismatch")
+        self._sync_stream(A, Cm)
+        self._check(self._lib.brpa_dsyrk(self._h, int(bool(trans)), int(bool(fill_lower)), n, k,
+                                         float(alpha), pa, lda, float(beta), pc, ldc))
+        return Cm
+
+    def dtrsm(self, U, B, side_right=False):
+        """B <- U^{-T} B (side_right False) or B U^{-1} (True), U upper triangular
+        (brpa_dtrsm), column-major tensors."""
+        pu, ru, cu, ldu = self._fmat(U, "U")
+        pb, m, n, ldb = self._fmat(B, "B")
+        if ru != cu or ru != (n if side_right else m):
+            raise ValueError("shape mismatch")
+        self._sync_stream(U, B)
+        self._check(self._lib.brpa_dtrsm(self._h, int(bool(side_right)), m, n, pu, ldu, pb, ldb))
+        return B
 
     def ao_mo(self, RG, pairs, Co, Cv, out=None, accumulate=False):
         """AO -> MO transform of the Cholesky vectors (rpa_CRG / rpa_C).

@@ -131,8 +131,51 @@ void stage_reset(brpa_ctx *ctx) {
         ctx->stage_ms[s] = 0.0;
         ctx->stage_launches[s] = 0;
     }
-    ctx->gemm_i8_ms = ctx->gemm_i8_flops = 0.0;
-    ctx->gemm_i8_launches = 0;
+    for (int k = 0; k < BRPA_NKERNELS; ++k) {
+        ctx->kst_ms[k] = ctx->kst_flops[k] = 0.0;
+        ctx->kst_launches[k] = 0;
+    }
+}
+
+int kstat_begin(brpa_ctx *ctx, size_t *slot) {
+    while (ctx->gev.size() < ctx->gev_used + 2) {
+        hipEvent_t e;
+        BRPA_HIP(ctx, hipEventCreate(&e));
+        ctx->gev.push_back(e);
+    }
+    *slot = ctx->gev_used;
+    BRPA_HIP(ctx, hipEventRecord(ctx->gev[*slot], ctx->stream));
+    ctx->gev_used += 2;
+    ctx->gev_flops.push_back(0.0);
+    ctx->gev_kind.push_back(-1);
+    return BRPA_OK;
+}
+
+int kstat_end(brpa_ctx *ctx, size_t slot, int kernel, double flops) {
+    BRPA_HIP(ctx, hipEventRecord(ctx->gev[slot + 1], ctx->stream));
+    ctx->gev_flops[slot / 2] = flops;
+    ctx->gev_kind[slot / 2] = kernel;
+    return BRPA_OK;
+}
+
+int kstat_collect(brpa_ctx *ctx) {
+    for (size_t i = 0; i + 1 < ctx->gev_used; i += 2) {
+        const int k = ctx->gev_kind[i / 2];
+        if (k < 0 || k >= BRPA_NKERNELS) continue;
+        float ms = 0.f;
+        BRPA_HIP(ctx, hipEventElapsedTime(&ms, ctx->gev[i], ctx->gev[i + 1]));
+        ctx->kst_ms[k] += ms;
+        ctx->kst_flops[k] += ctx->gev_flops[i / 2];
+        ctx->kst_launches[k] += 1;
+    }
+    kstat_drop(ctx);
+    return BRPA_OK;
+}
+
+void kstat_drop(brpa_ctx *ctx) {
+    ctx->gev_used = 0;
+    ctx->gev_flops.clear();
+    ctx->gev_kind.clear();
 }
 
 int stage_begin(brpa_ctx *ctx, int stage) {
@@ -248,9 +291,15 @@ double brpa_last_kernel_ms(const brpa_ctx *ctx) { return ctx ? ctx->last_kernel_
 
 int brpa_last_gemm_i8_stats(const brpa_ctx *ctx, double *ms, double *flops, int64_t *launches) {
     if (!ctx) return BRPA_ERR_ARG;
-    if (ms) *ms = ctx->gemm_i8_ms;
-    if (flops) *flops = ctx->gemm_i8_flops;
-    if (launches) *launches = ctx->gemm_i8_launches;
+    return brpa_last_kernel_stats(ctx, BRPA_KERNEL_GEMM_I8, ms, flops, launches);
+}
+
+int brpa_last_kernel_stats(const brpa_ctx *ctx, int kernel, double *ms, double *flops,
+                           int64_t *launches) {
+    if (!ctx || kernel < 0 || kernel >= BRPA_NKERNELS) return BRPA_ERR_ARG;
+    if (ms) *ms = ctx->kst_ms[kernel];
+    if (flops) *flops = ctx->kst_flops[kernel];
+    if (launches) *launches = ctx->kst_launches[kernel];
     return BRPA_OK;
 }
 

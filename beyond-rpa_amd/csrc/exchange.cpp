@@ -12,6 +12,7 @@
 // worker's kernels use it.
 #include <atomic>
 #include <cmath>
+#include <cstdlib>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -46,6 +47,10 @@ int brpa_exchange_batch(brpa_ctx *ctx, int precision, int64_t nfrag, const brpa_
         if (rc != BRPA_OK) return brpa_fail(ctx, rc, "brpa_exchange_batch: worker context");
         ctx->workers.push_back(w);
     }
+    // BRPA_EXCHANGE_FORM=pairs keeps the reference's order (amplitude eigenpairs or
+    // factored core, then the pair contraction) for every fragment
+    const char *form = std::getenv("BRPA_EXCHANGE_FORM");
+    const bool pairs = form && std::string(form) == "pairs";
     std::atomic<int64_t> next{0};
     std::atomic<bool> failed{false};
     std::mutex mu;
@@ -69,17 +74,29 @@ int brpa_exchange_batch(brpa_ctx *ctx, int precision, int64_t nfrag, const brpa_
             const int64_t f = next.fetch_add(1);
             if (f >= nfrag) break;
             const brpa_frag_t &fr = frags[f];
+            // the explicit-space form unless the S2 rescaling needs the eigenpairs, the
+            // caller asked for the pair order, or a quadrature weight is not positive
+            const bool xspace = !s2_operator && !pairs &&
+                                xspace_applicable(fr.weights, fr.nfreqs, lambda);
             const int64_t ng = (int64_t)std::ceil(guess_factor * (double)fr.naux);
             int64_t nt = 0;
             double ex = 0.0;
-            // energy only: the factored amplitude core (no eigensolve) unless the S2
-            // rescaling needs the amplitudes themselves
-            int rc = t2_build(w, precision, fr.R, fr.naux, fr.eocc, fr.nocc, fr.evir, fr.nvir,
+            int rc;
+            if (xspace) {
+                // energy only, in the explicit ov space (FP64)
+                rc = exchange_xspace(w, fr.R, fr.naux, fr.eocc, fr.nocc, fr.evir, fr.nvir,
+                                     fr.freqs, fr.weights, fr.nfreqs, lambda, cutoff, ng,
+                                     seed + (uint64_t)f, alpha, &ex, &nt);
+            } else {
+                // the reference's order: amplitudes (the factored core unless the S2
+                // rescaling needs the eigenpairs), then the pair contraction
+                rc = t2_build(w, precision, fr.R, fr.naux, fr.eocc, fr.nocc, fr.evir, fr.nvir,
                               fr.freqs, fr.weights, fr.nfreqs, lambda, cutoff, ng,
                               seed + (uint64_t)f, &nt, !s2_operator);
-            if (rc == BRPA_OK && nt > 0)
-                rc = brpa_sosex(w, precision, &ex, alpha, w->t2_V, w->t2_A, fr.R, fr.nocc,
-                                fr.nvir, fr.naux, nt, s2_operator);
+                if (rc == BRPA_OK && nt > 0)
+                    rc = brpa_sosex(w, precision, &ex, alpha, w->t2_V, w->t2_A, fr.R, fr.nocc,
+                                    fr.nvir, fr.naux, nt, s2_operator);
+            }
             if (rc != BRPA_OK) {
                 std::lock_guard<std::mutex> lk(mu);
                 if (first_rc == BRPA_OK) {
@@ -106,12 +123,14 @@ int brpa_exchange_batch(brpa_ctx *ctx, int precision, int64_t nfrag, const brpa_
     if (first_rc != BRPA_OK)
         return brpa_fail(ctx, first_rc, "brpa_exchange_batch: fragment %lld: %s",
                          (long long)first_f, first_msg.c_str());
-    ctx->gemm_i8_ms = ctx->gemm_i8_flops = 0.0;
-    ctx->gemm_i8_launches = 0;
-    for (int i = 0; i < nworkers; ++i) {
-        ctx->gemm_i8_ms += ctx->workers[i]->gemm_i8_ms;
-        ctx->gemm_i8_flops += ctx->workers[i]->gemm_i8_flops;
-        ctx->gemm_i8_launches += ctx->workers[i]->gemm_i8_launches;
+    for (int k = 0; k < BRPA_NKERNELS; ++k) {
+        ctx->kst_ms[k] = ctx->kst_flops[k] = 0.0;
+        ctx->kst_launches[k] = 0;
+        for (int i = 0; i < nworkers; ++i) {
+            ctx->kst_ms[k] += ctx->workers[i]->kst_ms[k];
+            ctx->kst_flops[k] += ctx->workers[i]->kst_flops[k];
+            ctx->kst_launches[k] += ctx->workers[i]->kst_launches[k];
+        }
     }
     for (int st = 0; st < BRPA_NSTAGES; ++st) {
         ctx->stage_ms[st] = 0.0;

@@ -27,14 +27,16 @@ struct brpa_ctx {
     // set on an exchange-batch worker: the amplitude and SOSEX calls add to the
     // stage times instead of starting them afresh
     bool stage_keep = false;
-    // device time (HIP events around each launch) and algorithmic FP64-
-    // equivalent flop of the split-int8 GEMM launches since the last
-    // brpa_gemm_i8_stats_reset; gev holds event pairs not yet collected
+    // device time (HIP events around each launch), algorithmic (FP64-
+    // equivalent) flop and launch count per measured kernel (BRPA_KERNEL_*)
+    // since the last stage_reset; gev holds event pairs not yet collected,
+    // gev_flops / gev_kind their flop and kernel
     std::vector<hipEvent_t> gev;
     std::vector<double> gev_flops;
+    std::vector<int> gev_kind;
     size_t gev_used = 0;
-    double gemm_i8_ms = 0.0, gemm_i8_flops = 0.0;
-    int64_t gemm_i8_launches = 0;
+    double kst_ms[BRPA_NKERNELS] = {}, kst_flops[BRPA_NKERNELS] = {};
+    int64_t kst_launches[BRPA_NKERNELS] = {};
     // amplitude (T2) results kept on the device until brpa_t2_copyout
     void *rocblas = nullptr;   // rocblas_handle, created on first use
     double *t2_A = nullptr, *t2_V = nullptr;
@@ -93,6 +95,15 @@ int stage_collect(brpa_ctx *ctx, int stage);
 void stage_reset(brpa_ctx *ctx);
 
 int brpa_fail(brpa_ctx *ctx, int code, const char *fmt, ...);
+
+// An event pair around one launch of a measured kernel: kstat_begin records
+// the first event before the launch, kstat_end the second after it together
+// with the kernel's algorithmic flop; kstat_collect adds the elapsed times to
+// kst_* once the stream is idle; kstat_drop forgets pairs of a failed call.
+int kstat_begin(brpa_ctx *ctx, size_t *slot);
+int kstat_end(brpa_ctx *ctx, size_t slot, int kernel, double flops);
+int kstat_collect(brpa_ctx *ctx);
+void kstat_drop(brpa_ctx *ctx);
 
 // One lock for every rocSOLVER call of the library: contexts driven from
 // several host threads (the exchange batch's workers) take turns in it.
@@ -226,3 +237,10 @@ int t2_build(brpa_ctx *ctx, int precision, const double *R, int64_t naux, const 
              uint64_t seed, int64_t *nvecs_t2, bool factored);
 int chol_single(brpa_ctx *ctx, const FragDev *d_fr, int64_t *d_pre, int64_t N, int64_t F);
 
+// Exchange energy of one fragment in the explicit ov space (energy-only form of
+// brpa_t2_compute + brpa_sosex, FP64): E_x and the kept amplitude count.
+bool xspace_applicable(const double *weights, int64_t nfreqs, double lambda);
+int exchange_xspace(brpa_ctx *ctx, const double *R, int64_t naux, const double *eocc, int64_t nocc,
+                    const double *evir, int64_t nvir, const double *freqs, const double *weights,
+                    int64_t nfreqs, double lambda, double cutoff, int64_t nguess, uint64_t seed,
+                    double alpha, double *ex, int64_t *nvecs_t2);

@@ -236,18 +236,13 @@ int launch_gemm_i8(brpa_ctx *ctx, const int8_t *SA, const double *sa, int64_t M,
                    const double *sb, int64_t N, int64_t kpad, double alpha, const double *rs,
                    double beta, double *out, int64_t sm, int64_t sn, int64_t kreal) {
     if (M <= 0 || N <= 0) return BRPA_OK;
-    // an event pair around the launch (collected by gemm_i8_collect)
-    while (ctx->gev.size() < ctx->gev_used + 2) {
-        hipEvent_t e;
-        BRPA_HIP(ctx, hipEventCreate(&e));
-        ctx->gev.push_back(e);
-    }
-    hipEvent_t e0 = ctx->gev[ctx->gev_used], e1 = ctx->gev[ctx->gev_used + 1];
-    BRPA_HIP(ctx, hipEventRecord(e0, ctx->stream));
     if (kpad <= 0 || kpad % 64 != 0 || kpad > I8_KMAX + 64)
         return brpa_fail(ctx, BRPA_ERR_ARG, "launch_gemm_i8: bad depth %lld", (long long)kpad);
     // 128 x 128 tiles (8 waves of 64 x 32, the whole 160 KB of LDS double-
     // buffered) when both sides fill them, else 128 x 64 (8 waves of 32 x 32)
+    // an event pair around the launch (collected by kstat_collect)
+    size_t slot = 0;
+    BRPA_TRY(kstat_begin(ctx, &slot));
     const bool big = N >= 128 && M >= 128;
     const int64_t TM = 128, TN = big ? 128 : 64;
     const int64_t ntm = (M + TM - 1) / TM, ntn = (N + TN - 1) / TN;
@@ -262,21 +257,6 @@ int launch_gemm_i8(brpa_ctx *ctx, const int8_t *SA, const double *sa, int64_t M,
                            ctx->stream, SA, sa, M, SB, sb, N, (int)kpad, (int)ntn, alpha, rs, beta,
                            out, sm, sn);
     LAUNCH_CHECK(ctx);
-    BRPA_HIP(ctx, hipEventRecord(e1, ctx->stream));
-    ctx->gev_used += 2;
-    ctx->gev_flops.push_back(2.0 * (double)M * (double)N * (double)(kreal > 0 ? kreal : kpad));
-    return BRPA_OK;
-}
-
-int gemm_i8_collect(brpa_ctx *ctx) {
-    for (size_t i = 0; i + 1 < ctx->gev_used; i += 2) {
-        float ms = 0.f;
-        BRPA_HIP(ctx, hipEventElapsedTime(&ms, ctx->gev[i], ctx->gev[i + 1]));
-        ctx->gemm_i8_ms += ms;
-        ctx->gemm_i8_flops += ctx->gev_flops[i / 2];
-        ctx->gemm_i8_launches += 1;
-    }
-    ctx->gev_used = 0;
-    ctx->gev_flops.clear();
-    return BRPA_OK;
+    return kstat_end(ctx, slot, BRPA_KERNEL_GEMM_I8,
+                     2.0 * (double)M * (double)N * (double)(kreal > 0 ? kreal : kpad));
 }

@@ -55,11 +55,10 @@ int64_t rowslice_i8_pm_len(int64_t v, int64_t ncols);
 int launch_rowslice_i8(brpa_ctx *, const double *X, int64_t v, int64_t ncols, const double *rs,
                        double *pm, double *scale, int8_t *out);
 // kreal: the contraction length (flop accounting); every launch is timed with
-// an event pair, gemm_i8_collect adds the times once the stream is idle
+// an event pair, kstat_collect adds the times once the stream is idle
 int launch_gemm_i8(brpa_ctx *, const int8_t *SA, const double *sa, int64_t M, const int8_t *SB,
                    const double *sb, int64_t N, int64_t kpad, double alpha, const double *rs,
                    double beta, double *out, int64_t sm, int64_t sn, int64_t kreal);
-int gemm_i8_collect(brpa_ctx *);
 int64_t sosex_i8_nparts(int64_t o, int64_t N, int64_t NT);
 int launch_sosex_i8_pairs(brpa_ctx *, const int8_t *SR, const int8_t *SV, const double *scR,
                           const double *scV, const double *B, int64_t o, int64_t v, int64_t N,
@@ -347,8 +346,7 @@ int t2_build(brpa_ctx *ctx, int precision, const double *R, int64_t naux, const 
     BRPA_TRY(copy_sync(ctx, dpre, pre.data(), 8 * pre.size(), hipMemcpyHostToDevice));
     BRPA_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
     if (!ctx->stage_keep) stage_reset(ctx);
-    ctx->gev_used = 0;   // event pairs left by a call that failed are dropped
-    ctx->gev_flops.clear();
+    kstat_drop(ctx);   // event pairs left by a call that failed are dropped
     // stages whose events were recorded in this call (collected at the end)
     bool staged[BRPA_NSTAGES] = {};
     auto begin = [&](int st) -> int {
@@ -407,7 +405,7 @@ int t2_build(brpa_ctx *ctx, int precision, const double *R, int64_t naux, const 
         BRPA_HIP(ctx, hipStreamSynchronize(ctx->stream));
         for (int st = 0; st < BRPA_NSTAGES; ++st)
             if (staged[st]) BRPA_TRY(stage_collect(ctx, st));
-        return gemm_i8_collect(ctx);
+        return kstat_collect(ctx);
     };
     const double one = 1.0;
     int64_t nt = 0;

@@ -120,6 +120,21 @@ int brpa_last_stage_launches(const brpa_ctx *ctx, int stage);
  * around each launch), algorithmic FP64-equivalent flop (2 m n k per launch)
  * and the launch count.  No reference counterpart (measurement). */
 int brpa_last_gemm_i8_stats(const brpa_ctx *ctx, double *ms, double *flops, int64_t *launches);
+/* The same for every measured kernel: BRPA_KERNEL_GEMM_I8 (as above),
+ * BRPA_KERNEL_GEMM_F64 (the FP64 MFMA GEMM of the explicit-space exchange and
+ * brpa_dgemm / brpa_dsyrk, 2 m n k flop per launch, m n k per launch on the
+ * triangle of a SYRK), BRPA_KERNEL_SOSEX_DOT (the exchange contraction,
+ * ov^2 N flop over the lower triangle of (ai|bj)), BRPA_KERNEL_TRSM_F64 (the
+ * triangular solves, n^2 nrhs flop).  No reference counterpart (measurement). */
+enum {
+    BRPA_KERNEL_GEMM_I8 = 0,
+    BRPA_KERNEL_GEMM_F64 = 1,
+    BRPA_KERNEL_SOSEX_DOT = 2,
+    BRPA_KERNEL_TRSM_F64 = 3,
+    BRPA_NKERNELS = 4
+};
+int brpa_last_kernel_stats(const brpa_ctx *ctx, int kernel, double *ms, double *flops,
+                           int64_t *launches);
 
 /* ---- frequency grid (host code) ----------------------------------------- */
 #define BRPA_HISTOGRAM_NBINS 100      /* RPA_HISTOGRAM_NBINS, rpa/rpa_definitions.f90:113 */
@@ -451,13 +466,48 @@ int brpa_mbpt3_2(brpa_ctx *ctx, double *ec_exchange, const double *R, const doub
  * overlap; the results do not depend on nworkers.  Per-stage device times
  * (summed over the workers) are left in the BRPA_STAGE_T2_* / BRPA_STAGE_SOSEX
  * slots.  ec_exchange[f] receives E_x; nvecs_t2 (may be NULL) the
- * retained NT2.  precision: arithmetic of the amplitude contractions and of
- * the SOSEX contraction (BRPA_FP64 or BRPA_SPLIT_I8, as in brpa_t2_compute
- * and brpa_sosex). */
+ * retained NT2.
+ * Unless s2_operator is set (the cumulant rescaling needs the amplitude
+ * eigenpairs), every quadrature weight is positive and BRPA_EXCHANGE_FORM is
+ * not "pairs" in the environment, each fragment runs in the explicit ov
+ * space, all FP64: I = -sum_k w_k/(4 pi) chi_k R (1 + l Pi_k)^{-1} R^T chi_k is
+ * formed as an ov x ov matrix, the basis of span(I Omega) is chosen by the
+ * reference's Gram eigenvalue cutoff, the amplitude matrix
+ * T2 = T f(T^T I T) T^T (f(a) = a/(2(1+a))) is formed as an ov x ov matrix, and
+ * E_x = -alpha sum (ai|bj) T2(aj,bi) with (ai|bj) = (R R^T)(ai,bj); this
+ * equals the reference's pair contraction of the same amplitudes.  Otherwise
+ * the reference's order runs (brpa_t2_compute, then brpa_sosex) and precision
+ * selects the arithmetic of its amplitude and SOSEX contractions (BRPA_FP64 or
+ * BRPA_SPLIT_I8). */
 int brpa_exchange_batch(brpa_ctx *ctx, int precision, int64_t nfrag, const brpa_frag_t *frags,
                         double lambda, double cutoff, double guess_factor, uint64_t seed,
                         double alpha, int s2_operator, int nworkers, double *ec_exchange,
                         int64_t *nvecs_t2);
+
+/* ---- dense FP64 building blocks -------------------------------------------------- */
+/* The library's own MFMA kernels behind the explicit-space exchange (the
+ * energy-only path of brpa_exchange_batch), exposed for callers that hold
+ * their matrices in HBM.  Column-major (Fortran) layout, device pointers,
+ * run on the context stream, complete on return.
+ * brpa_dgemm: C (m x n) = alpha op(A) op(B) + beta C, transa/transb 0 or 1; it
+ * replaces the dgemm wrappers the reference routes its products through,
+ * real_ab_x / real_aTb_x / real_abT_x / real_aTbT_x
+ * (linear-algebra/real_linalg.f90:1079-1110, 1233-1265, 1296-1328, 1363-1395).
+ * brpa_dsyrk: the upper triangle of C (n x n) = alpha op(A) op(A)^T + beta C,
+ * op(A) n x k (trans: A is k x n); fill_lower = 1 also writes the lower
+ * triangle (C symmetric on return).  No single reference counterpart (the
+ * reference forms these products with dgemm).
+ * brpa_dtrsm: B <- U^{-T} B (side_right = 0, B m x n, U m x m) or B <- B U^{-1}
+ * (side_right = 1, U n x n), U upper triangular (e.g. the factor A = U^T U of a
+ * Cholesky decomposition, real_Cholesky, real_linalg.f90:39-57).  No single
+ * reference counterpart. */
+int brpa_dgemm(brpa_ctx *ctx, int transa, int transb, int64_t m, int64_t n, int64_t k,
+               double alpha, const double *A, int64_t lda, const double *B, int64_t ldb,
+               double beta, double *C, int64_t ldc);
+int brpa_dsyrk(brpa_ctx *ctx, int trans, int fill_lower, int64_t n, int64_t k, double alpha,
+               const double *A, int64_t lda, double beta, double *C, int64_t ldc);
+int brpa_dtrsm(brpa_ctx *ctx, int side_right, int64_t m, int64_t n, const double *U, int64_t ldu,
+               double *B, int64_t ldb);
 
 /* ---- adjoint ----------------------------------------------------------------- */
 /* dE_RPA/dR (C R[ai][P], same shape as R) and dE_RPA/d(eps_occ), dE_RPA/d(eps_vir)

@@ -236,3 +236,25 @@ def test_sosex_split_i8_config5_vs_fp64(ctx):
     e8 = ctx.sosex(V, A, Rd, o, v, precision="split_i8")
     print("config 5: NT2 %d  E_x fp64 %.10f  split_i8 - fp64 = %.3e" % (A.numel(), e64, e8 - e64))
     assert abs(e8 - e64) <= 1e-7
+
+
+@pytest.mark.parametrize("seed0", [300])
+def test_exchange_xspace_matches_pair_order(ctx, monkeypatch, seed0):
+    """The explicit-space exchange (ov x ov I and T2 matrices, FP64) against the
+    reference's order (amplitudes, then the pair contraction) on the same guess
+    vectors, over fragments of both routes (o*v <= NG: T = 1 or the ov x ov
+    eigensolve; o*v > NG: Cholesky-QR or the NG x NG eigensolve): the same NT2
+    and E_x within 1e-8 Ha."""
+    shapes = [(8, 70, 351), (10, 40, 140), (8, 40, 80), (12, 60, 150), (6, 90, 160), (4, 20, 90)]
+    frags = []
+    for i, (o, v, n) in enumerate(shapes):
+        f = synth.make_fragment(o, v, n, seed0 + i)
+        x, w, _ = grid.freq_grid(f.eocc, f.evir, 16)
+        frags.append((f.R, f.eocc, f.evir, x, w))
+    ex, nt = ctx.exchange_batch(frags, seed=9)
+    monkeypatch.setenv("BRPA_EXCHANGE_FORM", "pairs")
+    exp, ntp = ctx.exchange_batch(frags, seed=9)
+    monkeypatch.delenv("BRPA_EXCHANGE_FORM")
+    print("NT2", nt, ntp, "xspace - pairs", ex - exp)
+    assert np.array_equal(nt, ntp)
+    assert np.max(np.abs(ex - exp)) <= 1e-8
