@@ -289,6 +289,22 @@ class Context:
                                                   _ptr(weights), nf, n, _ptr(eu)))
         return float(e[0]), eu
 
+    def ecrpa_analytic_noalloc(self, piu, freqs, weights):
+        """brpa_ecrpa_analytic_noalloc: the same energy with no device allocation
+        (buffers in the library's fixed per-GPU scratch array).  Returns (E, E(u))."""
+        nf, n, n2 = piu.shape
+        if n != n2:
+            raise ValueError("Pi(u) must be square")
+        freqs = np.ascontiguousarray(freqs, np.float64)
+        weights = np.ascontiguousarray(weights, np.float64)
+        e = np.zeros(1)
+        eu = np.zeros(nf)
+        self._sync_stream(piu)
+        self._check(self._lib.brpa_ecrpa_analytic_noalloc(self._h, _ptr(e), _ptr(piu),
+                                                          _ptr(freqs), _ptr(weights), nf, n,
+                                                          _ptr(eu)))
+        return float(e[0]), eu
+
     _BASES = {"random": 1, "eigen": 2, "full": 3, "full_cholesky": 3}
 
     def rpa_g(self, RWR, basis="eigen", target_error=10 ** -2.5, seed=0):

@@ -56,6 +56,7 @@ SIGNATURES = {
     "brpa_gpiug_mo": (C.c_int, [_vp, _vp, _vp, _i64, _vp, _i64, _vp, _i64, C.c_double,
                                 C.c_double]),
     "brpa_ecrpa_analytic": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _i64, _i64, _vp]),
+    "brpa_ecrpa_analytic_noalloc": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _i64, _i64, _vp]),
     "brpa_ecrpa": (C.c_int, [_vp, C.c_int, _vp, _i64, _vp, _i64, _vp, _i64, _vp, _vp, _i64,
                              _vp, _vp]),
     "brpa_ecrpa_batch": (C.c_int, [_vp, C.c_int, _i64, C.POINTER(brpa_frag_t), _vp]),

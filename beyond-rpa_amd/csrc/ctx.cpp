@@ -222,6 +222,12 @@ int brpa_ctx_create(int device, brpa_ctx **out) {
         delete ctx;
         return BRPA_ERR_HIP;
     }
+    // device address of the allocation-free entry point's scratch array (once per
+    // device, with the device current)
+    if (noalloc_setup(device) != BRPA_OK) {
+        brpa_ctx_destroy(ctx);
+        return BRPA_ERR_HIP;
+    }
     ctx->stream = ctx->own_stream;
     *out = ctx;
     return BRPA_OK;

@@ -658,58 +658,63 @@ int brpa_gpiug_mo(brpa_ctx *ctx, double *gpiug, const double *crg, int64_t naux,
 int launch_unpack_lower(brpa_ctx *, const double *AP, int64_t n, int64_t F, double *A);
 
 // E_RPA from stored Pi(u): full (F x N x N) or lower-packed (F x N(N+1)/2) input
-static int ecrpa_stored(brpa_ctx *ctx, double *ecrpa, const double *piu, bool packed,
-                        const double *weights, int64_t nfreqs, int64_t naux, double *eu,
-                        const char *who) {
-    if (!ctx) return BRPA_ERR_ARG;
-    if (!ecrpa || !piu || !weights || nfreqs <= 0 || naux <= 0 || naux > 65536)
-        return brpa_fail(ctx, BRPA_ERR_ARG, "%s: bad arguments", who);
-    BRPA_HIP(ctx, hipSetDevice(ctx->device));
-    const int64_t N = naux, F = nfreqs;
-    const int nsteps = (int)cdiv(N, CHOL_NB);
-    // prefix arrays: [setdiag, pdiag_0..S-1, trsm_0..S-1, update_0..S-1]
-    std::vector<int64_t> pre((size_t)(1 + 3 * nsteps) * 2, 0);
-    pre[1] = cdiv(F * N, 256);
-    for (int j = 0; j < nsteps; ++j) {
-        int64_t j0 = (int64_t)j * CHOL_NB;
-        pre[2 * (1 + j) + 1] = (N > j0) ? F : 0;
-        pre[2 * (1 + nsteps + j) + 1] = (N > j0 + CHOL_NB) ? F * (cdiv(N - j0, CHOL_NB) - 1) : 0;
-        pre[2 * (1 + 2 * nsteps + j) + 1] = F * chol_upd_items(N, j);
-    }
+// Device buffers of E_RPA from stored Pi(u) for nf frequencies of order N,
+// laid out by `ar` (count pass: ar.base == nullptr).  pk_host: a packed host
+// input is staged in the buffer as well.
+struct StoredBufs {
     FragDev hd{};
-    auto layout = [&](Arena &ar) {
+    FragDev *d_fr = nullptr;
+    int64_t *d_pre = nullptr;
+    double *d_pk = nullptr;
+    void layout(Arena &ar, int64_t N, int64_t nf, int nsteps, size_t npre, bool pk_host) {
+        d_fr = ar.take<FragDev>(1);
+        d_pre = ar.take<int64_t>(npre);
+        d_pk = pk_host ? ar.take<double>(nf * (N * (N + 1) / 2)) : nullptr;
         hd.K = 0;
         hd.N = (int)N;
-        hd.F = (int)F;
+        hd.F = (int)nf;
         hd.nsteps = nsteps;
-        hd.Pi = ar.take<double>(F * N * N);
-        hd.diag = ar.take<double>(F * N);
-        hd.ldpart = ar.take<double>(F * nsteps * 2);
-        hd.weights = ar.take<double>(F);
-        hd.eu = ar.take<double>(F);
+        hd.Pi = ar.take<double>(nf * N * N);
+        hd.diag = ar.take<double>(nf * N);
+        hd.ldpart = ar.take<double>(nf * nsteps * 2);
+        hd.weights = ar.take<double>(nf);
+        hd.eu = ar.take<double>(nf);
         hd.energy = ar.take<double>(1);
-        hd.status = ar.take<int>(F);
-        hd.linv = ar.take<double>(F * CHOL_NB * CHOL_NB);
-    };
-    const int64_t np = N * (N + 1) / 2;
-    const bool pk_host = packed && !is_device_ptr(piu);
-    Arena count;
-    count.take<FragDev>(1);
-    count.take<int64_t>(pre.size());
-    if (pk_host) count.take<double>(F * np);
-    layout(count);
-    Arena ar;
-    BRPA_TRY(ws_reserve(ctx, count.off, &ar));
-    FragDev *d_fr = ar.take<FragDev>(1);
-    int64_t *d_pre = ar.take<int64_t>(pre.size());
-    double *d_pk = pk_host ? ar.take<double>(F * np) : nullptr;
-    layout(ar);
+        hd.status = ar.take<int>(nf);
+        hd.linv = ar.take<double>(nf * CHOL_NB * CHOL_NB);
+    }
+};
+
+// Work-group prefix arrays [setdiag, pdiag_0..S-1, trsm_0..S-1, update_0..S-1]
+// of the blocked Cholesky of nf matrices of order N.
+static std::vector<int64_t> stored_prefix(int64_t N, int64_t nf, int nsteps) {
+    std::vector<int64_t> pre((size_t)(1 + 3 * nsteps) * 2, 0);
+    pre[1] = cdiv(nf * N, 256);
+    for (int j = 0; j < nsteps; ++j) {
+        int64_t j0 = (int64_t)j * CHOL_NB;
+        pre[2 * (1 + j) + 1] = (N > j0) ? nf : 0;
+        pre[2 * (1 + nsteps + j) + 1] = (N > j0 + CHOL_NB) ? nf * (cdiv(N - j0, CHOL_NB) - 1) : 0;
+        pre[2 * (1 + 2 * nsteps + j) + 1] = nf * chol_upd_items(N, j);
+    }
+    return pre;
+}
+
+// E_RPA of nf stored Pi(u) (piu: nf full or packed N x N matrices, host or
+// device) in the buffers `b` (laid out by StoredBufs::layout): the chunk's
+// energy goes to *ecrpa (host or device), its E(u) to eu (host or device, may
+// be NULL).
+static int ecrpa_stored_run(brpa_ctx *ctx, StoredBufs &b, const std::vector<int64_t> &pre,
+                            double *ecrpa, const double *piu, bool packed, const double *weights,
+                            int64_t nf, int64_t N, double *eu) {
+    const int nsteps = b.hd.nsteps;
+    const int64_t F = nf;
+    FragDev &hd = b.hd;
     if (packed) {
         // rpa_Unpack (rpa.f90:390-404) on the device, into the Cholesky's buffer
         const double *src = piu;
-        if (pk_host) {
-            BRPA_TRY(copy_sync(ctx, d_pk, piu, 8 * F * np, hipMemcpyHostToDevice));
-            src = d_pk;
+        if (b.d_pk) {
+            BRPA_TRY(copy_sync(ctx, b.d_pk, piu, 8 * F * (N * (N + 1) / 2), hipMemcpyHostToDevice));
+            src = b.d_pk;
         }
         BRPA_TRY(launch_unpack_lower(ctx, src, N, F, hd.Pi));
     } else {
@@ -717,9 +722,11 @@ static int ecrpa_stored(brpa_ctx *ctx, double *ecrpa, const double *piu, bool pa
     }
     BRPA_TRY(to_device(ctx, const_cast<double *>(hd.weights), weights, 8 * F));
     BRPA_TRY(launch_fill_int(ctx, hd.status, F, 0));
-    // completed copies: the descriptor and prefix sums live on this call's stack
-    BRPA_TRY(copy_sync(ctx, d_fr, &hd, sizeof(FragDev), hipMemcpyHostToDevice));
-    BRPA_TRY(copy_sync(ctx, d_pre, pre.data(), 8 * pre.size(), hipMemcpyHostToDevice));
+    // completed copies: the descriptor and prefix sums live on the caller's stack
+    BRPA_TRY(copy_sync(ctx, b.d_fr, &hd, sizeof(FragDev), hipMemcpyHostToDevice));
+    BRPA_TRY(copy_sync(ctx, b.d_pre, pre.data(), 8 * pre.size(), hipMemcpyHostToDevice));
+    const FragDev *d_fr = b.d_fr;
+    const int64_t *d_pre = b.d_pre;
     BRPA_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
     BRPA_TRY(launch_set_diag(ctx, d_fr, d_pre, 1, pre[1], 0));
     for (int j = 0; j < nsteps; ++j) {
@@ -745,6 +752,118 @@ static int ecrpa_stored(brpa_ctx *ctx, double *ecrpa, const double *piu, bool pa
     return BRPA_OK;
 }
 
+static int ecrpa_stored(brpa_ctx *ctx, double *ecrpa, const double *piu, bool packed,
+                        const double *weights, int64_t nfreqs, int64_t naux, double *eu,
+                        const char *who) {
+    if (!ctx) return BRPA_ERR_ARG;
+    if (!ecrpa || !piu || !weights || nfreqs <= 0 || naux <= 0 || naux > 65536)
+        return brpa_fail(ctx, BRPA_ERR_ARG, "%s: bad arguments", who);
+    BRPA_HIP(ctx, hipSetDevice(ctx->device));
+    const int64_t N = naux, F = nfreqs;
+    const int nsteps = (int)cdiv(N, CHOL_NB);
+    const std::vector<int64_t> pre = stored_prefix(N, F, nsteps);
+    const bool pk_host = packed && !is_device_ptr(piu);
+    StoredBufs b;
+    Arena count;
+    b.layout(count, N, F, nsteps, pre.size(), pk_host);
+    Arena ar;
+    BRPA_TRY(ws_reserve(ctx, count.off, &ar));
+    b.layout(ar, N, F, nsteps, pre.size(), pk_host);
+    return ecrpa_stored_run(ctx, b, pre, ecrpa, piu, packed, weights, F, N, eu);
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free E_RPA from stored Pi(u) (brpa_ecrpa_analytic_noalloc).
+//
+// Its device buffers live in `scratch`, one file-scope array per GPU that the
+// library holds until the process ends.  In host code the name `scratch` is a
+// host-side stand-in, not a device address: the device address is looked up
+// once per device (noalloc_setup, called when a context is created on it) and
+// only that address is handed to kernels and copies.  Every call on one GPU
+// shares that GPU's array: calls that overlap on one GPU race.
+namespace {
+constexpr size_t kNoallocBytes = (size_t)BRPA_NOALLOC_SCRATCH_MB << 20;
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mu;
+char *g_noalloc_ptr[kMaxDevices] = {};
+}  // namespace
+
+static __device__ __align__(256) char scratch[kNoallocBytes];
+
+int noalloc_setup(int device) {
+    if (device < 0 || device >= kMaxDevices) return BRPA_ERR_ARG;
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (g_noalloc_ptr[device]) return BRPA_OK;
+    // the caller has made `device` current
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+        (void)hipGetLastError();
+        return BRPA_ERR_HIP;
+    }
+    g_noalloc_ptr[device] = static_cast<char *>(p);
+    return BRPA_OK;
+}
+
+static int ecrpa_stored_noalloc(brpa_ctx *ctx, double *ecrpa, const double *piu, bool packed,
+                                const double *weights, int64_t nfreqs, int64_t naux, double *eu,
+                                const char *who) {
+    if (!ctx) return BRPA_ERR_ARG;
+    if (!ecrpa || !piu || !weights || nfreqs <= 0 || naux <= 0 || naux > 65536)
+        return brpa_fail(ctx, BRPA_ERR_ARG, "%s: bad arguments", who);
+    char *base = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_noalloc_mu);
+        if (ctx->device >= 0 && ctx->device < kMaxDevices) base = g_noalloc_ptr[ctx->device];
+    }
+    if (!base)
+        return brpa_fail(ctx, BRPA_ERR_HIP, "%s: the scratch array of device %d was not set up",
+                         who, ctx->device);
+    BRPA_HIP(ctx, hipSetDevice(ctx->device));
+    const int64_t N = naux, F = nfreqs;
+    const int nsteps = (int)cdiv(N, CHOL_NB);
+    const bool pk_host = packed && !is_device_ptr(piu);
+    // the largest frequency chunk whose buffers fit the scratch array
+    auto bytes_for = [&](int64_t nf) {
+        StoredBufs t;
+        Arena count;
+        t.layout(count, N, nf, nsteps, (size_t)(1 + 3 * nsteps) * 2, pk_host);
+        return count.off;
+    };
+    if (bytes_for(1) > kNoallocBytes)
+        return brpa_fail(ctx, BRPA_ERR_ARG,
+                         "%s: NAux = %lld needs %zu bytes of scratch per frequency, beyond the "
+                         "%zu-byte scratch array of the allocation-free entry point",
+                         who, (long long)N, bytes_for(1), kNoallocBytes);
+    int64_t fc = F;
+    while (fc > 1 && bytes_for(fc) > kNoallocBytes) fc = std::max<int64_t>(1, fc / 2);
+    while (fc < F && bytes_for(fc + 1) <= kNoallocBytes) ++fc;
+    const int64_t msize = packed ? N * (N + 1) / 2 : N * N;
+    long double etot = 0.0L;
+    double ms_tot = 0.0;
+    for (int64_t f0 = 0; f0 < F; f0 += fc) {
+        const int64_t nf = std::min(fc, F - f0);
+        const std::vector<int64_t> pre = stored_prefix(N, nf, nsteps);
+        StoredBufs b;
+        Arena ar;
+        ar.base = base;
+        b.layout(ar, N, nf, nsteps, pre.size(), pk_host);
+        // the chunk's energy comes back to the host (the run ends with a
+        // completed copy, so e is final on return)
+        double e = 0.0;
+        BRPA_TRY(ecrpa_stored_run(ctx, b, pre, &e, piu + f0 * msize, packed, weights + f0, nf, N,
+                                  eu ? eu + f0 : nullptr));
+        etot += (long double)e;
+        ms_tot += ctx->last_kernel_ms;
+    }
+    const double et = (double)etot;
+    if (is_device_ptr(ecrpa))
+        BRPA_TRY(copy_sync(ctx, ecrpa, &et, 8, hipMemcpyHostToDevice));
+    else
+        *ecrpa = et;
+    ctx->last_kernel_ms = ms_tot;
+    return BRPA_OK;
+}
+
 extern "C" {
 
 int brpa_ecrpa_analytic(brpa_ctx *ctx, double *ecrpa, const double *piu, const double *freqs,
@@ -758,6 +877,14 @@ int brpa_ecrpa_packed(brpa_ctx *ctx, double *ecrpa, const double *piu_packed, co
     (void)freqs;   // printed by the reference only (rpa.f90:1368-1372)
     return ecrpa_stored(ctx, ecrpa, piu_packed, true, weights, nfreqs, naux, eu,
                         "brpa_ecrpa_packed");
+}
+
+int brpa_ecrpa_analytic_noalloc(brpa_ctx *ctx, double *ecrpa, const double *piu,
+                                const double *freqs, const double *weights, int64_t nfreqs,
+                                int64_t naux, double *eu) {
+    (void)freqs;
+    return ecrpa_stored_noalloc(ctx, ecrpa, piu, false, weights, nfreqs, naux, eu,
+                                "brpa_ecrpa_analytic_noalloc");
 }
 
 }  // extern "C"

@@ -85,6 +85,9 @@ int sync_reserve(brpa_ctx *ctx, size_t n);
 void graphs_release(brpa_ctx *ctx);
 
 void t2_release(brpa_ctx *ctx);
+// Looks up the device address of the allocation-free entry point's scratch
+// array on `device` (current) once per process; later calls return at once.
+int noalloc_setup(int device);
 void mbpt3_release(brpa_ctx *ctx);
 
 // Records the start/end of a stage on the context stream; stage_end adds the

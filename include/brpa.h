@@ -223,6 +223,25 @@ int brpa_ecrpa_analytic(brpa_ctx *ctx, double *ecrpa, const double *piu,
                         const double *freqs, const double *weights, int64_t nfreqs,
                         int64_t naux, double *eu);
 
+/* Allocation-free form of brpa_ecrpa_analytic.  It has no counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * stream capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * call makes no hipMalloc, hipFree or other allocating call: its device
+ * buffers live in one file-scope scratch array of BRPA_NOALLOC_SCRATCH_MB MiB
+ * per GPU, which the library holds until the process ends (its device address
+ * is looked up when the first context is created on that GPU).  Frequencies
+ * are processed in chunks that fit the array; an NAux whose single-frequency
+ * buffers (about 8 N^2 bytes + 33 KB) do not fit is rejected with
+ * BRPA_ERR_ARG.  Same arguments, results and errors as brpa_ecrpa_analytic
+ * (within FP64 rounding when the frequencies need several chunks: the chunk
+ * energies are added on the host).  Every call on one GPU shares that GPU's
+ * array and leaves its contents behind: calls that overlap on one GPU (from
+ * several contexts or threads) race. */
+#define BRPA_NOALLOC_SCRATCH_MB 256
+int brpa_ecrpa_analytic_noalloc(brpa_ctx *ctx, double *ecrpa, const double *piu,
+                                const double *freqs, const double *weights, int64_t nfreqs,
+                                int64_t naux, double *eu);
+
 /* ---- fused E_RPA of one fragment ---------------------------------------- */
 /* Pi(u_k) build + exact diagonal/trace + Cholesky log-det + quadrature, Pi
  * kept in device memory.  Equivalent to rpa_CC_PiU's frequency loop
