@@ -656,6 +656,28 @@ class Context:
         self._check(self._lib.brpa_dtrsm(self._h, int(bool(side_right)), m, n, pu, ldu, pb, ldb))
         return B
 
+    def dtrtri(self, U, W):
+        """W_z = U_z^{-1} for upper triangular U (brpa_dtrtri): U and W column-major
+        (n, n) CUDA tensors, or batches (nmat, n, n) whose [z] are column-major
+        (i.e. C-contiguous (nmat, n, n) holding U_z^T in row-major order)."""
+        if U.dim() == 2:
+            pu, n, n2, ldu = self._fmat(U, "U")
+            pw, nw, nw2, ldw = self._fmat(W, "W")
+            nmat, su, sw = 1, 0, 0
+        else:
+            if not (U.is_cuda and W.is_cuda and U.is_contiguous() and W.is_contiguous()):
+                raise ValueError("batched U, W must be contiguous CUDA tensors")
+            nmat, n, n2 = U.shape
+            nw, nw2 = W.shape[1:]
+            pu, pw = _ptr(U), _ptr(W)
+            ldu = ldw = n
+            su = sw = n * n
+        if n != n2 or nw != n or nw2 != n:
+            raise ValueError("shape mismatch")
+        self._sync_stream(U, W)
+        self._check(self._lib.brpa_dtrtri(self._h, n, nmat, pu, ldu, su, pw, ldw, sw))
+        return W
+
     def ao_mo(self, RG, pairs, Co, Cv, out=None, accumulate=False):
         """AO -> MO transform of the Cholesky vectors (rpa_CRG / rpa_C).
 

@@ -14,7 +14,7 @@ T2_CUTOFF_EIG, T2_CUTOFF_EIG_DIV_MAXEIG, T2_CUTOFF_EIG_DIV_NELECTRON, T2_CUTOFF_
 STAGE_T2_PI, STAGE_T2_IOMEGA, STAGE_T2_BASIS, STAGE_T2_TIT, STAGE_T2_CORE, STAGE_SOSEX = range(4, 10)
 STAGE_THC_R, STAGE_THC_PI, STAGE_MBPT3, STAGE_T2_EIG = 10, 11, 12, 13
 # measured kernels (brpa_last_kernel_stats)
-KERNEL_GEMM_I8, KERNEL_GEMM_F64, KERNEL_SOSEX_DOT, KERNEL_TRSM_F64 = range(4)
+KERNEL_GEMM_I8, KERNEL_GEMM_F64, KERNEL_SOSEX_DOT, KERNEL_TRSM_F64, KERNEL_TRINV_F64 = range(5)
 HISTOGRAM_NBINS = 100          # BRPA_HISTOGRAM_NBINS
 CORE_ORB_THRESH = -3.0         # BRPA_CORE_ORB_THRESH
 
@@ -95,6 +95,7 @@ SIGNATURES = {
     "brpa_dsyrk": (C.c_int, [_vp, C.c_int, C.c_int, _i64, _i64, C.c_double, _vp, _i64,
                              C.c_double, _vp, _i64]),
     "brpa_dtrsm": (C.c_int, [_vp, C.c_int, _i64, _i64, _vp, _i64, _vp, _i64]),
+    "brpa_dtrtri": (C.c_int, [_vp, _i64, _i64, _vp, _i64, _i64, _vp, _i64, _i64]),
 }
 
 _lib = None

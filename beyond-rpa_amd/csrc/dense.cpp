@@ -13,10 +13,12 @@ int rocblas_handle_of(brpa_ctx *ctx, void **h);
 int launch_gemm_f64(brpa_ctx *, bool ta, bool tb, int64_t m, int64_t n, int64_t k, double alpha,
                     const double *A, int64_t lda, const double *B, int64_t ldb, double beta,
                     double *C, int64_t ldc, bool upper);
-int launch_trsm_f64(brpa_ctx *, bool brow, bool ident, int64_t n, int64_t nrhs, const double *L,
+int launch_trsm_f64(brpa_ctx *, bool brow, int64_t n, int64_t nrhs, const double *L,
                     int64_t ldl, int64_t strideL, double *Y, int64_t ldb, int64_t strideB,
                     int64_t nmat);
 int launch_symm_fill(brpa_ctx *, double *M, int64_t n);
+int launch_trinv_f64(brpa_ctx *, int64_t n, const double *U, int64_t ldu, int64_t su, double *W,
+                     int64_t ldw, int64_t sw, int64_t nmat, double *T);
 
 namespace {
 
@@ -54,16 +56,17 @@ int dense_symm_fill(brpa_ctx *ctx, double *M, int64_t n) { return launch_symm_fi
 int dense_trsm_right(brpa_ctx *ctx, int64_t m, int64_t n, const double *U, int64_t ldu, double *B,
                      int64_t ldb) {
     // B U^{-1} = (U^{-T} B^T)^T: forward substitution with L = U^T on the rows of B
-    return launch_trsm_f64(ctx, true, false, n, m, U, ldu, 0, B, ldb, 0, 1);
+    return launch_trsm_f64(ctx, true, n, m, U, ldu, 0, B, ldb, 0, 1);
 }
 
 int dense_trsm_left_t(brpa_ctx *ctx, int64_t m, int64_t n, const double *U, int64_t ldu,
                       int64_t strideU, double *B, int64_t ldb, int64_t strideB, int64_t nmat) {
-    return launch_trsm_f64(ctx, false, false, m, n, U, ldu, strideU, B, ldb, strideB, nmat);
+    return launch_trsm_f64(ctx, false, m, n, U, ldu, strideU, B, ldb, strideB, nmat);
 }
 
-int dense_tri_inv_t(brpa_ctx *ctx, int64_t n, const double *U, double *W) {
-    return launch_trsm_f64(ctx, false, true, n, n, U, n, 0, W, n, 0, 1);
+int dense_trinv(brpa_ctx *ctx, int64_t n, const double *U, int64_t ldu, int64_t su, double *W,
+                int64_t ldw, int64_t sw, int64_t nmat, double *T) {
+    return launch_trinv_f64(ctx, n, U, ldu, su, W, ldw, sw, nmat, T);
 }
 
 int dense_syevd(brpa_ctx *ctx, int64_t n, double *A, double *D, double *E, int *info) {
@@ -164,6 +167,27 @@ int brpa_dtrsm(brpa_ctx *ctx, int side_right, int64_t m, int64_t n, const double
         BRPA_TRY(dense_trsm_right(ctx, m, n, U, ldu, B, ldb));
     else
         BRPA_TRY(dense_trsm_left_t(ctx, m, n, U, ldu, 0, B, ldb, 0, 1));
+    BRPA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return kstat_collect(ctx);
+}
+
+int brpa_dtrtri(brpa_ctx *ctx, int64_t n, int64_t nmat, const double *U, int64_t ldu, int64_t su,
+                double *W, int64_t ldw, int64_t sw) {
+    if (!ctx) return BRPA_ERR_ARG;
+    if (n < 0 || nmat < 0 || (n > 0 && nmat > 0 && (!U || !W)))
+        return brpa_fail(ctx, BRPA_ERR_ARG, "brpa_dtrtri: bad arguments");
+    if (n == 0 || nmat == 0) return BRPA_OK;
+    if (ldu < n || ldw < n || (nmat > 1 && (su < ldu * n || sw < ldw * n)) || nmat > 65535)
+        return brpa_fail(ctx, BRPA_ERR_ARG, "brpa_dtrtri: leading dimension or stride too small");
+    if (!is_device_ptr(U) || !is_device_ptr(W))
+        return brpa_fail(ctx, BRPA_ERR_ARG, "brpa_dtrtri: operands must be device memory");
+    BRPA_HIP(ctx, hipSetDevice(ctx->device));
+    Arena ar;
+    BRPA_TRY(ws_reserve(ctx, 8 * (size_t)(nmat * trinv_scratch(n)) + 256, &ar));
+    double *T = ar.take<double>(nmat * trinv_scratch(n));
+    stage_reset(ctx);
+    kstat_drop(ctx);
+    BRPA_TRY(dense_trinv(ctx, n, U, ldu, su, W, ldw, sw, nmat, T));
     BRPA_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return kstat_collect(ctx);
 }

@@ -75,6 +75,10 @@ struct brpa_ctx {
     std::vector<brpa_ctx *> workers;
 };
 
+// Triangular operand of a batched GEMM (launch_gemm_f64_batched): the k range
+// where op(A) or op(B) is zero is skipped (the zeros must be stored).
+enum { TRI_NONE = 0, TRI_A_UPPER = 1, TRI_A_LOWER = 2, TRI_B_UPPER = 3, TRI_B_LOWER = 4 };
+
 // Fragment-size bucket of the autotuner (by NAux).
 inline int tune_bucket(int64_t naux) { return naux < 640 ? 0 : (naux < 1280 ? 1 : 2); }
 

@@ -4,14 +4,17 @@
 // at r + ld c), the layout of the reference's Fortran arrays and of the
 // explicit-space exchange.
 //
-//   k_gemm_f64<TA, TB>  C = alpha op(A) op(B) + beta C on 128 x 128 tiles; in the
-//                       upper mode only tiles on or above the diagonal (SYRK)
-//   k_trsm_f64<BROW, IDENT>
-//                       L Y = B by blocked forward substitution, L lower
+//   k_gemm_f64<TA, TB>  batched C = alpha op(A) op(B) diag(cs) + beta C on
+//                       128 x 128 tiles; upper mode (SYRK) and triangular
+//                       operands whose zero triangle is skipped
+//   k_trinv_diag        inverses of the 64 x 64 diagonal blocks of upper
+//                       triangular matrices; with GEMM levels on top, the
+//                       batched triangular inverse (launch_trinv_f64)
+//   k_trsm_f64<BROW>    L Y = B by blocked forward substitution, L lower
 //                       triangular (the row-major lower factor the Cholesky
 //                       kernels leave, i.e. U^T of the column-major upper U);
-//                       B column-major or row-major, or the identity (then
-//                       Y = L^{-1}); batched over matrices with strides
+//                       B column-major or row-major; batched over matrices
+//                       with strides (brpa_dtrsm)
 #include "device_common.h"
 #include "internal.h"
 
@@ -27,13 +30,16 @@ namespace {
 constexpr int GT = 128;    // GEMM tile edge
 constexpr int GKB = 16;    // k per LDS stage
 constexpr int GPAD = 16;   // LDS row padding: rows k and k+1 start 128 B apart mod 256 B
+constexpr int GGROUP = 8;  // tile rows per group of the tile order (panel reuse in L2)
 }
 
 // ---------------------------------------------------------------------------
-// One 128 x 128 tile of C per work-group, four waves of 64 x 64.  The A and B
-// panels of a stage (128 x 16 each) go to LDS as sA[k][row], sB[k][col]; the
-// next stage is prefetched into registers.  The MFMA takes the B fragment as
-// its first operand, so the accumulator rows run along the lanes
+// Batched C = alpha op(A) op(B) diag(cs) + beta C on 128 x 128 tiles, four
+// waves of 64 x 64 per work-group.  The A and B panels of a stage (128 x 16
+// each) go to LDS as sA[k][row], sB[k][col], double-buffered: the next stage
+// is loaded into registers while the current one is multiplied, then stored
+// into the other buffer, one barrier per stage.  The MFMA takes the B fragment
+// as its first operand, so the accumulator rows run along the lanes
 // (D[r] = C(row = lane%16, col = lane/16 + 4r) of a 16 x 16 block) and the
 // stores of C are contiguous across lanes.
 //
@@ -41,25 +47,69 @@ constexpr int GPAD = 16;   // LDS row padding: rows k and k+1 start 128 B apart 
 // consecutive doubles per k; TA reads A(k, row) along k, 16 consecutive
 // doubles per row.  op(B) = B (TB = false) reads B(k, col) along k; TB reads
 // B(col, k) along columns.
+//
+// Batches: work-group x = (item along dimension 1) * tiles + tile, y = item
+// along dimension 2; every operand moves by its own stride per item.  Tile
+// order: upper (SYRK: tiles ti <= tj only, by column) or groups of GGROUP tile
+// rows swept column by column, so that neighbouring work-groups share panels.
+// Triangular operands skip the k range where they are zero (they must hold
+// explicit zeros there):
+//   TRI_A_UPPER  op(A)(i, k) = 0 for k < i    TRI_A_LOWER  for k > i
+//   TRI_B_UPPER  op(B)(k, j) = 0 for k > j    TRI_B_LOWER  for k < j
+// cs (optional): column j of the product is scaled by cs[j] before beta C.
+struct GemmArgs {
+    int m, n, k;
+    int mt, nt, ntiles;
+    int upper, tri, nb1;
+    double alpha, beta;
+    const double *A;
+    int64_t lda, sA1, sA2;
+    const double *B;
+    int64_t ldb, sB1, sB2;
+    double *C;
+    int64_t ldc, sC1, sC2;
+    const double *cs;
+    int64_t scs1, scs2;
+};
+
 template <bool TA, bool TB>
-__global__ __launch_bounds__(256) void k_gemm_f64(int m, int n, int k, double alpha,
-                                                 const double *__restrict__ A, int64_t lda,
-                                                 const double *__restrict__ B, int64_t ldb,
-                                                 double beta, double *__restrict__ C, int64_t ldc,
-                                                 int upper, int mt) {
-    __shared__ double sA[GKB][GT + GPAD];
-    __shared__ double sB[GKB][GT + GPAD];
+__global__ __launch_bounds__(256) void k_gemm_f64(const GemmArgs g) {
+    // LDS layout follows the operand's memory order, so that both the stores
+    // from registers and the MFMA operand reads are free of bank conflicts:
+    //   rows contiguous (op(A) = A, op(B) = B^T): [k][GT + GPAD]
+    //   k contiguous (op(A) = A^T, op(B) = B):     [GT][GKB + 2]  (a row pitch of
+    //   36 dwords puts the 16 rows of an operand read on distinct bank pairs)
+    constexpr int KP = GKB + 2;
+    constexpr int SZ = (GKB * (GT + GPAD) > GT * KP) ? GKB * (GT + GPAD) : GT * KP;
+    __shared__ double sA[2][SZ];
+    __shared__ double sB[2][SZ];
+    const int item1 = (int)(blockIdx.x / (unsigned)g.ntiles);
+    const int t = (int)(blockIdx.x - (unsigned)item1 * (unsigned)g.ntiles);
+    const int item2 = (int)blockIdx.y;
     int ti, tj;
-    if (upper) {
+    if (g.upper) {
         // upper tiles ti <= tj in column order: tj (tj+1)/2 + ti
-        tri_decode((int)blockIdx.x, tj, ti);
+        tri_decode(t, tj, ti);
     } else {
-        ti = (int)(blockIdx.x % (unsigned)mt);
-        tj = (int)(blockIdx.x / (unsigned)mt);
+        const int gsz = GGROUP * g.nt;            // tiles per full group of tile rows
+        const int grp = t / gsz, r = t - grp * gsz;
+        const int rows = min(GGROUP, g.mt - grp * GGROUP);
+        ti = grp * GGROUP + r % rows;
+        tj = r / rows;
     }
+    const double *__restrict__ A = g.A + item1 * g.sA1 + item2 * g.sA2;
+    const double *__restrict__ B = g.B + item1 * g.sB1 + item2 * g.sB2;
+    double *__restrict__ C = g.C + item1 * g.sC1 + item2 * g.sC2;
+    const int m = g.m, n = g.n;
+    const int64_t lda = g.lda, ldb = g.ldb, ldc = g.ldc;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int wm = wave >> 1, wn = wave & 1;
     const int m0 = ti * GT, n0 = tj * GT;
+    int kbeg = 0, kend = g.k;
+    if (g.tri == TRI_A_UPPER) kbeg = m0;
+    else if (g.tri == TRI_A_LOWER) kend = min(kend, m0 + GT);
+    else if (g.tri == TRI_B_UPPER) kend = min(kend, n0 + GT);
+    else if (g.tri == TRI_B_LOWER) kbeg = n0;
 
     d4_t acc[4][4];
 #pragma unroll
@@ -67,65 +117,86 @@ __global__ __launch_bounds__(256) void k_gemm_f64(int m, int n, int k, double al
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = (d4_t){0.0, 0.0, 0.0, 0.0};
 
+    // element e = tid + 256 i of a 128 x 16 panel: rows-contiguous operands
+    // take row e % 128, k e / 128; k-contiguous ones k e % 16, row e / 16
+    // (16 lanes read 128 consecutive bytes of one row or column).  The row
+    // part is fixed per thread, so the addresses advance by a stride per stage.
+    const int ar_ = TA ? (tid >> 4) : (tid & 127), ak_ = TA ? (tid & 15) : (tid >> 7);
+    const int br_ = TB ? (tid & 127) : (tid >> 4), bk_ = TB ? (tid >> 7) : (tid & 15);
+    // per i: row offset 16 i (k-contiguous) or k offset 2 i (rows-contiguous)
+    bool aok[8], bok[8];
+    int64_t aoff[8], boff[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const int r = TA ? ar_ + 16 * i : ar_, kk = TA ? ak_ : ak_ + 2 * i;
+        aok[i] = m0 + r < m;
+        aoff[i] = TA ? (int64_t)kk + lda * (m0 + r) : (int64_t)(m0 + r) + lda * kk;
+        const int c = TB ? br_ : br_ + 16 * i, kb = TB ? bk_ + 2 * i : bk_;
+        bok[i] = n0 + c < n;
+        boff[i] = TB ? (int64_t)(n0 + c) + ldb * kb : (int64_t)kb + ldb * (n0 + c);
+    }
+    const int64_t astep = TA ? 1 : lda, bstep = TB ? ldb : 1;   // per unit of k
+
     double ra[8], rb[8];
     auto gload = [&](int k0) {
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
-            const int e = tid + 256 * i;
-            if (!TA) {
-                const int r = e & 127, kk = e >> 7;
-                ra[i] = (m0 + r < m && k0 + kk < k) ? A[(int64_t)(m0 + r) + lda * (k0 + kk)] : 0.0;
-            } else {
-                const int kk = e & 15, r = e >> 4;
-                ra[i] = (m0 + r < m && k0 + kk < k) ? A[(int64_t)(k0 + kk) + lda * (m0 + r)] : 0.0;
-            }
-            if (!TB) {
-                const int kk = e & 15, c = e >> 4;
-                rb[i] = (n0 + c < n && k0 + kk < k) ? B[(int64_t)(k0 + kk) + ldb * (n0 + c)] : 0.0;
-            } else {
-                const int c = e & 127, kk = e >> 7;
-                rb[i] = (n0 + c < n && k0 + kk < k) ? B[(int64_t)(n0 + c) + ldb * (k0 + kk)] : 0.0;
-            }
+            const int ka = k0 + (TA ? ak_ : ak_ + 2 * i);
+            ra[i] = (aok[i] && ka < kend) ? A[aoff[i] + astep * k0] : 0.0;
+            const int kb = k0 + (TB ? bk_ + 2 * i : bk_);
+            rb[i] = (bok[i] && kb < kend) ? B[boff[i] + bstep * k0] : 0.0;
         }
     };
-    auto lstore = [&]() {
+    auto lstore = [&](int buf) {
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
-            const int e = tid + 256 * i;
             if (!TA)
-                sA[e >> 7][e & 127] = ra[i];
+                sA[buf][(ak_ + 2 * i) * (GT + GPAD) + ar_] = ra[i];
             else
-                sA[e & 15][e >> 4] = ra[i];
-            if (!TB)
-                sB[e & 15][e >> 4] = rb[i];
+                sA[buf][(ar_ + 16 * i) * KP + ak_] = ra[i];
+            if (TB)
+                sB[buf][(bk_ + 2 * i) * (GT + GPAD) + br_] = rb[i];
             else
-                sB[e >> 7][e & 127] = rb[i];
+                sB[buf][(br_ + 16 * i) * KP + bk_] = rb[i];
         }
     };
+    auto aget = [&](int buf, int kr, int row) {
+        return TA ? sA[buf][row * KP + kr] : sA[buf][kr * (GT + GPAD) + row];
+    };
+    auto bget = [&](int buf, int kr, int col) {
+        return TB ? sB[buf][kr * (GT + GPAD) + col] : sB[buf][col * KP + kr];
+    };
 
-    gload(0);
-    for (int k0 = 0; k0 < k; k0 += GKB) {
+    if (kbeg < kend) {
+        gload(kbeg);
+        lstore(0);
         __syncthreads();
-        lstore();
-        __syncthreads();
-        if (k0 + GKB < k) gload(k0 + GKB);
+        if (kbeg + GKB < kend) gload(kbeg + GKB);
+        int buf = 0;
+        for (int k0 = kbeg; k0 < kend; k0 += GKB, buf ^= 1) {
 #pragma unroll
-        for (int kk = 0; kk < GKB; kk += 4) {
-            const int kr = kk + (lane >> 4);
-            double a[4], b[4];
+            for (int kk = 0; kk < GKB; kk += 4) {
+                const int kr = kk + (lane >> 4);
+                double a[4], b[4];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) a[i] = sA[kr][64 * wm + 16 * i + (lane & 15)];
+                for (int i = 0; i < 4; ++i) a[i] = aget(buf, kr, 64 * wm + 16 * i + (lane & 15));
 #pragma unroll
-            for (int j = 0; j < 4; ++j) b[j] = sB[kr][64 * wn + 16 * j + (lane & 15)];
-            // C^T block = op(B)^T op(A)^T: rows of C along the lanes
+                for (int j = 0; j < 4; ++j) b[j] = bget(buf, kr, 64 * wn + 16 * j + (lane & 15));
+                // C^T block = op(B)^T op(A)^T: rows of C along the lanes
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
+                for (int i = 0; i < 4; ++i)
 #pragma unroll
-                for (int j = 0; j < 4; ++j)
-                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[j], a[i], acc[i][j], 0, 0, 0);
+                    for (int j = 0; j < 4; ++j)
+                        acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[j], a[i], acc[i][j], 0, 0, 0);
+            }
+            // the other buffer was last read before the previous barrier
+            if (k0 + GKB < kend) lstore(buf ^ 1);
+            __syncthreads();
+            if (k0 + 2 * GKB < kend) gload(k0 + 2 * GKB);
         }
     }
 
+    const double *cs = g.cs ? g.cs + item1 * g.scs1 + item2 * g.scs2 : nullptr;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         const int row = m0 + 64 * wm + 16 * i + (lane & 15);
@@ -137,43 +208,186 @@ __global__ __launch_bounds__(256) void k_gemm_f64(int m, int n, int k, double al
                 const int col = n0 + 64 * wn + 16 * j + (lane >> 4) + 4 * r;
                 if (col >= n) continue;
                 double *p = C + (int64_t)row + ldc * col;
-                const double v = alpha * acc[i][j][r];
-                *p = beta == 0.0 ? v : fma(beta, *p, v);
+                double v = g.alpha * acc[i][j][r];
+                if (cs) v *= cs[col];
+                *p = g.beta == 0.0 ? v : fma(g.beta, *p, v);
             }
     }
+}
+
+int launch_gemm_f64_batched(brpa_ctx *ctx, bool ta, bool tb, int64_t m, int64_t n, int64_t k,
+                            double alpha, const double *A, int64_t lda, int64_t sA1, int64_t sA2,
+                            const double *B, int64_t ldb, int64_t sB1, int64_t sB2, double beta,
+                            double *C, int64_t ldc, int64_t sC1, int64_t sC2, int64_t nb1,
+                            int64_t nb2, bool upper, int tri, const double *cs, int64_t scs1,
+                            int64_t scs2) {
+    if (m <= 0 || n <= 0 || nb1 <= 0 || nb2 <= 0) return BRPA_OK;
+    if (m > INT32_MAX / 2 || n > INT32_MAX / 2 || k > INT32_MAX / 2 || (upper && m != n) ||
+        nb2 > 65535 || tri < 0 || tri > TRI_B_LOWER)
+        return brpa_fail(ctx, BRPA_ERR_ARG, "gemm: unsupported shape");
+    const int64_t mt = (m + GT - 1) / GT, nt = (n + GT - 1) / GT;
+    const int64_t ntiles = upper ? nt * (nt + 1) / 2 : mt * nt;
+    if (ntiles * nb1 > INT32_MAX) return brpa_fail(ctx, BRPA_ERR_ARG, "gemm: too many tiles");
+    GemmArgs g;
+    g.m = (int)m;
+    g.n = (int)n;
+    g.k = k > 0 ? (int)k : 0;
+    g.mt = (int)mt;
+    g.nt = (int)nt;
+    g.ntiles = (int)ntiles;
+    g.upper = upper ? 1 : 0;
+    g.tri = tri;
+    g.nb1 = (int)nb1;
+    g.alpha = alpha;
+    g.beta = beta;
+    g.A = A;
+    g.lda = lda;
+    g.sA1 = sA1;
+    g.sA2 = sA2;
+    g.B = B;
+    g.ldb = ldb;
+    g.sB1 = sB1;
+    g.sB2 = sB2;
+    g.C = C;
+    g.ldc = ldc;
+    g.sC1 = sC1;
+    g.sC2 = sC2;
+    g.cs = cs;
+    g.scs1 = scs1;
+    g.scs2 = scs2;
+    const dim3 grid((unsigned)(ntiles * nb1), (unsigned)nb2), block(256);
+    size_t slot = 0;
+    BRPA_TRY(kstat_begin(ctx, &slot));
+    if (!ta && !tb)
+        hipLaunchKernelGGL((k_gemm_f64<false, false>), grid, block, 0, ctx->stream, g);
+    else if (!ta && tb)
+        hipLaunchKernelGGL((k_gemm_f64<false, true>), grid, block, 0, ctx->stream, g);
+    else if (ta && !tb)
+        hipLaunchKernelGGL((k_gemm_f64<true, false>), grid, block, 0, ctx->stream, g);
+    else
+        hipLaunchKernelGGL((k_gemm_f64<true, true>), grid, block, 0, ctx->stream, g);
+    LAUNCH_CHECK(ctx);
+    // algorithmic flop: 2 m n k, n (n + 1) k on the triangle of a SYRK, m n k
+    // (n m k) with a triangular op(A) (op(B)) of order m = k (n = k)
+    double fl = upper ? (double)n * (double)(n + 1) * (double)g.k
+                      : 2.0 * (double)m * (double)n * (double)g.k;
+    if (tri != TRI_NONE) fl *= 0.5;
+    return kstat_end(ctx, slot, BRPA_KERNEL_GEMM_F64, fl * (double)nb1 * (double)nb2);
 }
 
 int launch_gemm_f64(brpa_ctx *ctx, bool ta, bool tb, int64_t m, int64_t n, int64_t k,
                     double alpha, const double *A, int64_t lda, const double *B, int64_t ldb,
                     double beta, double *C, int64_t ldc, bool upper) {
-    if (m <= 0 || n <= 0) return BRPA_OK;
-    if (m > INT32_MAX / 2 || n > INT32_MAX / 2 || k > INT32_MAX / 2 || (upper && m != n))
-        return brpa_fail(ctx, BRPA_ERR_ARG, "gemm: unsupported shape");
-    const int64_t mt = (m + GT - 1) / GT, nt = (n + GT - 1) / GT;
-    const int64_t nb = upper ? nt * (nt + 1) / 2 : mt * nt;
-    if (nb > INT32_MAX) return brpa_fail(ctx, BRPA_ERR_ARG, "gemm: too many tiles");
-    const dim3 g((unsigned)nb), b(256);
-    const int im = (int)m, in = (int)n, ik = (int)k > 0 ? (int)k : 0, iu = upper ? 1 : 0,
-              imt = (int)mt;
+    return launch_gemm_f64_batched(ctx, ta, tb, m, n, k, alpha, A, lda, 0, 0, B, ldb, 0, 0, beta,
+                                   C, ldc, 0, 0, 1, 1, upper, TRI_NONE, nullptr, 0, 0);
+}
+
+// ---------------------------------------------------------------------------
+// Triangular inverse W = U^{-1} of nmat upper triangular matrices (column-
+// major, U(r, c) at r + ldu c + z su; W(r, c) at r + ldw c + z sw), by
+// recursive doubling: the 64 x 64 diagonal blocks are inverted by
+// substitution (k_trinv_diag, which also writes the zeros below the diagonal),
+// then for block sizes b = 64, 128, ...  every pair of neighbouring inverted
+// blocks [W11 .; 0 W22] of a 2b block gets its corner
+//   W12 = -W11 U12 W22
+// by two batched triangular GEMMs (T = U12 W22, W12 = -W11 T).  Only the upper
+// triangle of U is read.
+namespace {
+constexpr int TIB = 64;
+}
+
+__global__ __launch_bounds__(256) void k_trinv_diag(int n, const double *__restrict__ U,
+                                                   int64_t ldu, int64_t su,
+                                                   double *__restrict__ W, int64_t ldw,
+                                                   int64_t sw) {
+    __shared__ double sU[TIB][TIB + 1];
+    __shared__ double sW[TIB][TIB + 1];
+    const int p = (int)blockIdx.x, z = (int)blockIdx.y;
+    const double *Um = U + (int64_t)z * su;
+    double *Wm = W + (int64_t)z * sw;
+    const int r0 = p * TIB;
+    const int nb = min(TIB, n - r0);
+    const int tid = threadIdx.x;
+    // the upper triangle of the diagonal block; identity past the matrix end
+    for (int e = tid; e < TIB * TIB; e += 256) {
+        const int c = e >> 6, r = e & 63;
+        double v = 0.0;
+        if (r < nb && c < nb) {
+            if (r <= c) v = Um[(int64_t)(r0 + r) + ldu * (r0 + c)];
+        } else if (r == c) {
+            v = 1.0;
+        }
+        sU[r][c] = v;
+    }
+    __syncthreads();
+    // column c of the inverse by back substitution: U w = e_c, w_i = 0 for i > c
+    if (tid < TIB) {
+        const int c = tid;
+        for (int i = TIB - 1; i > c; --i) sW[i][c] = 0.0;
+        for (int i = c; i >= 0; --i) {
+            double s0 = (i == c) ? 1.0 : 0.0, s1 = 0.0;
+            int q = i + 1;
+            for (; q + 1 <= c; q += 2) {
+                s0 = fma(-sU[i][q], sW[q][c], s0);
+                s1 = fma(-sU[i][q + 1], sW[q + 1][c], s1);
+            }
+            if (q <= c) s0 = fma(-sU[i][q], sW[q][c], s0);
+            sW[i][c] = (s0 + s1) / sU[i][i];
+        }
+    }
+    __syncthreads();
+    // the block, and zeros below it in its columns
+    for (int e = tid; e < TIB * TIB; e += 256) {
+        const int c = e >> 6, r = e & 63;
+        if (r < nb && c < nb) Wm[(int64_t)(r0 + r) + ldw * (r0 + c)] = sW[r][c];
+    }
+    for (int64_t e = tid; e < (int64_t)(n - r0 - nb) * nb; e += 256) {
+        const int c = (int)(e / (n - r0 - nb));
+        const int r = (int)(e - (int64_t)c * (n - r0 - nb));
+        Wm[(int64_t)(r0 + nb + r) + ldw * (r0 + c)] = 0.0;
+    }
+}
+
+int64_t trinv_scratch(int64_t n) { return n * n; }
+
+int launch_trinv_f64(brpa_ctx *ctx, int64_t n, const double *U, int64_t ldu, int64_t su, double *W,
+                     int64_t ldw, int64_t sw, int64_t nmat, double *T) {
+    if (n <= 0 || nmat <= 0) return BRPA_OK;
+    if (n > INT32_MAX / 2 || nmat > 65535 || ldu < n || ldw < n)
+        return brpa_fail(ctx, BRPA_ERR_ARG, "trinv: unsupported shape");
+    const int64_t nblk = (n + TIB - 1) / TIB;
     size_t slot = 0;
     BRPA_TRY(kstat_begin(ctx, &slot));
-    if (!ta && !tb)
-        hipLaunchKernelGGL((k_gemm_f64<false, false>), g, b, 0, ctx->stream, im, in, ik, alpha, A,
-                           lda, B, ldb, beta, C, ldc, iu, imt);
-    else if (!ta && tb)
-        hipLaunchKernelGGL((k_gemm_f64<false, true>), g, b, 0, ctx->stream, im, in, ik, alpha, A,
-                           lda, B, ldb, beta, C, ldc, iu, imt);
-    else if (ta && !tb)
-        hipLaunchKernelGGL((k_gemm_f64<true, false>), g, b, 0, ctx->stream, im, in, ik, alpha, A,
-                           lda, B, ldb, beta, C, ldc, iu, imt);
-    else
-        hipLaunchKernelGGL((k_gemm_f64<true, true>), g, b, 0, ctx->stream, im, in, ik, alpha, A,
-                           lda, B, ldb, beta, C, ldc, iu, imt);
+    hipLaunchKernelGGL(k_trinv_diag, dim3((unsigned)nblk, (unsigned)nmat), dim3(256), 0,
+                       ctx->stream, (int)n, U, ldu, su, W, ldw, sw);
     LAUNCH_CHECK(ctx);
-    // algorithmic flop: 2 m n k, or n (n + 1) k for the triangle of a SYRK
-    const double fl = upper ? (double)n * (double)(n + 1) * (double)ik
-                            : 2.0 * (double)m * (double)n * (double)ik;
-    return kstat_end(ctx, slot, BRPA_KERNEL_GEMM_F64, fl);
+    BRPA_TRY(kstat_end(ctx, slot, BRPA_KERNEL_TRINV_F64, (double)nmat * nblk * TIB * TIB * TIB / 3.0));
+    for (int64_t b = TIB; b < n; b *= 2) {
+        // full pairs [r0, r0 + 2b) and one ragged pair whose lower block is shorter
+        const int64_t nfull = n / (2 * b);
+        const int64_t rr = nfull * 2 * b;
+        const int64_t b2 = (rr + b < n) ? n - rr - b : 0;
+        const int64_t sT2 = (nfull + 1) * b * b;   // T per matrix
+        struct Part {
+            int64_t r0, cnt, bb;
+        } parts[2] = {{0, nfull, b}, {rr, b2 > 0 ? 1 : 0, b2}};
+        for (const Part &pt : parts) {
+            if (pt.cnt == 0) continue;
+            const int64_t r0 = pt.r0, bb = pt.bb;
+            double *Tp = T + (r0 / (2 * b)) * b * b;
+            // T = U12 W22: U12 (b x bb) at rows r0.., columns r0 + b..; W22 upper
+            BRPA_TRY(launch_gemm_f64_batched(
+                ctx, false, false, b, bb, bb, 1.0, U + r0 + ldu * (r0 + b), ldu, 2 * b * (1 + ldu),
+                su, W + (r0 + b) * (1 + ldw), ldw, 2 * b * (1 + ldw), sw, 0.0, Tp, b, b * b, sT2,
+                pt.cnt, nmat, false, TRI_B_UPPER, nullptr, 0, 0));
+            // W12 = -W11 T: W11 (b x b) upper
+            BRPA_TRY(launch_gemm_f64_batched(
+                ctx, false, false, b, bb, b, -1.0, W + r0 * (1 + ldw), ldw, 2 * b * (1 + ldw), sw, Tp,
+                b, b * b, sT2, 0.0, W + r0 + ldw * (r0 + b), ldw, 2 * b * (1 + ldw), sw, pt.cnt,
+                nmat, false, TRI_A_UPPER, nullptr, 0, 0));
+        }
+    }
+    return BRPA_OK;
 }
 
 // ---------------------------------------------------------------------------
@@ -181,8 +395,7 @@ int launch_gemm_f64(brpa_ctx *ctx, bool ta, bool tb, int64_t m, int64_t n, int64
 // row-major lower triangle (L(r, c) at r * ldl + c, c <= r), Y overwriting B.
 // B is n x nrhs: column-major (BROW = false, B(r, c) at r + ldb c) or row-major
 // (BROW, B(r, c) at r ldb + c, i.e. the rows of a column-major nrhs x n
-// matrix X: X <- X L^{-T}).  IDENT: B = 1 (n = nrhs), the result L^{-1} written
-// to Y (column-major, ldb), zero above the diagonal.
+// matrix X: X <- X L^{-T}).
 //
 // One work-group per (matrix, 64 right-hand sides), left-looking over 64-row
 // blocks p: acc = B_p - sum_{q<p} L_pq Y_q on MFMA (each wave 16 rows x 64
@@ -193,7 +406,7 @@ namespace {
 constexpr int TB_ = 64;
 }
 
-template <bool BROW, bool IDENT>
+template <bool BROW>
 __global__ __launch_bounds__(256) void k_trsm_f64(int n, int nrhs, const double *__restrict__ L,
                                                  int64_t ldl, int64_t strideL,
                                                  double *__restrict__ Y, int64_t ldb,
@@ -211,22 +424,13 @@ __global__ __launch_bounds__(256) void k_trsm_f64(int n, int nrhs, const double 
     auto addr = [&](int r, int c) -> int64_t {
         return BROW ? (int64_t)r * ldb + c : (int64_t)r + ldb * (int64_t)c;
     };
-    // L^{-1} is lower triangular: rows above the first column block are zero
-    const int pstart = IDENT ? ct : 0;
-    if (IDENT) {
-        // zero the blocks above the diagonal of this column range
-        for (int e = tid; e < c0 * TB_; e += 256) {
-            const int r = e / TB_, c = c0 + (e % TB_);
-            if (c < nrhs) Ym[addr(r, c)] = 0.0;
-        }
-    }
-    for (int p = pstart; p < nblk; ++p) {
+    for (int p = 0; p < nblk; ++p) {
         const int r0 = p * TB_;
         // acc: rows 16 * wave + (lane % 16) of the block, columns lane/16 + 4 r + 16 j
         d4_t acc[4];
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[j] = (d4_t){0.0, 0.0, 0.0, 0.0};
-        for (int q = pstart; q < p; ++q) {
+        for (int q = 0; q < p; ++q) {
             const int q0 = q * TB_;
             __syncthreads();
             for (int e = tid; e < TB_ * TB_; e += 256) {
@@ -269,11 +473,7 @@ __global__ __launch_bounds__(256) void k_trsm_f64(int n, int nrhs, const double 
             for (int rr = 0; rr < 4; ++rr) {
                 const int r = 16 * wave + (lane & 15);
                 const int c = 16 * j + (lane >> 4) + 4 * rr;
-                double bv;
-                if (IDENT)
-                    bv = (r0 + r == c0 + c) ? 1.0 : 0.0;
-                else
-                    bv = (r0 + r < n && c0 + c < nrhs) ? Ym[addr(r0 + r, c0 + c)] : 0.0;
+                const double bv = (r0 + r < n && c0 + c < nrhs) ? Ym[addr(r0 + r, c0 + c)] : 0.0;
                 sY[r][c] = bv - acc[j][rr];
             }
         __syncthreads();
@@ -312,29 +512,25 @@ __global__ __launch_bounds__(256) void k_trsm_f64(int n, int nrhs, const double 
     }
 }
 
-int launch_trsm_f64(brpa_ctx *ctx, bool brow, bool ident, int64_t n, int64_t nrhs,
-                    const double *L, int64_t ldl, int64_t strideL, double *Y, int64_t ldb,
-                    int64_t strideB, int64_t nmat) {
+int launch_trsm_f64(brpa_ctx *ctx, bool brow, int64_t n, int64_t nrhs, const double *L,
+                    int64_t ldl, int64_t strideL, double *Y, int64_t ldb, int64_t strideB,
+                    int64_t nmat) {
     if (n <= 0 || nrhs <= 0 || nmat <= 0) return BRPA_OK;
-    if (n > INT32_MAX / 2 || nrhs > INT32_MAX / 2 || (ident && (n != nrhs || brow)))
+    if (n > INT32_MAX / 2 || nrhs > INT32_MAX / 2)
         return brpa_fail(ctx, BRPA_ERR_ARG, "trsm: unsupported shape");
     const int64_t nct = (nrhs + TB_ - 1) / TB_;
     if (nct * nmat > INT32_MAX) return brpa_fail(ctx, BRPA_ERR_ARG, "trsm: too many blocks");
     const dim3 g((unsigned)(nct * nmat)), b(256);
     size_t slot = 0;
     BRPA_TRY(kstat_begin(ctx, &slot));
-    if (ident)
-        hipLaunchKernelGGL((k_trsm_f64<false, true>), g, b, 0, ctx->stream, (int)n, (int)nrhs, L,
-                           ldl, strideL, Y, ldb, strideB, (int)nct);
-    else if (brow)
-        hipLaunchKernelGGL((k_trsm_f64<true, false>), g, b, 0, ctx->stream, (int)n, (int)nrhs, L,
-                           ldl, strideL, Y, ldb, strideB, (int)nct);
+    if (brow)
+        hipLaunchKernelGGL((k_trsm_f64<true>), g, b, 0, ctx->stream, (int)n, (int)nrhs, L, ldl,
+                           strideL, Y, ldb, strideB, (int)nct);
     else
-        hipLaunchKernelGGL((k_trsm_f64<false, false>), g, b, 0, ctx->stream, (int)n, (int)nrhs, L,
-                           ldl, strideL, Y, ldb, strideB, (int)nct);
+        hipLaunchKernelGGL((k_trsm_f64<false>), g, b, 0, ctx->stream, (int)n, (int)nrhs, L, ldl,
+                           strideL, Y, ldb, strideB, (int)nct);
     LAUNCH_CHECK(ctx);
-    // algorithmic flop: n^2 per right-hand side (n^3 / 3 for the inverse)
-    const double fl = ident ? (double)n * (double)n * (double)n / 3.0
-                            : (double)n * (double)n * (double)nrhs * (double)nmat;
-    return kstat_end(ctx, slot, BRPA_KERNEL_TRSM_F64, fl);
+    // algorithmic flop: n^2 per right-hand side
+    return kstat_end(ctx, slot, BRPA_KERNEL_TRSM_F64,
+                     (double)n * (double)n * (double)nrhs * (double)nmat);
 }

@@ -9,7 +9,6 @@
 // contraction costs ov^2 N flop (G) instead of 2 o^2 v N NT2 for the pair
 // products of the reference's order.
 //
-//   k_stack_scale   Y[ai][off + P] = R[ai][P] s[ai]  (column-major Y, leading dim ld)
 //   k_symm_fill     lower triangle of a column-major symmetric matrix from its upper one
 //   k_sosex_dot     per tile pair of G: sum_{r,c} G(r,c) (gamma M(pi(r,c)) + delta [r==c])
 #include "device_common.h"
@@ -22,27 +21,6 @@
             return brpa_fail((ctx), BRPA_ERR_HIP, "kernel launch failed: %s (%s:%d)",          \
                              hipGetErrorString(_e), __FILE__, __LINE__);                       \
     } while (0)
-
-// ---------------------------------------------------------------------------
-// Y(off + P, ai) = R[ai][P] * s[ai] for P < N, ai < K; Y column-major with
-// leading dimension ld (one frequency's block of the stacked operand).
-__global__ void k_stack_scale(double *__restrict__ Y, int64_t ld, int64_t off,
-                              const double *__restrict__ R, const double *__restrict__ s,
-                              int64_t N, int64_t K) {
-    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (e >= N * K) return;
-    const int64_t ai = e / N, P = e - ai * N;
-    Y[ai * ld + off + P] = R[e] * s[ai];
-}
-
-int launch_stack_scale(brpa_ctx *ctx, double *Y, int64_t ld, int64_t off, const double *R,
-                       const double *s, int64_t N, int64_t K) {
-    if (N * K <= 0) return BRPA_OK;
-    hipLaunchKernelGGL(k_stack_scale, dim3((unsigned)((N * K + 255) / 256)), dim3(256), 0,
-                       ctx->stream, Y, ld, off, R, s, N, K);
-    LAUNCH_CHECK(ctx);
-    return BRPA_OK;
-}
 
 // ---------------------------------------------------------------------------
 // M(r, c) = M(c, r) for r > c, n x n column-major (element (r, c) at r + n c):

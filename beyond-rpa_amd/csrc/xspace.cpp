@@ -10,7 +10,9 @@
 // (ov <= 8.7 N):
 //   1. Pi_k and the Cholesky factors 1 + l Pi_k = U_k^T U_k (as for E_RPA);
 //   2. Y = [Y_1; ...; Y_F], Y_k = U_k^{-T} R^T diag(chi_k (w_k l/(4 pi))^{1/2}),
-//      and I = -Y^T Y  (one SYRK over F N);
+//      and I = -Y^T Y  (one SYRK over F N); U_k^{-1} by a batched triangular
+//      inverse, Y_k by one batched triangular GEMM with the column scale in
+//      its epilogue;
 //   3. X = normalised I Omega, Omega the normalised guess vectors;
 //   4. the orthonormal basis T of span(X) with the reference's eigenvalue
 //      cutoff (Cholesky-QR when every Gram eigenvalue clears it, the
@@ -18,6 +20,8 @@
 //      clears it, T = 1;
 //   5. TIT = T^T (I T), 1 + TIT = U^T U, W = T U^{-1},
 //      T2 = T (1/2) TIT (1 + TIT)^{-1} T^T = (T T^T - W W^T) / 2;
+// Every triangular solve is a triangular inverse (GEMM levels) followed by a
+// GEMM that skips the zero triangle.
 //   6. E_x = -alpha sum_{aibj} (ai|bj) T2(aj,bi) by k_sosex_dot, with
 //      (ai|bj) = (R R^T)(ai,bj) formed tile by tile on MFMA.
 // T2 = T f(T^T I T) T^T depends only on the subspace span(T), so every basis of
@@ -37,8 +41,6 @@ int launch_col_scale(brpa_ctx *, double *out, const double *in, const double *s,
                      int64_t n);
 int launch_scale(brpa_ctx *, double *x, double a, int64_t n);
 int launch_add_diag(brpa_ctx *, double *M, int64_t n, double a);
-int launch_stack_scale(brpa_ctx *, double *Y, int64_t ld, int64_t off, const double *R,
-                       const double *s, int64_t N, int64_t K);
 int64_t sosex_dot_nparts(int64_t K);
 int launch_sosex_dot(brpa_ctx *, const double *R, int64_t K, int64_t N, int64_t v, const double *M,
                      double gamma, double delta, double *part);
@@ -125,7 +127,7 @@ int exchange_xspace(brpa_ctx *ctx, const double *R, int64_t naux, const double *
     FragDev fd{};
     FragDev *dfd = nullptr;
     int64_t *dpre = nullptr, *dchol = nullptr;
-    double *dR = nullptr, *chis, *Y, *Im, *Om, *X, *Wb, *Sm, *TIT, *sev, *sE, *part;
+    double *dR = nullptr, *chis, *Y, *Im, *Om, *X, *Wb, *Sm, *TIT, *Winv, *Tinv, *sev, *sE, *part;
     int *dinfo;
     DenseChol dc;
     std::vector<int64_t> pre = {0, 0, 0, 0, 0, 0};
@@ -157,6 +159,8 @@ int exchange_xspace(brpa_ctx *ctx, const double *R, int64_t naux, const double *
         Wb = ar.take<double>(K * ntmax);
         Sm = ar.take<double>(nd * nd);
         TIT = ar.take<double>(ntmax * ntmax);
+        Winv = ar.take<double>(std::max(F * N * N, nd * nd));
+        Tinv = ar.take<double>(std::max(F * trinv_scratch(N), trinv_scratch(nd)));
         sev = ar.take<double>(std::max(K, NG));
         sE = ar.take<double>(std::max(K, NG));
         dinfo = ar.take<int>(2);
@@ -208,10 +212,12 @@ int exchange_xspace(brpa_ctx *ctx, const double *R, int64_t naux, const double *
     if (lambda != 1.0) BRPA_TRY(launch_scale(ctx, fd.Pi, lambda, F * N * N));
     BRPA_TRY(launch_set_diag(ctx, dfd, dpre + 4, 1, pre[5], 0));
     BRPA_TRY(chol_single(ctx, dfd, dchol, N, F));
-    // 2. Y_k = U_k^{-T} (R^T diag(chis_k)), stacked: column-major (F N) x K
-    for (int64_t k = 0; k < F; ++k)
-        BRPA_TRY(launch_stack_scale(ctx, Y, FN, k * N, dR, chis + k * K, N, K));
-    BRPA_TRY(dense_trsm_left_t(ctx, N, K, fd.Pi, N, N * N, Y, FN, N, F));
+    // 2. W_k = U_k^{-1} (the factor is the column-major upper triangle of Pi_k),
+    // then Y_k = W_k^T R^T diag(chis_k) stacked: column-major (F N) x K, block k
+    // at rows k N.. (R^T is R[ai][P] read as a column-major N x K matrix)
+    BRPA_TRY(dense_trinv(ctx, N, fd.Pi, N, N * N, Winv, N, N * N, F, Tinv));
+    BRPA_TRY(launch_gemm_f64_batched(ctx, true, false, N, K, N, 1.0, Winv, N, N * N, 0, dR, N, 0,
+                                     0, 0.0, Y, FN, N, 0, F, 1, false, TRI_A_LOWER, chis, K, 0));
     BRPA_TRY(stage_end(ctx, BRPA_STAGE_T2_PI));
 
     // 2-3. I = -Y^T Y, then I Omega with normalised columns (rpa_CC_Doubles.f90:572-588)
@@ -269,15 +275,25 @@ int exchange_xspace(brpa_ctx *ctx, const double *R, int64_t naux, const double *
         BRPA_TRY(dc.run(ctx, Sm, NG, &cholqr));
         if (cholqr) {
             // Cholesky-QR in two passes: X <- X U^{-1} with the shifted factor, then
-            // once more with the unshifted Gram matrix of the result
+            // once more with the unshifted Gram matrix of the result (each product
+            // goes to Wb, and X and Wb trade places)
             nt = NG;
-            BRPA_TRY(dense_trsm_right(ctx, K, NG, Sm, NG, X, K));
+            auto right_inv = [&]() -> int {
+                BRPA_TRY(dense_trinv(ctx, NG, Sm, NG, 0, Winv, NG, 0, 1, Tinv));
+                BRPA_TRY(launch_gemm_f64_batched(ctx, false, false, K, NG, NG, 1.0, X, K, 0, 0,
+                                                 Winv, NG, 0, 0, 0.0, Wb, K, 0, 0, 1, 1, false,
+                                                 TRI_B_UPPER, nullptr, 0, 0));
+                std::swap(X, Wb);
+                return BRPA_OK;
+            };
+            BRPA_TRY(right_inv());
             BRPA_TRY(dense_syrk(ctx, true, NG, K, 1.0, X, K, 0.0, Sm, NG));
             bool ok2 = false;
             BRPA_TRY(dc.run(ctx, Sm, NG, &ok2));
             if (!ok2)
                 return brpa_fail(ctx, BRPA_ERR_SOLVER, "Cholesky-QR pass 2 of the T2 basis broke down");
-            BRPA_TRY(dense_trsm_right(ctx, K, NG, Sm, NG, X, K));
+            BRPA_TRY(right_inv());
+            T = X;
         } else {
             BRPA_TRY(begin(BRPA_STAGE_T2_EIG));
             BRPA_TRY(dense_syrk(ctx, true, NG, K, 1.0, X, K, 0.0, Sm, NG));
@@ -288,7 +304,8 @@ int exchange_xspace(brpa_ctx *ctx, const double *R, int64_t naux, const double *
                 // T = X U_s s^{-1/2} (rpa_CC_Doubles.f90:610-624)
                 BRPA_TRY(launch_select_scale(ctx, Om, Sm, sev, NG, NG - nt, nt));
                 BRPA_TRY(dense_gemm(ctx, false, false, K, nt, NG, 1.0, X, K, Om, NG, 0.0, Wb, K));
-                BRPA_HIP(ctx, hipMemcpyAsync(X, Wb, 8 * K * nt, hipMemcpyDeviceToDevice, ctx->stream));
+                std::swap(X, Wb);
+                T = X;
             }
         }
     }
@@ -311,9 +328,12 @@ int exchange_xspace(brpa_ctx *ctx, const double *R, int64_t naux, const double *
         BRPA_TRY(dc.run(ctx, Im, K, &ok));
         if (!ok)
             return brpa_fail(ctx, BRPA_ERR_NOT_SPD, "1 + I is not positive definite");
-        // (1 + I)^{-1} = U^{-1} U^{-T} = W^T W with W = U^{-T}
-        BRPA_TRY(dense_tri_inv_t(ctx, K, Im, Wb));
-        BRPA_TRY(dense_syrk(ctx, true, K, K, 1.0, Wb, K, 0.0, Y, K));
+        // (1 + I)^{-1} = U^{-1} U^{-T} = W W^T with W = U^{-1} upper triangular:
+        // C(i, j) = sum_{k >= max(i, j)} W(i, k) W(j, k), the upper tiles
+        // starting at k = their column
+        BRPA_TRY(dense_trinv(ctx, K, Im, K, 0, Wb, K, 0, 1, Tinv));
+        BRPA_TRY(launch_gemm_f64_batched(ctx, false, true, K, K, K, 1.0, Wb, K, 0, 0, Wb, K, 0, 0,
+                                         0.0, Y, K, 0, 0, 1, 1, true, TRI_B_LOWER, nullptr, 0, 0));
         BRPA_TRY(dense_symm_fill(ctx, Y, K));
         gamma = -0.5;
         delta = 0.5;
@@ -331,8 +351,10 @@ int exchange_xspace(brpa_ctx *ctx, const double *R, int64_t naux, const double *
         BRPA_TRY(dc.run(ctx, TIT, nt, &ok));
         if (ok) {
             // W = T U^{-1}: W W^T = T (1 + TIT)^{-1} T^T, T2 = (T T^T - W W^T) / 2
-            BRPA_HIP(ctx, hipMemcpyAsync(Wb, T, 8 * K * nt, hipMemcpyDeviceToDevice, ctx->stream));
-            BRPA_TRY(dense_trsm_right(ctx, K, nt, TIT, nt, Wb, K));
+            BRPA_TRY(dense_trinv(ctx, nt, TIT, nt, 0, Winv, nt, 0, 1, Tinv));
+            BRPA_TRY(launch_gemm_f64_batched(ctx, false, false, K, nt, nt, 1.0, T, K, 0, 0, Winv,
+                                             nt, 0, 0, 0.0, Wb, K, 0, 0, 1, 1, false, TRI_B_UPPER,
+                                             nullptr, 0, 0));
             BRPA_TRY(dense_syrk(ctx, false, K, nt, 0.5, T, K, 0.0, Y, K));
             BRPA_TRY(dense_syrk(ctx, false, K, nt, -0.5, Wb, K, 1.0, Y, K));
             BRPA_TRY(dense_symm_fill(ctx, Y, K));

@@ -125,13 +125,17 @@ int brpa_last_gemm_i8_stats(const brpa_ctx *ctx, double *ms, double *flops, int6
  * brpa_dgemm / brpa_dsyrk, 2 m n k flop per launch, m n k per launch on the
  * triangle of a SYRK), BRPA_KERNEL_SOSEX_DOT (the exchange contraction,
  * ov^2 N flop over the lower triangle of (ai|bj)), BRPA_KERNEL_TRSM_F64 (the
- * triangular solves, n^2 nrhs flop).  No reference counterpart (measurement). */
+ * triangular solves, n^2 nrhs flop), BRPA_KERNEL_TRINV_F64 (the diagonal
+ * blocks of the batched triangular inverses, 64^3/3 flop per block; their
+ * off-diagonal blocks are GEMM launches).  No reference counterpart
+ * (measurement). */
 enum {
     BRPA_KERNEL_GEMM_I8 = 0,
     BRPA_KERNEL_GEMM_F64 = 1,
     BRPA_KERNEL_SOSEX_DOT = 2,
     BRPA_KERNEL_TRSM_F64 = 3,
-    BRPA_NKERNELS = 4
+    BRPA_KERNEL_TRINV_F64 = 4,
+    BRPA_NKERNELS = 5
 };
 int brpa_last_kernel_stats(const brpa_ctx *ctx, int kernel, double *ms, double *flops,
                            int64_t *launches);
@@ -519,7 +523,13 @@ int brpa_exchange_batch(brpa_ctx *ctx, int precision, int64_t nfrag, const brpa_
  * brpa_dtrsm: B <- U^{-T} B (side_right = 0, B m x n, U m x m) or B <- B U^{-1}
  * (side_right = 1, U n x n), U upper triangular (e.g. the factor A = U^T U of a
  * Cholesky decomposition, real_Cholesky, real_linalg.f90:39-57).  No single
- * reference counterpart. */
+ * reference counterpart.
+ * brpa_dtrtri: W_z = U_z^{-1} for nmat upper triangular n x n matrices U_z = U +
+ * z su (only the upper triangle is read), W_z = W + z sw written whole (zeros
+ * below the diagonal); the inverses of the 64 x 64 diagonal blocks by
+ * substitution, the rest by recursive doubling on the GEMM kernel (LAPACK
+ * dtrtri 'U' 'N' semantics; the library's substitute for its triangular
+ * solves).  No single reference counterpart. */
 int brpa_dgemm(brpa_ctx *ctx, int transa, int transb, int64_t m, int64_t n, int64_t k,
                double alpha, const double *A, int64_t lda, const double *B, int64_t ldb,
                double beta, double *C, int64_t ldc);
@@ -527,6 +537,8 @@ int brpa_dsyrk(brpa_ctx *ctx, int trans, int fill_lower, int64_t n, int64_t k, d
                const double *A, int64_t lda, double beta, double *C, int64_t ldc);
 int brpa_dtrsm(brpa_ctx *ctx, int side_right, int64_t m, int64_t n, const double *U, int64_t ldu,
                double *B, int64_t ldb);
+int brpa_dtrtri(brpa_ctx *ctx, int64_t n, int64_t nmat, const double *U, int64_t ldu,
+                int64_t su, double *W, int64_t ldw, int64_t sw);
 
 /* ---- adjoint ----------------------------------------------------------------- */
 /* dE_RPA/dR (C R[ai][P], same shape as R) and dE_RPA/d(eps_occ), dE_RPA/d(eps_vir)
