@@ -71,6 +71,10 @@ struct brpa_ctx {
     // fine-grained (XCD-coherent) counters of the persistent Cholesky
     int *sync = nullptr;
     size_t sync_cap = 0;   // ints
+    // scratch for the partial tiles of split-k GEMMs (doubles), lent by the
+    // caller of a dense sequence for its duration; null: no split
+    double *split_ws = nullptr;
+    size_t split_ws_cap = 0;
     // worker contexts (own stream, rocBLAS handle, workspace) of brpa_exchange_batch
     std::vector<brpa_ctx *> workers;
 };

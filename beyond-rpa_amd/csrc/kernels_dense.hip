@@ -15,6 +15,8 @@
 //                       kernels leave, i.e. U^T of the column-major upper U);
 //                       B column-major or row-major; batched over matrices
 //                       with strides (brpa_dtrsm)
+#include <algorithm>
+
 #include "device_common.h"
 #include "internal.h"
 
@@ -35,18 +37,25 @@ constexpr int GGROUP = 8;  // tile rows per group of the tile order (panel reuse
 
 // ---------------------------------------------------------------------------
 // Batched C = alpha op(A) op(B) diag(cs) + beta C on 128 x 128 tiles, four
-// waves of 64 x 64 per work-group.  The A and B panels of a stage (128 x 16
-// each) go to LDS as sA[k][row], sB[k][col], double-buffered: the next stage
-// is loaded into registers while the current one is multiplied, then stored
-// into the other buffer, one barrier per stage.  The MFMA takes the B fragment
-// as its first operand, so the accumulator rows run along the lanes
-// (D[r] = C(row = lane%16, col = lane/16 + 4r) of a 16 x 16 block) and the
-// stores of C are contiguous across lanes.
+// waves of 64 x 64 per work-group, v_mfma_f64_16x16x4_f64.
 //
-// Loads by orientation: op(A) = A (TA = false) reads A(row, k) along rows, 128
-// consecutive doubles per k; TA reads A(k, row) along k, 16 consecutive
-// doubles per row.  op(B) = B (TB = false) reads B(k, col) along k; TB reads
-// B(col, k) along columns.
+// Stages of 16 k go to LDS as [row][k] for both operands (row pitch 18
+// doubles), double-buffered: the next stage is loaded into registers while
+// the current one is multiplied, then stored into the other buffer, one
+// barrier per stage.  Every thread moves pairs of consecutive k of one row
+// (one ds_write_b128).  In a stage, lane l of a wave supplies the k values
+// 4 (l / 16) .. +3 of its row to the four MFMA steps, so its operands for the
+// whole stage are two ds_read_b128 per 16-row block; the 16 rows of such a
+// read land on distinct 4-bank groups (pitch 36 dwords).  The k order inside
+// a stage is a permutation of the sum, the same for both operands.  The MFMA
+// takes the B fragment as its first operand, so the accumulator rows run along
+// the lanes (D[r] = C(row = lane%16, col = lane/16 + 4r) of a 16 x 16 block)
+// and the stores of C are contiguous across lanes.
+//
+// Global loads by orientation: op(A) = A (TA = false) holds rows contiguous,
+// so lanes take consecutive rows at two k; TA holds k contiguous, so eight
+// lanes read one row's 16 consecutive k.  op(B) = B (TB = false) is k-
+// contiguous, TB rows-contiguous.
 //
 // Batches: work-group x = (item along dimension 1) * tiles + tile, y = item
 // along dimension 2; every operand moves by its own stride per item.  Tile
@@ -57,10 +66,14 @@ constexpr int GGROUP = 8;  // tile rows per group of the tile order (panel reuse
 //   TRI_A_UPPER  op(A)(i, k) = 0 for k < i    TRI_A_LOWER  for k > i
 //   TRI_B_UPPER  op(B)(k, j) = 0 for k > j    TRI_B_LOWER  for k < j
 // cs (optional): column j of the product is scaled by cs[j] before beta C.
+// Split k (nsplit > 1, one item only): work-group z takes the z-th chunk of k
+// and writes its partial tile to P + z m n (ld m); k_gemm_reduce then adds the
+// partials in order and applies alpha, cs and beta.
 struct GemmArgs {
     int m, n, k;
     int mt, nt, ntiles;
     int upper, tri, nb1;
+    int nsplit, kchunk;
     double alpha, beta;
     const double *A;
     int64_t lda, sA1, sA2;
@@ -70,19 +83,18 @@ struct GemmArgs {
     int64_t ldc, sC1, sC2;
     const double *cs;
     int64_t scs1, scs2;
+    double *P;
 };
 
+namespace {
+constexpr int KP = GKB + 2;   // LDS row pitch (doubles)
+typedef double d2_t __attribute__((ext_vector_type(2)));
+}
+
 template <bool TA, bool TB>
-__global__ __launch_bounds__(256) void k_gemm_f64(const GemmArgs g) {
-    // LDS layout follows the operand's memory order, so that both the stores
-    // from registers and the MFMA operand reads are free of bank conflicts:
-    //   rows contiguous (op(A) = A, op(B) = B^T): [k][GT + GPAD]
-    //   k contiguous (op(A) = A^T, op(B) = B):     [GT][GKB + 2]  (a row pitch of
-    //   36 dwords puts the 16 rows of an operand read on distinct bank pairs)
-    constexpr int KP = GKB + 2;
-    constexpr int SZ = (GKB * (GT + GPAD) > GT * KP) ? GKB * (GT + GPAD) : GT * KP;
-    __shared__ double sA[2][SZ];
-    __shared__ double sB[2][SZ];
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_gemm_f64(const GemmArgs g) {
+    __shared__ __attribute__((aligned(16))) double sA[2][GT * KP];
+    __shared__ __attribute__((aligned(16))) double sB[2][GT * KP];
     const int item1 = (int)(blockIdx.x / (unsigned)g.ntiles);
     const int t = (int)(blockIdx.x - (unsigned)item1 * (unsigned)g.ntiles);
     const int item2 = (int)blockIdx.y;
@@ -99,9 +111,8 @@ __global__ __launch_bounds__(256) void k_gemm_f64(const GemmArgs g) {
     }
     const double *__restrict__ A = g.A + item1 * g.sA1 + item2 * g.sA2;
     const double *__restrict__ B = g.B + item1 * g.sB1 + item2 * g.sB2;
-    double *__restrict__ C = g.C + item1 * g.sC1 + item2 * g.sC2;
     const int m = g.m, n = g.n;
-    const int64_t lda = g.lda, ldb = g.ldb, ldc = g.ldc;
+    const int64_t lda = g.lda, ldb = g.ldb;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int wm = wave >> 1, wn = wave & 1;
     const int m0 = ti * GT, n0 = tj * GT;
@@ -110,6 +121,11 @@ __global__ __launch_bounds__(256) void k_gemm_f64(const GemmArgs g) {
     else if (g.tri == TRI_A_LOWER) kend = min(kend, m0 + GT);
     else if (g.tri == TRI_B_UPPER) kend = min(kend, n0 + GT);
     else if (g.tri == TRI_B_LOWER) kbeg = n0;
+    if (g.nsplit > 1) {
+        const int z = (int)blockIdx.z;
+        kbeg = max(kbeg, z * g.kchunk);
+        kend = min(kend, (z + 1) * g.kchunk);
+    }
 
     d4_t acc[4][4];
 #pragma unroll
@@ -117,78 +133,75 @@ __global__ __launch_bounds__(256) void k_gemm_f64(const GemmArgs g) {
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = (d4_t){0.0, 0.0, 0.0, 0.0};
 
-    // element e = tid + 256 i of a 128 x 16 panel: rows-contiguous operands
-    // take row e % 128, k e / 128; k-contiguous ones k e % 16, row e / 16
-    // (16 lanes read 128 consecutive bytes of one row or column).  The row
-    // part is fixed per thread, so the addresses advance by a stride per stage.
-    const int ar_ = TA ? (tid >> 4) : (tid & 127), ak_ = TA ? (tid & 15) : (tid >> 7);
-    const int br_ = TB ? (tid & 127) : (tid >> 4), bk_ = TB ? (tid >> 7) : (tid & 15);
-    // per i: row offset 16 i (k-contiguous) or k offset 2 i (rows-contiguous)
-    bool aok[8], bok[8];
-    int64_t aoff[8], boff[8];
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        const int r = TA ? ar_ + 16 * i : ar_, kk = TA ? ak_ : ak_ + 2 * i;
-        aok[i] = m0 + r < m;
-        aoff[i] = TA ? (int64_t)kk + lda * (m0 + r) : (int64_t)(m0 + r) + lda * kk;
-        const int c = TB ? br_ : br_ + 16 * i, kb = TB ? bk_ + 2 * i : bk_;
-        bok[i] = n0 + c < n;
-        boff[i] = TB ? (int64_t)(n0 + c) + ldb * kb : (int64_t)kb + ldb * (n0 + c);
-    }
+    // pair q = tid + 256 i (i < 4) of a 128 x 16 panel: (row, k pair kp).
+    // Rows-contiguous operands: row q % 128 (fixed per thread), kp q / 128 =
+    // kp0 + 2 i; k-contiguous: row q / 8 = row0 + 32 i, kp q % 8 (fixed).  So
+    // per operand one base offset and one stride per i.
+    const int arow0 = TA ? (tid >> 3) : (tid & 127), akp0 = TA ? (tid & 7) : (tid >> 7);
+    const int brow0 = TB ? (tid & 127) : (tid >> 3), bkp0 = TB ? (tid >> 7) : (tid & 7);
     const int64_t astep = TA ? 1 : lda, bstep = TB ? ldb : 1;   // per unit of k
+    const double *__restrict__ pa0 = A + (TA ? (int64_t)(2 * akp0) + lda * (m0 + arow0)
+                                             : (int64_t)(m0 + arow0) + lda * (2 * akp0));
+    const double *__restrict__ pb0 = B + (TB ? (int64_t)(n0 + brow0) + ldb * (2 * bkp0)
+                                             : (int64_t)(2 * bkp0) + ldb * (n0 + brow0));
+    const int64_t ai_step = TA ? 32 * lda : 4 * lda;   // per i
+    const int64_t bi_step = TB ? 4 * ldb : 32 * ldb;
+    const int as_lds = arow0 * KP + 2 * akp0, bs_lds = brow0 * KP + 2 * bkp0;
+    const int ai_lds = TA ? 32 * KP : 4, bi_lds = TB ? 4 : 32 * KP;
 
-    double ra[8], rb[8];
+    d2_t ra[4], rb[4];
     auto gload = [&](int k0) {
+        const double *pa = pa0 + astep * k0, *pb = pb0 + bstep * k0;
 #pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            const int ka = k0 + (TA ? ak_ : ak_ + 2 * i);
-            ra[i] = (aok[i] && ka < kend) ? A[aoff[i] + astep * k0] : 0.0;
-            const int kb = k0 + (TB ? bk_ + 2 * i : bk_);
-            rb[i] = (bok[i] && kb < kend) ? B[boff[i] + bstep * k0] : 0.0;
+        for (int i = 0; i < 4; ++i) {
+            const bool aok = m0 + arow0 + (TA ? 32 * i : 0) < m;
+            const int ka = k0 + 2 * akp0 + (TA ? 0 : 4 * i);
+            ra[i].x = (aok && ka < kend) ? pa[i * ai_step] : 0.0;
+            ra[i].y = (aok && ka + 1 < kend) ? pa[i * ai_step + astep] : 0.0;
+            const bool bok = n0 + brow0 + (TB ? 0 : 32 * i) < n;
+            const int kb = k0 + 2 * bkp0 + (TB ? 4 * i : 0);
+            rb[i].x = (bok && kb < kend) ? pb[i * bi_step] : 0.0;
+            rb[i].y = (bok && kb + 1 < kend) ? pb[i * bi_step + bstep] : 0.0;
         }
     };
     auto lstore = [&](int buf) {
 #pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            if (!TA)
-                sA[buf][(ak_ + 2 * i) * (GT + GPAD) + ar_] = ra[i];
-            else
-                sA[buf][(ar_ + 16 * i) * KP + ak_] = ra[i];
-            if (TB)
-                sB[buf][(bk_ + 2 * i) * (GT + GPAD) + br_] = rb[i];
-            else
-                sB[buf][(br_ + 16 * i) * KP + bk_] = rb[i];
+        for (int i = 0; i < 4; ++i) {
+            *reinterpret_cast<d2_t *>(&sA[buf][as_lds + i * ai_lds]) = ra[i];
+            *reinterpret_cast<d2_t *>(&sB[buf][bs_lds + i * bi_lds]) = rb[i];
         }
-    };
-    auto aget = [&](int buf, int kr, int row) {
-        return TA ? sA[buf][row * KP + kr] : sA[buf][kr * (GT + GPAD) + row];
-    };
-    auto bget = [&](int buf, int kr, int col) {
-        return TB ? sB[buf][kr * (GT + GPAD) + col] : sB[buf][col * KP + kr];
     };
 
     if (kbeg < kend) {
+        const int kq = 4 * (lane >> 4), rl = lane & 15;
         gload(kbeg);
         lstore(0);
         __syncthreads();
         if (kbeg + GKB < kend) gload(kbeg + GKB);
         int buf = 0;
         for (int k0 = kbeg; k0 < kend; k0 += GKB, buf ^= 1) {
+            d2_t a[4][2], b[4][2];
 #pragma unroll
-            for (int kk = 0; kk < GKB; kk += 4) {
-                const int kr = kk + (lane >> 4);
-                double a[4], b[4];
+            for (int i = 0; i < 4; ++i) {
+                const double *p = &sA[buf][(64 * wm + 16 * i + rl) * KP + kq];
+                a[i][0] = *reinterpret_cast<const d2_t *>(p);
+                a[i][1] = *reinterpret_cast<const d2_t *>(p + 2);
+            }
 #pragma unroll
-                for (int i = 0; i < 4; ++i) a[i] = aget(buf, kr, 64 * wm + 16 * i + (lane & 15));
+            for (int j = 0; j < 4; ++j) {
+                const double *p = &sB[buf][(64 * wn + 16 * j + rl) * KP + kq];
+                b[j][0] = *reinterpret_cast<const d2_t *>(p);
+                b[j][1] = *reinterpret_cast<const d2_t *>(p + 2);
+            }
+            // C^T block = op(B)^T op(A)^T: rows of C along the lanes
 #pragma unroll
-                for (int j = 0; j < 4; ++j) b[j] = bget(buf, kr, 64 * wn + 16 * j + (lane & 15));
-                // C^T block = op(B)^T op(A)^T: rows of C along the lanes
+            for (int st = 0; st < 4; ++st)
 #pragma unroll
                 for (int i = 0; i < 4; ++i)
 #pragma unroll
                     for (int j = 0; j < 4; ++j)
-                        acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[j], a[i], acc[i][j], 0, 0, 0);
-            }
+                        acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+                            b[j][st >> 1][st & 1], a[i][st >> 1][st & 1], acc[i][j], 0, 0, 0);
             // the other buffer was last read before the previous barrier
             if (k0 + GKB < kend) lstore(buf ^ 1);
             __syncthreads();
@@ -196,6 +209,25 @@ __global__ __launch_bounds__(256) void k_gemm_f64(const GemmArgs g) {
         }
     }
 
+    if (g.nsplit > 1) {
+        // partial tile, reduced by k_gemm_reduce
+        double *P = g.P + (int64_t)blockIdx.z * m * n;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int row = m0 + 64 * wm + 16 * i + (lane & 15);
+            if (row >= m) continue;
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int col = n0 + 64 * wn + 16 * j + (lane >> 4) + 4 * r;
+                    if (col < n) P[(int64_t)row + (int64_t)m * col] = acc[i][j][r];
+                }
+        }
+        return;
+    }
+    double *__restrict__ C = g.C + item1 * g.sC1 + item2 * g.sC2;
+    const int64_t ldc = g.ldc;
     const double *cs = g.cs ? g.cs + item1 * g.scs1 + item2 * g.scs2 : nullptr;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
@@ -213,6 +245,23 @@ __global__ __launch_bounds__(256) void k_gemm_f64(const GemmArgs g) {
                 *p = g.beta == 0.0 ? v : fma(g.beta, *p, v);
             }
     }
+}
+
+// C(r, c) = alpha cs[c] sum_z P_z(r, c) + beta C(r, c), partials in split order;
+// in the upper mode only the elements of tiles ti <= tj (those the GEMM
+// writes).  One thread per element, rows along the threads.
+__global__ __launch_bounds__(256) void k_gemm_reduce(const GemmArgs g) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t mn = (int64_t)g.m * g.n;
+    if (e >= mn) return;
+    const int col = (int)(e / g.m), row = (int)(e - (int64_t)col * g.m);
+    if (g.upper && row / GT > col / GT) return;
+    double s = 0.0;
+    for (int z = 0; z < g.nsplit; ++z) s += g.P[z * mn + e];
+    double v = g.alpha * s;
+    if (g.cs) v *= g.cs[col];
+    double *p = g.C + (int64_t)row + g.ldc * col;
+    *p = g.beta == 0.0 ? v : fma(g.beta, *p, v);
 }
 
 int launch_gemm_f64_batched(brpa_ctx *ctx, bool ta, bool tb, int64_t m, int64_t n, int64_t k,
@@ -255,7 +304,25 @@ int launch_gemm_f64_batched(brpa_ctx *ctx, bool ta, bool tb, int64_t m, int64_t 
     g.cs = cs;
     g.scs1 = scs1;
     g.scs2 = scs2;
-    const dim3 grid((unsigned)(ntiles * nb1), (unsigned)nb2), block(256);
+    g.P = nullptr;
+    // split k when the tiles leave most of the chip idle and k is long: chunks
+    // of at least 256 k, work-groups up to ~2 per CU, partials in the
+    // context's split scratch (set by the caller for the duration of a call)
+    g.nsplit = 1;
+    g.kchunk = g.k;
+    const int64_t wgs = ntiles * nb1 * nb2;
+    if (nb1 == 1 && nb2 == 1 && ctx->split_ws && wgs < 384 && g.k >= 512) {
+        int64_t ns = std::min<int64_t>((512 + wgs - 1) / wgs, g.k / 256);
+        ns = std::min<int64_t>(ns, 32);
+        ns = std::min<int64_t>(ns, (int64_t)(ctx->split_ws_cap / (size_t)(m * n)));
+        if (ns > 1) {
+            const int64_t chunk = ((g.k + ns - 1) / ns + GKB - 1) / GKB * GKB;
+            g.nsplit = (int)((g.k + chunk - 1) / chunk);
+            g.kchunk = (int)chunk;
+            g.P = ctx->split_ws;
+        }
+    }
+    const dim3 grid((unsigned)(ntiles * nb1), (unsigned)nb2, (unsigned)g.nsplit), block(256);
     size_t slot = 0;
     BRPA_TRY(kstat_begin(ctx, &slot));
     if (!ta && !tb)
@@ -267,6 +334,11 @@ int launch_gemm_f64_batched(brpa_ctx *ctx, bool ta, bool tb, int64_t m, int64_t 
     else
         hipLaunchKernelGGL((k_gemm_f64<true, true>), grid, block, 0, ctx->stream, g);
     LAUNCH_CHECK(ctx);
+    if (g.nsplit > 1) {
+        hipLaunchKernelGGL(k_gemm_reduce, dim3((unsigned)((m * n + 255) / 256)), dim3(256), 0,
+                           ctx->stream, g);
+        LAUNCH_CHECK(ctx);
+    }
     // algorithmic flop: 2 m n k, n (n + 1) k on the triangle of a SYRK, m n k
     // (n m k) with a triangular op(A) (op(B)) of order m = k (n = k)
     double fl = upper ? (double)n * (double)(n + 1) * (double)g.k

@@ -47,6 +47,8 @@ int launch_sosex_dot(brpa_ctx *, const double *R, int64_t K, int64_t N, int64_t 
 
 namespace {
 constexpr double kPi = 3.14159265358979323846;
+// partial tiles of split-k GEMMs: up to ~512 work-groups of 128 x 128
+constexpr int64_t kSplitWs = (int64_t)9 << 20;
 int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 bool finite_all(const std::vector<double> &v) {
@@ -127,7 +129,8 @@ int exchange_xspace(brpa_ctx *ctx, const double *R, int64_t naux, const double *
     FragDev fd{};
     FragDev *dfd = nullptr;
     int64_t *dpre = nullptr, *dchol = nullptr;
-    double *dR = nullptr, *chis, *Y, *Im, *Om, *X, *Wb, *Sm, *TIT, *Winv, *Tinv, *sev, *sE, *part;
+    double *dR = nullptr, *chis, *Y, *Im, *Om, *X, *Wb, *Sm, *TIT, *Winv, *Tinv, *sev, *sE, *part,
+        *splitws;
     int *dinfo;
     DenseChol dc;
     std::vector<int64_t> pre = {0, 0, 0, 0, 0, 0};
@@ -165,6 +168,7 @@ int exchange_xspace(brpa_ctx *ctx, const double *R, int64_t naux, const double *
         sE = ar.take<double>(std::max(K, NG));
         dinfo = ar.take<int>(2);
         part = ar.take<double>(nparts);
+        splitws = ar.take<double>(kSplitWs);
         dc.layout(ar, nd);
     };
     Arena count;
@@ -172,6 +176,17 @@ int exchange_xspace(brpa_ctx *ctx, const double *R, int64_t naux, const double *
     Arena ar;
     BRPA_TRY(ws_reserve(ctx, count.off, &ar));
     layout(ar);
+    // the split-k GEMMs of small shapes keep their partial tiles in splitws
+    // for the duration of this call
+    struct SplitLend {
+        brpa_ctx *c;
+        ~SplitLend() {
+            c->split_ws = nullptr;
+            c->split_ws_cap = 0;
+        }
+    } lend{ctx};
+    ctx->split_ws = splitws;
+    ctx->split_ws_cap = kSplitWs;
     fd.R = dR;
     fd.K = (int)K;
     fd.N = (int)N;
