@@ -51,6 +51,22 @@ int from_device(brpa_ctx *ctx, void *dst, const void *src, size_t bytes) {
 
 int copy_sync(brpa_ctx *ctx, void *dst, const void *src, size_t bytes, hipMemcpyKind kind) {
     if (bytes == 0) return BRPA_OK;
+    // small transfers between pageable host memory and the device go through
+    // the context's pinned bounce buffer: a DMA on this stream only, where a
+    // pageable copy would take the runtime's shared staging path
+    if (ctx->bounce && bytes <= kBounceBytes &&
+        (kind == hipMemcpyHostToDevice || kind == hipMemcpyDeviceToHost)) {
+        if (kind == hipMemcpyHostToDevice) {
+            std::memcpy(ctx->bounce, src, bytes);
+            BRPA_HIP(ctx, hipMemcpyAsync(dst, ctx->bounce, bytes, kind, ctx->stream));
+            BRPA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        } else {
+            BRPA_HIP(ctx, hipMemcpyAsync(ctx->bounce, src, bytes, kind, ctx->stream));
+            BRPA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            std::memcpy(dst, ctx->bounce, bytes);
+        }
+        return BRPA_OK;
+    }
     BRPA_HIP(ctx, hipMemcpyAsync(dst, src, bytes, kind, ctx->stream));
     BRPA_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return BRPA_OK;
@@ -222,6 +238,12 @@ int brpa_ctx_create(int device, brpa_ctx **out) {
         delete ctx;
         return BRPA_ERR_HIP;
     }
+    if (hipHostMalloc(&ctx->bounce, kBounceBytes, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        ctx->bounce = nullptr;
+        brpa_ctx_destroy(ctx);
+        return BRPA_ERR_OOM;
+    }
     // device address of the allocation-free entry point's scratch array (once per
     // device, with the device current)
     if (noalloc_setup(device) != BRPA_OK) {
@@ -243,6 +265,7 @@ int brpa_ctx_destroy(brpa_ctx *ctx) {
     ctx->workers.clear();
     if (ctx->ws) (void)hipFree(ctx->ws);
     if (ctx->pin) (void)hipHostFree(ctx->pin);
+    if (ctx->bounce) (void)hipHostFree(ctx->bounce);
     if (ctx->sync) (void)hipFree(ctx->sync);
     t2_release(ctx);
     mbpt3_release(ctx);

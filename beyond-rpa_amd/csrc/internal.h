@@ -49,6 +49,8 @@ struct brpa_ctx {
     double *x_V = nullptr, *x_B = nullptr;
     size_t x_V_cap = 0, x_B_cap = 0;
     bool mbpt3_chol = false;   // the last MBPT3-1 factored -X by Cholesky (else eigenpairs)
+    // pinned bounce buffer of the small completed copies (copy_sync), kBounceBytes
+    void *bounce = nullptr;
     // pinned host staging of the batched E_RPA path (uploads and results)
     char *pin = nullptr;
     size_t pin_cap = 0;
@@ -82,6 +84,8 @@ struct brpa_ctx {
 // Triangular operand of a batched GEMM (launch_gemm_f64_batched): the k range
 // where op(A) or op(B) is zero is skipped (the zeros must be stored).
 enum { TRI_NONE = 0, TRI_A_UPPER = 1, TRI_A_LOWER = 2, TRI_B_UPPER = 3, TRI_B_LOWER = 4 };
+
+constexpr size_t kBounceBytes = (size_t)1 << 20;   // pinned bounce buffer of copy_sync
 
 // Fragment-size bucket of the autotuner (by NAux).
 inline int tune_bucket(int64_t naux) { return naux < 640 ? 0 : (naux < 1280 ? 1 : 2); }

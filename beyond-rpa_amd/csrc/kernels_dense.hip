@@ -305,18 +305,37 @@ int launch_gemm_f64_batched(brpa_ctx *ctx, bool ta, bool tb, int64_t m, int64_t 
     g.scs1 = scs1;
     g.scs2 = scs2;
     g.P = nullptr;
-    // split k when the tiles leave most of the chip idle and k is long: chunks
-    // of at least 256 k, work-groups up to ~2 per CU, partials in the
-    // context's split scratch (set by the caller for the duration of a call)
+    // split k when the last wave of tiles would leave much of the chip idle:
+    // the split count s (<= 8, chunks of >= 256 k) minimises the modelled time
+    // 1/eff(s) + (reduction traffic), eff(s) = tiles s / (slots ceil(tiles s /
+    // slots)) with two work-groups per CU; partials in the context's split
+    // scratch (set by the caller for the duration of a call)
     g.nsplit = 1;
     g.kchunk = g.k;
     const int64_t wgs = ntiles * nb1 * nb2;
-    if (nb1 == 1 && nb2 == 1 && ctx->split_ws && wgs < 384 && g.k >= 512) {
-        int64_t ns = std::min<int64_t>((512 + wgs - 1) / wgs, g.k / 256);
-        ns = std::min<int64_t>(ns, 32);
-        ns = std::min<int64_t>(ns, (int64_t)(ctx->split_ws_cap / (size_t)(m * n)));
-        if (ns > 1) {
-            const int64_t chunk = ((g.k + ns - 1) / ns + GKB - 1) / GKB * GKB;
+    if (nb1 == 1 && nb2 == 1 && ctx->split_ws && g.k >= 512) {
+        int ncu = 0;
+        if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device) !=
+                hipSuccess || ncu <= 0) {
+            (void)hipGetLastError();
+            ncu = 256;
+        }
+        const double slots = 2.0 * ncu;
+        double best = 1e300;
+        int64_t bs = 1;
+        const int64_t smax = std::min<int64_t>({8, g.k / 256,
+                                                (int64_t)(ctx->split_ws_cap / (size_t)(m * n))});
+        for (int64_t sp = 1; sp <= std::max<int64_t>(smax, 1); ++sp) {
+            const double w = (double)(wgs * sp);
+            const double eff = w / (slots * std::ceil(w / slots));
+            const double cost = 1.0 / eff + (sp > 1 ? 48.0 * (double)(sp + 1) / (double)g.k : 0.0);
+            if (cost < best - 1e-9) {
+                best = cost;
+                bs = sp;
+            }
+        }
+        if (bs > 1) {
+            const int64_t chunk = ((g.k + bs - 1) / bs + GKB - 1) / GKB * GKB;
             g.nsplit = (int)((g.k + chunk - 1) / chunk);
             g.kchunk = (int)chunk;
             g.P = ctx->split_ws;

@@ -24,6 +24,8 @@
 
 static inline unsigned grid1(int64_t total) { return (unsigned)total; }
 
+typedef double d2v_t __attribute__((ext_vector_type(2)));
+
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_dtab(const FragDev *__restrict__ fr,
                                              const int64_t *__restrict__ pre, int nfrag) {
@@ -528,18 +530,22 @@ __device__ __forceinline__ double wave_bcast(double v, int l) {
     return __hiloint2double(hi, lo);
 }
 
-// Diagonal block of step `step` of matrix w with one wave: lane r keeps row r
-// of the 64 x 64 block in registers (right-looking; the pivot is broadcast by
-// readlane, column c's multipliers through a double-buffered LDS column, one
-// barrier per column); then column r of L11^{-1} by forward substitution with
-// L11 broadcast from LDS.
-// Same outputs as potrf_body: linv, log-det partial, status, optional L11
-// write-back.
+// Diagonal block of step `step` of matrix w with one wave, left-looking: lane
+// r keeps row r of the 64 x 64 block in registers.  At column c every lane
+// subtracts the dot product of its row with row c (rows of L are complete up
+// to column c - 1 and are read from LDS as broadcast ds_read_b128, four
+// partial sums), the pivot comes from lane c by readlane, and the finished
+// column goes to LDS.  No barrier inside the factorisation (one wave).  Then
+// column `lane` of L11^{-1} by forward substitution over the rows of L in LDS
+// (reciprocal diagonal precomputed; uniform loops, x_k = 0 above the lane's
+// column).  Outputs: linv, log-det partial, status, optional L11 write-back.
 __global__ __launch_bounds__(64) void k_potrf_diag(const FragDev *__restrict__ fr,
                                                   const int64_t *__restrict__ pre, int nfrag,
                                                   int step) {
-    __shared__ double L[CHOL_NB][CHOL_NB + 1];
-    __shared__ double col[2][CHOL_NB];
+    // row pitch 66 doubles: 16-byte aligned rows; broadcast reads are conflict-free
+    constexpr int LP = CHOL_NB + 2;
+    __shared__ __attribute__((aligned(16))) double L[CHOL_NB][LP];
+    __shared__ double rdiag[CHOL_NB];
     __shared__ int64_t s_pre[FIND_LDS_MAX + 1];
     const int64_t b = blockIdx.x;
     const int f = find_frag_lds(pre, nfrag, b, s_pre);
@@ -563,43 +569,53 @@ __global__ __launch_bounds__(64) void k_potrf_diag(const FragDev *__restrict__ f
     int bad = 0;
 #pragma unroll
     for (int c = 0; c < CHOL_NB; ++c) {
+        if (c > 0) {
+            // a[c] -= sum_{k<c} L[lane][k] L[c][k]; row c of L from LDS
+            double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+#pragma unroll
+            for (int k = 0; k < c; k += 4) {
+                const d2v_t u = *reinterpret_cast<const d2v_t *>(&L[c][k]);
+                s0 = fma(a[k], u.x, s0);
+                if (k + 1 < c) s1 = fma(a[k + 1], u.y, s1);
+                if (k + 2 < c) {
+                    const d2v_t u2 = *reinterpret_cast<const d2v_t *>(&L[c][k + 2]);
+                    s2 = fma(a[k + 2], u2.x, s2);
+                    if (k + 3 < c) s3 = fma(a[k + 3], u2.y, s3);
+                }
+            }
+            a[c] -= (s0 + s1) + (s2 + s3);
+        }
         double piv = wave_bcast(a[c], c);
         if (!(piv > 0.0) || !isfinite(piv)) {
             bad = 1;
             piv = 1.0;
         }
         const double d = sqrt(piv);
-        // lanes above row c keep garbage (finite) in their upper triangle,
-        // never read; zeroing it here makes the compiler spill
-        const double q = a[c] / d;
-        a[c] = (lane == c) ? d : q;
-        col[c & 1][lane] = a[c];
-        __syncthreads();
-        const double m = lane > c ? a[c] : 0.0;
-        // the scheduling barriers keep at most 16 broadcast loads in flight
-        // (the fully unrolled loops would otherwise hoist them all and spill)
-#pragma unroll
-        for (int k = c + 1; k < CHOL_NB; ++k) {
-            a[k] -= m * col[c & 1][k];
-            if (k % 16 == 15) __builtin_amdgcn_sched_barrier(0);
-        }
+        const double rd = 1.0 / d;
+        // lanes above row c hold values of the upper triangle, never read
+        a[c] = (lane == c) ? d : a[c] * rd;
+        L[lane][c] = a[c];
+        if (lane == c) rdiag[c] = rd;
+        __builtin_amdgcn_sched_barrier(0);
     }
     __syncthreads();
-#pragma unroll
-    for (int c = 0; c < CHOL_NB; ++c) L[lane][c] = c <= lane ? a[c] : 0.0;
-    __syncthreads();
-    // column `lane` of X = L11^{-1}: x_i = (delta_i,lane - sum_{k<i} L_ik x_k) / L_ii
-    // (x_i = 0 for i < lane); a[] now holds x.  The sum runs as four interleaved
-    // partial sums (k mod 4), so the dependent FMA chain per row is a quarter as long.
+    // column `lane` of X = L11^{-1}: x_i = (delta_i,lane - sum_{k<i} L_ik x_k) / L_ii,
+    // x_i = 0 for i < lane; a[] now holds x
 #pragma unroll
     for (int i = 0; i < CHOL_NB; ++i) {
-        double s[4] = {(i == lane) ? 1.0 : 0.0, 0.0, 0.0, 0.0};
+        double s0 = (i == lane) ? 1.0 : 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
 #pragma unroll
-        for (int k = 0; k < i; ++k) {
-            s[k & 3] -= L[i][k] * a[k];
-            if (k % 16 == 15) __builtin_amdgcn_sched_barrier(0);
+        for (int k = 0; k < i; k += 4) {
+            const d2v_t u = *reinterpret_cast<const d2v_t *>(&L[i][k]);
+            s0 = fma(-u.x, a[k], s0);
+            if (k + 1 < i) s1 = fma(-u.y, a[k + 1], s1);
+            if (k + 2 < i) {
+                const d2v_t u2 = *reinterpret_cast<const d2v_t *>(&L[i][k + 2]);
+                s2 = fma(-u2.x, a[k + 2], s2);
+                if (k + 3 < i) s3 = fma(-u2.y, a[k + 3], s3);
+            }
         }
-        a[i] = ((s[0] + s[1]) + (s[2] + s[3])) / L[i][i];
+        a[i] = ((s0 + s1) + (s2 + s3)) * rdiag[i];
         __builtin_amdgcn_sched_barrier(0);
     }
     double *linv = F.linv + (int64_t)w * CHOL_NB * CHOL_NB;

@@ -47,8 +47,8 @@ int launch_sosex_dot(brpa_ctx *, const double *R, int64_t K, int64_t N, int64_t 
 
 namespace {
 constexpr double kPi = 3.14159265358979323846;
-// partial tiles of split-k GEMMs: up to ~512 work-groups of 128 x 128
-constexpr int64_t kSplitWs = (int64_t)9 << 20;
+// partial tiles of split-k GEMMs (up to 8 splits of the largest C of a call)
+constexpr int64_t kSplitWsMin = (int64_t)9 << 20;
 int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 bool finite_all(const std::vector<double> &v) {
@@ -139,6 +139,8 @@ int exchange_xspace(brpa_ctx *ctx, const double *R, int64_t naux, const double *
     pre[3] = ntile * (ntile + 1) / 2 * cdiv(F, 4);  // syrk
     pre[5] = cdiv(F * N, 256);                      // set_diag
     const int64_t nparts = sosex_dot_nparts(K);
+    const int64_t nsplitws = std::max<int64_t>(
+        kSplitWsMin, std::min<int64_t>((int64_t)1 << 28, 8 * std::max(K, NG) * std::max(K, NG)));
     auto layout = [&](Arena &ar) {
         dfd = ar.take<FragDev>(1);
         dpre = ar.take<int64_t>(pre.size());
@@ -168,7 +170,7 @@ int exchange_xspace(brpa_ctx *ctx, const double *R, int64_t naux, const double *
         sE = ar.take<double>(std::max(K, NG));
         dinfo = ar.take<int>(2);
         part = ar.take<double>(nparts);
-        splitws = ar.take<double>(kSplitWs);
+        splitws = ar.take<double>(nsplitws);
         dc.layout(ar, nd);
     };
     Arena count;
@@ -186,7 +188,7 @@ int exchange_xspace(brpa_ctx *ctx, const double *R, int64_t naux, const double *
         }
     } lend{ctx};
     ctx->split_ws = splitws;
-    ctx->split_ws_cap = kSplitWs;
+    ctx->split_ws_cap = (size_t)nsplitws;
     fd.R = dR;
     fd.K = (int)K;
     fd.N = (int)N;
