@@ -202,6 +202,7 @@ struct FragDev {
 };
 
 constexpr int CHOL_NB = 64;   // Cholesky block size
+constexpr int CHOL_OUTER = 4; // Cholesky steps per outer panel of the trailing update
 constexpr int SYRK_T = 64;    // FP64 SYRK tile
 constexpr int SPLIT_KB = 128; // split-BF16 exactness block (rows per scale)
 constexpr int NSLICE = 4;     // bf16 (or int8) slices per FP64 value in the split forms
@@ -219,7 +220,8 @@ int launch_potrf_diag(brpa_ctx *, const FragDev *, const int64_t *pre, int nfrag
 int launch_trsm(brpa_ctx *, const FragDev *, const int64_t *pre, int nfrag, int64_t total, int step);
 int launch_chol_update(brpa_ctx *, const FragDev *, const int64_t *pre, int nfrag, int64_t total, int step);
 // Work items per matrix of the trailing update after Cholesky step j (N x N):
-// the next 64-column strip after an even step, the rest after an odd one.
+// the next 64-column strip inside an outer panel of CHOL_OUTER steps, the rest
+// of the trailing matrix after its last step.
 int64_t chol_upd_items(int64_t N, int64_t j);
 // all Cholesky steps of the batch in one persistent launch (see kernels_f64.hip)
 int launch_chol_persistent(brpa_ctx *, const FragDev *, const int64_t *pre, int nfrag, int nfix,

@@ -674,12 +674,30 @@ int launch_trsm(brpa_ctx *ctx, const FragDev *fr, const int64_t *pre, int nfrag,
     return BRPA_OK;
 }
 
-// (standalone copy of update_body: inlining the shared body costs registers here)
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_chol_update(const FragDev *__restrict__ fr,
-                                                       const int64_t *__restrict__ pre,
-                                                       int nfrag, int step) {
-    __shared__ double sI[UPD_T][UPD_KH + 1];
-    __shared__ double sJ[UPD_T][UPD_KH + 1];
+// Trailing update of one 128 x 128 super-tile (TI >= TJ in 128-row units of the
+// trailing matrix that starts at t0 = j0 + 64): C -= P_i P_j^T over the panel
+// columns of the current outer panel (see below), C preloaded into the
+// accumulators.  Four waves of 64 x 64 (two work-groups per CU, two waves per
+// SIMD); waves entirely above the diagonal of a diagonal super-tile skip their
+// products; element writes keep col <= row.  The panels pass through LDS in
+// phases of 16 columns as [row][k] (pitch 18 doubles), double-buffered with one
+// barrier per phase; every thread moves pairs of consecutive columns (one
+// ds_write_b128), and in a phase lane group l/16 supplies the k values
+// 4 (l/16) .. +3 of its row to the four MFMA steps (a permutation of the
+// phase's k, the same for both operands), two ds_read_b128 per 16-row block.
+// The MFMA takes the P_i fragment first, so the accumulator columns run along
+// the lanes and the row-major C is read and written coalesced.
+template <bool STRIP>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_chol_update(
+    const FragDev *__restrict__ fr, const int64_t *__restrict__ pre, int nfrag, int step) {
+    // STRIP: the 64-column strip of a step inside an outer panel; the 128 x 64
+    // tile is split over the four waves as 2 x 2 blocks of 64 x 32.  Otherwise
+    // 128 x 128 tiles, four waves of 64 x 64.
+    constexpr int WC = STRIP ? 32 : 64;    // columns per wave
+    constexpr int NJ = WC / 16;            // 16-column MFMA blocks per wave
+    constexpr int KPU = UPD_KH + 2;
+    __shared__ __attribute__((aligned(16))) double sI[2][UPD_T * KPU];
+    __shared__ __attribute__((aligned(16))) double sJ[2][UPD_T * KPU];
     __shared__ int64_t s_pre[FIND_LDS_MAX + 1];
     int64_t b = blockIdx.x;
     int f = find_frag_lds(pre, nfrag, b, s_pre);
@@ -688,14 +706,18 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     const int j0 = step * CHOL_NB;
     const int t0 = j0 + CHOL_NB;
     const int nt = (N - t0 + UPD_T - 1) / UPD_T;
-    // Outer panels of 128 columns: an even step updates only the next 64-column
-    // strip (rows >= t0, columns t0 .. t0+63) with its own panel (rank 64); an
-    // odd step updates the rest of the trailing matrix with the two panels of
-    // steps step-1 and step (rank 128), so A22 is read and written once per two
-    // steps.  Work items: nt row tiles (even) or nt(nt+1)/2 tile pairs (odd).
-    const bool strip = (step % 2) == 0;
-    const int kbase = strip ? j0 : j0 - CHOL_NB;    // first panel column
-    const int nph = (strip ? CHOL_NB : 2 * CHOL_NB) / UPD_KH;
+    // Outer panels of CHOL_OUTER steps (256 columns): step s of an outer panel
+    // (s = step % CHOL_OUTER) updates, for s < CHOL_OUTER - 1, only the next
+    // 64-column strip (rows >= t0, columns t0 .. t0+63) with the panels of
+    // steps step-s .. step (rank 64 (s+1): the strip has seen no panel of this
+    // outer panel yet); the last step updates the rest of the trailing matrix
+    // with all CHOL_OUTER panels (rank 256), so A22 is read and written once
+    // per outer panel.  Work items: nt row tiles (strip) or nt(nt+1)/2 tile
+    // pairs (rest).
+    const int s_in = step % CHOL_OUTER;
+    constexpr bool strip = STRIP;
+    const int kbase = j0 - s_in * CHOL_NB;    // first panel column
+    const int nph = (s_in + 1) * CHOL_NB / UPD_KH;
     const int colmax = strip ? min(N, t0 + CHOL_NB) : N;   // C columns < colmax
     const int nitems = strip ? nt : nt * (nt + 1) / 2;
     int64_t lb = b - (nfrag <= FIND_LDS_MAX ? s_pre[f] : pre[f]);
@@ -712,44 +734,51 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     const int Ri = t0 + UPD_T * ti, Rj = t0 + UPD_T * tj;
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
-    const int wa = wave >> 2, wq = wave & 3;   // rows 64*wa.., cols 32*wq..
-    const int r0 = 64 * wa, c0 = 32 * wq;
+    const int wm = wave >> 1, wn = wave & 1;   // rows 64*wm.., cols WC*wn..
+    const int r0 = 64 * wm, c0 = WC * wn;
     const bool active = !(ti == tj && c0 > r0 + 63) && Ri + r0 < N && Rj + c0 < colmax;
 
-    // panel phase h (h < nph): rows Ri.. / Rj.. (128 each) x columns kbase + 16 h .. + 16;
-    // thread -> column tid%16, rows tid/16 + 32 u (u < 4) of both panels
-    constexpr int RS = 512 / UPD_KH;        // rows per load sweep
-    constexpr int PH = UPD_T / RS;
-    const int lc = tid % UPD_KH, lr = tid / UPD_KH;
-    auto load_half = [&](int h, double (&v)[2][PH]) {
+    // phase h (h < nph): rows Ri.. / Rj.. (128 each) x columns kbase + 16 h .. + 16
+    // (all < t0 <= N); pair q = tid + 256 u (u < 4): row q / 8, columns
+    // 2 (q % 8) + {0, 1}
+    d2v_t vi[4], vj[4];
+    auto gload = [&](int h) {
 #pragma unroll
-        for (int u = 0; u < PH; ++u) {
-            const int r = lr + RS * u;
+        for (int u = 0; u < 4; ++u) {
+            const int q = tid + 256 * u;
+            const int r = q >> 3, col = kbase + UPD_KH * h + 2 * (q & 7);
             const bool oki = Ri + r < N, okj = Rj + r < N;
-            v[0][u] = G[oki ? (int64_t)(Ri + r) * N + kbase + UPD_KH * h + lc : 0];
-            v[1][u] = G[okj ? (int64_t)(Rj + r) * N + kbase + UPD_KH * h + lc : 0];
-            v[0][u] = oki ? v[0][u] : 0.0;
-            v[1][u] = okj ? v[1][u] : 0.0;
+            const gdouble_t *pi = G + (oki ? (int64_t)(Ri + r) * N + col : 0);
+            const gdouble_t *pj = G + (okj ? (int64_t)(Rj + r) * N + col : 0);
+            const double i0 = pi[0], i1 = pi[1], ja = pj[0], jb = pj[1];
+            vi[u].x = oki ? i0 : 0.0;
+            vi[u].y = oki ? i1 : 0.0;
+            vj[u].x = okj ? ja : 0.0;
+            vj[u].y = okj ? jb : 0.0;
         }
     };
-    auto store_half = [&](const double (&v)[2][PH]) {
+    auto lstore = [&](int buf) {
 #pragma unroll
-        for (int u = 0; u < PH; ++u) {
-            sI[lr + RS * u][lc] = v[0][u];
-            sJ[lr + RS * u][lc] = v[1][u];
+        for (int u = 0; u < 4; ++u) {
+            const int q = tid + 256 * u;
+            const int o = (q >> 3) * KPU + 2 * (q & 7);
+            *reinterpret_cast<d2v_t *>(&sI[buf][o]) = vi[u];
+            *reinterpret_cast<d2v_t *>(&sJ[buf][o]) = vj[u];
         }
     };
 
-    double v0[2][PH];
-    load_half(0, v0);
-    store_half(v0);
+    gload(0);
+    lstore(0);
     __syncthreads();
-    // the C tile into the accumulators (out-of-range / upper elements read A[0], then 0)
-    d4_t acc[4][2];
+    if (nph > 1) gload(1);
+    // the C tile into the accumulators: D[q] = C(row = 16 i + lane/16 * 4 + q ...)
+    // in MFMA order, element (row r0 + 16 i + 4 (lane/16) + q, col c0 + 16 j + lane%16)
+    // (out-of-range / upper elements read A[0], then 0)
+    d4_t acc[4][NJ];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
+        for (int j = 0; j < NJ; ++j)
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const int row = Ri + r0 + 16 * i + (lane >> 4) + 4 * q;
@@ -758,36 +787,43 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
                 const double v = G[ok ? (int64_t)row * N + col : 0];
                 acc[i][j][q] = ok ? v : 0.0;
             }
-    auto mult = [&]() {
-        if (!active) return;
-#pragma unroll 4
-        for (int kk = 0; kk < UPD_KH; kk += 4) {
-            const int kc = kk + (lane >> 4);
-            double a[4], bv[2];
+    const int kq = 4 * (lane >> 4), rl = lane & 15;
+    int buf = 0;
+    for (int h = 0; h < nph; ++h, buf ^= 1) {
+        if (active) {
+            // two halves of two MFMA steps (one ds_read_b128 per operand block
+            // and half: the register budget)
 #pragma unroll
-            for (int i = 0; i < 4; ++i) a[i] = -sI[r0 + 16 * i + (lane & 15)][kc];
+            for (int hh = 0; hh < 2; ++hh) {
+                d2v_t a[4], bv[NJ];
 #pragma unroll
-            for (int j = 0; j < 2; ++j) bv[j] = sJ[c0 + 16 * j + (lane & 15)][kc];
+                for (int i = 0; i < 4; ++i)
+                    a[i] = -*reinterpret_cast<const d2v_t *>(
+                        &sI[buf][(r0 + 16 * i + rl) * KPU + kq + 2 * hh]);
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
+                for (int j = 0; j < NJ; ++j)
+                    bv[j] = *reinterpret_cast<const d2v_t *>(
+                        &sJ[buf][(c0 + 16 * j + rl) * KPU + kq + 2 * hh]);
 #pragma unroll
-                for (int j = 0; j < 2; ++j)
-                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], bv[j], acc[i][j], 0, 0, 0);
+                for (int st = 0; st < 2; ++st)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+#pragma unroll
+                        for (int j = 0; j < NJ; ++j)
+                            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i][st], bv[j][st],
+                                                                             acc[i][j], 0, 0, 0);
+            }
         }
-    };
-    mult();
-    for (int h = 1; h < nph; ++h) {
-        load_half(h, v0);
-        __syncthreads();   // everyone is done with the previous phase
-        store_half(v0);
+        // the other buffer was last read before the previous barrier
+        if (h + 1 < nph) lstore(buf ^ 1);
         __syncthreads();
-        mult();
+        if (h + 2 < nph) gload(h + 2);
     }
     if (!active) return;
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
+        for (int j = 0; j < NJ; ++j)
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const int row = Ri + r0 + 16 * i + (lane >> 4) + 4 * q;
@@ -801,14 +837,18 @@ int64_t chol_upd_items(int64_t N, int64_t j) {
     const int64_t t0 = (j + 1) * CHOL_NB;
     if (N <= t0) return 0;
     const int64_t t = (N - t0 + UPD_T - 1) / UPD_T;
-    return (j % 2 == 0) ? t : t * (t + 1) / 2;
+    return (j % CHOL_OUTER != CHOL_OUTER - 1) ? t : t * (t + 1) / 2;
 }
 
 int launch_chol_update(brpa_ctx *ctx, const FragDev *fr, const int64_t *pre, int nfrag,
                        int64_t total, int step) {
     if (total <= 0) return BRPA_OK;
-    hipLaunchKernelGGL(k_chol_update, dim3(grid1(total)), dim3(512), 0, ctx->stream, fr, pre,
-                       nfrag, step);
+    if (step % CHOL_OUTER != CHOL_OUTER - 1)
+        hipLaunchKernelGGL(k_chol_update<true>, dim3(grid1(total)), dim3(256), 0, ctx->stream, fr,
+                           pre, nfrag, step);
+    else
+        hipLaunchKernelGGL(k_chol_update<false>, dim3(grid1(total)), dim3(256), 0, ctx->stream, fr,
+                           pre, nfrag, step);
     LAUNCH_CHECK(ctx);
     return BRPA_OK;
 }
