@@ -116,26 +116,6 @@ int pin_reserve(brpa_ctx *ctx, size_t bytes) {
     return BRPA_OK;
 }
 
-int sync_reserve(brpa_ctx *ctx, size_t n) {
-    if (n <= ctx->sync_cap) return BRPA_OK;
-    BRPA_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    graphs_release(ctx);   // captured graphs hold the old address
-    if (ctx->sync) BRPA_HIP(ctx, hipFree(ctx->sync));
-    ctx->sync = nullptr;
-    ctx->sync_cap = 0;
-    const size_t cap = n + n / 4 + 1024;
-    hipError_t e = hipExtMallocWithFlags((void **)&ctx->sync, cap * sizeof(int),
-                                         hipDeviceMallocFinegrained);
-    if (e != hipSuccess) {
-        (void)hipGetLastError();
-        ctx->sync = nullptr;
-        return brpa_fail(ctx, BRPA_ERR_OOM, "fine-grained counters of %zu ints: %s", cap,
-                         hipGetErrorString(e));
-    }
-    ctx->sync_cap = cap;
-    return BRPA_OK;
-}
-
 void graphs_release(brpa_ctx *ctx) {
     for (auto &g : ctx->graphs)
         if (g.exec) (void)hipGraphExecDestroy(g.exec);
@@ -266,7 +246,6 @@ int brpa_ctx_destroy(brpa_ctx *ctx) {
     if (ctx->ws) (void)hipFree(ctx->ws);
     if (ctx->pin) (void)hipHostFree(ctx->pin);
     if (ctx->bounce) (void)hipHostFree(ctx->bounce);
-    if (ctx->sync) (void)hipFree(ctx->sync);
     t2_release(ctx);
     mbpt3_release(ctx);
     for (hipEvent_t e : ctx->gev) (void)hipEventDestroy(e);

@@ -177,21 +177,12 @@ int run_chunk(brpa_ctx *ctx, int precision, const std::vector<HostFrag> &fr, dou
         nfreq_all += fr[f].nfreqs;
     }
     auto pad = [](size_t b) { return ((b + 255) / 256) * 256; };
-    // persistent Cholesky: first work item of every (step, phase), in topological order
-    const bool persistent = ctx->chol_persistent != 0;
-    const int nph = 3 * nsteps_max;
-    std::vector<int64_t> poff(nph + 1, 0);
-    for (int j = 0; j < nsteps_max; ++j)
-        for (int k = 0; k < 3; ++k) {
-            const int ph = 3 * j + k;
-            poff[ph + 1] = poff[ph] + P(NFIX + k * nsteps_max + j)[nfrag];
-        }
     const size_t up_small = pad(8 * nsmall), up_fr = pad(sizeof(FragDev) * nfrag),
-                 up_pre = pad(sizeof(int64_t) * pre.size()), up_poff = pad(8 * poff.size());
-    const size_t up_bytes = up_small + up_fr + up_pre + up_poff;
+                 up_pre = pad(sizeof(int64_t) * pre.size());
+    const size_t up_bytes = up_small + up_fr + up_pre;
     const size_t dn_e = pad(8 * (size_t)nfrag), dn_eu = pad(8 * nfreq_all),
-                 dn_st = pad(4 * nfreq_all), dn_err = pad(4);
-    const size_t dn_bytes = dn_e + dn_eu + dn_st + dn_err;
+                 dn_st = pad(4 * nfreq_all);
+    const size_t dn_bytes = dn_e + dn_eu + dn_st;
     BRPA_TRY(pin_reserve(ctx, up_bytes + dn_bytes));
     char *h_up = ctx->pin, *h_dn = ctx->pin + up_bytes;
     double *hsmall = reinterpret_cast<double *>(h_up);
@@ -200,10 +191,6 @@ int run_chunk(brpa_ctx *ctx, int precision, const std::vector<HostFrag> &fr, dou
 
     // workspace layout (two passes: count, then assign)
     char *d_up = nullptr, *d_dn = nullptr;
-    // persistent Cholesky: fine-grained [completion counters | next item | error flag]
-    const size_t ncnt = persistent ? (size_t)nfreq_all * nsteps_max * 3 + 2 : 0;
-    if (persistent) BRPA_TRY(sync_reserve(ctx, ncnt));
-    int *d_cnt = persistent ? ctx->sync : nullptr;
     auto layout = [&](Arena &ar) {
         d_up = ar.take<char>(up_bytes);
         d_dn = ar.take<char>(dn_bytes);
@@ -277,16 +264,6 @@ int run_chunk(brpa_ctx *ctx, int precision, const std::vector<HostFrag> &fr, dou
             if (hd[f].R != h.R) all_dev = false;
         }
         std::memcpy(hpre, pre.data(), sizeof(int64_t) * pre.size());
-        std::memcpy(h_up + up_small + up_fr + up_pre, poff.data(), 8 * poff.size());
-    }
-    const int64_t *d_poff = reinterpret_cast<const int64_t *>(d_up + up_small + up_fr + up_pre);
-    int *d_err = persistent ? d_cnt + ncnt - 1 : reinterpret_cast<int *>(d_dn + dn_e + dn_eu + dn_st);
-    int nwg_persistent = 0;
-    if (persistent) {
-        int ncu = 0;
-        BRPA_HIP(ctx, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount,
-                                            ctx->device));
-        nwg_persistent = (int)std::min<int64_t>(poff[nph], (int64_t)std::max(ncu, 1));
     }
     auto DP = [&](int s) { return d_pre + (size_t)s * (nfrag + 1); };
     auto TOT = [&](int s) { return P(s)[nfrag]; };
@@ -296,10 +273,9 @@ int run_chunk(brpa_ctx *ctx, int precision, const std::vector<HostFrag> &fr, dou
         auto sb = [&](int st) { return timed ? stage_begin(ctx, st) : BRPA_OK; };
         auto se = [&](int st) { return timed ? stage_end(ctx, st) : BRPA_OK; };
         BRPA_HIP(ctx, hipMemcpyAsync(d_up, h_up, up_bytes, hipMemcpyHostToDevice, ctx->stream));
-        // status and error flag start at 0 (fill kernels: no memset nodes in graphs)
+        // status starts at 0 (a fill kernel: no memset nodes in graphs)
         BRPA_TRY(launch_fill_int(ctx, reinterpret_cast<int *>(d_dn + dn_e + dn_eu),
-                                 (int64_t)(dn_st + dn_err) / 4, 0));
-        if (persistent) BRPA_TRY(launch_fill_int(ctx, d_cnt, (int64_t)ncnt, 0));
+                                 (int64_t)dn_st / 4, 0));
         BRPA_TRY(sb(BRPA_STAGE_AUX));
         BRPA_TRY(launch_dtab(ctx, d_fr, DP(0), nfrag, TOT(0)));
         BRPA_TRY(se(BRPA_STAGE_AUX));
@@ -328,10 +304,7 @@ int run_chunk(brpa_ctx *ctx, int precision, const std::vector<HostFrag> &fr, dou
         const int diag_mode =
             precision == BRPA_SPLIT_BF16 ? 1 : (precision == BRPA_SPLIT_I8 ? 2 : 0);
         BRPA_TRY(launch_set_diag(ctx, d_fr, DP(2), nfrag, TOT(2), diag_mode));
-        if (persistent)
-            BRPA_TRY(launch_chol_persistent(ctx, d_fr, d_pre, nfrag, NFIX, nsteps_max, d_poff,
-                                            d_cnt, d_cnt + ncnt - 2, d_err, nwg_persistent));
-        for (int j = 0; j < nsteps_max && !persistent; ++j) {
+        for (int j = 0; j < nsteps_max; ++j) {
             BRPA_TRY(launch_potrf_diag(ctx, d_fr, DP(NFIX + j), nfrag, TOT(NFIX + j), j));
             BRPA_TRY(launch_trsm(ctx, d_fr, DP(NFIX + nsteps_max + j), nfrag,
                                  TOT(NFIX + nsteps_max + j), j));
@@ -341,9 +314,6 @@ int run_chunk(brpa_ctx *ctx, int precision, const std::vector<HostFrag> &fr, dou
         BRPA_TRY(se(BRPA_STAGE_CHOL));
         BRPA_TRY(launch_energy(ctx, d_fr, nfrag));
         BRPA_HIP(ctx, hipMemcpyAsync(h_dn, d_dn, dn_bytes, hipMemcpyDeviceToHost, ctx->stream));
-        if (persistent)
-            BRPA_HIP(ctx, hipMemcpyAsync(h_dn + dn_e + dn_eu + dn_st, d_err, sizeof(int),
-                                         hipMemcpyDeviceToHost, ctx->stream));
         return BRPA_OK;
     };
 
@@ -353,8 +323,7 @@ int run_chunk(brpa_ctx *ctx, int precision, const std::vector<HostFrag> &fr, dou
         std::string key = std::to_string(precision) + "/" + std::to_string(nfrag) + "/" +
                           std::to_string((uintptr_t)ctx->ws) + "/" +
                           std::to_string((uintptr_t)ctx->pin) + "/" +
-                          std::to_string((uintptr_t)ctx->stream) + "/" +
-                          std::to_string(ctx->chol_persistent);
+                          std::to_string((uintptr_t)ctx->stream);
         for (int f = 0; f < nfrag; ++f)
             key += ":" + std::to_string(fr[f].nocc) + "," + std::to_string(fr[f].nvir) + "," +
                    std::to_string(fr[f].naux) + "," + std::to_string(fr[f].nfreqs) + "," +
@@ -403,9 +372,6 @@ int run_chunk(brpa_ctx *ctx, int precision, const std::vector<HostFrag> &fr, dou
         if (precision != BRPA_FP64) BRPA_TRY(stage_collect(ctx, BRPA_STAGE_SPLIT));
         BRPA_TRY(stage_collect(ctx, BRPA_STAGE_CHOL));
     }
-    if (*reinterpret_cast<const int *>(h_dn + dn_e + dn_eu + dn_st) != 0)
-        return brpa_fail(ctx, BRPA_ERR_INTERNAL,
-                         "persistent Cholesky: a dependency wait exceeded its bound");
     const double *h_energy = reinterpret_cast<const double *>(h_dn);
     const double *h_eu = reinterpret_cast<const double *>(h_dn + dn_e);
     const int *h_status = reinterpret_cast<const int *>(h_dn + dn_e + dn_eu);
@@ -454,15 +420,6 @@ int run_ecrpa(brpa_ctx *ctx, int precision, const std::vector<HostFrag> &fr, dou
         BRPA_HIP(ctx, hipMemGetInfo(&free_b, &total_b));
         budget = (free_b + ctx->ws_cap) / 4 * 3;
     }
-    if (const char *e = std::getenv("BRPA_CHOL_PERSISTENT")) ctx->chol_persistent = std::atoi(e) != 0;
-    // The persistent Cholesky launch did not complete on the GPU (twice: with
-    // agent-scope counters in device memory, then with system-scope atomics on
-    // fine-grained counters); its dependency bookkeeping checks out on the host,
-    // the cause is not known, so the path is switched off.
-    if (ctx->chol_persistent)
-        return brpa_fail(ctx, BRPA_ERR_ARG,
-                         "the persistent Cholesky launch is disabled (it did not complete on the "
-                         "GPU); use the per-step launches");
     double ms = 0.0;
     stage_reset(ctx);
     ctx->last_graph_captures = ctx->last_graph_replays = 0;

@@ -66,13 +66,6 @@ struct brpa_ctx {
     int tune_force = -1;                // variant forced for every fragment (tuning runs)
     int i8_choice[3] = {-1, -1, -1};
     double i8_tune_ms[3][3] = {};      // [bucket][variant], 0 = not measured
-    // Cholesky of the batched E_RPA path: 0 one launch per step and phase, 1 one
-    // persistent launch for all steps (chosen by brpa_autotune or set by the caller)
-    int chol_persistent = 0;
-    double chol_tune_ms[2] = {};
-    // fine-grained (XCD-coherent) counters of the persistent Cholesky
-    int *sync = nullptr;
-    size_t sync_cap = 0;   // ints
     // scratch for the partial tiles of split-k GEMMs (doubles), lent by the
     // caller of a dense sequence for its duration; null: no split
     double *split_ws = nullptr;
@@ -92,8 +85,6 @@ inline int tune_bucket(int64_t naux) { return naux < 640 ? 0 : (naux < 1280 ? 1 
 
 // Pinned host buffer of at least `bytes` owned by the context.
 int pin_reserve(brpa_ctx *ctx, size_t bytes);
-// Fine-grained device buffer of at least n ints owned by the context.
-int sync_reserve(brpa_ctx *ctx, size_t n);
 void graphs_release(brpa_ctx *ctx);
 
 void t2_release(brpa_ctx *ctx);
@@ -223,9 +214,6 @@ int launch_chol_update(brpa_ctx *, const FragDev *, const int64_t *pre, int nfra
 // the next 64-column strip inside an outer panel of CHOL_OUTER steps, the rest
 // of the trailing matrix after its last step.
 int64_t chol_upd_items(int64_t N, int64_t j);
-// all Cholesky steps of the batch in one persistent launch (see kernels_f64.hip)
-int launch_chol_persistent(brpa_ctx *, const FragDev *, const int64_t *pre, int nfrag, int nfix,
-                           int nsteps, const int64_t *poff, int *cnt, int *next, int *err, int nwg);
 int launch_energy(brpa_ctx *, const FragDev *, int nfrag);
 int launch_fill_int(brpa_ctx *, int *p, int64_t n, int v);
 int launch_split(brpa_ctx *, const FragDev *, const int64_t *pre, int nfrag, int64_t total);

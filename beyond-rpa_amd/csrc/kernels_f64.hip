@@ -294,84 +294,9 @@ __device__ __forceinline__ void load_panel64(double (*s)[CHOL_NB + 1], const dou
 }
 
 // ---------------------------------------------------------------------------
-// Item bodies of the blocked Cholesky, shared by the per-step kernels and the
-// persistent kernel.  NT is the work-group size; every thread calls the body
-// (the barriers are uniform) and the work is guarded by the thread index.
-
-// Diagonal block of step `step` of matrix w: 64 x 64 Cholesky in LDS (lanes
-// tid < 64 own one row each), L11^{-1} to linv, log-det partial, optional
-// write-back of L11.  L: [64][65] LDS, Xd: [64] LDS.
-template <int NT>
-__device__ __forceinline__ void potrf_body(const FragDev &F, int w, int step, double (*L)[CHOL_NB + 1], double *Xd,
-                           int tid) {
-    static_assert(NT >= CHOL_NB && NT % CHOL_NB == 0, "work-group too small");
-    const int N = F.N;
-    const int j0 = step * CHOL_NB;
-    const int nbj = min(CHOL_NB, N - j0);
-    const double *A = F.Pi + (int64_t)w * N * N;
-    const bool act = tid < CHOL_NB;
-    const int lane = tid;
-    // rows/cols past nbj are padded with the identity; upper triangle zero
-    load_panel64<CHOL_NB, NT>(L, A, N, j0, j0 + nbj, j0, j0 + nbj, tid);
-    __syncthreads();
-    if (act)
-        for (int c = 0; c < CHOL_NB; ++c) {
-            const double v = L[lane][c];
-            L[lane][c] = (lane < nbj && c < nbj) ? (c <= lane ? v : 0.0) : (lane == c ? 1.0 : 0.0);
-        }
-    __syncthreads();
-    int bad = 0;
-    for (int c = 0; c < CHOL_NB; ++c) {
-        double piv = L[c][c];
-        if (!(piv > 0.0) || !isfinite(piv)) {
-            bad = 1;
-            piv = 1.0;
-        }
-        const double d = sqrt(piv);
-        __syncthreads();
-        if (act && lane == c) L[c][c] = d;
-        if (act && lane > c) L[lane][c] = L[lane][c] / d;
-        __syncthreads();
-        if (act && lane > c) {
-            const double lic = L[lane][c];
-            for (int k = c + 1; k <= lane; ++k) L[lane][k] -= lic * L[k][c];
-        }
-        __syncthreads();
-    }
-    // column `lane` of L^{-1} by forward substitution: x_i = X[i][lane], i >= lane,
-    // stored at L[lane][i] (i > lane, upper triangle) and Xd[lane] (i == lane)
-    if (act) {
-        Xd[lane] = 1.0 / L[lane][lane];
-        for (int i = lane + 1; i < CHOL_NB; ++i) {
-            double s = -L[i][lane] * Xd[lane];
-            for (int k = lane + 1; k < i; ++k) s -= L[i][k] * L[lane][k];
-            L[lane][i] = s / L[i][i];
-        }
-    }
-    __syncthreads();
-    if (act) {
-        double *linv = F.linv + (int64_t)w * CHOL_NB * CHOL_NB;
-        for (int i = 0; i < CHOL_NB; ++i) {
-            // X[i][lane]: zero above the diagonal, Xd on it, upper-stored below it
-            const double x = (i < lane) ? 0.0 : (i == lane ? Xd[lane] : L[lane][i]);
-            linv[i * CHOL_NB + lane] = x;
-        }
-        if (F.keep_l11 && lane < nbj) {
-            // the factored block back into Pi (lower triangle and diagonal); no other
-            // work item reads this diagonal block
-            double *Aw = F.Pi + (int64_t)w * N * N + (int64_t)(j0 + lane) * N + j0;
-            for (int c = 0; c <= lane; ++c) Aw[c] = L[lane][c];
-        }
-    }
-    if (tid == 0) {
-        dd_t s = {0.0, 0.0};
-        for (int c = 0; c < nbj; ++c) s = dd_add(s, log(L[c][c]));
-        double *part = F.ldpart + ((int64_t)w * F.nsteps + step) * 2;
-        part[0] = s.hi;
-        part[1] = s.lo;
-        if (bad) F.status[w] = 1;
-    }
-}
+// Panel body of the blocked Cholesky.  NT is the work-group size; every thread
+// calls the body (the barriers are uniform) and the work is guarded by the
+// thread index.
 
 // L21 chunk r (rows j0 + 64 r ..) of step `step`: L21 = A21 L11^{-T} on MFMA
 // (waves 0..3).  sB, sL: [64][65] LDS.
@@ -404,117 +329,7 @@ __device__ __forceinline__ void trsm_body(const FragDev &F, int w, int step, int
     }
 }
 
-// Trailing update on 128x128 super-tiles (TI >= TJ in 128-row units of the
-// trailing matrix that starts at t0 = j0 + 64), K = 64, 8 waves as 2 (rows) x
-// 4 (cols) of 64x32.  Waves entirely above the diagonal of a diagonal
-// super-tile skip their products; element writes keep col <= row.  The C tile
-// is loaded straight into the accumulators (the MFMA subtracts via a negated
-// A operand); the two panels pass through LDS in four K-phases of 16 columns.
-constexpr int UPD_T = 128;
-constexpr int UPD_KH = 16;   // K columns per LDS phase
-
-__device__ __forceinline__ void update_body(const FragDev &F, int w, int step, int pair,
-                            double (*sI)[UPD_KH + 1], double (*sJ)[UPD_KH + 1], int tid) {
-    const int N = F.N;
-    const int j0 = step * CHOL_NB;
-    const int t0 = j0 + CHOL_NB;
-    int ti, tj;
-    tri_decode(pair, ti, tj);
-    double *A = F.Pi + (int64_t)w * N * N;
-    const gdouble_t *G = (const gdouble_t *)A;
-    const int Ri = t0 + UPD_T * ti, Rj = t0 + UPD_T * tj;
-    const int wave = tid >> 6, lane = tid & 63;
-    const int wa = wave >> 2, wq = wave & 3;   // rows 64*wa.., cols 32*wq..
-    const int r0 = 64 * wa, c0 = 32 * wq;
-    const bool active = !(ti == tj && c0 > r0 + 63) && Ri + r0 < N && Rj + c0 < N;
-
-    // panel phase h: rows Ri.. / Rj.. (128 each) x columns j0 + 16 h .. + 16;
-    // thread -> column tid%16, rows tid/16 + 32 u (u < 4) of both panels
-    constexpr int RS = 512 / UPD_KH;        // rows per load sweep
-    constexpr int PH = UPD_T / RS;
-    const int lc = tid % UPD_KH, lr = tid / UPD_KH;
-    auto load_half = [&](int h, double (&v)[2][PH]) {
-#pragma unroll
-        for (int u = 0; u < PH; ++u) {
-            const int r = lr + RS * u;
-            const bool oki = Ri + r < N, okj = Rj + r < N;
-            v[0][u] = G[oki ? (int64_t)(Ri + r) * N + j0 + UPD_KH * h + lc : 0];
-            v[1][u] = G[okj ? (int64_t)(Rj + r) * N + j0 + UPD_KH * h + lc : 0];
-            v[0][u] = oki ? v[0][u] : 0.0;
-            v[1][u] = okj ? v[1][u] : 0.0;
-        }
-    };
-    auto store_half = [&](const double (&v)[2][PH]) {
-#pragma unroll
-        for (int u = 0; u < PH; ++u) {
-            sI[lr + RS * u][lc] = v[0][u];
-            sJ[lr + RS * u][lc] = v[1][u];
-        }
-    };
-    double v0[2][PH];
-    load_half(0, v0);
-    store_half(v0);
-    __syncthreads();
-    // the C tile into the accumulators (out-of-range / upper elements read A[0], then 0)
-    d4_t acc[4][2];
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j)
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int row = Ri + r0 + 16 * i + (lane >> 4) + 4 * q;
-                const int col = Rj + c0 + 16 * j + (lane & 15);
-                const bool ok = active && row < N && col < N && col <= row;
-                const double v = G[ok ? (int64_t)row * N + col : 0];
-                acc[i][j][q] = ok ? v : 0.0;
-            }
-    auto mult = [&]() {
-        if (!active) return;
-#pragma unroll 4
-        for (int kk = 0; kk < UPD_KH; kk += 4) {
-            const int kc = kk + (lane >> 4);
-            double a[4], bv[2];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) a[i] = -sI[r0 + 16 * i + (lane & 15)][kc];
-#pragma unroll
-            for (int j = 0; j < 2; ++j) bv[j] = sJ[c0 + 16 * j + (lane & 15)][kc];
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-#pragma unroll
-                for (int j = 0; j < 2; ++j)
-                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], bv[j], acc[i][j], 0, 0, 0);
-        }
-    };
-    mult();
-    for (int h = 1; h < CHOL_NB / UPD_KH; ++h) {
-        load_half(h, v0);
-        __syncthreads();   // everyone is done with the previous phase
-        store_half(v0);
-        __syncthreads();
-        mult();
-    }
-    if (!active) return;
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j)
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int row = Ri + r0 + 16 * i + (lane >> 4) + 4 * q;
-                const int col = Rj + c0 + 16 * j + (lane & 15);
-                if (row < N && col < N && col <= row)
-                    ((gdouble_mut_t *)A)[(int64_t)row * N + col] = acc[i][j][q];
-            }
-}
-
-// Number of trailing-update super-tile pairs / L21 chunks of step j.
-__device__ __forceinline__ int upd_pairs(int N, int j) {
-    const int t0 = j * CHOL_NB + CHOL_NB;
-    if (N <= t0) return 0;
-    const int nt = (N - t0 + UPD_T - 1) / UPD_T;
-    return nt * (nt + 1) / 2;
-}
+// Number of L21 chunks of step j.
 __device__ __forceinline__ int trsm_chunks(int N, int j) {
     const int j0 = j * CHOL_NB;
     return N > j0 + CHOL_NB ? (N - j0 + CHOL_NB - 1) / CHOL_NB - 1 : 0;
@@ -673,6 +488,9 @@ int launch_trsm(brpa_ctx *ctx, const FragDev *fr, const int64_t *pre, int nfrag,
     LAUNCH_CHECK(ctx);
     return BRPA_OK;
 }
+
+constexpr int UPD_T = 128;   // trailing-update super-tile edge
+constexpr int UPD_KH = 16;   // panel columns per LDS phase
 
 // Trailing update of one 128 x 128 super-tile (TI >= TJ in 128-row units of the
 // trailing matrix that starts at t0 = j0 + 64): C -= P_i P_j^T over the panel
@@ -849,140 +667,6 @@ int launch_chol_update(brpa_ctx *ctx, const FragDev *fr, const int64_t *pre, int
     else
         hipLaunchKernelGGL(k_chol_update<false>, dim3(grid1(total)), dim3(256), 0, ctx->stream, fr,
                            pre, nfrag, step);
-    LAUNCH_CHECK(ctx);
-    return BRPA_OK;
-}
-
-// ---------------------------------------------------------------------------
-// Persistent Cholesky: one launch runs every step of every matrix of the batch.
-//
-// The work items are those of the per-step kernels, in topological order:
-// phase 3j + 0 holds the diagonal factorisations of step j, 3j + 1 the L21
-// chunks, 3j + 2 the trailing updates; poff[ph] is the first item of phase ph
-// and pre + (nfix + ph/3 + (ph%3) * nsteps) * (nfrag + 1) the per-fragment
-// prefix sums of that phase (the per-step launch arrays).  Work-groups take
-// items from a global counter.  Per matrix m = F.mat0 + w and step j three
-// counters record completions: a diagonal item waits until every update of
-// step j-1 of its matrix is done, an L21 item until the diagonal of its step
-// is done, an update until every L21 chunk of its step is done.  A work-group
-// only ever waits for items earlier in the order, which running work-groups
-// have already taken, so the earliest unfinished item can always proceed and
-// the launch drains.  Completion: every thread fences at agent scope, then
-// one release increment; waiting: one thread polls with acquire loads, then
-// all threads fence.  A wait is bounded (CHOL_SPIN polls); on expiry the
-// work-group sets the error flag and stops, and every work-group stops taking
-// items once the flag is set.
-//
-// The counters (work index, completions, error flag) live in fine-grained
-// device memory and are accessed with system-scope atomics: the 8 XCDs each
-// have their own L2, and atomics on ordinary (coarse-grained) device memory
-// are performed in the issuing XCD's L2, which other XCDs do not see within a
-// kernel.  The matrix data itself goes through the agent-scope release /
-// acquire fences (L2 write-back / invalidate).
-constexpr int CHOL_SPIN = 1 << 20;
-
-__device__ __forceinline__ int spin_wait(const int *cnt, int target) {
-    for (int it = 0; it < CHOL_SPIN; ++it) {
-        if (__hip_atomic_load(cnt, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM) >= target) return 1;
-        __builtin_amdgcn_s_sleep(1);
-    }
-    return 0;
-}
-
-__global__ __launch_bounds__(512) void k_chol_persistent(
-    const FragDev *__restrict__ fr, const int64_t *__restrict__ pre, int nfrag, int nfix,
-    int nsteps, const int64_t *__restrict__ poff, int *cnt, int *next, int *err) {
-    // one LDS arena for the three item kinds (the L21 item is the largest: 2 x 64 x 65)
-    __shared__ double smem[2 * CHOL_NB * (CHOL_NB + 1)];
-    __shared__ int s_go, s_f, s_w, s_j, s_kind, s_sub;
-    const int tid = threadIdx.x;
-    const int nph = 3 * nsteps;
-    const int64_t nitems = poff[nph];
-    for (;;) {
-        if (tid == 0) {
-            int go = 0;
-            if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) == 0) {
-                const int64_t idx = __hip_atomic_fetch_add(next, 1, __ATOMIC_RELAXED,
-                                                           __HIP_MEMORY_SCOPE_SYSTEM);
-                if (idx < nitems) {
-                    // phase: largest ph with poff[ph] <= idx
-                    int lo = 0, hi = nph - 1;
-                    while (lo < hi) {
-                        const int mid = (lo + hi + 1) >> 1;
-                        if (poff[mid] <= idx) lo = mid; else hi = mid - 1;
-                    }
-                    const int ph = lo, j = ph / 3, kind = ph % 3;
-                    const int64_t *pp = pre + (int64_t)(nfix + j + kind * nsteps) * (nfrag + 1);
-                    const int64_t local = idx - poff[ph];
-                    const int f = find_frag(pp, nfrag, local);
-                    const int64_t lb = local - pp[f];
-                    const int N = fr[f].N;
-                    int w = 0, sub = 0;
-                    if (kind == 0) {
-                        w = (int)lb;
-                    } else if (kind == 1) {
-                        const int nsub = trsm_chunks(N, j);
-                        w = (int)(lb / nsub);
-                        sub = 1 + (int)(lb - (int64_t)w * nsub);
-                    } else {
-                        const int np = upd_pairs(N, j);
-                        w = (int)(lb / np);
-                        sub = (int)(lb - (int64_t)w * np);
-                    }
-                    const int64_t m = fr[f].mat0 + w;
-                    int ok = 1;
-                    if (kind == 0 && j > 0)
-                        ok = spin_wait(cnt + (m * nsteps + (j - 1)) * 3 + 2, upd_pairs(N, j - 1));
-                    else if (kind == 1)
-                        ok = spin_wait(cnt + (m * nsteps + j) * 3 + 0, 1);
-                    else if (kind == 2)
-                        ok = spin_wait(cnt + (m * nsteps + j) * 3 + 1, trsm_chunks(N, j));
-                    if (ok) {
-                        go = 1;
-                        s_f = f;
-                        s_w = w;
-                        s_j = j;
-                        s_kind = kind;
-                        s_sub = sub;
-                    } else {
-                        __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-                    }
-                }
-            }
-            s_go = go;
-        }
-        __syncthreads();
-        if (!s_go) return;   // uniform: every thread read the same flag
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        const int f = s_f, w = s_w, j = s_j, kind = s_kind, sub = s_sub;
-        const FragDev &F = fr[f];
-        if (kind == 0) {
-            potrf_body<512>(F, w, j, reinterpret_cast<double(*)[CHOL_NB + 1]>(smem),
-                            smem + CHOL_NB * (CHOL_NB + 1), tid);
-        } else if (kind == 1) {
-            trsm_body<512>(F, w, j, sub, reinterpret_cast<double(*)[CHOL_NB + 1]>(smem),
-                           reinterpret_cast<double(*)[CHOL_NB + 1]>(smem + CHOL_NB * (CHOL_NB + 1)),
-                           tid);
-        } else {
-            update_body(F, w, j, sub, reinterpret_cast<double(*)[UPD_KH + 1]>(smem),
-                        reinterpret_cast<double(*)[UPD_KH + 1]>(smem + UPD_T * (UPD_KH + 1)), tid);
-        }
-        __threadfence();   // this item's stores are visible at agent scope
-        __syncthreads();   // every thread has fenced and is done with smem and s_*
-        if (tid == 0) {
-            const int64_t m = F.mat0 + w;
-            __hip_atomic_fetch_add(cnt + (m * nsteps + j) * 3 + kind, 1, __ATOMIC_RELEASE,
-                                   __HIP_MEMORY_SCOPE_SYSTEM);
-        }
-    }
-}
-
-int launch_chol_persistent(brpa_ctx *ctx, const FragDev *fr, const int64_t *pre, int nfrag,
-                           int nfix, int nsteps, const int64_t *poff, int *cnt, int *next, int *err,
-                           int nwg) {
-    if (nwg <= 0) return BRPA_OK;
-    hipLaunchKernelGGL(k_chol_persistent, dim3((unsigned)nwg), dim3(512), 0, ctx->stream, fr, pre,
-                       nfrag, nfix, nsteps, poff, cnt, next, err);
     LAUNCH_CHECK(ctx);
     return BRPA_OK;
 }

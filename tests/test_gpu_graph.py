@@ -91,8 +91,9 @@ def test_autotune_chooses_and_agrees(ctx):
     assert np.max(np.abs(e - np.array(refs))) <= 1e-7
 
 
-@pytest.mark.skip(reason="the persistent Cholesky launch did not complete on the GPU (twice); "
-                         "the path is switched off and raises an error until the cause is found")
+@pytest.mark.skip(reason="the persistent Cholesky launch did not complete on the GPU (twice), "
+                         "cause unknown; the kernel was taken out of the library and the batch "
+                         "runs as graph-replayed per-step launches")
 @pytest.mark.parametrize("prec", ["fp64", "split_i8"])
 def test_persistent_cholesky_matches_per_step(ctx, prec, monkeypatch):
     """The one-launch persistent Cholesky (dependency counters between work
@@ -123,6 +124,8 @@ def test_persistent_cholesky_matches_per_step(ctx, prec, monkeypatch):
         ctx.ecrpa_batch(frags[:1], prec)   # back to the per-step mode for later tests
 
 
+@pytest.mark.skip(reason="the persistent Cholesky launch was taken out of the library (it did not "
+                         "complete on the GPU, twice); there is no switch left to test")
 def test_persistent_cholesky_is_switched_off(ctx, monkeypatch):
     import brpa
     f = synth.make_fragment(3, 20, 100, 5)
