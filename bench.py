@@ -23,8 +23,9 @@ aomo (the AO->MO transform of the Cholesky vectors, one fragment), thc1b2g
 (the THC SOSEX + 2g cumulant terms, one fragment).
 
 Prints one JSON line.  `roofline` is measured for the dominant kernel of the job,
-the split-int8 GEMM of the amplitude contractions (k_gemm_i8), with HIP events
-inside the library around every launch of a one-worker exchange pass;
+the FP64 MFMA GEMM of the explicit-space exchange (k_gemm_f64), with HIP events
+inside the library around every launch of a one-worker exchange pass over
+fragments 9..16;
 `roofline_erpa` is the E_RPA product kernel's over the timed steps; `cpu_baseline` times the
 CPU oracle (numpy float64 + threaded OpenBLAS: E_RPA, T2 v5 and SOSEX) on the
 first fragments of the batch for about --cpu-seconds and checks the GPU
@@ -688,6 +689,9 @@ def main():
     ap.add_argument("--exchange-frags", type=int, default=-1,
                     help="fragments whose T2 + SOSEX is part of each step (default -1: all "
                          "of them; 0 times E_RPA only)")
+    ap.add_argument("--exchange-first", type=int, default=0,
+                    help="index of the first exchange fragment (the exchange runs on fragments "
+                         "first .. first + exchange-frags - 1; a counter-profiling subset)")
     ap.add_argument("--exchange-workers", type=int, default=2,
                     help="worker threads of brpa_exchange_batch (fragments in flight at once)")
     ap.add_argument("--workload", default="batch",
@@ -765,8 +769,9 @@ def main():
     # the exchange runs in the explicit ov space, FP64, in every form (the split
     # forms cover the Pi products of E_RPA)
     xprec_of = {p: "fp64" for p in PRECISIONS}
-    kx = a.nfrag if a.exchange_frags < 0 else min(a.exchange_frags, a.nfrag)
-    xsub = batch[:kx]
+    x0 = max(0, min(a.exchange_first, a.nfrag))
+    kx = a.nfrag - x0 if a.exchange_frags < 0 else min(a.exchange_frags, a.nfrag - x0)
+    xsub = batch[x0:x0 + kx]
     nw = a.exchange_workers
 
     def erpa_eager(prec):
@@ -778,7 +783,7 @@ def main():
         return e, {k: v[0] for k, v in st.items()}, {k: v[1] for k, v in st.items()}
 
     def exchange(xprec):
-        exs, nts = ctx.exchange_batch(xsub, seed=synth.BASE_SEED, nworkers=nw, precision=xprec)
+        exs, nts = ctx.exchange_batch(xsub, seed=synth.BASE_SEED + x0, nworkers=nw, precision=xprec)
         stages = {k: v[0] for k, v in ctx.last_exchange_stage_ms().items()}
         return exs, nts, stages
 
@@ -866,7 +871,7 @@ def main():
     roofline_exch = None
     if kx > 0:
         xp = xprec_of[p0]
-        exch = {"n_fragments": kx, "workers": nw,
+        exch = {"n_fragments": kx, "first_fragment": x0, "workers": nw,
                 "ms_per_step": round(xres[xp]["ms"], 3),
                 "ms_per_fragment": round(xres[xp]["ms"] / kx, 3),
                 "ms_per_fragment_by_precision": {q: round(r["ms"] / kx, 3)
@@ -876,7 +881,7 @@ def main():
                     for q, r in xres.items()},
                 "precision": "fp64 (explicit ov-space amplitudes and exchange contraction)",
                 "nguess": "ceil(3.0 N), not capped at o*v",
-                "n_oversampled": int(sum(1 for (o, v, n) in shapes[:kx]
+                "n_oversampled": int(sum(1 for (o, v, n) in shapes[x0:x0 + kx]
                                          if o * v <= int(np.ceil(3.0 * n)))),
                 "mean_nt2": float(np.mean(xres[xp]["nt"]))}
         if "fp64" in xres and "split_i8" in xres:
@@ -885,23 +890,32 @@ def main():
             exch["n_nt2_differ_split_i8_vs_fp64"] = int(
                 np.sum(xres["split_i8"]["nt"] != xres["fp64"]["nt"]))
         # the FP64 MFMA GEMM of the explicit-space exchange (I, I Omega, Gram
-        # matrices, T^T I T, T2), timed in the library (HIP events around every
-        # launch, on the worker's stream) over a one-worker pass of the first 16
-        # fragments, so that its launches do not overlap with another fragment's
+        # matrices, triangular-inverse levels, T^T I T, T2), timed in the library
+        # (HIP events around every launch, on the worker's stream) over a
+        # one-worker pass of fragments 9..16 of the batch, so that its launches
+        # do not overlap with another fragment's; the committed counter runs
+        # (profiles/exchange_frags9-16_gemm_f64_pmc.md) cover the same fragments
         from brpa import _lib as blib
-        kr = min(16, kx)
-        ctx.exchange_batch(xsub[:kr], seed=synth.BASE_SEED, nworkers=1, precision="fp64")
+        r0, kr = (9, 8) if a.nfrag >= 17 else (0, min(16, a.nfrag))
+        ctx.exchange_batch(batch[r0:r0 + kr], seed=synth.BASE_SEED + r0, nworkers=1,
+                           precision="fp64")
         kst = {name: ctx.last_kernel_stats(kid) for name, kid in
-               (("k_gemm_f64", blib.KERNEL_GEMM_F64), ("k_trsm_f64", blib.KERNEL_TRSM_F64),
+               (("k_gemm_f64", blib.KERNEL_GEMM_F64), ("k_trinv_diag", blib.KERNEL_TRINV_F64),
                 ("k_sosex_dot", blib.KERNEL_SOSEX_DOT))}
         gms, gfl, gn = kst["k_gemm_f64"]
         ach = gfl / (gms * 1e-3) / 1e12
+        xtraffic = (pmc_traffic("k_gemm_f64 (all variants)",
+                                profile="exchange_frags9-16_gemm_f64_pmc.md")
+                    if (a.nfrag == 256 and a.nfreqs == 16 and r0 == 9) else None)
         roofline_exch = {
             "bound": "mfma", "achieved": round(ach, 3), "peak": PEAK_FP64_TFLOPS,
-            "unit": "TFLOP/s", "frac": round(ach / PEAK_FP64_TFLOPS, 4), "traffic": None,
+            "unit": "TFLOP/s", "frac": round(ach / PEAK_FP64_TFLOPS, 4), "traffic": xtraffic,
+            "traffic_source": ("profiles/exchange_frags9-16_gemm_f64_pmc.md (FETCH_SIZE x 2.0 + "
+                               "WRITE_SIZE x 1.0 per dispatch, copy-calibrated, same fragments)")
+            if xtraffic else None,
             "kernel": "k_gemm_f64 (FP64 MFMA GEMM/SYRK of the explicit-space exchange)",
             "kernel_ms_per_launch": round(gms / max(gn, 1), 4), "launches": gn,
-            "sample": "one-worker exchange pass over the first %d fragments" % kr,
+            "sample": "one-worker exchange pass over fragments %d..%d" % (r0, r0 + kr - 1),
             "other_kernels": {name: {"ms": round(ms, 3), "launches": n,
                                      "tflops": round(fl / max(ms, 1e-9) / 1e9, 3)}
                               for name, (ms, fl, n) in kst.items()}}
@@ -927,11 +941,11 @@ def main():
             eref = oerpa.ecrpa_analytic(Pis, x, w)
             for p in forms:
                 dmax[p] = max(dmax[p], abs(forms[p]["energies"][k] - eref))
-            if k < kx:
+            if x0 <= k < x0 + kx:
                 A, V = ot2.t2_v5(f.R, g, x, w, Pis, seed=synth.BASE_SEED + k)
                 xref = osx.sosex(f.R, V, A, f.nocc, f.nvir)
                 for q in xres:
-                    dxmax[q] = max(dxmax[q], abs(xres[q]["ex"][k] - xref))
+                    dxmax[q] = max(dxmax[q], abs(xres[q]["ex"][k - x0] - xref))
             k += 1
         tc = time.perf_counter() - t
         cpu = {"value": round(k / tc, 4), "unit": "fragments/sec", "cores": threads,
@@ -956,7 +970,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": DTYPES[p0],
+        "dtype": DTYPES[p0] + ("; exchange f64" if kx > 0 and p0 != "fp64" else ""),
         "data": "synthetic (seeded PCG64, brpa.synth)",
         "config": {"workload": "batch of %d uneven fragments, o~U{8..40}, v~U{70..350}, "
                                "N=round(4.5(o+v)), %d frequencies" % (a.nfrag, a.nfreqs),

@@ -7,7 +7,6 @@
 #include <rocsolver/rocsolver.h>
 
 #include <algorithm>
-#include <mutex>
 
 int rocblas_handle_of(brpa_ctx *ctx, void **h);
 int launch_gemm_f64(brpa_ctx *, bool ta, bool tb, int64_t m, int64_t n, int64_t k, double alpha,
@@ -73,16 +72,16 @@ int dense_syevd(brpa_ctx *ctx, int64_t n, double *A, double *D, double *E, int *
     rocblas_handle h;
     BRPA_TRY(hnd(ctx, &h));
     int hinfo = 0;
-    {
-        // one rocSOLVER call at a time over the library (worker threads of the
-        // exchange batch share it)
-        std::lock_guard<std::mutex> lk(solver_mutex());
-        BRPA_TRY(rb(ctx,
-                    rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_upper, (rocblas_int)n,
-                                     A, (rocblas_int)n, D, E, info),
-                    "rocsolver_dsyevd"));
-        BRPA_TRY(copy_sync(ctx, &hinfo, info, sizeof(int), hipMemcpyDeviceToHost));
-    }
+    // Each exchange worker calls this on its own context (own rocBLAS handle and
+    // stream), concurrently with the other workers: the host fault that
+    // rocsolver_dsyevd takes under counter collection (rocprofv3 --pmc) also
+    // occurs with a single worker, so it is not a concurrency effect, and the
+    // eigensolves of different fragments are not serialised.
+    BRPA_TRY(rb(ctx,
+                rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_upper, (rocblas_int)n, A,
+                                 (rocblas_int)n, D, E, info),
+                "rocsolver_dsyevd"));
+    BRPA_TRY(copy_sync(ctx, &hinfo, info, sizeof(int), hipMemcpyDeviceToHost));
     if (hinfo != 0)
         return brpa_fail(ctx, BRPA_ERR_SOLVER, "dsyevd did not converge (info=%d, n=%lld)", hinfo,
                          (long long)n);
