@@ -229,7 +229,7 @@ class Context:
         names = {"t2_pi": _lib.STAGE_T2_PI, "t2_iomega": _lib.STAGE_T2_IOMEGA,
                  "t2_basis": _lib.STAGE_T2_BASIS, "t2_tit": _lib.STAGE_T2_TIT,
                  "t2_core": _lib.STAGE_T2_CORE, "sosex": _lib.STAGE_SOSEX,
-                 "t2_basis_eig": _lib.STAGE_T2_EIG}
+                 "t2_basis_eig": _lib.STAGE_T2_EIG, "t2_basis_drop": _lib.STAGE_T2_DROP}
         return {k: (self._lib.brpa_last_stage_ms(self._h, v),
                     self._lib.brpa_last_stage_launches(self._h, v)) for k, v in names.items()}
 

@@ -12,7 +12,7 @@ STAGE_PI, STAGE_SPLIT, STAGE_CHOL, STAGE_AUX = range(4)
 # amplitude-eigenvalue cutoff rules (brpa_t2_cutoff)
 T2_CUTOFF_EIG, T2_CUTOFF_EIG_DIV_MAXEIG, T2_CUTOFF_EIG_DIV_NELECTRON, T2_CUTOFF_SUM_REJECTED = range(4)
 STAGE_T2_PI, STAGE_T2_IOMEGA, STAGE_T2_BASIS, STAGE_T2_TIT, STAGE_T2_CORE, STAGE_SOSEX = range(4, 10)
-STAGE_THC_R, STAGE_THC_PI, STAGE_MBPT3, STAGE_T2_EIG = 10, 11, 12, 13
+STAGE_THC_R, STAGE_THC_PI, STAGE_MBPT3, STAGE_T2_EIG, STAGE_T2_DROP = 10, 11, 12, 13, 14
 # measured kernels (brpa_last_kernel_stats)
 KERNEL_GEMM_I8, KERNEL_GEMM_F64, KERNEL_SOSEX_DOT, KERNEL_TRSM_F64, KERNEL_TRINV_F64 = range(5)
 HISTOGRAM_NBINS = 100          # BRPA_HISTOGRAM_NBINS

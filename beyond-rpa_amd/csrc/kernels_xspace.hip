@@ -11,6 +11,7 @@
 //
 //   k_symm_fill     lower triangle of a column-major symmetric matrix from its upper one
 //   k_sosex_dot     per tile pair of G: sum_{r,c} G(r,c) (gamma M(pi(r,c)) + delta [r==c])
+//   k_ritz_resid    residual norms |A z_j - theta_j z_j| of Ritz pairs
 #include "device_common.h"
 #include "internal.h"
 
@@ -51,6 +52,39 @@ int launch_symm_fill(brpa_ctx *ctx, double *M, int64_t n) {
     const int64_t nt = (n + 31) / 32;
     hipLaunchKernelGGL(k_symm_fill, dim3((unsigned)(nt * (nt + 1) / 2)), dim3(256), 0, ctx->stream,
                        M, n);
+    LAUNCH_CHECK(ctx);
+    return BRPA_OK;
+}
+
+// ---------------------------------------------------------------------------
+// out[j] = |AZ(:, j) - theta[j] Z(:, j)|_2 for the m columns of the n-row
+// column-major AZ and Z (leading dimension n).  One work-group per column; the
+// partial sums are combined in a fixed order (wave butterfly, then the waves).
+__global__ __launch_bounds__(256) void k_ritz_resid(const double *__restrict__ AZ,
+                                                    const double *__restrict__ Z,
+                                                    const double *__restrict__ theta, int64_t n,
+                                                    double *__restrict__ out) {
+    __shared__ double sred[4];
+    const int64_t j = blockIdx.x;
+    const double t = theta[j];
+    const double *a = AZ + j * n, *z = Z + j * n;
+    double s = 0.0;
+    for (int64_t r = threadIdx.x; r < n; r += 256) {
+        const double d = a[r] - t * z[r];
+        s = fma(d, d, s);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if ((threadIdx.x & 63) == 0) sred[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) out[j] = sqrt((sred[0] + sred[1]) + (sred[2] + sred[3]));
+}
+
+int launch_ritz_resid(brpa_ctx *ctx, const double *AZ, const double *Z, const double *theta,
+                      int64_t n, int64_t m, double *out) {
+    if (m <= 0) return BRPA_OK;
+    hipLaunchKernelGGL(k_ritz_resid, dim3((unsigned)m), dim3(256), 0, ctx->stream, AZ, Z, theta,
+                       n, out);
     LAUNCH_CHECK(ctx);
     return BRPA_OK;
 }

@@ -15,9 +15,10 @@
 //      its epilogue;
 //   3. X = normalised I Omega, Omega the normalised guess vectors;
 //   4. the orthonormal basis T of span(X) with the reference's eigenvalue
-//      cutoff (Cholesky-QR when every Gram eigenvalue clears it, the
-//      eigenvectors otherwise); when NG >= ov and every eigenvalue of X X^T
-//      clears it, T = 1;
+//      cutoff (Cholesky-QR when every Gram eigenvalue clears it; when a few do
+//      not, the complement of their invariant subspace, found by inverse
+//      subspace iteration; the eigenvectors otherwise); when NG >= ov and every
+//      eigenvalue of X X^T clears it, T = 1;
 //   5. TIT = T^T (I T), 1 + TIT = U^T U, W = T U^{-1},
 //      T2 = T (1/2) TIT (1 + TIT)^{-1} T^T = (T T^T - W W^T) / 2;
 // Every triangular solve is a triangular inverse (GEMM levels) followed by a
@@ -41,6 +42,8 @@ int launch_col_scale(brpa_ctx *, double *out, const double *in, const double *s,
                      int64_t n);
 int launch_scale(brpa_ctx *, double *x, double a, int64_t n);
 int launch_add_diag(brpa_ctx *, double *M, int64_t n, double a);
+int launch_ritz_resid(brpa_ctx *, const double *AZ, const double *Z, const double *theta,
+                      int64_t n, int64_t m, double *out);
 int64_t sosex_dot_nparts(int64_t K);
 int launch_sosex_dot(brpa_ctx *, const double *R, int64_t K, int64_t N, int64_t v, const double *M,
                      double gamma, double delta, double *part);
@@ -49,6 +52,12 @@ namespace {
 constexpr double kPi = 3.14159265358979323846;
 // partial tiles of split-k GEMMs (up to 8 splits of the largest C of a call)
 constexpr int64_t kSplitWsMin = (int64_t)9 << 20;
+// dropped-subspace basis: first block size, largest block, Ritz pairs kept
+// above the dropped ones, subspace steps
+constexpr int64_t kDropBlock0 = 192;
+constexpr int64_t kDropBlockMax = 512;
+constexpr int64_t kDropMargin = 64;
+constexpr int kDropSteps = 10;
 int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 bool finite_all(const std::vector<double> &v) {
@@ -253,9 +262,10 @@ int exchange_xspace(brpa_ctx *ctx, const double *R, int64_t naux, const double *
     // when it minus cutoff * 1 has a Cholesky factor -- the kept subspace is
     // all of span(X) and any orthonormal basis of it gives the same energy.
     BRPA_TRY(begin(BRPA_STAGE_T2_BASIS));
-    int64_t nt = 0;
-    bool ident = false;            // T = 1 (the whole ov space is kept)
-    const double *T = X;           // K x nt orthonormal basis, leading dimension K
+    int64_t nt = 0;                // kept dimension (the reference's NT2)
+    int64_t nc = 0;                // columns of T (nt, or NG for the padded basis below)
+    bool ident = false;            // T2 = (1 - (1 + I)^{-1}) / 2 with the (projected) I
+    const double *T = X;           // K x nc basis, leading dimension K
     auto count_kept = [&](int64_t n) -> int {
         std::vector<double> hs(n);
         BRPA_TRY(copy_sync(ctx, hs.data(), sev, 8 * n, hipMemcpyDeviceToHost));
@@ -269,37 +279,222 @@ int exchange_xspace(brpa_ctx *ctx, const double *R, int64_t naux, const double *
         }
         return BRPA_OK;
     };
-    if (over) {
-        BRPA_TRY(dense_syrk(ctx, false, K, NG, 1.0, X, K, 0.0, Sm, K));
-        BRPA_TRY(launch_add_diag(ctx, Sm, K, -cutoff));
-        bool full = false;
-        BRPA_TRY(dc.run(ctx, Sm, K, &full));
-        if (full) {
-            nt = K;
+    auto gram = [&]() -> int {
+        if (over) return dense_syrk(ctx, false, K, NG, 1.0, X, K, 0.0, Sm, K);
+        return dense_syrk(ctx, true, NG, K, 1.0, X, K, 0.0, Sm, NG);
+    };
+    // C (m x n) = op(A) op(B) with a triangular operand (TRI_*), no scale
+    auto tri_gemm = [&](bool ta, bool tb, int64_t m, int64_t n, int64_t k, double alpha,
+                        const double *A, int64_t lda, const double *B, int64_t ldb, double *C,
+                        int64_t ldc, int tri) -> int {
+        return launch_gemm_f64_batched(ctx, ta, tb, m, n, k, alpha, A, lda, 0, 0, B, ldb, 0, 0,
+                                       0.0, C, ldc, 0, 0, 1, 1, false, tri, nullptr, 0, 0);
+    };
+
+    // Dropped-subspace basis.  G (order nd, in Sm) failed the Cholesky test of
+    // G - cutoff: some of its eigenvalues lie at or below the cutoff, and on
+    // these fragments they are few (1 to ~100 of thousands).  Their invariant
+    // subspace Zd (nd x md) is found by inverse subspace iteration with
+    // (G + cutoff/4)^{-1} on a block of p >= md + 64 vectors (Cholesky-QR in two
+    // passes after each step) and Rayleigh-Ritz; the kept subspace of the
+    // reference (the eigenvectors above the cutoff) is the orthogonal
+    // complement of Zd, and T2 depends on the kept subspace only:
+    //  - oversampled (G = X X^T, order K): T T^T = P = 1 - Zd Zd^T and
+    //    T f(T^T I T) T^T = (1 - (1 + P I P)^{-1}) / 2, the whole-space form with
+    //    I replaced by P I P = I - Zd E^T - E Zd^T, E = I Zd - Zd (Zd^T I Zd) / 2;
+    //  - otherwise (G = X^T X, order NG): the kept subspace is span(X P).  The
+    //    (K + md) x NG matrix [X P; Zd^T] has full column rank; its span is
+    //    span(X P) plus the last md coordinates, on which the zero-padded I
+    //    vanishes, so Cholesky-QR of it and the usual core leave T2 in the top
+    //    K x K block: T is its top K rows (nc = NG columns for nt = NG - md kept
+    //    directions), the bottom md rows enter the two Gram matrices only.
+    // Exactly md eigenvalues of G lie at or below the cutoff when the md Ritz
+    // values of Zd do (Courant-Fischer: at least md) and the compression of
+    // G - cutoff to the complement of Zd has a Cholesky factor (interlacing: at
+    // most md): that factor is the first Cholesky-QR factor, or P G P + Zd Zd^T
+    // - cutoff in the oversampled case.  A failure anywhere (a factor that
+    // breaks down, no convergence, md too close to p) leaves *done false and
+    // the eigensolve runs instead.
+    auto drop_basis = [&](bool *done) -> int {
+        *done = false;
+        const int64_t n = nd, r = std::max(K, nd);
+        const int64_t ysz = std::max(FN * K, K * K);
+        // the trace of G is NG (normalised columns); its largest eigenvalues are
+        // ~10 NG / n on this workload, the residual floor ~3e-15 NG / n
+        const double scale = (double)NG / (double)n;
+        const double tol_conv = 3e-14 * scale, tol_stall = 1e-13 * scale;
+        // G in full and (G + cutoff / 4)^{-1} = W W^T, W = U^{-1} in Winv
+        BRPA_TRY(gram());
+        BRPA_TRY(dense_symm_fill(ctx, Sm, n));
+        BRPA_HIP(ctx, hipMemcpyAsync(TIT, Sm, 8 * n * n, hipMemcpyDeviceToDevice, ctx->stream));
+        BRPA_TRY(launch_add_diag(ctx, TIT, n, 0.25 * cutoff));
+        bool ok = false;
+        BRPA_TRY(dc.run(ctx, TIT, n, &ok));
+        if (!ok) return BRPA_OK;
+        BRPA_TRY(dense_trinv(ctx, n, TIT, n, 0, Winv, n, 0, 1, Tinv));
+        int64_t p = kDropBlock0, md = 0;
+        double *Zd = nullptr, *GZd = nullptr, *A2 = nullptr, *B2 = nullptr;
+        bool conv = false;
+        for (int attempt = 0; attempt < 2 && !conv; ++attempt) {
+            if (p > kDropBlockMax || 2 * p > n || 4 * n * p + 4 * r * p > ysz ||
+                2 * p * p > K * NG)
+                return BRPA_OK;
+            double *Z = Y, *Z2 = Y + n * p, *GZ = Y + 2 * n * p;
+            GZd = Y + 3 * n * p;
+            A2 = Y + 4 * n * p;
+            B2 = A2 + 2 * r * p;
+            double *Gp = Om, *Wp = Om + p * p;
+            BRPA_TRY(launch_guess(ctx, Z, n, p, seed ^ 0x6a09e667f3bcc909ull));
+            double res_prev = INFINITY;
+            int64_t grow = 0;
+            for (int it = 1; it <= kDropSteps && !conv; ++it) {
+                // Z <- (G + cutoff / 4)^{-1} Z = W (W^T Z)
+                BRPA_TRY(tri_gemm(true, false, n, p, n, 1.0, Winv, n, Z, n, Z2, n, TRI_A_LOWER));
+                BRPA_TRY(tri_gemm(false, false, n, p, n, 1.0, Winv, n, Z2, n, Z, n, TRI_A_UPPER));
+                for (int pass = 0; pass < 2; ++pass) {
+                    BRPA_TRY(dense_syrk(ctx, true, p, n, 1.0, Z, n, 0.0, Gp, p));
+                    bool okq = false;
+                    BRPA_TRY(dc.run(ctx, Gp, p, &okq));
+                    if (!okq) return BRPA_OK;
+                    BRPA_TRY(dense_trinv(ctx, p, Gp, p, 0, Wp, p, 0, 1, Tinv));
+                    BRPA_TRY(tri_gemm(false, false, n, p, p, 1.0, Z, n, Wp, p, Z2, n, TRI_B_UPPER));
+                    std::swap(Z, Z2);
+                }
+                if (it < 3) continue;
+                // Rayleigh-Ritz: Z^T G Z = E diag(theta) E^T, theta ascending
+                BRPA_TRY(dense_gemm(ctx, false, false, n, p, n, 1.0, Sm, n, Z, n, 0.0, GZ, n));
+                BRPA_TRY(dense_gemm(ctx, true, false, p, p, n, 1.0, Z, n, GZ, n, 0.0, Gp, p));
+                BRPA_TRY(dense_syevd(ctx, p, Gp, sev, sE, dinfo));
+                std::vector<double> th(p);
+                BRPA_TRY(copy_sync(ctx, th.data(), sev, 8 * p, hipMemcpyDeviceToHost));
+                if (!finite_all(th)) return BRPA_OK;
+                int64_t m = 0;
+                while (m < p && th[m] <= cutoff) ++m;
+                if (m == 0) continue;
+                if (m > p - kDropMargin) {
+                    grow = m;
+                    break;
+                }
+                // the dropped Ritz pairs: Zd = Z E_d, G Zd = (G Z) E_d
+                BRPA_TRY(dense_gemm(ctx, false, false, n, m, p, 1.0, Z, n, Gp, p, 0.0, Z2, n));
+                BRPA_TRY(dense_gemm(ctx, false, false, n, m, p, 1.0, GZ, n, Gp, p, 0.0, GZd, n));
+                BRPA_TRY(launch_ritz_resid(ctx, GZd, Z2, sev, n, m, sE));
+                std::vector<double> rn(m);
+                BRPA_TRY(copy_sync(ctx, rn.data(), sE, 8 * m, hipMemcpyDeviceToHost));
+                if (!finite_all(rn)) return BRPA_OK;
+                const double res = *std::max_element(rn.begin(), rn.end());
+                conv = res <= tol_conv || (res <= tol_stall && res > 0.5 * res_prev);
+                res_prev = res;
+                md = m;
+                Zd = Z2;
+            }
+            if (!conv) {
+                if (grow == 0) return BRPA_OK;
+                p = cdiv(grow + 2 * kDropMargin, 64) * 64;
+            }
+        }
+        if (!conv) return BRPA_OK;
+        // M -= Zd D^T + D Zd^T as one GEMM of depth 2 md: [Zd D] [D Zd]^T
+        auto sym_update = [&](double *M, const double *D) -> int {
+            BRPA_HIP(ctx, hipMemcpyAsync(A2, Zd, 8 * n * md, hipMemcpyDeviceToDevice, ctx->stream));
+            BRPA_HIP(ctx, hipMemcpyAsync(A2 + n * md, D, 8 * n * md, hipMemcpyDeviceToDevice,
+                                         ctx->stream));
+            BRPA_HIP(ctx, hipMemcpyAsync(B2, D, 8 * n * md, hipMemcpyDeviceToDevice, ctx->stream));
+            BRPA_HIP(ctx, hipMemcpyAsync(B2 + n * md, Zd, 8 * n * md, hipMemcpyDeviceToDevice,
+                                         ctx->stream));
+            return dense_gemm(ctx, false, true, n, n, 2 * md, -1.0, A2, n, B2, n, 1.0, M, n);
+        };
+        double *Cd = Om;   // md x md
+        if (over) {
+            // P G P + Zd Zd^T - cutoff = G - Zd D^T - D Zd^T - cutoff,
+            // D = G Zd - Zd (Zd^T G Zd + 1) / 2
+            BRPA_TRY(dense_gemm(ctx, true, false, md, md, n, 1.0, Zd, n, GZd, n, 0.0, Cd, md));
+            BRPA_TRY(launch_add_diag(ctx, Cd, md, 1.0));
+            BRPA_TRY(dense_gemm(ctx, false, false, n, md, md, -0.5, Zd, n, Cd, md, 1.0, GZd, n));
+            BRPA_TRY(sym_update(Sm, GZd));
+            BRPA_TRY(launch_add_diag(ctx, Sm, n, -cutoff));
+            bool okv = false;
+            BRPA_TRY(dc.run(ctx, Sm, n, &okv));
+            if (!okv) return BRPA_OK;
+            // I <- P I P = I - Zd E^T - E Zd^T, E = I Zd - Zd (Zd^T I Zd) / 2
+            double *E = GZd;
+            BRPA_TRY(dense_gemm(ctx, false, false, K, md, K, 1.0, Im, K, Zd, K, 0.0, E, K));
+            BRPA_TRY(dense_gemm(ctx, true, false, md, md, K, 1.0, Zd, K, E, K, 0.0, Cd, md));
+            BRPA_TRY(dense_gemm(ctx, false, false, K, md, md, -0.5, Zd, K, Cd, md, 1.0, E, K));
+            BRPA_TRY(sym_update(Im, E));
+            nt = K - md;
+            nc = K;
             ident = true;
         } else {
+            // [X P; Zd^T]: top block X P = X - (X Zd) Zd^T in Wb (X stays for the
+            // eigensolve should the test below fail), bottom block Zd^T
+            double *XZ = A2, *B1 = B2;   // K x md, md x NG
+            BRPA_HIP(ctx, hipMemcpyAsync(Wb, X, 8 * K * NG, hipMemcpyDeviceToDevice, ctx->stream));
+            BRPA_TRY(dense_gemm(ctx, false, false, K, md, NG, 1.0, X, K, Zd, NG, 0.0, XZ, K));
+            BRPA_TRY(dense_gemm(ctx, false, true, K, NG, md, -1.0, XZ, K, Zd, NG, 1.0, Wb, K));
+            // Gram matrix P G P + Zd Zd^T of the padded matrix, minus the cutoff
+            BRPA_TRY(dense_syrk(ctx, true, NG, K, 1.0, Wb, K, 0.0, Sm, NG));
+            BRPA_TRY(dense_syrk(ctx, false, NG, md, 1.0, Zd, NG, 1.0, Sm, NG));
+            BRPA_TRY(launch_add_diag(ctx, Sm, NG, -cutoff));
+            bool okv = false;
+            BRPA_TRY(dc.run(ctx, Sm, NG, &okv));
+            if (!okv) return BRPA_OK;
+            // Cholesky-QR pass 1 with the shifted factor: top X <- (X P) U^{-1},
+            // bottom B1 = Zd^T U^{-1}
+            BRPA_TRY(dense_trinv(ctx, NG, Sm, NG, 0, Winv, NG, 0, 1, Tinv));
+            BRPA_TRY(tri_gemm(false, false, K, NG, NG, 1.0, Wb, K, Winv, NG, X, K, TRI_B_UPPER));
+            BRPA_TRY(tri_gemm(true, false, md, NG, NG, 1.0, Zd, NG, Winv, NG, B1, md, TRI_B_UPPER));
+            // pass 2 with the Gram matrix of both blocks; only the top block goes on
+            BRPA_TRY(dense_syrk(ctx, true, NG, K, 1.0, X, K, 0.0, Sm, NG));
+            BRPA_TRY(dense_syrk(ctx, true, NG, md, 1.0, B1, md, 1.0, Sm, NG));
+            bool ok2 = false;
+            BRPA_TRY(dc.run(ctx, Sm, NG, &ok2));
+            if (!ok2)
+                return brpa_fail(ctx, BRPA_ERR_SOLVER, "Cholesky-QR pass 2 of the T2 basis broke down");
+            BRPA_TRY(dense_trinv(ctx, NG, Sm, NG, 0, Winv, NG, 0, 1, Tinv));
+            BRPA_TRY(tri_gemm(false, false, K, NG, NG, 1.0, X, K, Winv, NG, Wb, K, TRI_B_UPPER));
+            std::swap(X, Wb);
+            T = X;
+            nt = NG - md;
+            nc = NG;
+        }
+        *done = true;
+        return BRPA_OK;
+    };
+
+    BRPA_TRY(gram());
+    BRPA_TRY(launch_add_diag(ctx, Sm, nd, -cutoff));
+    bool pass = false;
+    BRPA_TRY(dc.run(ctx, Sm, nd, &pass));
+    bool dropped = false;
+    if (!pass) {
+        BRPA_TRY(begin(BRPA_STAGE_T2_DROP));
+        BRPA_TRY(drop_basis(&dropped));
+        BRPA_TRY(stage_end(ctx, BRPA_STAGE_T2_DROP));
+    }
+    if (over) {
+        if (pass) {
+            nt = nc = K;
+            ident = true;
+        } else if (!dropped) {
             BRPA_TRY(begin(BRPA_STAGE_T2_EIG));
-            BRPA_TRY(dense_syrk(ctx, false, K, NG, 1.0, X, K, 0.0, Sm, K));
+            BRPA_TRY(gram());
             BRPA_TRY(dense_syevd(ctx, K, Sm, sev, sE, dinfo));
             BRPA_TRY(stage_end(ctx, BRPA_STAGE_T2_EIG));
             BRPA_TRY(count_kept(K));
+            nc = nt;
             T = Sm + K * (K - nt);   // eigenvectors of the nt largest eigenvalues
         }
     } else {
-        BRPA_TRY(dense_syrk(ctx, true, NG, K, 1.0, X, K, 0.0, Sm, NG));
-        BRPA_TRY(launch_add_diag(ctx, Sm, NG, -cutoff));
-        bool cholqr = false;
-        BRPA_TRY(dc.run(ctx, Sm, NG, &cholqr));
-        if (cholqr) {
+        if (pass) {
             // Cholesky-QR in two passes: X <- X U^{-1} with the shifted factor, then
             // once more with the unshifted Gram matrix of the result (each product
             // goes to Wb, and X and Wb trade places)
-            nt = NG;
+            nt = nc = NG;
             auto right_inv = [&]() -> int {
                 BRPA_TRY(dense_trinv(ctx, NG, Sm, NG, 0, Winv, NG, 0, 1, Tinv));
-                BRPA_TRY(launch_gemm_f64_batched(ctx, false, false, K, NG, NG, 1.0, X, K, 0, 0,
-                                                 Winv, NG, 0, 0, 0.0, Wb, K, 0, 0, 1, 1, false,
-                                                 TRI_B_UPPER, nullptr, 0, 0));
+                BRPA_TRY(tri_gemm(false, false, K, NG, NG, 1.0, X, K, Winv, NG, Wb, K,
+                                  TRI_B_UPPER));
                 std::swap(X, Wb);
                 return BRPA_OK;
             };
@@ -311,12 +506,13 @@ int exchange_xspace(brpa_ctx *ctx, const double *R, int64_t naux, const double *
                 return brpa_fail(ctx, BRPA_ERR_SOLVER, "Cholesky-QR pass 2 of the T2 basis broke down");
             BRPA_TRY(right_inv());
             T = X;
-        } else {
+        } else if (!dropped) {
             BRPA_TRY(begin(BRPA_STAGE_T2_EIG));
-            BRPA_TRY(dense_syrk(ctx, true, NG, K, 1.0, X, K, 0.0, Sm, NG));
+            BRPA_TRY(gram());
             BRPA_TRY(dense_syevd(ctx, NG, Sm, sev, sE, dinfo));
             BRPA_TRY(stage_end(ctx, BRPA_STAGE_T2_EIG));
             BRPA_TRY(count_kept(NG));
+            nc = nt;
             if (nt > 0) {
                 // T = X U_s s^{-1/2} (rpa_CC_Doubles.f90:610-624)
                 BRPA_TRY(launch_select_scale(ctx, Om, Sm, sev, NG, NG - nt, nt));
@@ -337,8 +533,9 @@ int exchange_xspace(brpa_ctx *ctx, const double *R, int64_t naux, const double *
     // (rpa_CC_Doubles.f90:625-641), in the buffer of Y
     double gamma = 1.0, delta = 0.0;
     if (ident) {
-        // T = 1: T2 = I (1 + I)^{-1} / 2 = (1 - (1 + I)^{-1}) / 2; the contraction
-        // takes (1 + I)^{-1} with gamma = -1/2 and the identity with delta = 1/2
+        // T = 1: T2 = I (1 + I)^{-1} / 2 = (1 - (1 + I)^{-1}) / 2 (with I = P I P
+        // when a dropped subspace was projected out); the contraction takes
+        // (1 + I)^{-1} with gamma = -1/2 and the identity with delta = 1/2
         BRPA_TRY(begin(BRPA_STAGE_T2_CORE));
         BRPA_TRY(launch_add_diag(ctx, Im, K, 1.0));
         bool ok = false;
@@ -358,34 +555,34 @@ int exchange_xspace(brpa_ctx *ctx, const double *R, int64_t naux, const double *
     } else {
         // T^T I T (rpa_CC_MIM_v5, 425-488) from the explicit I
         BRPA_TRY(begin(BRPA_STAGE_T2_TIT));
-        BRPA_TRY(dense_gemm(ctx, false, false, K, nt, K, 1.0, Im, K, T, K, 0.0, Om, K));
-        BRPA_TRY(dense_gemm(ctx, true, false, nt, nt, K, 1.0, T, K, Om, K, 0.0, TIT, nt));
+        BRPA_TRY(dense_gemm(ctx, false, false, K, nc, K, 1.0, Im, K, T, K, 0.0, Om, K));
+        BRPA_TRY(dense_gemm(ctx, true, false, nc, nc, K, 1.0, T, K, Om, K, 0.0, TIT, nc));
         BRPA_TRY(stage_end(ctx, BRPA_STAGE_T2_TIT));
         BRPA_TRY(begin(BRPA_STAGE_T2_CORE));
-        BRPA_HIP(ctx, hipMemcpyAsync(Wb, TIT, 8 * nt * nt, hipMemcpyDeviceToDevice, ctx->stream));
-        BRPA_TRY(launch_add_diag(ctx, TIT, nt, 1.0));
+        BRPA_HIP(ctx, hipMemcpyAsync(Wb, TIT, 8 * nc * nc, hipMemcpyDeviceToDevice, ctx->stream));
+        BRPA_TRY(launch_add_diag(ctx, TIT, nc, 1.0));
         bool ok = false;
-        BRPA_TRY(dc.run(ctx, TIT, nt, &ok));
+        BRPA_TRY(dc.run(ctx, TIT, nc, &ok));
         if (ok) {
             // W = T U^{-1}: W W^T = T (1 + TIT)^{-1} T^T, T2 = (T T^T - W W^T) / 2
-            BRPA_TRY(dense_trinv(ctx, nt, TIT, nt, 0, Winv, nt, 0, 1, Tinv));
-            BRPA_TRY(launch_gemm_f64_batched(ctx, false, false, K, nt, nt, 1.0, T, K, 0, 0, Winv,
-                                             nt, 0, 0, 0.0, Wb, K, 0, 0, 1, 1, false, TRI_B_UPPER,
+            BRPA_TRY(dense_trinv(ctx, nc, TIT, nc, 0, Winv, nc, 0, 1, Tinv));
+            BRPA_TRY(launch_gemm_f64_batched(ctx, false, false, K, nc, nc, 1.0, T, K, 0, 0, Winv,
+                                             nc, 0, 0, 0.0, Wb, K, 0, 0, 1, 1, false, TRI_B_UPPER,
                                              nullptr, 0, 0));
-            BRPA_TRY(dense_syrk(ctx, false, K, nt, 0.5, T, K, 0.0, Y, K));
-            BRPA_TRY(dense_syrk(ctx, false, K, nt, -0.5, Wb, K, 1.0, Y, K));
+            BRPA_TRY(dense_syrk(ctx, false, K, nc, 0.5, T, K, 0.0, Y, K));
+            BRPA_TRY(dense_syrk(ctx, false, K, nc, -0.5, Wb, K, 1.0, Y, K));
             BRPA_TRY(dense_symm_fill(ctx, Y, K));
         } else {
             // eigenpairs of T^T I T: A = a / (2 (1 + a)), V = T E, T2 = V diag(A) V^T
-            BRPA_HIP(ctx, hipMemcpyAsync(TIT, Wb, 8 * nt * nt, hipMemcpyDeviceToDevice, ctx->stream));
-            BRPA_TRY(dense_syevd(ctx, nt, TIT, sev, sE, dinfo + 1));
-            std::vector<double> ha(nt);
-            BRPA_TRY(copy_sync(ctx, ha.data(), sev, 8 * nt, hipMemcpyDeviceToHost));
-            for (int64_t i = 0; i < nt; ++i) ha[i] = 0.5 * (ha[i] / (ha[i] + 1.0));
-            BRPA_TRY(copy_sync(ctx, sev, ha.data(), 8 * nt, hipMemcpyHostToDevice));
-            BRPA_TRY(dense_gemm(ctx, false, false, K, nt, nt, 1.0, T, K, TIT, nt, 0.0, Om, K));
-            BRPA_TRY(launch_col_scale(ctx, Wb, Om, sev, K, nt));
-            BRPA_TRY(dense_gemm(ctx, false, true, K, K, nt, 1.0, Wb, K, Om, K, 0.0, Y, K));
+            BRPA_HIP(ctx, hipMemcpyAsync(TIT, Wb, 8 * nc * nc, hipMemcpyDeviceToDevice, ctx->stream));
+            BRPA_TRY(dense_syevd(ctx, nc, TIT, sev, sE, dinfo + 1));
+            std::vector<double> ha(nc);
+            BRPA_TRY(copy_sync(ctx, ha.data(), sev, 8 * nc, hipMemcpyDeviceToHost));
+            for (int64_t i = 0; i < nc; ++i) ha[i] = 0.5 * (ha[i] / (ha[i] + 1.0));
+            BRPA_TRY(copy_sync(ctx, sev, ha.data(), 8 * nc, hipMemcpyHostToDevice));
+            BRPA_TRY(dense_gemm(ctx, false, false, K, nc, nc, 1.0, T, K, TIT, nc, 0.0, Om, K));
+            BRPA_TRY(launch_col_scale(ctx, Wb, Om, sev, K, nc));
+            BRPA_TRY(dense_gemm(ctx, false, true, K, K, nc, 1.0, Wb, K, Om, K, 0.0, Y, K));
         }
         BRPA_TRY(stage_end(ctx, BRPA_STAGE_T2_CORE));
     }

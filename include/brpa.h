@@ -109,7 +109,11 @@ enum {
     BRPA_STAGE_T2_EIG = 13,   /* amplitudes: the basis eigensolve, inside BRPA_STAGE_T2_BASIS,
                                  when the Gram matrix fails the Cholesky test (its count is
                                  the number of amplitude builds that needed it) */
-    BRPA_NSTAGES = 14
+    BRPA_STAGE_T2_DROP = 14,  /* explicit-space exchange: the dropped-subspace basis (inverse
+                                 subspace iteration for the Gram eigenvalues at or below the
+                                 cutoff), inside BRPA_STAGE_T2_BASIS (its count is the number
+                                 of fragments that took it) */
+    BRPA_NSTAGES = 15
 };
 /* For BRPA_STAGE_MBPT3 after brpa_mbpt3_1 the count is the number of
  * sign-iteration steps of the Lyapunov solve. */
