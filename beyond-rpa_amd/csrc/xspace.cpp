@@ -252,7 +252,9 @@ int exchange_xspace(brpa_ctx *ctx, const double *R, int64_t naux, const double *
     BRPA_TRY(dense_symm_fill(ctx, Im, K));
     BRPA_TRY(launch_guess(ctx, Om, K, NG, seed));
     BRPA_TRY(launch_normalize_columns(ctx, Om, K, NG));
-    BRPA_TRY(dense_gemm(ctx, false, false, K, NG, K, 1.0, Im, K, Om, K, 0.0, X, K));
+    // I is symmetric (both triangles stored): I Omega as I^T Omega, whose
+    // operands are both contiguous in k
+    BRPA_TRY(dense_gemm(ctx, true, false, K, NG, K, 1.0, Im, K, Om, K, 0.0, X, K));
     BRPA_TRY(launch_normalize_columns(ctx, X, K, NG));
     BRPA_TRY(stage_end(ctx, BRPA_STAGE_T2_IOMEGA));
 
@@ -360,9 +362,12 @@ int exchange_xspace(brpa_ctx *ctx, const double *R, int64_t naux, const double *
                     BRPA_TRY(tri_gemm(false, false, n, p, p, 1.0, Z, n, Wp, p, Z2, n, TRI_B_UPPER));
                     std::swap(Z, Z2);
                 }
-                if (it < 3) continue;
-                // Rayleigh-Ritz: Z^T G Z = E diag(theta) E^T, theta ascending
-                BRPA_TRY(dense_gemm(ctx, false, false, n, p, n, 1.0, Sm, n, Z, n, 0.0, GZ, n));
+                // Rayleigh-Ritz at steps 4, 6, 8, 10 (the p x p eigensolve costs more
+                // than two subspace steps)
+                if (it < 4 || (it & 1)) continue;
+                // Rayleigh-Ritz: Z^T G Z = E diag(theta) E^T, theta ascending (G Z
+                // as G^T Z: G is symmetric with both triangles stored)
+                BRPA_TRY(dense_gemm(ctx, true, false, n, p, n, 1.0, Sm, n, Z, n, 0.0, GZ, n));
                 BRPA_TRY(dense_gemm(ctx, true, false, p, p, n, 1.0, Z, n, GZ, n, 0.0, Gp, p));
                 BRPA_TRY(dense_syevd(ctx, p, Gp, sev, sE, dinfo));
                 std::vector<double> th(p);
@@ -418,7 +423,7 @@ int exchange_xspace(brpa_ctx *ctx, const double *R, int64_t naux, const double *
             if (!okv) return BRPA_OK;
             // I <- P I P = I - Zd E^T - E Zd^T, E = I Zd - Zd (Zd^T I Zd) / 2
             double *E = GZd;
-            BRPA_TRY(dense_gemm(ctx, false, false, K, md, K, 1.0, Im, K, Zd, K, 0.0, E, K));
+            BRPA_TRY(dense_gemm(ctx, true, false, K, md, K, 1.0, Im, K, Zd, K, 0.0, E, K));
             BRPA_TRY(dense_gemm(ctx, true, false, md, md, K, 1.0, Zd, K, E, K, 0.0, Cd, md));
             BRPA_TRY(dense_gemm(ctx, false, false, K, md, md, -0.5, Zd, K, Cd, md, 1.0, E, K));
             BRPA_TRY(sym_update(Im, E));
@@ -555,7 +560,7 @@ int exchange_xspace(brpa_ctx *ctx, const double *R, int64_t naux, const double *
     } else {
         // T^T I T (rpa_CC_MIM_v5, 425-488) from the explicit I
         BRPA_TRY(begin(BRPA_STAGE_T2_TIT));
-        BRPA_TRY(dense_gemm(ctx, false, false, K, nc, K, 1.0, Im, K, T, K, 0.0, Om, K));
+        BRPA_TRY(dense_gemm(ctx, true, false, K, nc, K, 1.0, Im, K, T, K, 0.0, Om, K));
         BRPA_TRY(dense_gemm(ctx, true, false, nc, nc, K, 1.0, T, K, Om, K, 0.0, TIT, nc));
         BRPA_TRY(stage_end(ctx, BRPA_STAGE_T2_TIT));
         BRPA_TRY(begin(BRPA_STAGE_T2_CORE));
