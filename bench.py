@@ -8,9 +8,10 @@ N = round(4.5 (o+v)), 16 frequencies each, seeded synthetic inputs
 fragments in batched launches (Pi(u) build, exact diagonal/trace, blocked FP64
 Cholesky log-det, quadrature; replayed as a hipGraph per workspace chunk, one
 eager pass supplies the per-stage kernel times), then the exchange correction
-of all 256 fragments: the factorised direct-ring amplitudes (T2, guess count
-ceil(3 N)) and the SOSEX contraction per fragment (brpa_exchange_batch, two
-fragments in flight on worker threads).  The headline form (--precision,
+of all 256 fragments: the direct-ring amplitudes (T2, guess count ceil(3 N))
+and the SOSEX contraction per fragment in the explicit ov space
+(brpa_exchange_batch, four fragments in flight on worker threads, each on a
+quarter of the CUs).  The headline form (--precision,
 split-int8 by default) runs W warm-up steps and exactly K timed steps; the
 other forms (split-BF16 and FP64, their exchange in FP64) are timed over one
 step each and reported under `forms`.  The frequency grids (host code,
@@ -692,7 +693,7 @@ def main():
     ap.add_argument("--exchange-first", type=int, default=0,
                     help="index of the first exchange fragment (the exchange runs on fragments "
                          "first .. first + exchange-frags - 1; a counter-profiling subset)")
-    ap.add_argument("--exchange-workers", type=int, default=2,
+    ap.add_argument("--exchange-workers", type=int, default=4,
                     help="worker threads of brpa_exchange_batch (fragments in flight at once)")
     ap.add_argument("--workload", default="batch",
                     choices=["batch", "single", "adjoint", "sosex", "thc", "mbpt3", "aomo",

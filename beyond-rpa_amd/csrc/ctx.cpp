@@ -192,9 +192,7 @@ int stage_collect(brpa_ctx *ctx, int stage) {
     return BRPA_OK;
 }
 
-extern "C" {
-
-int brpa_ctx_create(int device, brpa_ctx **out) {
+int ctx_create_cu_share(int device, int part, int nparts, brpa_ctx **out) {
     if (!out) return BRPA_ERR_ARG;
     *out = nullptr;
     int n = 0;
@@ -202,12 +200,36 @@ int brpa_ctx_create(int device, brpa_ctx **out) {
         (void)hipGetLastError();
         return BRPA_ERR_HIP;
     }
-    if (device < 0 || device >= n) return BRPA_ERR_ARG;
+    if (device < 0 || device >= n || (nparts > 1 && (part < 0 || part >= nparts)))
+        return BRPA_ERR_ARG;
     brpa_ctx *ctx = new brpa_ctx();
     ctx->device = device;
-    if (hipSetDevice(device) != hipSuccess ||
-        hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) {
+    if (hipSetDevice(device) != hipSuccess) {
+        (void)hipGetLastError();
+        delete ctx;
+        return BRPA_ERR_HIP;
+    }
+    hipError_t se = hipSuccess;
+    if (nparts > 1) {
+        int ncu = 0;
+        if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) !=
+                hipSuccess || ncu < nparts) {
+            (void)hipGetLastError();
+            delete ctx;
+            return BRPA_ERR_HIP;
+        }
+        std::vector<uint32_t> mask((ncu + 31) / 32, 0u);
+        // a contiguous range of mask bits; the runtime spreads mask bits over
+        // the XCDs, so each share has CUs on every XCD
+        const int c0 = (int)((int64_t)part * ncu / nparts);
+        const int c1 = (int)((int64_t)(part + 1) * ncu / nparts);
+        for (int c = c0; c < c1; ++c) mask[c / 32] |= 1u << (c % 32);
+        se = hipExtStreamCreateWithCUMask(&ctx->own_stream, (uint32_t)mask.size(), mask.data());
+    } else {
+        se = hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking);
+    }
+    if (se != hipSuccess || hipEventCreate(&ctx->ev0) != hipSuccess ||
+        hipEventCreate(&ctx->ev1) != hipSuccess) {
         (void)hipGetLastError();
         delete ctx;
         return BRPA_ERR_HIP;
@@ -234,6 +256,10 @@ int brpa_ctx_create(int device, brpa_ctx **out) {
     *out = ctx;
     return BRPA_OK;
 }
+
+extern "C" {
+
+int brpa_ctx_create(int device, brpa_ctx **out) { return ctx_create_cu_share(device, 0, 1, out); }
 
 int brpa_ctx_destroy(brpa_ctx *ctx) {
     if (!ctx) return BRPA_OK;

@@ -41,9 +41,22 @@ int brpa_exchange_batch(brpa_ctx *ctx, int precision, int64_t nfrag, const brpa_
     nworkers = (int)std::min<int64_t>(nworkers, nfrag);
     BRPA_HIP(ctx, hipSetDevice(ctx->device));
     BRPA_HIP(ctx, hipStreamSynchronize(ctx->stream));   // inputs written on the caller's stream
+    // With four workers each worker's stream runs on its own quarter of the CUs,
+    // so a worker's short kernels (the Cholesky steps of its fragment) never wait
+    // behind the grid of another worker's GEMM.  Measured on the 256-fragment
+    // batch (MI355X, 256 CUs): 4 workers on CU quarters 13.6 s, 4 or 2 workers
+    // on the whole chip 14.2 s, 2 workers on CU halves 14.7 s; 3, 6 and 8
+    // workers on CU shares were slower still.  Other counts share the chip.
+    // Workers made for another worker count are replaced.
+    const int nshares = nworkers == 4 ? 4 : 1;
+    if ((int)ctx->workers.size() != nworkers) {
+        for (brpa_ctx *w : ctx->workers) brpa_ctx_destroy(w);
+        ctx->workers.clear();
+    }
     while ((int)ctx->workers.size() < nworkers) {
         brpa_ctx *w = nullptr;
-        int rc = brpa_ctx_create(ctx->device, &w);
+        const int part = nshares > 1 ? (int)ctx->workers.size() : 0;
+        int rc = ctx_create_cu_share(ctx->device, part, nshares, &w);
         if (rc != BRPA_OK) return brpa_fail(ctx, rc, "brpa_exchange_batch: worker context");
         ctx->workers.push_back(w);
     }

@@ -70,9 +70,14 @@ struct brpa_ctx {
     // caller of a dense sequence for its duration; null: no split
     double *split_ws = nullptr;
     size_t split_ws_cap = 0;
-    // worker contexts (own stream, rocBLAS handle, workspace) of brpa_exchange_batch
+    // worker contexts (own stream, rocBLAS handle, workspace) of brpa_exchange_batch;
+    // their streams run on disjoint CU sets, one share of the chip per worker
     std::vector<brpa_ctx *> workers;
 };
+
+// A context whose stream runs on the CUs [part ncu / nparts, (part + 1) ncu /
+// nparts) of the device (nparts <= 1: every CU, as brpa_ctx_create).
+int ctx_create_cu_share(int device, int part, int nparts, brpa_ctx **out);
 
 // Triangular operand of a batched GEMM (launch_gemm_f64_batched): the k range
 // where op(A) or op(B) is zero is skipped (the zeros must be stored).

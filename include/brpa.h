@@ -490,7 +490,8 @@ int brpa_mbpt3_2(brpa_ctx *ctx, double *ec_exchange, const double *R, const doub
  * nworkers (1..8, 0 = 1) worker contexts, each with its own stream, rocBLAS
  * handle and workspace and driven by its own host thread, take the fragments
  * one at a time, so that the latency-bound stages of different fragments
- * overlap; the results do not depend on nworkers.  Per-stage device times
+ * overlap; with exactly four workers each worker's stream runs on its own
+ * quarter of the CUs.  The results do not depend on nworkers.  Per-stage device times
  * (summed over the workers) are left in the BRPA_STAGE_T2_* / BRPA_STAGE_SOSEX
  * slots.  ec_exchange[f] receives E_x; nvecs_t2 (may be NULL) the
  * retained NT2.
