@@ -10,6 +10,7 @@
 // eigensolver, small GEMMs) overlap with the work of the others.  Every host
 // input of a worker goes to the device with a completed copy before the
 // worker's kernels use it.
+#include <algorithm>
 #include <atomic>
 #include <cmath>
 #include <cstdlib>
@@ -64,6 +65,21 @@ int brpa_exchange_batch(brpa_ctx *ctx, int precision, int64_t nfrag, const brpa_
     // factored core, then the pair contraction) for every fragment
     const char *form = std::getenv("BRPA_EXCHANGE_FORM");
     const bool pairs = form && std::string(form) == "pairs";
+    // fragments are handed out largest first (modelled cost o^2 v^2 (F N + 2 NG),
+    // the I and I Omega products), so that the last fragments in flight are
+    // small ones and no worker's share of the chip idles long at the end
+    std::vector<int64_t> order(nfrag);
+    {
+        std::vector<double> cost(nfrag);
+        for (int64_t f = 0; f < nfrag; ++f) {
+            const double K = (double)frags[f].nocc * (double)frags[f].nvir;
+            const double N = (double)frags[f].naux;
+            cost[f] = K * K * ((double)frags[f].nfreqs * N + 2.0 * guess_factor * N);
+            order[f] = f;
+        }
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int64_t a, int64_t b) { return cost[a] > cost[b]; });
+    }
     std::atomic<int64_t> next{0};
     std::atomic<bool> failed{false};
     std::mutex mu;
@@ -84,8 +100,9 @@ int brpa_exchange_batch(brpa_ctx *ctx, int precision, int64_t nfrag, const brpa_
         stage_reset(w);
         w->stage_keep = true;
         while (!failed) {
-            const int64_t f = next.fetch_add(1);
-            if (f >= nfrag) break;
+            const int64_t q = next.fetch_add(1);
+            if (q >= nfrag) break;
+            const int64_t f = order[q];
             const brpa_frag_t &fr = frags[f];
             // the explicit-space form unless the S2 rescaling needs the eigenpairs, the
             // caller asked for the pair order, or a quadrature weight is not positive
@@ -111,8 +128,9 @@ int brpa_exchange_batch(brpa_ctx *ctx, int precision, int64_t nfrag, const brpa_
                                     fr.nvir, fr.naux, nt, s2_operator);
             }
             if (rc != BRPA_OK) {
+                // the lowest failing fragment among those that ran is reported
                 std::lock_guard<std::mutex> lk(mu);
-                if (first_rc == BRPA_OK) {
+                if (first_rc == BRPA_OK || f < first_f) {
                     first_rc = rc;
                     first_f = f;
                     first_msg = w->err;
