@@ -224,6 +224,7 @@ int ctx_create_cu_share(int device, int part, int nparts, brpa_ctx **out) {
         const int c0 = (int)((int64_t)part * ncu / nparts);
         const int c1 = (int)((int64_t)(part + 1) * ncu / nparts);
         for (int c = c0; c < c1; ++c) mask[c / 32] |= 1u << (c % 32);
+        ctx->ncu = c1 - c0;
         se = hipExtStreamCreateWithCUMask(&ctx->own_stream, (uint32_t)mask.size(), mask.data());
     } else {
         se = hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking);

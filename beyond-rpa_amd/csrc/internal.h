@@ -70,6 +70,8 @@ struct brpa_ctx {
     // caller of a dense sequence for its duration; null: no split
     double *split_ws = nullptr;
     size_t split_ws_cap = 0;
+    // CUs the context's stream runs on (0: every CU of the device)
+    int ncu = 0;
     // worker contexts (own stream, rocBLAS handle, workspace) of brpa_exchange_batch;
     // their streams run on disjoint CU sets, one share of the chip per worker
     std::vector<brpa_ctx *> workers;

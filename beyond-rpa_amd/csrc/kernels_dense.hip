@@ -314,9 +314,11 @@ int launch_gemm_f64_batched(brpa_ctx *ctx, bool ta, bool tb, int64_t m, int64_t 
     g.kchunk = g.k;
     const int64_t wgs = ntiles * nb1 * nb2;
     if (nb1 == 1 && nb2 == 1 && ctx->split_ws && g.k >= 512) {
-        int ncu = 0;
-        if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device) !=
-                hipSuccess || ncu <= 0) {
+        // CUs of the context's stream (its share of the chip, or all of them)
+        int ncu = ctx->ncu;
+        if (ncu <= 0 &&
+            (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device) !=
+                 hipSuccess || ncu <= 0)) {
             (void)hipGetLastError();
             ncu = 256;
         }

@@ -491,7 +491,9 @@ int brpa_mbpt3_2(brpa_ctx *ctx, double *ec_exchange, const double *R, const doub
  * handle and workspace and driven by its own host thread, take the fragments
  * one at a time, so that the latency-bound stages of different fragments
  * overlap; with exactly four workers each worker's stream runs on its own
- * quarter of the CUs.  The results do not depend on nworkers.  Per-stage device times
+ * quarter of the CUs.  Across worker counts the results agree to rounding (the
+ * split of long GEMM reductions follows the CUs of each worker's share; within
+ * one worker count they are reproducible bit for bit).  Per-stage device times
  * (summed over the workers) are left in the BRPA_STAGE_T2_* / BRPA_STAGE_SOSEX
  * slots.  ec_exchange[f] receives E_x; nvecs_t2 (may be NULL) the
  * retained NT2.
