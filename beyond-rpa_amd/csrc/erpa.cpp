@@ -93,7 +93,7 @@ Counts counts_of(const HostFrag &f, int nsteps_max, int precision, int i8_varian
     c.syrk = (precision == BRPA_FP64) ? nt * (nt + 1) / 2 * cdiv(F, 4) : 0;
     c.setdiag = cdiv(F * N, 256);
     const int64_t kpad = cdiv(K, SPLIT_KB) * SPLIT_KB;
-    c.split = (precision == BRPA_SPLIT_BF16) ? F * (kpad / SPLIT_KB) * cdiv(N, 64) : 0;
+    c.split = (precision == BRPA_SPLIT_BF16) ? (kpad / SPLIT_KB) * cdiv(N, 64) : 0;
     const int64_t nt16 = cdiv(N, 128);
     c.syrk16 = (precision == BRPA_SPLIT_BF16) ? F * (nt16 * (nt16 + 1) / 2) : 0;
     const bool i8 = precision == BRPA_SPLIT_I8;

@@ -31,98 +31,114 @@ __device__ __forceinline__ uint16_t int_to_bf16_bits(double n) {
     return (uint16_t)(__float_as_uint(f) >> 16);
 }
 
-// Work-groups per fragment: F * nblk * ceil(N/64).  One work-group splits
-// a 128-row x 64-column block of X_w.
+// Work-groups per fragment: nblk * ceil(N/64).  One work-group reads a
+// 128-row x 64-column block of R once and splits X_w = diag(sqrt(c_w)) R on
+// that block for every frequency w in turn (R is read once, not once per
+// frequency; the slices of all frequencies are written).
 __global__ __launch_bounds__(256) void k_split(const FragDev *__restrict__ fr,
                                               const int64_t *__restrict__ pre, int nfrag) {
-    __shared__ double sX[SPLIT_KB][65];
+    __shared__ double sR[SPLIT_KB][65];
+    __shared__ double sScw[SPLIT_KB];
     __shared__ double sMax[4][64];
     __shared__ int sExp[64];
+    __shared__ double sDh[4][64], sDl[4][64];
     int64_t b = blockIdx.x;
     int f = find_frag(pre, nfrag, b);
     const FragDev &F = fr[f];
     const int N = F.N, K = F.K;
     const int ncc = (N + 63) / 64;
-    int64_t lb = b - pre[f];
-    const int w = (int)(lb / ((int64_t)F.nblk * ncc));
-    int64_t rem = lb - (int64_t)w * F.nblk * ncc;
-    const int blk = (int)(rem / ncc), cc = (int)(rem - (int64_t)blk * ncc);
+    const int64_t lb = b - pre[f];
+    const int blk = (int)(lb / ncc), cc = (int)(lb - (int64_t)blk * ncc);
     const int k0 = blk * SPLIT_KB, P0 = cc * 64;
     const int tid = threadIdx.x;
 
-    // thread tid always handles column tid%64 (256 is a multiple of 64), rows
-    // tid/64 + 4 i; it also accumulates that column's exact diagonal partial
-    // sum_k c_w(k) R[k][P]^2 in double-double.
-    dd_t dsum = {0.0, 0.0};
     for (int e = tid; e < SPLIT_KB * 64; e += 256) {
-        int r = e >> 6, c = e & 63;
-        int k = k0 + r, P = P0 + c;
-        double x = 0.0;
-        if (k < K && P < N) {
-            const double cw = F.dtab[(int64_t)w * K + k];
-            const double rv = F.R[(int64_t)k * N + P];
-            x = sqrt(cw) * rv;
-            const double r2 = rv * rv, r2e = fma(rv, rv, -r2);
-            const double p = cw * r2, pe = fma(cw, r2, -p) + cw * r2e;
-            dsum = dd_add(dsum, p);
-            dsum.lo += pe;
+        const int r = e >> 6, c = e & 63;
+        const int k = k0 + r, P = P0 + c;
+        sR[r][c] = (k < K && P < N) ? F.R[(int64_t)k * N + P] : 0.0;
+    }
+    // slice phase: thread -> column c = tid/4, rows (tid%4)*32 .. +32 (contiguous
+    // in the output)
+    const int cs = tid >> 2, rs0 = (tid & 3) * 32;
+    const int Ps = P0 + cs;
+    for (int w = 0; w < F.F; ++w) {
+        if (tid < SPLIT_KB) {
+            const int k = k0 + tid;
+            sScw[tid] = k < K ? sqrt(F.dtab[(int64_t)w * K + k]) : 0.0;
         }
-        sX[r][c] = x;
-    }
-    __shared__ double sDh[4][64], sDl[4][64];
-    sDh[tid >> 6][tid & 63] = dsum.hi;
-    sDl[tid >> 6][tid & 63] = dsum.lo;
-    __syncthreads();
-    if (tid < 64 && P0 + tid < N) {
-        dd_t t = {sDh[0][tid], sDl[0][tid]};
-        for (int g = 1; g < 4; ++g) t = dd_add(t, (dd_t){sDh[g][tid], sDl[g][tid]});
-        double *q = F.dpart + (((int64_t)w * F.nblk + blk) * N + P0 + tid) * 2;
-        q[0] = t.hi;
-        q[1] = t.lo;
-    }
-    {
-        int c = tid & 63, g = tid >> 6;
-        double m = 0.0;
-        for (int r = g * 32; r < g * 32 + 32; ++r) m = fmax(m, fabs(sX[r][c]));
-        sMax[g][c] = m;
-    }
-    __syncthreads();
-    if (tid < 64) {
-        double m = fmax(fmax(sMax[0][tid], sMax[1][tid]), fmax(sMax[2][tid], sMax[3][tid]));
-        int e = 0;
-        if (m > 0.0) frexp(m, &e);   // m = f 2^e, 0.5 <= f < 1  ->  m < 2^e
-        sExp[tid] = e;
-        if (P0 + tid < N)
-            F.scale[((int64_t)w * F.nblk + blk) * N + P0 + tid] = (m > 0.0) ? ldexpf(1.0f, e - 8) : 0.0f;
-    }
-    __syncthreads();
-    // thread -> column c = tid/4, rows (tid%4)*32 .. +32 (contiguous in the output)
-    const int c = tid >> 2, r0 = (tid & 3) * 32;
-    const int P = P0 + c;
-    if (P >= N) return;
-    const int e = sExp[c];
-    for (int r = 0; r < 32; r += 8) {
-        uint16_t v[NSLICE][8];
-#pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            double t = ldexp(sX[r0 + r + i][c], 8 - e);   // |t| < 256, exact
-#pragma unroll
-            for (int sl = 0; sl < NSLICE; ++sl) {
-                double n = rint(t);
-                t = (t - n) * 256.0;                      // exact
-                v[sl][i] = int_to_bf16_bits(ldexp(n, -8 * sl));
+        __syncthreads();
+        // thread tid handles column tid%64, rows tid/64 + 4 i: that column's
+        // exact diagonal partial sum_k c_w(k) R[k][P]^2 in double-double
+        dd_t dsum = {0.0, 0.0};
+        for (int e = tid; e < SPLIT_KB * 64; e += 256) {
+            const int r = e >> 6, c = e & 63;
+            const int k = k0 + r, P = P0 + c;
+            if (k < K && P < N) {
+                const double cw = F.dtab[(int64_t)w * K + k];
+                const double rv = sR[r][c];
+                const double r2 = rv * rv, r2e = fma(rv, rv, -r2);
+                const double p = cw * r2, pe = fma(cw, r2, -p) + cw * r2e;
+                dsum = dd_add(dsum, p);
+                dsum.lo += pe;
             }
         }
-#pragma unroll
-        for (int sl = 0; sl < NSLICE; ++sl) {
-            uint4 pk;
-            pk.x = v[sl][0] | ((uint32_t)v[sl][1] << 16);
-            pk.y = v[sl][2] | ((uint32_t)v[sl][3] << 16);
-            pk.z = v[sl][4] | ((uint32_t)v[sl][5] << 16);
-            pk.w = v[sl][6] | ((uint32_t)v[sl][7] << 16);
-            uint16_t *dst = F.sl + (((int64_t)w * NSLICE + sl) * N + P) * F.Kpad + k0 + r0 + r;
-            *reinterpret_cast<uint4 *>(dst) = pk;
+        sDh[tid >> 6][tid & 63] = dsum.hi;
+        sDl[tid >> 6][tid & 63] = dsum.lo;
+        {
+            const int c = tid & 63, g = tid >> 6;
+            double m = 0.0;
+            for (int r = g * 32; r < g * 32 + 32; ++r) m = fmax(m, fabs(sScw[r] * sR[r][c]));
+            sMax[g][c] = m;
         }
+        __syncthreads();
+        if (tid < 64) {
+            if (P0 + tid < N) {
+                dd_t t = {sDh[0][tid], sDl[0][tid]};
+                for (int g = 1; g < 4; ++g) t = dd_add(t, (dd_t){sDh[g][tid], sDl[g][tid]});
+                double *q = F.dpart + (((int64_t)w * F.nblk + blk) * N + P0 + tid) * 2;
+                q[0] = t.hi;
+                q[1] = t.lo;
+            }
+            const double m =
+                fmax(fmax(sMax[0][tid], sMax[1][tid]), fmax(sMax[2][tid], sMax[3][tid]));
+            int e = 0;
+            if (m > 0.0) frexp(m, &e);   // m = f 2^e, 0.5 <= f < 1  ->  m < 2^e
+            sExp[tid] = e;
+            if (P0 + tid < N)
+                F.scale[((int64_t)w * F.nblk + blk) * N + P0 + tid] =
+                    (m > 0.0) ? ldexpf(1.0f, e - 8) : 0.0f;
+        }
+        __syncthreads();
+        if (Ps < N) {
+            const int e = sExp[cs];
+            for (int r = 0; r < 32; r += 8) {
+                uint16_t v[NSLICE][8];
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    const int rr = rs0 + r + i;
+                    double t = ldexp(sScw[rr] * sR[rr][cs], 8 - e);   // |t| < 256, exact
+#pragma unroll
+                    for (int sl = 0; sl < NSLICE; ++sl) {
+                        double n = rint(t);
+                        t = (t - n) * 256.0;                      // exact
+                        v[sl][i] = int_to_bf16_bits(ldexp(n, -8 * sl));
+                    }
+                }
+#pragma unroll
+                for (int sl = 0; sl < NSLICE; ++sl) {
+                    uint4 pk;
+                    pk.x = v[sl][0] | ((uint32_t)v[sl][1] << 16);
+                    pk.y = v[sl][2] | ((uint32_t)v[sl][3] << 16);
+                    pk.z = v[sl][4] | ((uint32_t)v[sl][5] << 16);
+                    pk.w = v[sl][6] | ((uint32_t)v[sl][7] << 16);
+                    uint16_t *dst =
+                        F.sl + (((int64_t)w * NSLICE + sl) * N + Ps) * F.Kpad + k0 + rs0 + r;
+                    *reinterpret_cast<uint4 *>(dst) = pk;
+                }
+            }
+        }
+        // sScw, sMax, sExp and sDh/sDl are rewritten for the next frequency
+        __syncthreads();
     }
 }
 
