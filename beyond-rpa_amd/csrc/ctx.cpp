@@ -210,25 +210,31 @@ int ctx_create_cu_share(int device, int part, int nparts, brpa_ctx **out) {
         return BRPA_ERR_HIP;
     }
     hipError_t se = hipSuccess;
+    int ncu = 0;
+    if (nparts > 1 &&
+        (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) !=
+             hipSuccess || ncu < nparts)) {
+        (void)hipGetLastError();
+        nparts = 1;   // no share to give: the whole chip
+    }
     if (nparts > 1) {
-        int ncu = 0;
-        if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) !=
-                hipSuccess || ncu < nparts) {
-            (void)hipGetLastError();
-            delete ctx;
-            return BRPA_ERR_HIP;
-        }
+        // share `part`: a contiguous range of the CU mask bits
         std::vector<uint32_t> mask((ncu + 31) / 32, 0u);
-        // a contiguous range of mask bits; the runtime spreads mask bits over
-        // the XCDs, so each share has CUs on every XCD
         const int c0 = (int)((int64_t)part * ncu / nparts);
         const int c1 = (int)((int64_t)(part + 1) * ncu / nparts);
         for (int c = c0; c < c1; ++c) mask[c / 32] |= 1u << (c % 32);
         ctx->ncu = c1 - c0;
         se = hipExtStreamCreateWithCUMask(&ctx->own_stream, (uint32_t)mask.size(), mask.data());
-    } else {
-        se = hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking);
+        if (se != hipSuccess) {
+            // a runtime that refuses the mask: the share only serves performance,
+            // so the context runs on the whole chip instead
+            (void)hipGetLastError();
+            ctx->own_stream = nullptr;
+            ctx->ncu = 0;
+            nparts = 1;
+        }
     }
+    if (nparts <= 1) se = hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking);
     if (se != hipSuccess || hipEventCreate(&ctx->ev0) != hipSuccess ||
         hipEventCreate(&ctx->ev1) != hipSuccess) {
         (void)hipGetLastError();

@@ -78,7 +78,8 @@ struct brpa_ctx {
 };
 
 // A context whose stream runs on the CUs [part ncu / nparts, (part + 1) ncu /
-// nparts) of the device (nparts <= 1: every CU, as brpa_ctx_create).
+// nparts) of the device (nparts <= 1: every CU, as brpa_ctx_create; also when
+// the runtime refuses the CU mask).
 int ctx_create_cu_share(int device, int part, int nparts, brpa_ctx **out);
 
 // Triangular operand of a batched GEMM (launch_gemm_f64_batched): the k range
