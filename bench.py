@@ -902,7 +902,7 @@ def main():
                            precision="fp64")
         kst = {name: ctx.last_kernel_stats(kid) for name, kid in
                (("k_gemm_f64", blib.KERNEL_GEMM_F64), ("k_trinv_diag", blib.KERNEL_TRINV_F64),
-                ("k_sosex_dot", blib.KERNEL_SOSEX_DOT))}
+                ("k_sosex_contract", blib.KERNEL_SOSEX_DOT))}
         gms, gfl, gn = kst["k_gemm_f64"]
         ach = gfl / (gms * 1e-3) / 1e12
         xtraffic = (pmc_traffic("k_gemm_f64 (all variants)",

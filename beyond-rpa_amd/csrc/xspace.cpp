@@ -23,8 +23,8 @@
 //      T2 = T (1/2) TIT (1 + TIT)^{-1} T^T = (T T^T - W W^T) / 2;
 // Every triangular solve is a triangular inverse (GEMM levels) followed by a
 // GEMM that skips the zero triangle.
-//   6. E_x = -alpha sum_{aibj} (ai|bj) T2(aj,bi) by k_sosex_dot, with
-//      (ai|bj) = (R R^T)(ai,bj) formed tile by tile on MFMA.
+//   6. E_x = -alpha sum_{aibj} (ai|bj) T2(aj,bi): (ai|bj) = (R R^T)(ai,bj) by
+//      one SYRK, then the index-permuted contraction (k_sosex_contract).
 // T2 = T f(T^T I T) T^T depends only on the subspace span(T), so every basis of
 // the reference's kept subspace gives the same energy.
 #include <algorithm>
@@ -44,9 +44,8 @@ int launch_scale(brpa_ctx *, double *x, double a, int64_t n);
 int launch_add_diag(brpa_ctx *, double *M, int64_t n, double a);
 int launch_ritz_resid(brpa_ctx *, const double *AZ, const double *Z, const double *theta,
                       int64_t n, int64_t m, double *out);
-int64_t sosex_dot_nparts(int64_t K);
-int launch_sosex_dot(brpa_ctx *, const double *R, int64_t K, int64_t N, int64_t v, const double *M,
-                     double gamma, double delta, double *part);
+int launch_sosex_contract(brpa_ctx *, const double *G, int64_t K, int64_t v, const double *M,
+                          double gamma, double delta, double *part);
 
 namespace {
 constexpr double kPi = 3.14159265358979323846;
@@ -147,7 +146,7 @@ int exchange_xspace(brpa_ctx *ctx, const double *R, int64_t naux, const double *
     pre[1] = cdiv(F * K, 256);                      // dtab
     pre[3] = ntile * (ntile + 1) / 2 * cdiv(F, 4);  // syrk
     pre[5] = cdiv(F * N, 256);                      // set_diag
-    const int64_t nparts = sosex_dot_nparts(K);
+    const int64_t nparts = K;   // one partial sum per column of the contraction
     const int64_t nsplitws = std::max<int64_t>(
         kSplitWsMin, std::min<int64_t>((int64_t)1 << 28, 8 * std::max(K, NG) * std::max(K, NG)));
     auto layout = [&](Arena &ar) {
@@ -594,7 +593,11 @@ int exchange_xspace(brpa_ctx *ctx, const double *R, int64_t naux, const double *
 
     // 6. E_x = -alpha sum (ai|bj) T2(aj, bi)
     BRPA_TRY(begin(BRPA_STAGE_SOSEX));
-    BRPA_TRY(launch_sosex_dot(ctx, dR, K, N, nvir, Y, gamma, delta, part));
+    // (ai|bj) = (R R^T)(ai, bj) as a SYRK into the buffer of I (no longer needed;
+    // R[ai][P] is the column-major N x K matrix R^T, contiguous in P), then the
+    // contraction with the T2 matrix
+    BRPA_TRY(dense_syrk(ctx, true, K, N, 1.0, dR, N, 0.0, Im, K));
+    BRPA_TRY(launch_sosex_contract(ctx, Im, K, nvir, Y, gamma, delta, part));
     BRPA_TRY(stage_end(ctx, BRPA_STAGE_SOSEX));
     BRPA_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
     std::vector<double> hp(nparts);

@@ -127,8 +127,10 @@ int brpa_last_gemm_i8_stats(const brpa_ctx *ctx, double *ms, double *flops, int6
 /* The same for every measured kernel: BRPA_KERNEL_GEMM_I8 (as above),
  * BRPA_KERNEL_GEMM_F64 (the FP64 MFMA GEMM of the explicit-space exchange and
  * brpa_dgemm / brpa_dsyrk, 2 m n k flop per launch, m n k per launch on the
- * triangle of a SYRK), BRPA_KERNEL_SOSEX_DOT (the exchange contraction,
- * ov^2 N flop over the lower triangle of (ai|bj)), BRPA_KERNEL_TRSM_F64 (the
+ * triangle of a SYRK), BRPA_KERNEL_SOSEX_DOT (the contraction of the explicit
+ * (ai|bj) = R R^T, itself a GEMM launch, with the occupied-swapped amplitude
+ * matrix: (ov)(ov+1) flop, one multiply-add per element of the triangle),
+ * BRPA_KERNEL_TRSM_F64 (the
  * triangular solves, n^2 nrhs flop), BRPA_KERNEL_TRINV_F64 (the diagonal
  * blocks of the batched triangular inverses, 64^3/3 flop per block; their
  * off-diagonal blocks are GEMM launches).  No reference counterpart
