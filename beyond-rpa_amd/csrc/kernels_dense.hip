@@ -41,8 +41,8 @@ constexpr int GGROUP = 8;  // tile rows per group of the tile order (panel reuse
 //
 // Stages of 16 k go to LDS as [row][k] for both operands (row pitch 18
 // doubles), double-buffered: the next stage is loaded into registers while
-// the current one is multiplied, then stored into the other buffer, one
-// barrier per stage.  Every thread moves pairs of consecutive k of one row
+// the current one is multiplied, and stored into the other buffer halfway
+// through it, one barrier per stage.  Every thread moves pairs of consecutive k of one row
 // (one ds_write_b128).  In a stage, lane l of a wave supplies the k values
 // 4 (l / 16) .. +3 of its row to the four MFMA steps, so its operands for the
 // whole stage are two ds_read_b128 per 16-row block; the 16 rows of such a
@@ -69,11 +69,22 @@ constexpr int GGROUP = 8;  // tile rows per group of the tile order (panel reuse
 // Split k (nsplit > 1, one item only): work-group z takes the z-th chunk of k
 // and writes its partial tile to P + z m n (ld m); k_gemm_reduce then adds the
 // partials in order and applies alpha, cs and beta.
+// TN with even leading dimensions and strides and 16-byte aligned operands
+// (dma = 1): both operands are k-contiguous, so each stage goes from global
+// memory straight into LDS (global_load_lds_dwordx4, 16 bytes per lane, one
+// 8-row x 16-k block per wave instruction) with no register staging and no LDS
+// store instructions.  The LDS rows are unpadded (16 doubles); lane L of a block
+// loads the k pair (L % 8) ^ ((row >> 1) & 7) of its row, so that the 16 rows of
+// a fragment read land on distinct 16-byte bank groups.  Rows past m (n) repeat
+// row m - 1 (n - 1): they only reach products that are not stored.  A last stage
+// with fewer than 16 k goes through registers with zeros past kend.  The k order
+// of the sum is the same as in the register-staged form.
 struct GemmArgs {
     int m, n, k;
     int mt, nt, ntiles;
     int upper, tri, nb1;
     int nsplit, kchunk;
+    int dma;   // TN only: operands staged by direct global -> LDS loads
     double alpha, beta;
     const double *A;
     int64_t lda, sA1, sA2;
@@ -172,7 +183,82 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         }
     };
 
-    if (kbeg < kend) {
+    constexpr bool kDmaForm = TA && !TB;
+    if (kDmaForm && g.dma) {
+        if (kbeg < kend) {
+            const int rl = lane & 15, g4 = lane >> 4;
+            // per lane: source of its 16 bytes in the four 8-row blocks it loads
+            const double *da[4], *db[4];
+            int dk[4];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const int row = 32 * wave + 8 * c + (lane >> 3);
+                dk[c] = 2 * ((lane & 7) ^ ((row >> 1) & 7));
+                da[c] = A + lda * (int64_t)min(m0 + row, m - 1) + dk[c];
+                db[c] = B + ldb * (int64_t)min(n0 + row, n - 1) + dk[c];
+            }
+            auto stage = [&](int k0, int buf) {
+                if (k0 + GKB <= kend) {
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) {
+                        __builtin_amdgcn_global_load_lds(da[c] + k0, &sA[buf][(32 * wave + 8 * c) * GKB],
+                                                         16, 0, 0);
+                        __builtin_amdgcn_global_load_lds(db[c] + k0, &sB[buf][(32 * wave + 8 * c) * GKB],
+                                                         16, 0, 0);
+                    }
+                } else {
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) {
+                        const int kk = k0 + dk[c];
+                        d2_t va, vb;
+                        va.x = kk < kend ? da[c][k0] : 0.0;
+                        va.y = kk + 1 < kend ? da[c][k0 + 1] : 0.0;
+                        vb.x = kk < kend ? db[c][k0] : 0.0;
+                        vb.y = kk + 1 < kend ? db[c][k0 + 1] : 0.0;
+                        *reinterpret_cast<d2_t *>(&sA[buf][(32 * wave + 8 * c) * GKB + 2 * lane]) = va;
+                        *reinterpret_cast<d2_t *>(&sB[buf][(32 * wave + 8 * c) * GKB + 2 * lane]) = vb;
+                    }
+                }
+            };
+            stage(kbeg, 0);
+            __syncthreads();
+            int buf = 0;
+            for (int k0 = kbeg; k0 < kend; k0 += GKB, buf ^= 1) {
+                d2_t a[4][2], b[4][2];
+#pragma unroll
+                for (int hh = 0; hh < 2; ++hh) {
+                    const int kp = 2 * g4 + hh;
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const int row = 64 * wm + 16 * i + rl;
+                        a[i][hh] = *reinterpret_cast<const d2_t *>(
+                            &sA[buf][row * GKB + 2 * (kp ^ ((row >> 1) & 7))]);
+                    }
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const int row = 64 * wn + 16 * j + rl;
+                        b[j][hh] = *reinterpret_cast<const d2_t *>(
+                            &sB[buf][row * GKB + 2 * (kp ^ ((row >> 1) & 7))]);
+                    }
+                }
+                // the other buffer was last read before the previous barrier
+                if (k0 + GKB < kend) stage(k0 + GKB, buf ^ 1);
+#pragma unroll
+                for (int hh = 0; hh < 2; ++hh)
+#pragma unroll
+                    for (int st = 0; st < 2; ++st)
+#pragma unroll
+                        for (int i = 0; i < 4; ++i)
+#pragma unroll
+                            for (int j = 0; j < 4; ++j)
+                                acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+                                    b[j][hh][st], a[i][hh][st], acc[i][j], 0, 0, 0);
+                // keep the barrier (and its wait for the loads in flight) after the MFMAs
+                __builtin_amdgcn_sched_barrier(0);
+                __syncthreads();
+            }
+        }
+    } else if (kbeg < kend) {
         const int kq = 4 * (lane >> 4), rl = lane & 15;
         gload(kbeg);
         lstore(0);
@@ -180,30 +266,31 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         if (kbeg + GKB < kend) gload(kbeg + GKB);
         int buf = 0;
         for (int k0 = kbeg; k0 < kend; k0 += GKB, buf ^= 1) {
-            d2_t a[4][2], b[4][2];
+            // the stage in two halves of two k each; the next stage goes to the
+            // other buffer between them (that buffer was last read before the
+            // previous barrier), so its LDS stores overlap the second half's MFMAs
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const double *p = &sA[buf][(64 * wm + 16 * i + rl) * KP + kq];
-                a[i][0] = *reinterpret_cast<const d2_t *>(p);
-                a[i][1] = *reinterpret_cast<const d2_t *>(p + 2);
-            }
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const double *p = &sB[buf][(64 * wn + 16 * j + rl) * KP + kq];
-                b[j][0] = *reinterpret_cast<const d2_t *>(p);
-                b[j][1] = *reinterpret_cast<const d2_t *>(p + 2);
-            }
-            // C^T block = op(B)^T op(A)^T: rows of C along the lanes
-#pragma unroll
-            for (int st = 0; st < 4; ++st)
+            for (int hh = 0; hh < 2; ++hh) {
+                d2_t a[4], b[4];
 #pragma unroll
                 for (int i = 0; i < 4; ++i)
+                    a[i] = *reinterpret_cast<const d2_t *>(
+                        &sA[buf][(64 * wm + 16 * i + rl) * KP + kq + 2 * hh]);
 #pragma unroll
-                    for (int j = 0; j < 4; ++j)
-                        acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(
-                            b[j][st >> 1][st & 1], a[i][st >> 1][st & 1], acc[i][j], 0, 0, 0);
-            // the other buffer was last read before the previous barrier
-            if (k0 + GKB < kend) lstore(buf ^ 1);
+                for (int j = 0; j < 4; ++j)
+                    b[j] = *reinterpret_cast<const d2_t *>(
+                        &sB[buf][(64 * wn + 16 * j + rl) * KP + kq + 2 * hh]);
+                // C^T block = op(B)^T op(A)^T: rows of C along the lanes
+#pragma unroll
+                for (int st = 0; st < 2; ++st)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+#pragma unroll
+                        for (int j = 0; j < 4; ++j)
+                            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[j][st], a[i][st],
+                                                                             acc[i][j], 0, 0, 0);
+                if (hh == 0 && k0 + GKB < kend) lstore(buf ^ 1);
+            }
             __syncthreads();
             if (k0 + 2 * GKB < kend) gload(k0 + 2 * GKB);
         }
@@ -305,6 +392,11 @@ int launch_gemm_f64_batched(brpa_ctx *ctx, bool ta, bool tb, int64_t m, int64_t 
     g.scs1 = scs1;
     g.scs2 = scs2;
     g.P = nullptr;
+    g.dma = (ta && !tb && lda % 2 == 0 && ldb % 2 == 0 && sA1 % 2 == 0 && sA2 % 2 == 0 &&
+             sB1 % 2 == 0 && sB2 % 2 == 0 && (reinterpret_cast<uintptr_t>(A) & 15) == 0 &&
+             (reinterpret_cast<uintptr_t>(B) & 15) == 0)
+                ? 1
+                : 0;
     // split k when the last wave of tiles would leave much of the chip idle:
     // the split count s (<= 8, chunks of >= 256 k) minimises the modelled time
     // 1/eff(s) + (reduction traffic), eff(s) = tiles s / (slots ceil(tiles s /

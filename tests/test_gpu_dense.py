@@ -45,6 +45,27 @@ def test_dgemm_matches_cpu(ctx, m, n, k, ta, tb):
     assert np.max(np.abs(got - ref)) <= 1e-13 * scale
 
 
+@pytest.mark.parametrize("m,n,k", [(1, 129, 1), (130, 257, 48), (257, 300, 517), (64, 64, 2000),
+                                   (300, 17, 33)])
+def test_dgemm_tn_even_ld_matches_cpu(ctx, m, n, k):
+    """TN products with even leading dimensions and aligned operands, which
+    stage their k panels by direct global -> LDS loads: ragged row and column
+    tiles (rows past m, n repeat the last row), k not a multiple of the 16-k
+    stage (the last stage goes through registers with zeros), k = 1."""
+    rng = np.random.default_rng(m * 7 + n * 13 + k)
+    A = rng.standard_normal((k, m))
+    B = rng.standard_normal((k, n))
+    C0 = rng.standard_normal((m, n))
+    ref = 1.5 * A.T @ B - 0.5 * C0
+    Cd = _fmat(C0)
+    ld = k + (k % 2)
+    ctx.dgemm(_fmat(A, ld=ld), _fmat(B, ld=ld + 2), Cd, transa=True, transb=False, alpha=1.5,
+              beta=-0.5)
+    got = _host(Cd)
+    scale = max(1.0, np.max(np.abs(ref)))
+    assert np.max(np.abs(got - ref)) <= 1e-13 * scale
+
+
 @pytest.mark.parametrize("n,k", [(1, 5), (100, 37), (129, 300), (384, 1000)])
 @pytest.mark.parametrize("trans", [False, True])
 def test_dsyrk_matches_cpu(ctx, n, k, trans):
