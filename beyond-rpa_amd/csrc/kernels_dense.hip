@@ -69,22 +69,20 @@ constexpr int GGROUP = 8;  // tile rows per group of the tile order (panel reuse
 // Split k (nsplit > 1, one item only): work-group z takes the z-th chunk of k
 // and writes its partial tile to P + z m n (ld m); k_gemm_reduce then adds the
 // partials in order and applies alpha, cs and beta.
-// TN with even leading dimensions and strides and 16-byte aligned operands
-// (dma = 1): both operands are k-contiguous, so each stage goes from global
-// memory straight into LDS (global_load_lds_dwordx4, 16 bytes per lane, one
-// 8-row x 16-k block per wave instruction) with no register staging and no LDS
-// store instructions.  The LDS rows are unpadded (16 doubles); lane L of a block
-// loads the k pair (L % 8) ^ ((row >> 1) & 7) of its row, so that the 16 rows of
-// a fragment read land on distinct 16-byte bank groups.  Rows past m (n) repeat
-// row m - 1 (n - 1): they only reach products that are not stored.  A last stage
-// with fewer than 16 k goes through registers with zeros past kend.  The k order
-// of the sum is the same as in the register-staged form.
+// With even leading dimensions and strides, 16-byte aligned operands and, for
+// an operand contiguous in rows, an even row count (dma = 1), each stage goes
+// from global memory straight into LDS (global_load_lds_dwordx4, 16 bytes per
+// lane, 1 KB per wave instruction) with no register staging and no LDS store
+// instructions (layouts at dma_sources).  Rows past m (n) repeat the last row
+// (pair): they only reach products that are not stored.  A last stage with
+// fewer than 16 k goes through registers with zeros past kend.  The k order of
+// the sum is the same as in the register-staged form.
 struct GemmArgs {
     int m, n, k;
     int mt, nt, ntiles;
     int upper, tri, nb1;
     int nsplit, kchunk;
-    int dma;   // TN only: operands staged by direct global -> LDS loads
+    int dma;   // operands staged by direct global -> LDS loads
     double alpha, beta;
     const double *A;
     int64_t lda, sA1, sA2;
@@ -100,6 +98,85 @@ struct GemmArgs {
 namespace {
 constexpr int KP = GKB + 2;   // LDS row pitch (doubles)
 typedef double d2_t __attribute__((ext_vector_type(2)));
+}
+
+// Direct-to-LDS staging of one operand of a 128-row tile and 16-k stage.
+// k-contiguous (KC): [row][k] with unpadded 16-double rows; wave w loads the
+// 8-row blocks 32 w + 8 c (c < 4), lane L the k pair (L % 8) ^ ((row >> 1) & 7)
+// of row 8 c + L / 8 of its block.  Rows-contiguous: [k][row] with rows of
+// DMA_KPR doubles (a pitch of 4 rows moves by half of the 64 LDS banks, so
+// the four lane groups of a fragment read hit disjoint halves); wave w loads
+// the k rows 4 w + c, lane L the rows 2 L, 2 L + 1.  dk: the lane's k offset
+// within a stage (KC, pair start) or its k row.
+constexpr int DMA_KPR = 132;
+template <bool KC>
+__host__ __device__ __forceinline__ void dma_sources(const double *X, int64_t ldx, int r0, int rows, int wave,
+                                            int lane, const double *(&p)[4], int (&dk)[4]) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        if (KC) {
+            const int row = 32 * wave + 8 * c + (lane >> 3);
+            dk[c] = 2 * ((lane & 7) ^ ((row >> 1) & 7));
+            p[c] = X + ldx * (int64_t)min(r0 + row, rows - 1) + dk[c];
+        } else {
+            // rows is even here: the last pair is rows - 2, rows - 1
+            dk[c] = 4 * wave + c;
+            p[c] = X + min(r0 + 2 * lane, rows - 2) + ldx * (int64_t)dk[c];
+        }
+    }
+}
+template <bool KC>
+__host__ __device__ __forceinline__ int dma_block(int wave, int c) {
+    return KC ? (32 * wave + 8 * c) * GKB : (4 * wave + c) * DMA_KPR;
+}
+// the lane's 16 bytes of a last stage with k < kend, zeros past it
+template <bool KC>
+__host__ __device__ __forceinline__ d2_t dma_tail(const double *p, int64_t ldx, int k0, int dk, int kend) {
+    d2_t v;
+    if (KC) {
+        v.x = k0 + dk < kend ? p[k0] : 0.0;
+        v.y = k0 + dk + 1 < kend ? p[k0 + 1] : 0.0;
+    } else {
+        const bool ok = k0 + dk < kend;
+        v.x = ok ? p[ldx * k0] : 0.0;
+        v.y = ok ? p[ldx * k0 + 1] : 0.0;
+    }
+    return v;
+}
+// fragment of row `row` at the k pair 2 g4 + hh of a stage
+template <bool KC>
+__host__ __device__ __forceinline__ d2_t dma_frag(const double *L, int row, int g4, int hh) {
+    if (KC) {
+        const int kp = 2 * g4 + hh;
+        return *reinterpret_cast<const d2_t *>(&L[row * GKB + 2 * (kp ^ ((row >> 1) & 7))]);
+    }
+    const int kr = 4 * g4 + 2 * hh;
+    d2_t v;
+    v.x = L[kr * DMA_KPR + row];
+    v.y = L[(kr + 1) * DMA_KPR + row];
+    return v;
+}
+
+// one operand's stage at k0 into the LDS buffer L: direct loads when the stage
+// is full, else through registers with zeros past kend
+template <bool KC>
+__device__ __forceinline__ void dma_stage(const double *(&p)[4], int (&dk)[4],
+                                          int64_t ldx, double *L, int wave, int lane, int k0,
+                                          int kend) {
+    if (k0 + GKB <= kend) {
+        // the builtin exists for the device pass only; the host pass never runs this
+#if defined(__HIP_DEVICE_COMPILE__)
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+            __builtin_amdgcn_global_load_lds(p[c] + (KC ? (int64_t)k0 : ldx * k0),
+                                             L + dma_block<KC>(wave, c), 16, 0, 0);
+#endif
+    } else {
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+            *reinterpret_cast<d2_t *>(L + dma_block<KC>(wave, c) + 2 * lane) =
+                dma_tail<KC>(p[c], ldx, k0, dk[c], kend);
+    }
 }
 
 template <bool TA, bool TB>
@@ -183,66 +260,34 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         }
     };
 
-    constexpr bool kDmaForm = TA && !TB;
-    if (kDmaForm && g.dma) {
+    if (g.dma) {
         if (kbeg < kend) {
             const int rl = lane & 15, g4 = lane >> 4;
-            // per lane: source of its 16 bytes in the four 8-row blocks it loads
-            const double *da[4], *db[4];
-            int dk[4];
-#pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                const int row = 32 * wave + 8 * c + (lane >> 3);
-                dk[c] = 2 * ((lane & 7) ^ ((row >> 1) & 7));
-                da[c] = A + lda * (int64_t)min(m0 + row, m - 1) + dk[c];
-                db[c] = B + ldb * (int64_t)min(n0 + row, n - 1) + dk[c];
-            }
-            auto stage = [&](int k0, int buf) {
-                if (k0 + GKB <= kend) {
-#pragma unroll
-                    for (int c = 0; c < 4; ++c) {
-                        __builtin_amdgcn_global_load_lds(da[c] + k0, &sA[buf][(32 * wave + 8 * c) * GKB],
-                                                         16, 0, 0);
-                        __builtin_amdgcn_global_load_lds(db[c] + k0, &sB[buf][(32 * wave + 8 * c) * GKB],
-                                                         16, 0, 0);
-                    }
-                } else {
-#pragma unroll
-                    for (int c = 0; c < 4; ++c) {
-                        const int kk = k0 + dk[c];
-                        d2_t va, vb;
-                        va.x = kk < kend ? da[c][k0] : 0.0;
-                        va.y = kk + 1 < kend ? da[c][k0 + 1] : 0.0;
-                        vb.x = kk < kend ? db[c][k0] : 0.0;
-                        vb.y = kk + 1 < kend ? db[c][k0 + 1] : 0.0;
-                        *reinterpret_cast<d2_t *>(&sA[buf][(32 * wave + 8 * c) * GKB + 2 * lane]) = va;
-                        *reinterpret_cast<d2_t *>(&sB[buf][(32 * wave + 8 * c) * GKB + 2 * lane]) = vb;
-                    }
-                }
-            };
-            stage(kbeg, 0);
+            constexpr bool KCA = TA, KCB = !TB;   // operand contiguous in k
+            const double *pa[4], *pb[4];
+            int ka[4], kb[4];
+            dma_sources<KCA>(A, lda, m0, m, wave, lane, pa, ka);
+            dma_sources<KCB>(B, ldb, n0, n, wave, lane, pb, kb);
+            dma_stage<KCA>(pa, ka, lda, sA[0], wave, lane, kbeg, kend);
+            dma_stage<KCB>(pb, kb, ldb, sB[0], wave, lane, kbeg, kend);
             __syncthreads();
             int buf = 0;
             for (int k0 = kbeg; k0 < kend; k0 += GKB, buf ^= 1) {
                 d2_t a[4][2], b[4][2];
 #pragma unroll
                 for (int hh = 0; hh < 2; ++hh) {
-                    const int kp = 2 * g4 + hh;
 #pragma unroll
-                    for (int i = 0; i < 4; ++i) {
-                        const int row = 64 * wm + 16 * i + rl;
-                        a[i][hh] = *reinterpret_cast<const d2_t *>(
-                            &sA[buf][row * GKB + 2 * (kp ^ ((row >> 1) & 7))]);
-                    }
+                    for (int i = 0; i < 4; ++i)
+                        a[i][hh] = dma_frag<KCA>(sA[buf], 64 * wm + 16 * i + rl, g4, hh);
 #pragma unroll
-                    for (int j = 0; j < 4; ++j) {
-                        const int row = 64 * wn + 16 * j + rl;
-                        b[j][hh] = *reinterpret_cast<const d2_t *>(
-                            &sB[buf][row * GKB + 2 * (kp ^ ((row >> 1) & 7))]);
-                    }
+                    for (int j = 0; j < 4; ++j)
+                        b[j][hh] = dma_frag<KCB>(sB[buf], 64 * wn + 16 * j + rl, g4, hh);
                 }
                 // the other buffer was last read before the previous barrier
-                if (k0 + GKB < kend) stage(k0 + GKB, buf ^ 1);
+                if (k0 + GKB < kend) {
+                    dma_stage<KCA>(pa, ka, lda, sA[buf ^ 1], wave, lane, k0 + GKB, kend);
+                    dma_stage<KCB>(pb, kb, ldb, sB[buf ^ 1], wave, lane, k0 + GKB, kend);
+                }
 #pragma unroll
                 for (int hh = 0; hh < 2; ++hh)
 #pragma unroll
@@ -392,9 +437,13 @@ int launch_gemm_f64_batched(brpa_ctx *ctx, bool ta, bool tb, int64_t m, int64_t 
     g.scs1 = scs1;
     g.scs2 = scs2;
     g.P = nullptr;
-    g.dma = (ta && !tb && lda % 2 == 0 && ldb % 2 == 0 && sA1 % 2 == 0 && sA2 % 2 == 0 &&
-             sB1 % 2 == 0 && sB2 % 2 == 0 && (reinterpret_cast<uintptr_t>(A) & 15) == 0 &&
-             (reinterpret_cast<uintptr_t>(B) & 15) == 0)
+    // direct-to-LDS stages: 16-byte aligned sources (even leading dimensions and
+    // strides, aligned bases); a rows-contiguous operand also needs an even row
+    // count, so that its last row pair lies inside the matrix
+    g.dma = (lda % 2 == 0 && ldb % 2 == 0 && sA1 % 2 == 0 && sA2 % 2 == 0 && sB1 % 2 == 0 &&
+             sB2 % 2 == 0 && (reinterpret_cast<uintptr_t>(A) & 15) == 0 &&
+             (reinterpret_cast<uintptr_t>(B) & 15) == 0 && (ta || m % 2 == 0) &&
+             (!tb || n % 2 == 0))
                 ? 1
                 : 0;
     // split k when the last wave of tiles would leave much of the chip idle:

@@ -45,22 +45,25 @@ def test_dgemm_matches_cpu(ctx, m, n, k, ta, tb):
     assert np.max(np.abs(got - ref)) <= 1e-13 * scale
 
 
-@pytest.mark.parametrize("m,n,k", [(1, 129, 1), (130, 257, 48), (257, 300, 517), (64, 64, 2000),
+@pytest.mark.parametrize("m,n,k", [(2, 129, 1), (130, 257, 48), (258, 300, 517), (64, 64, 2000),
                                    (300, 17, 33)])
-def test_dgemm_tn_even_ld_matches_cpu(ctx, m, n, k):
-    """TN products with even leading dimensions and aligned operands, which
-    stage their k panels by direct global -> LDS loads: ragged row and column
-    tiles (rows past m, n repeat the last row), k not a multiple of the 16-k
-    stage (the last stage goes through registers with zeros), k = 1."""
-    rng = np.random.default_rng(m * 7 + n * 13 + k)
-    A = rng.standard_normal((k, m))
-    B = rng.standard_normal((k, n))
+@pytest.mark.parametrize("ta,tb", [(False, False), (False, True), (True, False), (True, True)])
+def test_dgemm_even_ld_matches_cpu(ctx, m, n, k, ta, tb):
+    """Products with even leading dimensions and aligned operands, which stage
+    their k panels by direct global -> LDS loads (an operand contiguous in
+    rows also needs an even row count; NT with odd n takes the register-staged
+    form): ragged row and column tiles (rows past m, n repeat the last row or
+    row pair), k not a multiple of the 16-k stage (the last stage goes through
+    registers with zeros), k = 1."""
+    rng = np.random.default_rng(m * 7 + n * 13 + k + 100 * ta + 1000 * tb)
+    A = rng.standard_normal((k, m) if ta else (m, k))
+    B = rng.standard_normal((n, k) if tb else (k, n))
     C0 = rng.standard_normal((m, n))
-    ref = 1.5 * A.T @ B - 0.5 * C0
+    ref = 1.5 * (A.T if ta else A) @ (B.T if tb else B) - 0.5 * C0
     Cd = _fmat(C0)
-    ld = k + (k % 2)
-    ctx.dgemm(_fmat(A, ld=ld), _fmat(B, ld=ld + 2), Cd, transa=True, transb=False, alpha=1.5,
-              beta=-0.5)
+    lda = A.shape[0] + A.shape[0] % 2
+    ldb = B.shape[0] + B.shape[0] % 2 + 2
+    ctx.dgemm(_fmat(A, ld=lda), _fmat(B, ld=ldb), Cd, transa=ta, transb=tb, alpha=1.5, beta=-0.5)
     got = _host(Cd)
     scale = max(1.0, np.max(np.abs(ref)))
     assert np.max(np.abs(got - ref)) <= 1e-13 * scale
