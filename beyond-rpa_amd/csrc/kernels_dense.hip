@@ -196,6 +196,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         const int rows = min(GGROUP, g.mt - grp * GGROUP);
         ti = grp * GGROUP + r % rows;
         tj = r / rows;
+        // a triangular operand gives tiles k ranges that grow with tj (TRI_B_UPPER)
+        // or ti (TRI_A_LOWER): those tiles go first, so the grid ends on short ones
+        if (g.tri == TRI_B_UPPER) tj = g.nt - 1 - tj;
+        else if (g.tri == TRI_A_LOWER) ti = g.mt - 1 - ti;
     }
     const double *__restrict__ A = g.A + item1 * g.sA1 + item2 * g.sA2;
     const double *__restrict__ B = g.B + item1 * g.sB1 + item2 * g.sB2;
