@@ -6,7 +6,8 @@
 //
 //   k_gemm_f64<TA, TB>  batched C = alpha op(A) op(B) diag(cs) + beta C on
 //                       128 x 128 tiles; upper mode (SYRK) and triangular
-//                       operands whose zero triangle is skipped
+//                       operands whose zero triangle is skipped; 16-k stages
+//                       loaded straight into LDS where alignment allows
 //   k_trinv_diag        inverses of the 64 x 64 diagonal blocks of upper
 //                       triangular matrices; with GEMM levels on top, the
 //                       batched triangular inverse (launch_trinv_f64)
@@ -60,8 +61,9 @@ constexpr int GGROUP = 8;  // tile rows per group of the tile order (panel reuse
 // Batches: work-group x = (item along dimension 1) * tiles + tile, y = item
 // along dimension 2; every operand moves by its own stride per item.  Tile
 // order: upper (SYRK: tiles ti <= tj only, by column) or groups of GGROUP tile
-// rows swept column by column, so that neighbouring work-groups share panels.
-// Triangular operands skip the k range where they are zero (they must hold
+// rows swept column by column, so that neighbouring work-groups share panels
+// (reversed along the tile index whose k range grows, for TRI_B_UPPER and
+// TRI_A_LOWER, so that the longest tiles start first).  Triangular operands skip the k range where they are zero (they must hold
 // explicit zeros there):
 //   TRI_A_UPPER  op(A)(i, k) = 0 for k < i    TRI_A_LOWER  for k > i
 //   TRI_B_UPPER  op(B)(k, j) = 0 for k > j    TRI_B_LOWER  for k < j
