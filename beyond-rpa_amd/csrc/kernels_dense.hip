@@ -43,8 +43,8 @@ constexpr int GGROUP = 8;  // tile rows per group of the tile order (panel reuse
 // Stages of 16 k go to LDS as [row][k] for both operands (row pitch 18
 // doubles), double-buffered: the next stage is loaded into registers while
 // the current one is multiplied, and stored into the other buffer halfway
-// through it, one barrier per stage.  Every thread moves pairs of consecutive k of one row
-// (one ds_write_b128).  In a stage, lane l of a wave supplies the k values
+// through it, one barrier per stage.  Every thread moves pairs of consecutive
+// k of one row (one ds_write_b128).  In a stage, lane l of a wave supplies the k values
 // 4 (l / 16) .. +3 of its row to the four MFMA steps, so its operands for the
 // whole stage are two ds_read_b128 per 16-row block; the 16 rows of such a
 // read land on distinct 4-bank groups (pitch 36 dwords).  The k order inside
@@ -63,8 +63,8 @@ constexpr int GGROUP = 8;  // tile rows per group of the tile order (panel reuse
 // order: upper (SYRK: tiles ti <= tj only, by column) or groups of GGROUP tile
 // rows swept column by column, so that neighbouring work-groups share panels
 // (reversed along the tile index whose k range grows, for TRI_B_UPPER and
-// TRI_A_LOWER, so that the longest tiles start first).  Triangular operands skip the k range where they are zero (they must hold
-// explicit zeros there):
+// TRI_A_LOWER, so that the longest tiles start first).  Triangular operands
+// skip the k range where they are zero (they must hold explicit zeros there):
 //   TRI_A_UPPER  op(A)(i, k) = 0 for k < i    TRI_A_LOWER  for k > i
 //   TRI_B_UPPER  op(B)(k, j) = 0 for k > j    TRI_B_LOWER  for k < j
 // cs (optional): column j of the product is scaled by cs[j] before beta C.
@@ -112,8 +112,9 @@ typedef double d2_t __attribute__((ext_vector_type(2)));
 // within a stage (KC, pair start) or its k row.
 constexpr int DMA_KPR = 132;
 template <bool KC>
-__host__ __device__ __forceinline__ void dma_sources(const double *X, int64_t ldx, int r0, int rows, int wave,
-                                            int lane, const double *(&p)[4], int (&dk)[4]) {
+__host__ __device__ __forceinline__ void dma_sources(const double *X, int64_t ldx, int r0,
+                                                     int rows, int wave, int lane,
+                                                     const double *(&p)[4], int (&dk)[4]) {
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
         if (KC) {
@@ -133,7 +134,8 @@ __host__ __device__ __forceinline__ int dma_block(int wave, int c) {
 }
 // the lane's 16 bytes of a last stage with k < kend, zeros past it
 template <bool KC>
-__host__ __device__ __forceinline__ d2_t dma_tail(const double *p, int64_t ldx, int k0, int dk, int kend) {
+__host__ __device__ __forceinline__ d2_t dma_tail(const double *p, int64_t ldx, int k0, int dk,
+                                                  int kend) {
     d2_t v;
     if (KC) {
         v.x = k0 + dk < kend ? p[k0] : 0.0;
