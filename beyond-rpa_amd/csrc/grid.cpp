@@ -60,10 +60,18 @@ int dai_histogram(const double *eocc, int64_t nocc, const double *evir, int64_t 
             double d = evir[a] - eocc[i];
             if (d <= cap) {
                 double t = (d - (dmin + width / 2.0)) / width;
-                // Fortran nint: round half away from zero
-                int64_t k = 1 + (int64_t)(t >= 0 ? std::floor(t + 0.5) : -std::floor(-t + 0.5));
-                if (k < 1) k = 1;
-                if (k > kNBins) k = kNBins;
+                int64_t k;
+                if (!(std::fabs(t) <= (double)kNBins)) {
+                    // NaN (all gaps equal: width 0) or beyond the range: the
+                    // clamp of the bin rule decides; a NaN quotient goes to the
+                    // first bin, where nint of it lands on x86 before max(1, k)
+                    k = t > 0.0 ? kNBins : 1;
+                } else {
+                    // Fortran nint: round half away from zero
+                    k = 1 + (int64_t)(t >= 0 ? std::floor(t + 0.5) : -std::floor(-t + 0.5));
+                    if (k < 1) k = 1;
+                    if (k > kNBins) k = kNBins;
+                }
                 count[k - 1] += 1;
                 total += 1;
             }

@@ -90,8 +90,12 @@ def dai_histogram(eocc, evir, nbins=NBINS, dai_max_thresh=None, core_orb_thresh=
     values = dmin + np.arange(nbins, dtype=np.float64) * width
     d = (evir[None, :] - eocc[nc:, None]).ravel()
     d = d[d <= cap]
-    # Fortran nint rounds half away from zero
-    t = (d - (dmin + width / 2.0)) / width
+    # Fortran nint rounds half away from zero.  With every gap equal the width
+    # is 0 and the quotient NaN: nint of it is the most negative integer on
+    # x86 compilers, so max(1, k) puts the gap in the first bin.
+    with np.errstate(divide="ignore", invalid="ignore"):
+        t = (d - (dmin + width / 2.0)) / width
+    t = np.where(np.isnan(t), -1.0, np.clip(t, -2.0 * nbins, 2.0 * nbins))
     k = 1 + (np.sign(t) * np.floor(np.abs(t) + 0.5)).astype(np.int64)
     k = np.clip(k, 1, nbins)
     counts = np.bincount(k - 1, minlength=nbins)

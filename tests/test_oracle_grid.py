@@ -232,3 +232,53 @@ def test_subsystem_grid_capped_at_the_complex():
     x, w, _ = grid.freq_grid_sets(V, W, 16)
     xl, wl, _ = brpa.freq_grid_sets(V, W, 16)
     assert np.max(np.abs(xl - x) / x) <= 1e-13 and np.max(np.abs(wl - w) / w) <= 1e-13
+
+
+@pytest.mark.parametrize("eo,ev", [
+    ([-0.7], [0.4]),                      # one active pair
+    ([-0.9, -0.7, -0.7], [0.4]),          # two distinct gaps (the ordinary rule)
+    ([-12.0, -0.5, -0.5], [0.3, 0.3]),    # core frozen, every active gap equal
+])
+def test_degenerate_histogram_first_bin(eo, ev):
+    """All active gaps equal (range of width 0): the bin quotient is 0/0 and the
+    reference's clamp max(1, k) (OrbDiffHist.f90:95-97) puts every gap in the
+    first bin; library and oracle agree, and the grid built on it integrates
+    1/(d^2+u^2) at that one gap."""
+    import brpa
+    eo, ev = np.array(eo), np.array(ev)
+    vals, wts = grid.dai_histogram(eo, ev)
+    vl, wl = brpa.dai_histogram(eo, ev)
+    assert np.array_equal(vl, vals) and np.array_equal(wl, wts)
+    nc = grid.ncore(eo)
+    d = (ev[None, :] - eo[nc:, None]).ravel()
+    if np.ptp(d) == 0.0:
+        assert wts[0] == 1.0 and np.all(wts[1:] == 0.0) and np.all(vals == d[0])
+        x, w, _ = brpa.freq_grid(eo, ev, 8)
+        xr, wr, _ = grid.freq_grid(eo, ev, 8)
+        assert np.allclose(x, xr, rtol=1e-13) and np.allclose(w, wr, rtol=1e-13)
+        g = d[0]
+        assert abs(np.sum(w / (g * g + x * x)) * (2.0 * g / math.pi) - 1.0) <= 1e-6
+    else:
+        assert np.isclose(wts.sum(), 1.0)
+
+
+def test_invalid_gap_and_arguments_are_errors():
+    """A non-positive HOMO-LUMO gap stops the reference (OrbDiffHist.f90:76-79);
+    the library returns an error code for it and for sizes it does not take."""
+    import brpa
+    eo, ev = _valence(5)
+    with pytest.raises(ValueError):
+        grid.dai_histogram([-0.5, 0.2], ev)
+    with pytest.raises(brpa.BrpaError) as e:
+        brpa.dai_histogram([-0.5, 0.2], ev)
+    assert e.value.code == brpa._lib.ERR_GAP
+    with pytest.raises(brpa.BrpaError) as e:
+        brpa.dai_histogram(np.array([np.nan, -0.5]), ev)
+    assert e.value.code == brpa._lib.ERR_NONFINITE
+    for nf in (1, 441):
+        with pytest.raises(brpa.BrpaError) as e:
+            brpa.freq_grid(eo, ev, nf)
+        assert e.value.code == brpa._lib.ERR_ARG
+    with pytest.raises(brpa.BrpaError) as e:     # every occupied orbital frozen
+        brpa.freq_grid(np.array([-9.0, -4.0]), ev, 8)
+    assert e.value.code == brpa._lib.ERR_ARG
