@@ -32,7 +32,10 @@ int brpa_exchange_batch(brpa_ctx *ctx, int precision, int64_t nfrag, const brpa_
                         double alpha, int s2_operator, int nworkers, double *ec_exchange,
                         int64_t *nvecs_t2) {
     if (!ctx) return BRPA_ERR_ARG;
-    if (nfrag < 0 || (nfrag > 0 && (!frags || !ec_exchange)) || !(guess_factor > 0.0))
+    // guess_factor: GuessNVecsT2 (3 in the reference); bounded so that the
+    // guess count ceil(guess_factor * NAux) is a representable integer
+    if (nfrag < 0 || (nfrag > 0 && (!frags || !ec_exchange)) ||
+        !(guess_factor > 0.0 && guess_factor <= 1.0e6))
         return brpa_fail(ctx, BRPA_ERR_ARG, "brpa_exchange_batch: bad arguments");
     if (nworkers <= 0) nworkers = 1;
     if (nworkers > MAX_WORKERS)

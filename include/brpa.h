@@ -488,7 +488,8 @@ int brpa_mbpt3_2(brpa_ctx *ctx, double *ec_exchange, const double *R, const doub
  * on the device-resident amplitudes, per fragment (the reference's
  * rpa_CC.f90:884-892 then 963-976 sequence, subsystem after subsystem).
  * Fragment f uses GuessNVecsT2 = ceil(guess_factor * naux) (the reference's
- * default factor is 3.0, rpa_definitions.f90:428) and guess seed seed + f.
+ * default factor is 3.0, rpa_definitions.f90:428; 0 < guess_factor <= 1e6, else
+ * BRPA_ERR_ARG) and guess seed seed + f.
  * nworkers (1..8, 0 = 1) worker contexts, each with its own stream, rocBLAS
  * handle and workspace and driven by its own host thread, take the fragments
  * one at a time, so that the latency-bound stages of different fragments
