@@ -427,20 +427,7 @@ class Context:
 
         frags: sequence of (R, eocc, evir, freqs, weights).  Returns an array of E.
         """
-        keep = []
-        arr = (_lib.brpa_frag_t * len(frags))()
-        for i, (R, eocc, evir, freqs, weights) in enumerate(frags):
-            eocc = np.ascontiguousarray(eocc, np.float64)
-            evir = np.ascontiguousarray(evir, np.float64)
-            freqs = np.ascontiguousarray(freqs, np.float64)
-            weights = np.ascontiguousarray(weights, np.float64)
-            keep.append((R, eocc, evir, freqs, weights))
-            nai, naux = R.shape
-            if nai != eocc.size * evir.size or freqs.size != weights.size:
-                raise ValueError("fragment %d: shape mismatch" % i)
-            arr[i] = _lib.brpa_frag_t(eocc.size, evir.size, naux, freqs.size, _ptr(R).value,
-                                      _ptr(eocc).value, _ptr(evir).value, _ptr(freqs).value,
-                                      _ptr(weights).value)
+        arr, keep = self._frag_array(frags)   # keep: the arrays arr points into
         e = np.zeros(len(frags))
         self._sync_stream(*[k[0] for k in keep])
         self._check(self._lib.brpa_ecrpa_batch(self._h, _prec(precision), len(frags), arr,
