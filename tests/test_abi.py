@@ -51,3 +51,30 @@ def test_fortran_shim_exports_reference_routines():
     out = subprocess.run([nm, "-D", "--defined-only", lib], capture_output=True, text=True).stdout.lower()
     for name in ("rpa_gpiug_mo", "rpa_cc_ecrpa_analytic", "rpa_cc_t2", "rpa_cc_exchange_sosex"):
         assert name in out, name
+
+
+def test_fragment_array_points_at_kept_arrays():
+    """The brpa_frag_t array handed to the batch entry points points into arrays
+    the binding keeps alive through the call, and it rejects what the library
+    would misread (ragged shapes, non-contiguous or non-float64 R)."""
+    import numpy as np
+
+    import brpa
+    from brpa import synth
+    f = synth.make_fragment(3, 10, 20, seed=1)
+    x = np.linspace(0.1, 1.0, 4)
+    frags = [(f.R, f.eocc.tolist(), f.evir, x, np.ones(4)), (f.R, f.eocc, f.evir, x[:3], np.ones(3))]
+    arr, keep = brpa.Context._frag_array(None, frags)
+    for i, (R, eo, ev, fx, fw) in enumerate(keep):
+        a = arr[i]
+        assert (a.nocc, a.nvir, a.naux, a.nfreqs) == (3, 10, 20, fx.size)
+        assert a.R == R.ctypes.data and a.eocc == eo.ctypes.data and a.evir == ev.ctypes.data
+        assert a.freqs == fx.ctypes.data and a.weights == fw.ctypes.data
+    with pytest.raises(ValueError):          # o*v rows do not match the energies
+        brpa.Context._frag_array(None, [(f.R, f.eocc[:2], f.evir, x, np.ones(4))])
+    with pytest.raises(ValueError):          # frequencies and weights differ in length
+        brpa.Context._frag_array(None, [(f.R, f.eocc, f.evir, x, np.ones(3))])
+    with pytest.raises(ValueError):          # a transposed view is not C-contiguous
+        brpa.Context._frag_array(None, [(np.asfortranarray(f.R), f.eocc, f.evir, x, np.ones(4))])
+    with pytest.raises(ValueError):
+        brpa.Context._frag_array(None, [(f.R.astype(np.float32), f.eocc, f.evir, x, np.ones(4))])
