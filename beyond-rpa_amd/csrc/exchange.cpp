@@ -107,28 +107,39 @@ int brpa_exchange_batch(brpa_ctx *ctx, int precision, int64_t nfrag, const brpa_
             if (q >= nfrag) break;
             const int64_t f = order[q];
             const brpa_frag_t &fr = frags[f];
-            // the explicit-space form unless the S2 rescaling needs the eigenpairs, the
-            // caller asked for the pair order, or a quadrature weight is not positive
-            const bool xspace = !s2_operator && !pairs &&
-                                xspace_applicable(fr.weights, fr.nfreqs, lambda);
-            const int64_t ng = (int64_t)std::ceil(guess_factor * (double)fr.naux);
             int64_t nt = 0;
             double ex = 0.0;
             int rc;
-            if (xspace) {
-                // energy only, in the explicit ov space (FP64)
-                rc = exchange_xspace(w, fr.R, fr.naux, fr.eocc, fr.nocc, fr.evir, fr.nvir,
-                                     fr.freqs, fr.weights, fr.nfreqs, lambda, cutoff, ng,
-                                     seed + (uint64_t)f, alpha, &ex, &nt);
+            if (!fr.R || !fr.eocc || !fr.evir || !fr.freqs || !fr.weights || fr.nocc <= 0 ||
+                fr.nvir <= 0 || fr.naux <= 0 || fr.nfreqs <= 0 || fr.nocc > INT32_MAX ||
+                fr.nvir > INT32_MAX || fr.naux > 65536 || fr.nfreqs > INT32_MAX) {
+                // checked before the guess count and the weights are read
+                rc = brpa_fail(w, BRPA_ERR_ARG, "bad fragment arguments (nocc %lld, nvir %lld, "
+                               "naux %lld, nfreqs %lld)", (long long)fr.nocc, (long long)fr.nvir,
+                               (long long)fr.naux, (long long)fr.nfreqs);
             } else {
-                // the reference's order: amplitudes (the factored core unless the S2
-                // rescaling needs the eigenpairs), then the pair contraction
-                rc = t2_build(w, precision, fr.R, fr.naux, fr.eocc, fr.nocc, fr.evir, fr.nvir,
-                              fr.freqs, fr.weights, fr.nfreqs, lambda, cutoff, ng,
-                              seed + (uint64_t)f, &nt, !s2_operator);
-                if (rc == BRPA_OK && nt > 0)
-                    rc = brpa_sosex(w, precision, &ex, alpha, w->t2_V, w->t2_A, fr.R, fr.nocc,
-                                    fr.nvir, fr.naux, nt, s2_operator);
+                const int64_t ng = (int64_t)std::ceil(guess_factor * (double)fr.naux);
+                // the explicit-space form unless the S2 rescaling needs the eigenpairs,
+                // the caller asked for the pair order, the fragment is beyond the
+                // explicit form's sizes, or a quadrature weight is not positive
+                const bool xspace = !s2_operator && !pairs &&
+                                    xspace_fits(fr.nocc * fr.nvir, fr.naux, ng, fr.nfreqs) &&
+                                    xspace_applicable(fr.weights, fr.nfreqs, lambda);
+                if (xspace) {
+                    // energy only, in the explicit ov space (FP64)
+                    rc = exchange_xspace(w, fr.R, fr.naux, fr.eocc, fr.nocc, fr.evir, fr.nvir,
+                                         fr.freqs, fr.weights, fr.nfreqs, lambda, cutoff, ng,
+                                         seed + (uint64_t)f, alpha, &ex, &nt);
+                } else {
+                    // the reference's order: amplitudes (the factored core unless the S2
+                    // rescaling needs the eigenpairs), then the pair contraction
+                    rc = t2_build(w, precision, fr.R, fr.naux, fr.eocc, fr.nocc, fr.evir,
+                                  fr.nvir, fr.freqs, fr.weights, fr.nfreqs, lambda, cutoff, ng,
+                                  seed + (uint64_t)f, &nt, !s2_operator);
+                    if (rc == BRPA_OK && nt > 0)
+                        rc = brpa_sosex(w, precision, &ex, alpha, w->t2_V, w->t2_A, fr.R,
+                                        fr.nocc, fr.nvir, fr.naux, nt, s2_operator);
+                }
             }
             if (rc != BRPA_OK) {
                 // the lowest failing fragment among those that ran is reported

@@ -252,6 +252,9 @@ int chol_single(brpa_ctx *ctx, const FragDev *d_fr, int64_t *d_pre, int64_t N, i
 
 // Exchange energy of one fragment in the explicit ov space (energy-only form of
 // brpa_t2_compute + brpa_sosex, FP64): E_x and the kept amplitude count.
+// xspace_fits: the sizes the explicit form takes (K = o v, NG guess vectors,
+// F frequencies); larger fragments take the reference's order instead.
+bool xspace_fits(int64_t K, int64_t N, int64_t NG, int64_t F);
 bool xspace_applicable(const double *weights, int64_t nfreqs, double lambda);
 int exchange_xspace(brpa_ctx *ctx, const double *R, int64_t naux, const double *eocc, int64_t nocc,
                     const double *evir, int64_t nvir, const double *freqs, const double *weights,

@@ -73,6 +73,12 @@ int host_copy(brpa_ctx *ctx, std::vector<double> &v, const double *p, int64_t n)
 }
 }  // namespace
 
+bool xspace_fits(int64_t K, int64_t N, int64_t NG, int64_t F) {
+    // ov x ov and NG x NG matrices with 32-bit element offsets within one
+    // matrix, and the F N x ov stack Y
+    return K <= 46340 && N <= 65536 && NG <= 46340 && F <= INT32_MAX && F * N * K <= INT32_MAX;
+}
+
 bool xspace_applicable(const double *weights, int64_t nfreqs, double lambda) {
     if (!(lambda > 0.0)) return false;
     std::vector<double> w(nfreqs);
@@ -97,7 +103,7 @@ int exchange_xspace(brpa_ctx *ctx, const double *R, int64_t naux, const double *
         return brpa_fail(ctx, BRPA_ERR_ARG, "exchange: bad arguments");
     const int64_t K = nocc * nvir, N = naux, F = nfreqs;
     const int64_t NG = nguess > 0 ? nguess : (int64_t)std::ceil(3.0 * (double)N);
-    if (K > 46340 || N > 65536 || NG > 46340 || F * N * K > INT32_MAX)
+    if (!xspace_fits(K, N, NG, F))
         return brpa_fail(ctx, BRPA_ERR_ARG, "exchange: sizes beyond the supported range "
                          "(o*v = %lld, N = %lld, NG = %lld)", (long long)K, (long long)N,
                          (long long)NG);

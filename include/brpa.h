@@ -501,8 +501,10 @@ int brpa_mbpt3_2(brpa_ctx *ctx, double *ec_exchange, const double *R, const doub
  * slots.  ec_exchange[f] receives E_x; nvecs_t2 (may be NULL) the
  * retained NT2.
  * Unless s2_operator is set (the cumulant rescaling needs the amplitude
- * eigenpairs), every quadrature weight is positive and BRPA_EXCHANGE_FORM is
- * not "pairs" in the environment, each fragment runs in the explicit ov
+ * eigenpairs), every quadrature weight is positive, BRPA_EXCHANGE_FORM is
+ * not "pairs" in the environment and the fragment is within the explicit
+ * form's sizes (o*v <= 46340, GuessNVecsT2 <= 46340, nfreqs*naux*o*v < 2^31),
+ * each fragment runs in the explicit ov
  * space, all FP64: I = -sum_k w_k/(4 pi) chi_k R (1 + l Pi_k)^{-1} R^T chi_k is
  * formed as an ov x ov matrix, the basis of span(I Omega) is chosen by the
  * reference's Gram eigenvalue cutoff, the amplitude matrix
