@@ -174,6 +174,12 @@ def test_exchange_batch_matches_sequential(ctx):
     with pytest.raises(brpa.BrpaError) as ei:
         ctx.exchange_batch(frags[:2], seed=77, nworkers=9)
     assert ei.value.code == brpa._lib.ERR_ARG
+    # a fragment without frequencies is rejected before any of its arrays are
+    # read, and the error names it (fragment 1)
+    R, eo, ev, _, _ = frags[1]
+    with pytest.raises(brpa.BrpaError) as ei:
+        ctx.exchange_batch([frags[0], (R, eo, ev, np.zeros(0), np.zeros(0))], seed=77)
+    assert ei.value.code == brpa._lib.ERR_ARG and "fragment 1" in str(ei.value)
 
 
 @pytest.mark.parametrize("precision", ["fp64", "split_i8"])
