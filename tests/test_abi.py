@@ -78,3 +78,43 @@ def test_fragment_array_points_at_kept_arrays():
         brpa.Context._frag_array(None, [(np.asfortranarray(f.R), f.eocc, f.evir, x, np.ones(4))])
     with pytest.raises(ValueError):
         brpa.Context._frag_array(None, [(f.R.astype(np.float32), f.eocc, f.evir, x, np.ones(4))])
+
+
+def _prototypes():
+    text = open(os.path.join(ROOT, "include", "brpa.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return dict(re.findall(r"\b(brpa_[a-z0-9_]+)\s*\(([^;{]*?)\)\s*;", text, flags=re.S))
+
+
+def test_binding_argument_kinds_match_the_header():
+    """Every ctypes signature has the header's parameters in order: pointer,
+    int64_t, int, double or uint64_t (an int passed where the C side reads
+    int64_t would leave the upper half of the register undefined)."""
+    import ctypes as C
+
+    from brpa import _lib
+
+    def kind_h(arg):
+        arg = arg.strip()
+        if "*" in arg:
+            return "ptr"
+        t = arg.rsplit(None, 1)[0].replace("const ", "").strip()
+        return {"int64_t": "i64", "int": "i32", "double": "f64", "uint64_t": "u64"}[t]
+
+    def kind_c(x):
+        if x in (C.c_void_p, C.c_char_p) or (isinstance(x, type) and issubclass(x, C._Pointer)):
+            return "ptr"
+        return {C.c_int64: "i64", C.c_int: "i32", C.c_double: "f64", C.c_uint64: "u64"}[x]
+
+    protos = _prototypes()
+    assert set(protos) == set(declared_functions())
+    for name, args in protos.items():
+        want = [] if args.strip() in ("", "void") else [kind_h(a) for a in args.split(",")]
+        have = [kind_c(x) for x in _lib.SIGNATURES[name][1]]
+        assert have == want, (name, want, have)
+    text = re.sub(r"/\*.*?\*/", "", open(os.path.join(ROOT, "include", "brpa.h")).read(), flags=re.S)
+    rets = dict((n, r) for r, n in re.findall(r"(int|double|const char \*)\s*(brpa_[a-z0-9_]+)\s*\(", text))
+    assert set(rets) == set(protos)
+    for name, r in rets.items():
+        assert _lib.SIGNATURES[name][0] is {"int": C.c_int, "double": C.c_double,
+                                             "const char *": C.c_char_p}[r], name
