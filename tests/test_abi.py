@@ -118,3 +118,50 @@ def test_binding_argument_kinds_match_the_header():
     for name, r in rets.items():
         assert _lib.SIGNATURES[name][0] is {"int": C.c_int, "double": C.c_double,
                                              "const char *": C.c_char_p}[r], name
+
+
+def test_fortran_interfaces_match_the_header():
+    """The bind(C) interfaces of the Fortran module (brpa_mod.f90) pass each
+    parameter as the header declares it: by value for int64_t / int / double
+    (uint64_t as a 64-bit integer, Fortran has no unsigned kind), by
+    reference or as type(c_ptr) for pointers."""
+    src = open(os.path.join(ROOT, "beyond-rpa_amd", "fortran", "brpa_mod.f90")).read()
+    src = re.sub(r"&\s*\n\s*", " ", src)            # join continuation lines
+    blocks = re.findall(r"function\s+\w+\s*\(([^)]*)\)\s*bind\(C,\s*name=\"(brpa_\w+)\"\)(.*?)end function",
+                        src, flags=re.S | re.I)
+    assert len(blocks) >= 15
+    protos = _prototypes()
+
+    def kind_h(arg):
+        arg = arg.strip()
+        if "*" in arg:
+            return "ptr"
+        t = arg.rsplit(None, 1)[0].replace("const ", "").strip()
+        return {"int64_t": "i64", "uint64_t": "i64", "int": "i32", "double": "f64"}[t]
+
+    for arglist, cname, body in blocks:
+        names = [a.strip().lower() for a in arglist.split(",") if a.strip()]
+        decl = {}
+        for line in body.splitlines():
+            if "::" not in line:
+                continue
+            spec, vars_ = line.split("::", 1)
+            spec = spec.strip().lower().replace(" ", "")
+            value = ",value" in spec
+            for v in re.split(r",(?![^(]*\))", vars_):
+                v = v.strip().lower()
+                if not v:
+                    continue
+                v = v.split("(")[0].strip()
+                if not value or spec.startswith("type(c_ptr)"):
+                    decl[v] = "ptr"
+                elif spec.startswith("integer(c_int64_t)"):
+                    decl[v] = "i64"
+                elif spec.startswith("integer(c_int)"):
+                    decl[v] = "i32"
+                elif spec.startswith("real(c_double)"):
+                    decl[v] = "f64"
+        have = [decl[n] for n in names]
+        args = protos[cname]
+        want = [] if args.strip() in ("", "void") else [kind_h(a) for a in args.split(",")]
+        assert have == want, (cname, want, have)
